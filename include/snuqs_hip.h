@@ -1,0 +1,191 @@
+/*
+ * snuqs_hip.h -- C ABI of the MI355X state-vector engine (libsnuqs_hip.so).
+ *
+ * Plain pointers and sizes only.  Every call returns SQ_OK (0) or an error
+ * code; sq_last_error() gives the message of the calling thread's last
+ * failure.  Nothing here exits the process.
+ *
+ * Conventions (the reference's, src/braket/snuqs/csrc/operation/
+ * gate_operations_impl_cpu.cc:22-66):
+ *   - amplitude index i = sum_q b_q * 2^(num_qubits-1-q): qubit 0 is the MSB;
+ *   - a gate on targets [t_0..t_{k-1}] is a row-major 2^k x 2^k complex128
+ *     matrix whose local index has t_0 as its MSB;
+ *   - complex numbers are interleaved (re, im);
+ *   - a state is 2^m contiguous amplitudes of dtype SQ_COMPLEX64 (float re,im)
+ *     or SQ_COMPLEX128 (double re,im) in device memory.
+ *
+ * `stream` arguments are hipStream_t values (NULL = the null stream).  All
+ * device work is asynchronous on that stream unless the comment says the call
+ * returns a host value, in which case the call synchronises the stream.
+ *
+ * Each declaration names the reference interface it replaces
+ * (paths relative to src/braket/snuqs/csrc/).
+ */
+#ifndef SNUQS_HIP_H_
+#define SNUQS_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SQ_OK = 0,
+  SQ_ERR_INVALID = 1,     /* bad argument (targets, sizes, dtype, ...)  */
+  SQ_ERR_HIP = 2,         /* HIP runtime error                          */
+  SQ_ERR_OOM = 3,         /* device allocation failed                   */
+  SQ_ERR_UNSUPPORTED = 4  /* valid request outside what the engine does */
+};
+
+typedef enum { SQ_COMPLEX64 = 0, SQ_COMPLEX128 = 1 } sq_dtype_t;
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char *sq_last_error(void);
+/* Library version string. */
+const char *sq_version(void);
+
+/* ---- device runtime: core/cuda/runtime.cu:4-25 (cu::mem_info,
+ *      device_count, set_device, get_device, device_synchronize) ---------- */
+int sq_mem_info(size_t *free_bytes, size_t *total_bytes);
+int sq_device_count(int *count);
+int sq_set_device(int device);
+int sq_get_device(int *device);
+int sq_device_synchronize(void);
+
+/* ---- memory: buffer/buffer_cuda.cc:13-20 (cudaMalloc/cudaFree),
+ *      buffer/buffer_cpu.cc:13-28 (pinned host), core/runtime.cc:46-65
+ *      (memcpyH2D/D2H/D2D) ---------------------------------------------- */
+int sq_malloc(void **ptr, size_t bytes);
+int sq_free(void *ptr);
+int sq_host_alloc_pinned(void **ptr, size_t bytes);
+int sq_host_free_pinned(void *ptr);
+int sq_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int sq_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int sq_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+
+/* ---- streams and events: stream/stream.cc:6-34, event/event.cc:5-26 ---- */
+int sq_stream_create(void **stream, int nonblocking);
+int sq_stream_destroy(void *stream);
+int sq_stream_synchronize(void *stream);
+int sq_stream_wait_event(void *stream, void *event);
+int sq_event_create(void **event);
+int sq_event_destroy(void *event);
+int sq_event_record(void *event, void *stream);
+int sq_event_synchronize(void *event);
+int sq_event_elapsed_ms(float *ms, void *start, void *stop);
+
+/* ---- initialisation: functionals/functionals.cc:30-50 ->
+ *      operation/initialization_impl_cuda.cu:9-38 ------------------------ */
+int sq_initialize_zero(void *sv, sq_dtype_t dtype, uint64_t num_elems,
+                       void *stream);
+int sq_initialize_basis_z(void *sv, sq_dtype_t dtype, uint64_t num_elems,
+                          void *stream);
+
+/* ---- gate application: functionals::apply (functionals/functionals.cc:15-28)
+ *      -> cu::applyGate (operation/gate_operations_impl_cuda.cu:159-175).
+ *
+ * Applies `matrix` (row-major complex128, 2^k x 2^k, k = num_targets <= 5) on
+ * `targets` of a `num_qubits`-qubit index space to the state `sv` of
+ * `num_elems` amplitudes, in place.  As in the reference, num_qubits may
+ * exceed log2(num_elems) (the state is then the first slice of a larger
+ * space); every target stride 2^(num_qubits-1-t) must be < num_elems.
+ * k = 0 multiplies every amplitude by matrix[0] (GPhase).  The gate class
+ * (phase / diagonal / permutation / dense, with control qubits factored out)
+ * is chosen from the matrix entries.  Asynchronous on `stream`. */
+int sq_apply(void *sv, sq_dtype_t dtype, uint64_t num_elems,
+             uint32_t num_qubits, const uint32_t *targets,
+             uint32_t num_targets, const double *matrix, void *stream);
+
+/* ---- gate programs: a whole gate list uploaded once, run many times.
+ * Replaces the per-gate Python loop of simulation.py:274-284 (each gate
+ * matrix is classified and copied to the device once, at creation).
+ * gates: num_gates entries; gate g has num_targets[g] targets taken in order
+ * from `targets` and a (2^k)^2 complex128 matrix taken in order from
+ * `matrices`.  k <= 5 for any gate. */
+int sq_program_create(void **program, sq_dtype_t dtype, uint32_t num_qubits,
+                      uint32_t num_gates, const uint32_t *num_targets,
+                      const uint32_t *targets, const double *matrices);
+int sq_program_destroy(void *program);
+/* Number of kernels a run launches and its algorithmic HBM bytes
+ * (2 x touched amplitudes x itemsize, summed over the gates). */
+int sq_program_stats(void *program, uint64_t *num_launches,
+                     double *algorithmic_bytes);
+int sq_program_run(void *program, void *sv, uint64_t num_elems, void *stream);
+
+/* ---- results: simulation.py:196-220 (state_vector, probabilities,
+ *      retrieve_samples) and the expectation/inner-product reductions that
+ *      the reference lacks (result_types.py:149-151 registers StateVector
+ *      only).  Reductions accumulate in FP64 for both dtypes. --------------- */
+
+/* probs[i] = |sv[i]|^2 in float64, device -> device (np.abs(sv)**2). */
+int sq_probabilities(const void *sv, sq_dtype_t dtype, uint64_t num_elems,
+                     double *probs, void *stream);
+/* sum_i |sv[i]|^2 in FP64; returns a host value. */
+int sq_norm2(const void *sv, sq_dtype_t dtype, uint64_t num_elems,
+             double *out, void *stream);
+/* sv *= (re + i im) in place (norm renormalisation, global phase). */
+int sq_scale(void *sv, sq_dtype_t dtype, uint64_t num_elems, double re,
+             double im, void *stream);
+/* out[2*j], out[2*j+1] = sv[indices[j]] as float64 (host arrays). */
+int sq_amplitudes(const void *sv, sq_dtype_t dtype, uint64_t num_elems,
+                  const uint64_t *indices, uint64_t count, double *out,
+                  void *stream);
+/* <a|b> in FP64, returns (re, im) on the host. */
+int sq_inner(const void *a, const void *b, sq_dtype_t dtype,
+             uint64_t num_elems, double *out, void *stream);
+/* sum_t coeffs[t] <sv| P_t |sv> in FP64; P_t given by xmask[t], zmask[t]
+ * (amplitude-index bit masks: X or Y set the x bit, Z or Y the z bit) and the
+ * number of Y factors ny[t].  Returns the real part on the host; per-term
+ * values go to term_values (host, may be NULL). */
+int sq_expect_pauli_sum(const void *sv, sq_dtype_t dtype, uint64_t num_elems,
+                        uint32_t num_terms, const uint64_t *xmask,
+                        const uint64_t *zmask, const uint32_t *ny,
+                        const double *coeffs, double *out,
+                        double *term_values, void *stream);
+/* out = sum_t coeffs[t] P_t |in>  (device -> device, out != in). */
+int sq_apply_pauli_sum(const void *in, void *out, sq_dtype_t dtype,
+                       uint64_t num_elems, uint32_t num_terms,
+                       const uint64_t *xmask, const uint64_t *zmask,
+                       const uint32_t *ny, const double *coeffs, void *stream);
+
+/* Sampling: simulation.py:217-220 (np.random.choice(N, p=|sv|^2, size)).
+ * For each host uniform u[s] in [0,1), out[s] = the first index i with
+ * cdf[i] > u[s], where cdf = cumsum(p) / cumsum(p)[N-1], p = probs (device,
+ * float64, from sq_probabilities).  exact != 0 reproduces numpy's sequential
+ * float64 cumsum bit for bit; exact == 0 uses an FP64 parallel scan.  out is
+ * a host array of int64.  Returns when the samples are on the host. */
+int sq_sample(const double *probs, uint64_t num_elems, const double *uniforms,
+              uint64_t shots, int exact, int64_t *out, void *stream);
+
+/* ---- adjoint differentiation (new; no reference counterpart).
+ * For the program U = G_{m-1}...G_0 and observable H = sum_t c_t P_t the
+ * call runs the forward pass itself (sv <- U|0..0>), returns the energy
+ * <psi|H|psi> in *energy and
+ *   grads[p] = sum over derivative entries d with deriv_param[d] == p of
+ *              2 Re <psi| H U_{>g} dG_d U_{<g} |0..0>,   g = deriv_gate[d],
+ * i.e. d<psi|H|psi>/d theta_p when dG_d = dG_g/d theta_p (a gate with several
+ * parameters has one entry per parameter; a parameter shared by several gates
+ * accumulates).  dmatrices holds the entries' (2^k)^2 complex128 matrices
+ * back to back, k = targets of gate deriv_gate[d].
+ * The reverse sweep applies G_g^dagger to the state and to lambda = H psi
+ * (one FP64 inner product per entry).  num_checkpoints device buffers of
+ * num_elems amplitudes store forward states at evenly spaced gates; the
+ * reverse sweep reloads them instead of undoing those segments, bounding the
+ * drift of complex64 reverse sweeps.  work0/work1: scratch states.
+ * Host arrays: deriv_*, dmatrices, xmask/zmask/ny/coeffs, energy, grads. */
+int sq_adjoint_gradient(void *program, uint32_t num_derivs,
+                        const uint32_t *deriv_gate, const uint32_t *deriv_param,
+                        const double *dmatrices, uint32_t num_params,
+                        void *sv, void *work0, void *work1,
+                        void *const *checkpoints, uint32_t num_checkpoints,
+                        uint64_t num_elems, uint32_t num_terms,
+                        const uint64_t *xmask, const uint64_t *zmask,
+                        const uint32_t *ny, const double *coeffs,
+                        double *energy, double *grads, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SNUQS_HIP_H_ */

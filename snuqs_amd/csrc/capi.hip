@@ -1,0 +1,701 @@
+// capi.hip -- the extern "C" boundary of libsnuqs_hip.so (include/snuqs_hip.h).
+//
+// Reference counterparts (src/braket/snuqs/csrc/):
+//   runtime  core/cuda/runtime.cu:4-25          memory  buffer/buffer_cuda.cc:13-20
+//   streams  stream/stream.cc:6-34              events  event/event.cc:5-26
+//   init     functionals/functionals.cc:30-50   apply   functionals/functionals.cc:15-28
+// The reference exits the process on any CUDA error (utils_cuda.h:8-17); here
+// every entry point returns a status and records a message instead.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "kernels.hpp"
+#include "kernels_reduce.hpp"
+
+namespace sq {
+const std::string &last_error();
+
+// ---- per-device workspace ---------------------------------------------------
+static std::mutex g_ws_mutex[64];
+static Workspace g_ws[64];
+
+static int current_device(int *dev) {
+  SQ_HIP(hipGetDevice(dev));
+  if (*dev < 0 || *dev >= 64) return fail(SQ_ERR_UNSUPPORTED, "device index >= 64");
+  return SQ_OK;
+}
+
+// Caller holds g_ws_mutex[dev].
+static int ws_reserve(int dev, size_t dev_bytes, size_t host_bytes, Workspace **out) {
+  Workspace &w = g_ws[dev];
+  if (w.dev_bytes < dev_bytes) {
+    if (w.dev) SQ_HIP(hipFree(w.dev));
+    w.dev = nullptr;
+    w.dev_bytes = 0;
+    size_t b = std::max(dev_bytes, (size_t)1 << 20);
+    if (hipMalloc(&w.dev, b) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(SQ_ERR_OOM, "workspace: device allocation of " + std::to_string(b) +
+                                  " bytes failed");
+    }
+    w.dev_bytes = b;
+  }
+  if (w.host_bytes < host_bytes) {
+    if (w.host) SQ_HIP(hipHostFree(w.host));
+    w.host = nullptr;
+    w.host_bytes = 0;
+    size_t b = std::max(host_bytes, (size_t)1 << 16);
+    SQ_HIP(hipHostMalloc(&w.host, b, hipHostMallocDefault));
+    w.host_bytes = b;
+  }
+  *out = &w;
+  return SQ_OK;
+}
+
+static inline char *al(char *p) { return (char *)(((uintptr_t)p + 255) & ~(uintptr_t)255); }
+
+static int check_state(const void *sv, int dt, uint64_t n) {
+  if (!valid_dtype(dt)) return fail(SQ_ERR_INVALID, "dtype must be SQ_COMPLEX64 or SQ_COMPLEX128");
+  if (!is_pow2(n)) return fail(SQ_ERR_INVALID, "num_elems must be a power of two");
+  if (!sv) return fail(SQ_ERR_INVALID, "null state pointer");
+  return SQ_OK;
+}
+
+}  // namespace sq
+
+using namespace sq;
+
+extern "C" {
+
+const char *sq_last_error(void) { return last_error().c_str(); }
+const char *sq_version(void) { return "snuqs-hip 0.1 (gfx950)"; }
+
+// ---- runtime ----------------------------------------------------------------
+int sq_mem_info(size_t *free_bytes, size_t *total_bytes) {
+  SQ_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return SQ_OK;
+}
+int sq_device_count(int *count) {
+  SQ_HIP(hipGetDeviceCount(count));
+  return SQ_OK;
+}
+int sq_set_device(int device) {
+  SQ_HIP(hipSetDevice(device));
+  return SQ_OK;
+}
+int sq_get_device(int *device) {
+  SQ_HIP(hipGetDevice(device));
+  return SQ_OK;
+}
+int sq_device_synchronize(void) {
+  SQ_HIP(hipDeviceSynchronize());
+  return SQ_OK;
+}
+
+// ---- memory -----------------------------------------------------------------
+int sq_malloc(void **ptr, size_t bytes) {
+  if (!ptr) return fail(SQ_ERR_INVALID, "null output pointer");
+  *ptr = nullptr;
+  if (bytes == 0) return SQ_OK;
+  if (hipMalloc(ptr, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    return fail(SQ_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  return SQ_OK;
+}
+int sq_free(void *ptr) {
+  if (ptr) SQ_HIP(hipFree(ptr));
+  return SQ_OK;
+}
+int sq_host_alloc_pinned(void **ptr, size_t bytes) {
+  if (!ptr) return fail(SQ_ERR_INVALID, "null output pointer");
+  if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    return fail(SQ_ERR_OOM, "hipHostMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  return SQ_OK;
+}
+int sq_host_free_pinned(void *ptr) {
+  if (ptr) SQ_HIP(hipHostFree(ptr));
+  return SQ_OK;
+}
+int sq_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream) {
+  SQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  return SQ_OK;
+}
+int sq_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
+  SQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  return SQ_OK;
+}
+int sq_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream) {
+  SQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return SQ_OK;
+}
+
+// ---- streams / events (default priority only) ------------------------------
+int sq_stream_create(void **stream, int nonblocking) {
+  hipStream_t s;
+  if (nonblocking)
+    SQ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  else
+    SQ_HIP(hipStreamCreate(&s));
+  *stream = (void *)s;
+  return SQ_OK;
+}
+int sq_stream_destroy(void *stream) {
+  if (stream) SQ_HIP(hipStreamDestroy((hipStream_t)stream));
+  return SQ_OK;
+}
+int sq_stream_synchronize(void *stream) {
+  SQ_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return SQ_OK;
+}
+int sq_stream_wait_event(void *stream, void *event) {
+  SQ_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
+  return SQ_OK;
+}
+int sq_event_create(void **event) {
+  hipEvent_t e;
+  SQ_HIP(hipEventCreate(&e));
+  *event = (void *)e;
+  return SQ_OK;
+}
+int sq_event_destroy(void *event) {
+  if (event) SQ_HIP(hipEventDestroy((hipEvent_t)event));
+  return SQ_OK;
+}
+int sq_event_record(void *event, void *stream) {
+  SQ_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+  return SQ_OK;
+}
+int sq_event_synchronize(void *event) {
+  SQ_HIP(hipEventSynchronize((hipEvent_t)event));
+  return SQ_OK;
+}
+int sq_event_elapsed_ms(float *ms, void *start, void *stop) {
+  SQ_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return SQ_OK;
+}
+
+// ---- init -------------------------------------------------------------------
+int sq_initialize_zero(void *sv, sq_dtype_t dt, uint64_t n, void *stream) {
+  SQ_TRY(check_state(sv, dt, n));
+  return launch_init(sv, dt, n, false, (hipStream_t)stream);
+}
+int sq_initialize_basis_z(void *sv, sq_dtype_t dt, uint64_t n, void *stream) {
+  SQ_TRY(check_state(sv, dt, n));
+  return launch_init(sv, dt, n, true, (hipStream_t)stream);
+}
+
+// ---- apply ------------------------------------------------------------------
+int sq_apply(void *sv, sq_dtype_t dt, uint64_t n, uint32_t num_qubits,
+             const uint32_t *targets, uint32_t k, const double *matrix, void *stream) {
+  SQ_TRY(check_state(sv, dt, n));
+  if (!matrix || (k && !targets)) return fail(SQ_ERR_INVALID, "null matrix or targets");
+  if (num_qubits > 63) return fail(SQ_ERR_INVALID, "num_qubits must be < 64");
+  GatePlan p;
+  SQ_TRY(make_plan(matrix, targets, (int)k, (int)num_qubits, log2u(n), &p));
+  hipStream_t st = (hipStream_t)stream;
+  if (p.kind == Kind::DENSE && p.kk >= 4) {
+    // 4-5 target dense gates need the matrix in device memory: stage it in the
+    // workspace and finish before the workspace is released.
+    int dev;
+    SQ_TRY(current_device(&dev));
+    std::lock_guard<std::mutex> lk(g_ws_mutex[dev]);
+    std::vector<double> g = fused_real_block(p);
+    const size_t esz = dt == SQ_COMPLEX64 ? 4 : 8;
+    const size_t bytes = g.size() * esz;
+    Workspace *w;
+    SQ_TRY(ws_reserve(dev, bytes, bytes, &w));
+    if (dt == SQ_COMPLEX64) {
+      float *h = (float *)w->host;
+      for (size_t i = 0; i < g.size(); ++i) h[i] = (float)g[i];
+    } else {
+      std::memcpy(w->host, g.data(), bytes);
+    }
+    SQ_HIP(hipMemcpyAsync(w->dev, w->host, bytes, hipMemcpyHostToDevice, st));
+    SQ_TRY(launch_plan(p, sv, dt, n, st, w->dev));
+    SQ_HIP(hipStreamSynchronize(st));
+    return SQ_OK;
+  }
+  return launch_plan(p, sv, dt, n, st, nullptr);
+}
+
+}  // extern "C"
+
+// ---- programs -----------------------------------------------------------------
+namespace sq {
+
+struct Program {
+  sq_dtype_t dt;
+  int nq;
+  int device;
+  std::vector<GatePlan> fwd, inv;
+  std::vector<int> nt;                 // targets per gate
+  std::vector<size_t> toff;            // offset of a gate's targets
+  std::vector<uint32_t> targets;
+  std::vector<const void *> fwd_dev, inv_dev;  // real-block matrices (kk >= 4)
+  void *dev_mats = nullptr;
+  uint64_t launches = 0;
+  double bytes = 0.0;
+};
+
+// Upload the real-block matrices of the dense 4-5 target plans in `plans`.
+static int upload_blocks(const std::vector<const GatePlan *> &plans, sq_dtype_t dt,
+                         void **dev_base, std::vector<const void *> *ptrs) {
+  const size_t esz = dt == SQ_COMPLEX64 ? 4 : 8;
+  std::vector<size_t> off(plans.size(), (size_t)-1);
+  size_t total = 0;
+  for (size_t i = 0; i < plans.size(); ++i) {
+    const GatePlan &p = *plans[i];
+    if (p.kind == Kind::DENSE && p.kk >= 4) {
+      const size_t D = (size_t)2 << p.kk;
+      off[i] = total;
+      total += ((D * D * esz + 255) / 256) * 256;
+    }
+  }
+  ptrs->assign(plans.size(), nullptr);
+  *dev_base = nullptr;
+  if (total == 0) return SQ_OK;
+  std::vector<char> host(total);
+  for (size_t i = 0; i < plans.size(); ++i) {
+    if (off[i] == (size_t)-1) continue;
+    std::vector<double> g = fused_real_block(*plans[i]);
+    if (dt == SQ_COMPLEX64) {
+      float *h = (float *)(host.data() + off[i]);
+      for (size_t e = 0; e < g.size(); ++e) h[e] = (float)g[e];
+    } else {
+      std::memcpy(host.data() + off[i], g.data(), g.size() * 8);
+    }
+  }
+  if (hipMalloc(dev_base, total) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(SQ_ERR_OOM, "program matrices: device allocation failed");
+  }
+  SQ_HIP(hipMemcpy(*dev_base, host.data(), total, hipMemcpyHostToDevice));
+  for (size_t i = 0; i < plans.size(); ++i)
+    if (off[i] != (size_t)-1) (*ptrs)[i] = (const char *)*dev_base + off[i];
+  return SQ_OK;
+}
+
+}  // namespace sq
+
+extern "C" {
+
+int sq_program_create(void **program, sq_dtype_t dt, uint32_t nq, uint32_t ng,
+                      const uint32_t *num_targets, const uint32_t *targets,
+                      const double *matrices) {
+  if (!program) return fail(SQ_ERR_INVALID, "null output pointer");
+  *program = nullptr;
+  if (!valid_dtype(dt)) return fail(SQ_ERR_INVALID, "bad dtype");
+  if (nq < 1 || nq > 40) return fail(SQ_ERR_INVALID, "num_qubits must be in [1, 40]");
+  auto prog = std::make_unique<Program>();
+  prog->dt = dt;
+  prog->nq = (int)nq;
+  SQ_TRY(current_device(&prog->device));
+  size_t toff = 0, moff = 0;
+  prog->fwd.resize(ng);
+  prog->inv.resize(ng);
+  for (uint32_t g = 0; g < ng; ++g) {
+    const int k = (int)num_targets[g];
+    if (k > kMaxK) return fail(SQ_ERR_UNSUPPORTED, "gate with more than 5 targets");
+    const uint32_t *t = targets + toff;
+    const double *m = matrices + moff;
+    SQ_TRY(make_plan(m, t, k, (int)nq, (int)nq, &prog->fwd[g]));
+    std::vector<double> md = dagger(m, k);
+    SQ_TRY(make_plan(md.data(), t, k, (int)nq, (int)nq, &prog->inv[g]));
+    prog->nt.push_back(k);
+    prog->toff.push_back(toff);
+    prog->targets.insert(prog->targets.end(), t, t + k);
+    toff += k;
+    moff += 2ull << (2 * k);
+    if (prog->fwd[g].kind != Kind::NOOP) {
+      prog->launches++;
+      prog->bytes += 2.0 * prog->fwd[g].touched * std::ldexp(1.0, (int)nq) * itemsize(dt);
+    }
+  }
+  std::vector<const GatePlan *> all;
+  for (auto &p : prog->fwd) all.push_back(&p);
+  for (auto &p : prog->inv) all.push_back(&p);
+  std::vector<const void *> ptrs;
+  SQ_TRY(upload_blocks(all, dt, &prog->dev_mats, &ptrs));
+  prog->fwd_dev.assign(ptrs.begin(), ptrs.begin() + ng);
+  prog->inv_dev.assign(ptrs.begin() + ng, ptrs.end());
+  *program = prog.release();
+  return SQ_OK;
+}
+
+int sq_program_destroy(void *program) {
+  Program *p = (Program *)program;
+  if (!p) return SQ_OK;
+  if (p->dev_mats) SQ_HIP(hipFree(p->dev_mats));
+  delete p;
+  return SQ_OK;
+}
+
+int sq_program_stats(void *program, uint64_t *num_launches, double *bytes) {
+  Program *p = (Program *)program;
+  if (!p) return fail(SQ_ERR_INVALID, "null program");
+  if (num_launches) *num_launches = p->launches;
+  if (bytes) *bytes = p->bytes;
+  return SQ_OK;
+}
+
+int sq_program_run(void *program, void *sv, uint64_t n, void *stream) {
+  Program *p = (Program *)program;
+  if (!p) return fail(SQ_ERR_INVALID, "null program");
+  SQ_TRY(check_state(sv, p->dt, n));
+  if (n != (1ull << p->nq))
+    return fail(SQ_ERR_INVALID, "program built for " + std::to_string(p->nq) +
+                                    " qubits, state has " + std::to_string(n) + " amplitudes");
+  hipStream_t st = (hipStream_t)stream;
+  for (size_t g = 0; g < p->fwd.size(); ++g)
+    SQ_TRY(launch_plan(p->fwd[g], sv, p->dt, n, st, p->fwd_dev[g]));
+  return SQ_OK;
+}
+
+// ---- results ----------------------------------------------------------------
+int sq_probabilities(const void *sv, sq_dtype_t dt, uint64_t n, double *probs, void *stream) {
+  SQ_TRY(check_state(sv, dt, n));
+  if (!probs) return fail(SQ_ERR_INVALID, "null output");
+  return launch_probabilities(sv, dt, n, probs, (hipStream_t)stream);
+}
+
+int sq_norm2(const void *sv, sq_dtype_t dt, uint64_t n, double *out, void *stream) {
+  SQ_TRY(check_state(sv, dt, n));
+  int dev;
+  SQ_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mutex[dev]);
+  Workspace *w;
+  SQ_TRY(ws_reserve(dev, (kRedBlocks + 64) * sizeof(double), 64, &w));
+  hipStream_t st = (hipStream_t)stream;
+  double *res = (double *)w->dev;
+  double *partial = res + 32;
+  SQ_TRY(launch_norm2(sv, dt, n, partial, res, st));
+  SQ_HIP(hipMemcpyAsync(w->host, res, sizeof(double), hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  *out = *(double *)w->host;
+  return SQ_OK;
+}
+
+int sq_scale(void *sv, sq_dtype_t dt, uint64_t n, double re, double im, void *stream) {
+  SQ_TRY(check_state(sv, dt, n));
+  return launch_scale(sv, dt, n, re, im, (hipStream_t)stream);
+}
+
+int sq_amplitudes(const void *sv, sq_dtype_t dt, uint64_t n, const uint64_t *indices,
+                  uint64_t count, double *out, void *stream) {
+  SQ_TRY(check_state(sv, dt, n));
+  if (count == 0) return SQ_OK;
+  for (uint64_t j = 0; j < count; ++j)
+    if (indices[j] >= n) return fail(SQ_ERR_INVALID, "amplitude index out of range");
+  int dev;
+  SQ_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mutex[dev]);
+  const size_t ib = count * 8, ob = count * 16;
+  Workspace *w;
+  SQ_TRY(ws_reserve(dev, ib + ob + 512, ib + ob + 512, &w));
+  hipStream_t st = (hipStream_t)stream;
+  char *hb = (char *)w->host;
+  std::memcpy(hb, indices, ib);
+  uint64_t *didx = (uint64_t *)w->dev;
+  double *dout = (double *)al((char *)w->dev + ib);
+  SQ_HIP(hipMemcpyAsync(didx, hb, ib, hipMemcpyHostToDevice, st));
+  SQ_TRY(launch_gather(sv, dt, didx, count, dout, st));
+  char *hout = al(hb + ib);
+  SQ_HIP(hipMemcpyAsync(hout, dout, ob, hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  std::memcpy(out, hout, ob);
+  return SQ_OK;
+}
+
+int sq_inner(const void *a, const void *b, sq_dtype_t dt, uint64_t n, double *out,
+             void *stream) {
+  SQ_TRY(check_state(a, dt, n));
+  SQ_TRY(check_state(b, dt, n));
+  int dev;
+  SQ_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mutex[dev]);
+  Workspace *w;
+  SQ_TRY(ws_reserve(dev, (2 * kRedBlocks + 64) * sizeof(double), 64, &w));
+  hipStream_t st = (hipStream_t)stream;
+  double *res = (double *)w->dev;
+  double *partial = res + 32;
+  SQ_TRY(launch_inner(a, b, dt, n, partial, res, nullptr, 0.0, st));
+  SQ_HIP(hipMemcpyAsync(w->host, res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  out[0] = ((double *)w->host)[0];
+  out[1] = ((double *)w->host)[1];
+  return SQ_OK;
+}
+
+}  // extern "C"
+
+namespace sq {
+
+static int check_terms(uint32_t nterms, const uint64_t *x, const uint64_t *z,
+                       const uint32_t *ny, uint64_t n) {
+  for (uint32_t t = 0; t < nterms; ++t) {
+    if (x[t] >= n || z[t] >= n) return fail(SQ_ERR_INVALID, "Pauli mask beyond the state");
+    if ((uint32_t)__builtin_popcountll(x[t] & z[t]) != ny[t])
+      return fail(SQ_ERR_INVALID, "ny must equal popcount(xmask & zmask)");
+  }
+  return SQ_OK;
+}
+
+// Device layout of an expectation call: partial sums, term table, per-term
+// factors, per-term values.
+struct ExpectLayout {
+  double *partial;
+  void *terms;
+  double *alpha;
+  double *vals;
+};
+static size_t expect_partial_bytes() { return (size_t)(kRedBlocks * 64 + 64) * sizeof(double); }
+static size_t expect_ws_dev(uint32_t nterms) {
+  return expect_partial_bytes() + nterms * (pauli_term_bytes() + 2 * sizeof(double)) + 1024;
+}
+static ExpectLayout expect_layout(Workspace *w, uint32_t nterms) {
+  ExpectLayout L;
+  char *d = (char *)w->dev;
+  L.partial = (double *)d;
+  L.terms = al(d + expect_partial_bytes());
+  L.alpha = (double *)al((char *)L.terms + nterms * pauli_term_bytes());
+  L.vals = (double *)al((char *)L.alpha + nterms * sizeof(double));
+  return L;
+}
+
+// Per-term values <P_t> (times coeffs[t]) into L.vals.  Caller holds the
+// workspace lock and reserved expect_ws_dev(nterms) device bytes.
+static int expect_terms_locked(Workspace *w, const void *sv, sq_dtype_t dt, uint64_t n,
+                               uint32_t nterms, const uint64_t *x, const uint64_t *z,
+                               const uint32_t *ny, const double *coeffs, hipStream_t st) {
+  ExpectLayout L = expect_layout(w, nterms);
+  const size_t tb = nterms * pauli_term_bytes(), ab = nterms * sizeof(double);
+  char *h = (char *)w->host;
+  double *halpha = (double *)al(h + tb);
+  fill_pauli_terms((int)nterms, x, z, ny, coeffs, h, halpha);
+  SQ_HIP(hipMemcpyAsync(L.terms, h, tb, hipMemcpyHostToDevice, st));
+  SQ_HIP(hipMemcpyAsync(L.alpha, halpha, ab, hipMemcpyHostToDevice, st));
+  // at most 64 terms per launch (partials hold 64 x kRedBlocks values)
+  for (uint32_t t0 = 0; t0 < nterms; t0 += 64) {
+    const uint32_t c = std::min<uint32_t>(64, nterms - t0);
+    SQ_TRY(launch_pauli_terms(sv, dt, n, (int)c,
+                              (const char *)L.terms + t0 * pauli_term_bytes(),
+                              L.alpha + t0, L.partial, L.vals + t0, st));
+  }
+  return SQ_OK;
+}
+
+}  // namespace sq
+
+extern "C" {
+
+int sq_expect_pauli_sum(const void *sv, sq_dtype_t dt, uint64_t n, uint32_t nterms,
+                        const uint64_t *x, const uint64_t *z, const uint32_t *ny,
+                        const double *coeffs, double *out, double *term_values,
+                        void *stream) {
+  SQ_TRY(check_state(sv, dt, n));
+  SQ_TRY(check_terms(nterms, x, z, ny, n));
+  if (nterms == 0) {
+    *out = 0.0;
+    return SQ_OK;
+  }
+  int dev;
+  SQ_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mutex[dev]);
+  Workspace *w;
+  SQ_TRY(ws_reserve(dev, expect_ws_dev(nterms), nterms * 64 + 4096, &w));
+  hipStream_t st = (hipStream_t)stream;
+  // per-term values without the coefficient, then weight on the host
+  std::vector<double> ones(nterms, 1.0);
+  SQ_TRY(expect_terms_locked(w, sv, dt, n, nterms, x, z, ny, ones.data(), st));
+  std::vector<double> vals(nterms);
+  SQ_HIP(hipMemcpyAsync(w->host, expect_layout(w, nterms).vals, nterms * sizeof(double),
+                        hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  std::memcpy(vals.data(), w->host, nterms * sizeof(double));
+  double acc = 0.0;
+  for (uint32_t t = 0; t < nterms; ++t) acc += (coeffs ? coeffs[t] : 1.0) * vals[t];
+  *out = acc;
+  if (term_values) std::memcpy(term_values, vals.data(), nterms * sizeof(double));
+  return SQ_OK;
+}
+
+int sq_apply_pauli_sum(const void *in, void *out, sq_dtype_t dt, uint64_t n, uint32_t nterms,
+                       const uint64_t *x, const uint64_t *z, const uint32_t *ny,
+                       const double *coeffs, void *stream) {
+  SQ_TRY(check_state(in, dt, n));
+  SQ_TRY(check_state(out, dt, n));
+  if (in == out) return fail(SQ_ERR_INVALID, "apply_pauli_sum: out must differ from in");
+  SQ_TRY(check_terms(nterms, x, z, ny, n));
+  if (!coeffs) return fail(SQ_ERR_INVALID, "null coeffs");
+  int dev;
+  SQ_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mutex[dev]);
+  const size_t tb = nterms * pauli_apply_term_bytes() + 256;
+  Workspace *w;
+  SQ_TRY(ws_reserve(dev, tb, tb, &w));
+  hipStream_t st = (hipStream_t)stream;
+  fill_pauli_apply_terms((int)nterms, x, z, ny, coeffs, w->host);
+  SQ_HIP(hipMemcpyAsync(w->dev, w->host, tb, hipMemcpyHostToDevice, st));
+  SQ_TRY(launch_apply_pauli_sum(in, out, dt, n, (int)nterms, w->dev, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  return SQ_OK;
+}
+
+int sq_sample(const double *probs, uint64_t n, const double *uniforms, uint64_t shots,
+              int exact, int64_t *out, void *stream) {
+  if (!probs || !is_pow2(n)) return fail(SQ_ERR_INVALID, "bad probabilities");
+  if (shots == 0) return SQ_OK;
+  if (!uniforms || !out) return fail(SQ_ERR_INVALID, "null uniforms or output");
+  int dev;
+  SQ_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mutex[dev]);
+  const size_t cdf_b = n * sizeof(double) + 256;
+  const size_t scr_b = sample_scratch_bytes(n, exact != 0) + 256;
+  const size_t io_b = shots * 16 + 512;
+  Workspace *w;
+  SQ_TRY(ws_reserve(dev, cdf_b + scr_b + io_b + 1024, io_b, &w));
+  hipStream_t st = (hipStream_t)stream;
+  char *d = (char *)w->dev;
+  double *cdf = (double *)d;
+  char *scr = al(d + cdf_b);
+  double *du = (double *)al(scr + scr_b);
+  int64_t *dout = (int64_t *)al((char *)du + shots * 8);
+  SQ_TRY(sample_cdf(probs, n, cdf, exact != 0, scr, scr_b, st));
+  std::memcpy(w->host, uniforms, shots * 8);
+  SQ_HIP(hipMemcpyAsync(du, w->host, shots * 8, hipMemcpyHostToDevice, st));
+  SQ_TRY(launch_search(cdf, n, du, shots, dout, st));
+  SQ_HIP(hipMemcpyAsync(w->host, dout, shots * 8, hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  std::memcpy(out, w->host, shots * 8);
+  return SQ_OK;
+}
+
+// ---- adjoint ----------------------------------------------------------------
+int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deriv_gate,
+                        const uint32_t *deriv_param, const double *dmatrices,
+                        uint32_t num_params, void *sv, void *work0, void *work1,
+                        void *const *checkpoints, uint32_t num_checkpoints,
+                        uint64_t n, uint32_t nterms, const uint64_t *x, const uint64_t *z,
+                        const uint32_t *ny, const double *coeffs, double *energy,
+                        double *grads, void *stream) {
+  Program *p = (Program *)program;
+  if (!p) return fail(SQ_ERR_INVALID, "null program");
+  const sq_dtype_t dt = p->dt;
+  SQ_TRY(check_state(sv, dt, n));
+  SQ_TRY(check_state(work0, dt, n));
+  SQ_TRY(check_state(work1, dt, n));
+  if (n != (1ull << p->nq)) return fail(SQ_ERR_INVALID, "state size does not match program");
+  if (sv == work0 || sv == work1 || work0 == work1)
+    return fail(SQ_ERR_INVALID, "sv, work0 and work1 must be distinct");
+  SQ_TRY(check_terms(nterms, x, z, ny, n));
+  if (!coeffs && nterms) return fail(SQ_ERR_INVALID, "null coeffs");
+  for (uint32_t c = 0; c < num_checkpoints; ++c)
+    SQ_TRY(check_state(checkpoints[c], dt, n));
+  const uint32_t ng = (uint32_t)p->fwd.size();
+
+  // derivative plans, grouped by gate
+  std::vector<GatePlan> dplans(num_derivs);
+  std::vector<std::vector<uint32_t>> by_gate(ng);
+  size_t moff = 0;
+  for (uint32_t d = 0; d < num_derivs; ++d) {
+    const uint32_t g = deriv_gate[d];
+    if (g >= ng) return fail(SQ_ERR_INVALID, "deriv_gate out of range");
+    if (deriv_param[d] >= num_params) return fail(SQ_ERR_INVALID, "deriv_param out of range");
+    const int k = p->nt[g];
+    SQ_TRY(make_plan(dmatrices + moff, p->targets.data() + p->toff[g], k, p->nq, p->nq,
+                     &dplans[d]));
+    moff += 2ull << (2 * k);
+    by_gate[g].push_back(d);
+  }
+  std::vector<const GatePlan *> dptr;
+  for (auto &dp : dplans) dptr.push_back(&dp);
+  void *dmat_dev = nullptr;
+  std::vector<const void *> dmat_ptr;
+  SQ_TRY(upload_blocks(dptr, dt, &dmat_dev, &dmat_ptr));
+  struct Free {
+    void *q;
+    ~Free() {
+      if (q) (void)hipFree(q);
+    }
+  } free_guard{dmat_dev};
+
+  // checkpoint c holds the state before gate bound[c]
+  std::vector<uint32_t> bound(num_checkpoints);
+  for (uint32_t c = 0; c < num_checkpoints; ++c)
+    bound[c] = (uint32_t)(((uint64_t)(c + 1) * ng) / (num_checkpoints + 1));
+
+  int dev;
+  SQ_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mutex[dev]);
+  const size_t grad_b = ((2 * (size_t)num_params + 64) * sizeof(double) + 255) / 256 * 256;
+  const size_t edev = expect_ws_dev(nterms);
+  const size_t apb = nterms * pauli_apply_term_bytes() + 256;
+  Workspace *w;
+  SQ_TRY(ws_reserve(dev, edev + grad_b + apb + 4096, nterms * 64 + grad_b + apb + 4096, &w));
+  hipStream_t st = (hipStream_t)stream;
+  char *dbase = (char *)w->dev;
+  double *partial = (double *)dbase;                      // reduction partials
+  double *gslots = (double *)al(dbase + edev);            // 2 per param + energy
+  void *aterms = al((char *)gslots + grad_b);             // Pauli apply terms
+  double *two = gslots + 2 * num_params;                  // {2, 0}
+  double *eslot = two + 2;                                // energy (re, im)
+
+  // forward pass with checkpoints
+  SQ_TRY(launch_init(sv, dt, n, true, st));
+  uint32_t cnext = 0;
+  for (uint32_t g = 0; g < ng; ++g) {
+    while (cnext < num_checkpoints && bound[cnext] == g) {
+      SQ_HIP(hipMemcpyAsync(checkpoints[cnext], sv, n * itemsize(dt), hipMemcpyDeviceToDevice, st));
+      ++cnext;
+    }
+    SQ_TRY(launch_plan(p->fwd[g], sv, dt, n, st, p->fwd_dev[g]));
+  }
+  // lambda = H psi ; energy = <psi|lambda>
+  fill_pauli_apply_terms((int)nterms, x, z, ny, coeffs, w->host);
+  SQ_HIP(hipMemcpyAsync(aterms, w->host, nterms * pauli_apply_term_bytes(),
+                        hipMemcpyHostToDevice, st));
+  SQ_TRY(launch_apply_pauli_sum(sv, work0, dt, n, (int)nterms, aterms, st));
+  {
+    double *hg = (double *)al((char *)w->host + apb);
+    std::memset(hg, 0, grad_b);
+    hg[2 * num_params] = 2.0;
+    hg[2 * num_params + 1] = 0.0;
+    SQ_HIP(hipMemcpyAsync(gslots, hg, grad_b, hipMemcpyHostToDevice, st));
+  }
+  SQ_TRY(launch_inner(sv, work0, dt, n, partial, eslot, nullptr, 0.0, st));
+  // reverse sweep
+  int cidx = (int)num_checkpoints - 1;
+  for (int g = (int)ng - 1; g >= 0; --g) {
+    if (cidx >= 0 && bound[cidx] == (uint32_t)g) {
+      SQ_HIP(hipMemcpyAsync(sv, checkpoints[cidx], n * itemsize(dt), hipMemcpyDeviceToDevice, st));
+      while (cidx >= 0 && bound[cidx] == (uint32_t)g) --cidx;
+    } else {
+      SQ_TRY(launch_plan(p->inv[g], sv, dt, n, st, p->inv_dev[g]));
+    }
+    for (uint32_t d : by_gate[g]) {
+      SQ_HIP(hipMemcpyAsync(work1, sv, n * itemsize(dt), hipMemcpyDeviceToDevice, st));
+      SQ_TRY(launch_plan(dplans[d], work1, dt, n, st, dmat_ptr[d]));
+      // gslots[2p] += 2 Re <lambda|work1>
+      SQ_TRY(launch_inner(work0, work1, dt, n, partial, gslots + 2 * deriv_param[d], two,
+                          1.0, st));
+    }
+    if (g > 0) SQ_TRY(launch_plan(p->inv[g], work0, dt, n, st, p->inv_dev[g]));
+  }
+  double *hg = (double *)al((char *)w->host + apb);
+  SQ_HIP(hipMemcpyAsync(hg, gslots, grad_b, hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  for (uint32_t q = 0; q < num_params; ++q) grads[q] = hg[2 * q];
+  *energy = hg[2 * num_params + 2];
+  return SQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,707 @@
+// pymodule.cc -- pybind11 module snuqs_amd._C.
+//
+// Keeps the layout and names of the reference module braket.snuqs._C
+// (src/braket/snuqs/csrc/pymodule.cc:66-210): submodules operation,
+// operation.gate_operations, stream, event, functionals, result_types, core,
+// core.cuda and the DeviceType enum.  Everything that touches the GPU goes
+// through the C ABI of libsnuqs_hip.so (include/snuqs_hip.h); this file holds
+// no device code.  DeviceType.CUDA names the HIP device.
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+#include <sys/sysinfo.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+
+#include "gates.h"
+#include "snuqs_hip.h"
+
+namespace py = pybind11;
+using namespace snuqs;
+
+namespace {
+
+void check(int rc) {
+  if (rc == SQ_OK) return;
+  std::string msg = sq_last_error();
+  switch (rc) {
+    case SQ_ERR_INVALID: throw py::value_error(msg);
+    case SQ_ERR_OOM: throw std::bad_alloc();
+    case SQ_ERR_UNSUPPORTED: throw std::invalid_argument(msg);
+    default: throw std::runtime_error("HIP error: " + msg);
+  }
+}
+
+// ---- Stream / Event (stream/stream.cc:6-34, event/event.cc:5-26) ----------
+class Event;
+class Stream {
+ public:
+  explicit Stream(uintptr_t s, bool owned = false) : s_((void *)s), owned_(owned) {}
+  ~Stream() {
+    if (owned_ && s_) sq_stream_destroy(s_);
+  }
+  uintptr_t get() const { return (uintptr_t)s_; }
+  void *raw() const { return s_; }
+  void synchronize() { check(sq_stream_synchronize(s_)); }
+  void wait_event(std::shared_ptr<Event> e);
+  static std::shared_ptr<Stream> create() {
+    void *s;
+    check(sq_stream_create(&s, 0));
+    return std::make_shared<Stream>((uintptr_t)s, true);
+  }
+  static std::shared_ptr<Stream> create_nonblocking() {
+    void *s;
+    check(sq_stream_create(&s, 1));
+    return std::make_shared<Stream>((uintptr_t)s, true);
+  }
+
+ private:
+  void *s_;
+  bool owned_;
+};
+
+class Event {
+ public:
+  explicit Event(uintptr_t e, bool owned = false) : e_((void *)e), owned_(owned) {}
+  ~Event() {
+    if (owned_ && e_) sq_event_destroy(e_);
+  }
+  uintptr_t get() const { return (uintptr_t)e_; }
+  void *raw() const { return e_; }
+  void record(std::shared_ptr<Stream> s) { check(sq_event_record(e_, s ? s->raw() : nullptr)); }
+  void synchronize() { check(sq_event_synchronize(e_)); }
+  float elapsed_ms(const Event &end) const {
+    float ms;
+    check(sq_event_elapsed_ms(&ms, e_, end.e_));
+    return ms;
+  }
+  static std::shared_ptr<Event> create() {
+    void *e;
+    check(sq_event_create(&e));
+    return std::make_shared<Event>((uintptr_t)e, true);
+  }
+
+ private:
+  void *e_;
+  bool owned_;
+};
+
+void Stream::wait_event(std::shared_ptr<Event> e) { check(sq_stream_wait_event(s_, e->raw())); }
+
+void *sraw(const std::shared_ptr<Stream> &s) { return s ? s->raw() : nullptr; }
+
+enum class DeviceType { CPU = 0, CUDA = 1, STORAGE = 2 };
+
+// ---- buffers (buffer/buffer_cpu.cc, buffer/buffer_cuda.cc) ----------------
+struct Buffer {
+  DeviceType device;
+  bool pinned = false;
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  Buffer(DeviceType d, size_t b, bool pin) : device(d), pinned(pin), bytes(b) {
+    if (d == DeviceType::CUDA) {
+      check(sq_malloc(&ptr, b));
+    } else if (d == DeviceType::CPU) {
+      if (pin) {
+        check(sq_host_alloc_pinned(&ptr, b));
+      } else {
+        ptr = std::aligned_alloc(4096, ((b + 4095) / 4096) * 4096);
+        if (!ptr) throw std::bad_alloc();
+      }
+    } else {
+      throw std::invalid_argument("the STORAGE tier is not part of this engine");
+    }
+  }
+  ~Buffer() {
+    if (!ptr) return;
+    if (device == DeviceType::CUDA) sq_free(ptr);
+    else if (pinned) sq_host_free_pinned(ptr);
+    else std::free(ptr);
+  }
+};
+
+sq_dtype_t parse_dtype(const std::string &s) {
+  if (s == "complex128" || s == "c128" || s == "complex") return SQ_COMPLEX128;
+  if (s == "complex64" || s == "c64") return SQ_COMPLEX64;
+  throw py::value_error("dtype must be 'complex64' or 'complex128'");
+}
+const char *dtype_name(sq_dtype_t d) { return d == SQ_COMPLEX64 ? "complex64" : "complex128"; }
+
+// ---- StateVector (result_types/state_vector.h:12-42) ----------------------
+class StateVector : public std::enable_shared_from_this<StateVector> {
+ public:
+  StateVector(DeviceType device, size_t num_qubits, sq_dtype_t dt, bool pinned = false)
+      : device_(device), num_qubits_(num_qubits), dtype_(dt), offset_(0) {
+    if (num_qubits > 40) throw py::value_error("num_qubits must be <= 40");
+    buffer_ = std::make_shared<Buffer>(device, bytes(), pinned);
+  }
+  StateVector(DeviceType device, size_t num_qubits, sq_dtype_t dt,
+              std::shared_ptr<Buffer> buf, size_t offset)
+      : device_(device), num_qubits_(num_qubits), dtype_(dt), buffer_(buf), offset_(offset) {}
+
+  size_t itemsize() const { return dtype_ == SQ_COMPLEX64 ? 8 : 16; }
+  size_t num_elems() const { return (size_t)1 << num_qubits_; }
+  size_t bytes() const { return num_elems() * itemsize(); }
+  void *ptr() const { return (char *)buffer_->ptr + offset_; }
+  DeviceType device() const { return device_; }
+  size_t num_qubits() const { return num_qubits_; }
+  sq_dtype_t dtype() const { return dtype_; }
+
+  std::shared_ptr<StateVector> cpu(std::shared_ptr<Stream> stream) {
+    if (device_ == DeviceType::CPU) return shared_from_this();
+    auto out = std::make_shared<StateVector>(DeviceType::CPU, num_qubits_, dtype_, true);
+    check(sq_memcpy_d2h(out->ptr(), ptr(), bytes(), sraw(stream)));
+    check(sq_stream_synchronize(sraw(stream)));
+    return out;
+  }
+  std::shared_ptr<StateVector> cuda(std::shared_ptr<Stream> stream) {
+    if (device_ == DeviceType::CUDA) return shared_from_this();
+    auto out = std::make_shared<StateVector>(DeviceType::CUDA, num_qubits_, dtype_);
+    check(sq_memcpy_h2d(out->ptr(), ptr(), bytes(), sraw(stream)));
+    check(sq_stream_synchronize(sraw(stream)));
+    return out;
+  }
+  void copy(StateVector &other, std::shared_ptr<Stream> stream) {
+    if (other.num_elems() != num_elems() || other.dtype_ != dtype_)
+      throw py::value_error("copy: size or dtype mismatch");
+    void *s = sraw(stream);
+    if (device_ == DeviceType::CUDA && other.device_ == DeviceType::CUDA)
+      check(sq_memcpy_d2d(ptr(), other.ptr(), bytes(), s));
+    else if (device_ == DeviceType::CUDA)
+      check(sq_memcpy_h2d(ptr(), other.ptr(), bytes(), s));
+    else if (other.device_ == DeviceType::CUDA)
+      check(sq_memcpy_d2h(ptr(), other.ptr(), bytes(), s));
+    else
+      std::memcpy(ptr(), other.ptr(), bytes());
+    check(sq_stream_synchronize(s));
+  }
+  std::shared_ptr<StateVector> slice(size_t num_sliced_qubits, size_t index) {
+    if (num_sliced_qubits > num_qubits_ || ((index + 1) << num_sliced_qubits) > num_elems())
+      throw py::value_error("slice out of range");
+    return std::make_shared<StateVector>(device_, num_sliced_qubits, dtype_, buffer_,
+                                         offset_ + (index << num_sliced_qubits) * itemsize());
+  }
+  void sync() {}
+  std::string repr() const {
+    std::stringstream ss;
+    ss << "StateVector<" << num_qubits_ << " qubits, device: "
+       << (device_ == DeviceType::CPU ? "CPU" : "CUDA") << ", dtype: " << dtype_name(dtype_) << ">";
+    return ss.str();
+  }
+  void require_device(const char *what) const {
+    if (device_ != DeviceType::CUDA)
+      throw std::invalid_argument(std::string(what) +
+                                  " runs on the GPU: use StateVector(DeviceType.CUDA, n)");
+  }
+
+ private:
+  DeviceType device_;
+  size_t num_qubits_;
+  sq_dtype_t dtype_;
+  std::shared_ptr<Buffer> buffer_;
+  size_t offset_;
+};
+
+std::vector<uint32_t> to_u32(const std::vector<size_t> &v) {
+  return std::vector<uint32_t>(v.begin(), v.end());
+}
+
+const double *mat_ptr(const GateOperation &op) {
+  return reinterpret_cast<const double *>(op.matrix().data());
+}
+
+// ---- functionals (functionals/functionals.cc:15-50) ------------------------
+void apply(StateVector &sv, GateOperation &op, size_t num_qubits, std::vector<size_t> targets,
+           std::shared_ptr<Stream> stream) {
+  sv.require_device("apply");
+  if (targets.size() != op.num_matrix_qubits())
+    throw py::value_error("apply: " + std::to_string(targets.size()) + " targets for a " +
+                          std::to_string(op.num_matrix_qubits()) + "-qubit gate matrix");
+  auto t = to_u32(targets);
+  py::gil_scoped_release nogil;
+  check(sq_apply(sv.ptr(), sv.dtype(), sv.num_elems(), (uint32_t)num_qubits, t.data(),
+                 (uint32_t)t.size(), mat_ptr(op), sraw(stream)));
+}
+void initialize_zero(StateVector &sv, std::shared_ptr<Stream> stream) {
+  sv.require_device("initialize_zero");
+  check(sq_initialize_zero(sv.ptr(), sv.dtype(), sv.num_elems(), sraw(stream)));
+}
+void initialize_basis_z(StateVector &sv, std::shared_ptr<Stream> stream) {
+  sv.require_device("initialize_basis_z");
+  check(sq_initialize_basis_z(sv.ptr(), sv.dtype(), sv.num_elems(), sraw(stream)));
+}
+
+// Pauli-sum observable in mask form.
+struct PauliSum {
+  std::vector<uint64_t> x, z;
+  std::vector<uint32_t> ny;
+  std::vector<double> c;
+};
+PauliSum make_pauli_sum(size_t nq, const std::vector<std::string> &paulis,
+                        const std::vector<std::vector<size_t>> &qubits,
+                        const std::vector<double> &coeffs) {
+  if (paulis.size() != qubits.size() || paulis.size() != coeffs.size())
+    throw py::value_error("paulis, qubits and coeffs must have equal length");
+  PauliSum s;
+  for (size_t t = 0; t < paulis.size(); ++t) {
+    if (paulis[t].size() != qubits[t].size())
+      throw py::value_error("Pauli string and qubit list lengths differ");
+    uint64_t x = 0, z = 0;
+    uint32_t ny = 0;
+    for (size_t j = 0; j < paulis[t].size(); ++j) {
+      const size_t q = qubits[t][j];
+      if (q >= nq) throw py::value_error("Pauli qubit out of range");
+      const uint64_t b = 1ull << (nq - 1 - q);
+      if ((x | z) & b) throw py::value_error("repeated qubit in a Pauli string");
+      switch (paulis[t][j]) {
+        case 'I': break;
+        case 'X': x |= b; break;
+        case 'Y': x |= b; z |= b; ++ny; break;
+        case 'Z': z |= b; break;
+        default: throw py::value_error("Pauli letters must be I, X, Y or Z");
+      }
+    }
+    s.x.push_back(x);
+    s.z.push_back(z);
+    s.ny.push_back(ny);
+    s.c.push_back(coeffs[t]);
+  }
+  return s;
+}
+
+py::array_t<double> probabilities(StateVector &sv, std::shared_ptr<Stream> stream) {
+  sv.require_device("probabilities");
+  const size_t n = sv.num_elems();
+  void *d;
+  check(sq_malloc(&d, n * sizeof(double)));
+  std::unique_ptr<void, int (*)(void *)> g(d, sq_free);
+  py::array_t<double> out(n);
+  {
+    py::gil_scoped_release nogil;
+    check(sq_probabilities(sv.ptr(), sv.dtype(), n, (double *)d, sraw(stream)));
+    check(sq_memcpy_d2h(out.mutable_data(), d, n * sizeof(double), sraw(stream)));
+    check(sq_stream_synchronize(sraw(stream)));
+  }
+  return out;
+}
+
+py::array_t<int64_t> sample(StateVector &sv, py::array_t<double, py::array::c_style> uniforms,
+                            bool exact, std::shared_ptr<Stream> stream) {
+  sv.require_device("sample");
+  const size_t n = sv.num_elems();
+  void *d;
+  check(sq_malloc(&d, n * sizeof(double)));
+  std::unique_ptr<void, int (*)(void *)> g(d, sq_free);
+  py::array_t<int64_t> out(uniforms.size());
+  const double *u = uniforms.data();
+  int64_t *o = out.mutable_data();
+  const size_t shots = uniforms.size();
+  {
+    py::gil_scoped_release nogil;
+    check(sq_probabilities(sv.ptr(), sv.dtype(), n, (double *)d, sraw(stream)));
+    check(sq_sample((const double *)d, n, u, shots, exact ? 1 : 0, o, sraw(stream)));
+  }
+  return out;
+}
+
+// ---- Program: a gate list uploaded once (replaces the per-gate loop of
+// simulation.py:274-284) ---------------------------------------------------
+class Program {
+ public:
+  Program(sq_dtype_t dt, size_t nq, const std::vector<std::shared_ptr<GateOperation>> &ops)
+      : dt_(dt), nq_(nq), ops_(ops) {
+    std::vector<uint32_t> nt, tg;
+    std::vector<double> mats;
+    for (auto &op : ops) {
+      auto t = op->get_targets();
+      if (t.size() != op->num_matrix_qubits())
+        throw py::value_error("gate " + op->formatted_string() + ": targets do not match matrix");
+      nt.push_back((uint32_t)t.size());
+      for (size_t q : t) tg.push_back((uint32_t)q);
+      const double *m = mat_ptr(*op);
+      mats.insert(mats.end(), m, m + 2 * op->matrix().size());
+    }
+    check(sq_program_create(&prog_, dt, (uint32_t)nq, (uint32_t)ops.size(), nt.data(),
+                            tg.data(), mats.data()));
+  }
+  ~Program() {
+    if (prog_) sq_program_destroy(prog_);
+  }
+  void run(StateVector &sv, std::shared_ptr<Stream> stream) {
+    sv.require_device("Program.run");
+    if (sv.dtype() != dt_) throw py::value_error("state dtype differs from the program's");
+    py::gil_scoped_release nogil;
+    check(sq_program_run(prog_, sv.ptr(), sv.num_elems(), sraw(stream)));
+  }
+  py::dict stats() const {
+    uint64_t l;
+    double b;
+    check(sq_program_stats(prog_, &l, &b));
+    py::dict d;
+    d["launches"] = l;
+    d["algorithmic_bytes"] = b;
+    return d;
+  }
+  // Energy and gradient w.r.t. `num_params` parameters; param_map[g] lists
+  // (angle index, parameter index) pairs of gate g.
+  py::tuple adjoint(StateVector &sv, StateVector &w0, StateVector &w1,
+                    std::vector<std::shared_ptr<StateVector>> checkpoints,
+                    const std::vector<std::vector<std::pair<size_t, size_t>>> &param_map,
+                    size_t num_params, const std::vector<std::string> &paulis,
+                    const std::vector<std::vector<size_t>> &qubits,
+                    const std::vector<double> &coeffs, std::shared_ptr<Stream> stream) {
+    sv.require_device("adjoint");
+    if (param_map.size() != ops_.size()) throw py::value_error("param_map needs one entry per gate");
+    PauliSum ps = make_pauli_sum(nq_, paulis, qubits, coeffs);
+    std::vector<uint32_t> dg, dp;
+    std::vector<double> dm;
+    for (size_t g = 0; g < ops_.size(); ++g)
+      for (auto &ap : param_map[g]) {
+        if (ap.second >= num_params) throw py::value_error("parameter index out of range");
+        auto d = ops_[g]->derivative(ap.first);
+        dg.push_back((uint32_t)g);
+        dp.push_back((uint32_t)ap.second);
+        const double *m = reinterpret_cast<const double *>(d.data());
+        dm.insert(dm.end(), m, m + 2 * d.size());
+      }
+    std::vector<void *> cps;
+    for (auto &c : checkpoints) {
+      c->require_device("checkpoint");
+      cps.push_back(c->ptr());
+    }
+    double energy = 0;
+    py::array_t<double> grads(num_params);
+    double *gp = grads.mutable_data();
+    {
+      py::gil_scoped_release nogil;
+      check(sq_adjoint_gradient(prog_, (uint32_t)dg.size(), dg.data(), dp.data(), dm.data(),
+                                (uint32_t)num_params, sv.ptr(), w0.ptr(), w1.ptr(),
+                                cps.empty() ? nullptr : cps.data(), (uint32_t)cps.size(),
+                                sv.num_elems(), (uint32_t)ps.x.size(), ps.x.data(), ps.z.data(),
+                                ps.ny.data(), ps.c.data(), &energy, gp, sraw(stream)));
+    }
+    return py::make_tuple(energy, grads);
+  }
+
+ private:
+  sq_dtype_t dt_;
+  size_t nq_;
+  std::vector<std::shared_ptr<GateOperation>> ops_;
+  void *prog_ = nullptr;
+};
+
+py::buffer_info gate_buffer(GateOperation &g) {
+  return py::buffer_info(g.data(), sizeof(cplx), py::format_descriptor<cplx>::format(), 2,
+                         g.shape(), g.stride());
+}
+
+}  // namespace
+
+// One C++ subclass per gate type, so isinstance() and the reference's
+// constructor signatures (pymodule.cc:20-64) hold.
+template <GateOperationType T>
+struct Gate0 : GateOperation {
+  Gate0(const std::vector<size_t> &t, const std::vector<size_t> &c, long p)
+      : GateOperation(T, t, {}, c, p) {}
+};
+template <GateOperationType T>
+struct Gate1 : GateOperation {
+  Gate1(const std::vector<size_t> &t, double a, const std::vector<size_t> &c, long p)
+      : GateOperation(T, t, {a}, c, p) {}
+};
+struct GateU : GateOperation {
+  GateU(const std::vector<size_t> &t, double a1, double a2, double a3,
+        const std::vector<size_t> &c, long p)
+      : GateOperation(GateOperationType::U, t, {a1, a2, a3}, c, p) {}
+};
+struct GateUnitary : GateOperation {
+  GateUnitary(const std::vector<size_t> &t, const std::vector<cplx> &m,
+              const std::vector<size_t> &c, long p)
+      : GateOperation(t, m, c, p) {}
+};
+
+template <GateOperationType T>
+void bind0(py::module_ &mg, const char *name) {
+  py::class_<Gate0<T>, GateOperation, std::shared_ptr<Gate0<T>>>(mg, name, py::buffer_protocol())
+      .def_buffer([](Gate0<T> &g) { return gate_buffer(g); })
+      .def(py::init<const std::vector<size_t> &, const std::vector<size_t> &, long>(),
+           py::arg("targets"), py::kw_only(), py::arg("ctrl_modifiers") = std::vector<size_t>(),
+           py::arg("power") = 1);
+}
+template <GateOperationType T>
+void bind1(py::module_ &mg, const char *name) {
+  py::class_<Gate1<T>, GateOperation, std::shared_ptr<Gate1<T>>>(mg, name, py::buffer_protocol())
+      .def_buffer([](Gate1<T> &g) { return gate_buffer(g); })
+      .def(py::init<const std::vector<size_t> &, double, const std::vector<size_t> &, long>(),
+           py::arg("targets"), py::arg("angle"), py::kw_only(),
+           py::arg("ctrl_modifiers") = std::vector<size_t>(), py::arg("power") = 1);
+}
+
+PYBIND11_MODULE(_C, m) {
+  m.doc() = "SnuQS MI355X state-vector engine (HIP, gfx950).";
+
+  py::enum_<DeviceType>(m, "DeviceType")
+      .value("CPU", DeviceType::CPU)
+      .value("CUDA", DeviceType::CUDA)
+      .value("STORAGE", DeviceType::STORAGE)
+      .export_values();
+
+  // operation
+  auto m_op = m.def_submodule("operation");
+  py::class_<Operation, std::shared_ptr<Operation>>(m_op, "Operation")
+      .def(py::init<const std::vector<size_t> &>())
+      .def_property("targets", &Operation::get_targets, &Operation::set_targets);
+
+  py::enum_<GateOperationType> gt(m_op, "GateOperationType");
+  for (int t = 0; t <= (int)GateOperationType::Unitary; ++t)
+    gt.value(type_name((GateOperationType)t).c_str(), (GateOperationType)t);
+
+  py::class_<GateOperation, Operation, std::shared_ptr<GateOperation>>(m_op, "GateOperation",
+                                                                        py::buffer_protocol())
+      .def(py::init<GateOperationType, const std::vector<size_t> &, const std::vector<double> &,
+                    const std::vector<size_t> &, long>(),
+           py::arg("type"), py::arg("targets"), py::arg("angles") = std::vector<double>(),
+           py::arg("ctrl_modifiers") = std::vector<size_t>(), py::arg("power") = 1)
+      .def_buffer(&gate_buffer)
+      .def("sliceable", &GateOperation::sliceable)
+      .def("slice", &GateOperation::slice)
+      .def("__repr__", &GateOperation::formatted_string)
+      .def("__eq__", &GateOperation::operator==)
+      .def("__hash__", [](const GateOperation &g) { return std::hash<std::string>()(g.formatted_string()); })
+      .def_property_readonly("name", &GateOperation::name)
+      .def_property_readonly("type", &GateOperation::type)
+      .def_property_readonly("angles", &GateOperation::angles)
+      .def_property_readonly("ctrl_modifiers", &GateOperation::ctrl_modifiers)
+      .def_property_readonly("power", &GateOperation::power)
+      .def_property_readonly("num_params", &GateOperation::num_params)
+      .def("matrix", [](GateOperation &g) { return py::array(gate_buffer(g)).attr("copy")(); })
+      .def("derivative", [](GateOperation &g, size_t i) {
+        auto d = g.derivative(i);
+        const size_t n = g.ncols();
+        py::array_t<cplx> a({n, n});
+        std::memcpy(a.mutable_data(), d.data(), d.size() * sizeof(cplx));
+        return a;
+      });
+
+  // operation.gate_operations: the 33 gate classes of pymodule.cc:87-120 as
+  // subclasses with the reference's constructor signatures, plus Unitary.
+  auto m_gates = m_op.def_submodule("gate_operations");
+  using G = GateOperationType;
+  bind0<G::Identity>(m_gates, "Identity");
+  bind0<G::Hadamard>(m_gates, "Hadamard");
+  bind0<G::PauliX>(m_gates, "PauliX");
+  bind0<G::PauliY>(m_gates, "PauliY");
+  bind0<G::PauliZ>(m_gates, "PauliZ");
+  bind0<G::CX>(m_gates, "CX");
+  bind0<G::CY>(m_gates, "CY");
+  bind0<G::CZ>(m_gates, "CZ");
+  bind0<G::S>(m_gates, "S");
+  bind0<G::Si>(m_gates, "Si");
+  bind0<G::T>(m_gates, "T");
+  bind0<G::Ti>(m_gates, "Ti");
+  bind0<G::V>(m_gates, "V");
+  bind0<G::Vi>(m_gates, "Vi");
+  bind1<G::PhaseShift>(m_gates, "PhaseShift");
+  bind1<G::CPhaseShift>(m_gates, "CPhaseShift");
+  bind1<G::CPhaseShift00>(m_gates, "CPhaseShift00");
+  bind1<G::CPhaseShift01>(m_gates, "CPhaseShift01");
+  bind1<G::CPhaseShift10>(m_gates, "CPhaseShift10");
+  bind1<G::RotX>(m_gates, "RotX");
+  bind1<G::RotY>(m_gates, "RotY");
+  bind1<G::RotZ>(m_gates, "RotZ");
+  bind0<G::Swap>(m_gates, "Swap");
+  bind0<G::ISwap>(m_gates, "ISwap");
+  bind1<G::PSwap>(m_gates, "PSwap");
+  bind1<G::XY>(m_gates, "XY");
+  bind1<G::XX>(m_gates, "XX");
+  bind1<G::YY>(m_gates, "YY");
+  bind1<G::ZZ>(m_gates, "ZZ");
+  bind0<G::CCNot>(m_gates, "CCNot");
+  bind0<G::CSwap>(m_gates, "CSwap");
+  py::class_<GateU, GateOperation, std::shared_ptr<GateU>>(m_gates, "U", py::buffer_protocol())
+      .def_buffer([](GateU &g) { return gate_buffer(g); })
+      .def(py::init<const std::vector<size_t> &, double, double, double,
+                    const std::vector<size_t> &, long>(),
+           py::arg("targets"), py::arg("angle_1"), py::arg("angle_2"), py::arg("angle_3"),
+           py::kw_only(), py::arg("ctrl_modifiers") = std::vector<size_t>(),
+           py::arg("power") = 1);
+  bind1<G::GPhase>(m_gates, "GPhase");
+  py::class_<GateUnitary, GateOperation, std::shared_ptr<GateUnitary>>(m_gates, "Unitary",
+                                                                       py::buffer_protocol())
+      .def_buffer([](GateUnitary &g) { return gate_buffer(g); })
+      .def(py::init([](const std::vector<size_t> &targets,
+                       py::array_t<cplx, py::array::c_style | py::array::forcecast> matrix,
+                       const std::vector<size_t> &ctrl, long power) {
+             if (matrix.ndim() != 2 || matrix.shape(0) != matrix.shape(1))
+               throw py::value_error("matrix is not a two-dimensional square matrix");
+             std::vector<cplx> m(matrix.data(), matrix.data() + matrix.size());
+             return std::make_shared<GateUnitary>(targets, m, ctrl, power);
+           }),
+           py::arg("targets"), py::arg("matrix"), py::kw_only(),
+           py::arg("ctrl_modifiers") = std::vector<size_t>(), py::arg("power") = 1);
+
+  // stream / event
+  auto m_stream = m.def_submodule("stream");
+  py::class_<Stream, std::shared_ptr<Stream>>(m_stream, "Stream")
+      .def(py::init<uintptr_t>())
+      .def("get", &Stream::get)
+      .def("synchronize", &Stream::synchronize)
+      .def("wait_event", &Stream::wait_event)
+      .def_static("create", &Stream::create)
+      .def_static("create_nonblocking", &Stream::create_nonblocking);
+  auto m_event = m.def_submodule("event");
+  py::class_<Event, std::shared_ptr<Event>>(m_event, "Event")
+      .def(py::init<uintptr_t>())
+      .def("get", &Event::get)
+      .def("record", &Event::record, py::arg("stream") = nullptr)
+      .def("synchronize", &Event::synchronize)
+      .def("elapsed_ms", &Event::elapsed_ms)
+      .def_static("create", &Event::create);
+
+  // result_types
+  auto m_rt = m.def_submodule("result_types");
+  py::class_<StateVector, std::shared_ptr<StateVector>>(m_rt, "StateVector", py::buffer_protocol())
+      .def_buffer([](StateVector &sv) -> py::buffer_info {
+        if (sv.device() != DeviceType::CPU)
+          throw std::invalid_argument("buffer protocol needs a host state: call .cpu() first");
+        if (sv.dtype() == SQ_COMPLEX128)
+          return py::buffer_info(sv.ptr(), 16, py::format_descriptor<std::complex<double>>::format(),
+                                 1, {sv.num_elems()}, {(size_t)16});
+        return py::buffer_info(sv.ptr(), 8, py::format_descriptor<std::complex<float>>::format(), 1,
+                               {sv.num_elems()}, {(size_t)8});
+      })
+      .def(py::init([](size_t n, std::string dtype) {
+             return std::make_shared<StateVector>(DeviceType::CPU, n, parse_dtype(dtype));
+           }), py::arg("num_qubits"), py::kw_only(), py::arg("dtype") = "complex128")
+      .def(py::init([](size_t n, bool pinned, std::string dtype) {
+             return std::make_shared<StateVector>(DeviceType::CPU, n, parse_dtype(dtype), pinned);
+           }), py::arg("num_qubits"), py::arg("pinned"), py::kw_only(), py::arg("dtype") = "complex128")
+      .def(py::init([](DeviceType d, size_t n, std::string dtype) {
+             return std::make_shared<StateVector>(d, n, parse_dtype(dtype));
+           }), py::arg("device"), py::arg("num_qubits"), py::kw_only(), py::arg("dtype") = "complex128")
+      .def("cpu", &StateVector::cpu, py::kw_only(), py::arg("stream") = nullptr)
+      .def("cuda", &StateVector::cuda, py::kw_only(), py::arg("stream") = nullptr)
+      .def("hip", &StateVector::cuda, py::kw_only(), py::arg("stream") = nullptr)
+      .def("copy", &StateVector::copy, py::arg("other"), py::kw_only(), py::arg("stream") = nullptr)
+      .def("slice", &StateVector::slice)
+      .def("__repr__", &StateVector::repr)
+      .def("num_elems", &StateVector::num_elems)
+      .def("sync", &StateVector::sync)
+      .def_property_readonly("num_qubits", &StateVector::num_qubits)
+      .def_property_readonly("device", &StateVector::device)
+      .def_property_readonly("dtype", [](const StateVector &s) { return dtype_name(s.dtype()); })
+      .def_property_readonly("data_ptr", [](const StateVector &s) { return (uintptr_t)s.ptr(); });
+
+  // functionals
+  auto m_fn = m.def_submodule("functionals");
+  m_fn.def("apply", &apply, py::arg("state_vector"), py::arg("op"), py::arg("num_qubits"),
+           py::arg("targets"), py::kw_only(), py::arg("stream") = nullptr);
+  m_fn.def("initialize_zero", &initialize_zero, py::arg("state_vector"), py::kw_only(),
+           py::arg("stream") = nullptr);
+  m_fn.def("initialize_basis_z", &initialize_basis_z, py::arg("state_vector"), py::kw_only(),
+           py::arg("stream") = nullptr);
+  m_fn.def("probabilities", &probabilities, py::arg("state_vector"), py::kw_only(),
+           py::arg("stream") = nullptr);
+  m_fn.def("sample", &sample, py::arg("state_vector"), py::arg("uniforms"), py::kw_only(),
+           py::arg("exact") = true, py::arg("stream") = nullptr);
+  m_fn.def("norm2", [](StateVector &sv, std::shared_ptr<Stream> s) {
+    sv.require_device("norm2");
+    double out;
+    check(sq_norm2(sv.ptr(), sv.dtype(), sv.num_elems(), &out, sraw(s)));
+    return out;
+  }, py::arg("state_vector"), py::kw_only(), py::arg("stream") = nullptr);
+  m_fn.def("scale", [](StateVector &sv, cplx f, std::shared_ptr<Stream> s) {
+    sv.require_device("scale");
+    check(sq_scale(sv.ptr(), sv.dtype(), sv.num_elems(), f.real(), f.imag(), sraw(s)));
+  }, py::arg("state_vector"), py::arg("factor"), py::kw_only(), py::arg("stream") = nullptr);
+  m_fn.def("inner", [](StateVector &a, StateVector &b, std::shared_ptr<Stream> s) {
+    a.require_device("inner");
+    b.require_device("inner");
+    if (a.dtype() != b.dtype() || a.num_elems() != b.num_elems())
+      throw py::value_error("inner: states differ in size or dtype");
+    double out[2];
+    check(sq_inner(a.ptr(), b.ptr(), a.dtype(), a.num_elems(), out, sraw(s)));
+    return cplx(out[0], out[1]);
+  }, py::arg("a"), py::arg("b"), py::kw_only(), py::arg("stream") = nullptr);
+  m_fn.def("amplitudes", [](StateVector &sv, std::vector<uint64_t> idx, std::shared_ptr<Stream> s) {
+    sv.require_device("amplitudes");
+    py::array_t<cplx> out(idx.size());
+    check(sq_amplitudes(sv.ptr(), sv.dtype(), sv.num_elems(), idx.data(), idx.size(),
+                        reinterpret_cast<double *>(out.mutable_data()), sraw(s)));
+    return out;
+  }, py::arg("state_vector"), py::arg("indices"), py::kw_only(), py::arg("stream") = nullptr);
+  m_fn.def("expectation", [](StateVector &sv, const std::vector<std::string> &paulis,
+                             const std::vector<std::vector<size_t>> &qubits,
+                             const std::vector<double> &coeffs, std::shared_ptr<Stream> s) {
+    sv.require_device("expectation");
+    PauliSum ps = make_pauli_sum(sv.num_qubits(), paulis, qubits, coeffs);
+    double out;
+    py::array_t<double> terms(ps.x.size());
+    check(sq_expect_pauli_sum(sv.ptr(), sv.dtype(), sv.num_elems(), (uint32_t)ps.x.size(),
+                              ps.x.data(), ps.z.data(), ps.ny.data(), ps.c.data(), &out,
+                              terms.mutable_data(), sraw(s)));
+    return py::make_tuple(out, terms);
+  }, py::arg("state_vector"), py::arg("paulis"), py::arg("qubits"), py::arg("coeffs"),
+     py::kw_only(), py::arg("stream") = nullptr);
+  m_fn.def("apply_pauli_sum", [](StateVector &in, StateVector &out,
+                                 const std::vector<std::string> &paulis,
+                                 const std::vector<std::vector<size_t>> &qubits,
+                                 const std::vector<double> &coeffs, std::shared_ptr<Stream> s) {
+    in.require_device("apply_pauli_sum");
+    out.require_device("apply_pauli_sum");
+    PauliSum ps = make_pauli_sum(in.num_qubits(), paulis, qubits, coeffs);
+    check(sq_apply_pauli_sum(in.ptr(), out.ptr(), in.dtype(), in.num_elems(),
+                             (uint32_t)ps.x.size(), ps.x.data(), ps.z.data(), ps.ny.data(),
+                             ps.c.data(), sraw(s)));
+  }, py::arg("input"), py::arg("output"), py::arg("paulis"), py::arg("qubits"),
+     py::arg("coeffs"), py::kw_only(), py::arg("stream") = nullptr);
+
+  py::class_<Program, std::shared_ptr<Program>>(m_fn, "Program")
+      .def(py::init([](size_t nq, const std::vector<std::shared_ptr<GateOperation>> &ops,
+                       std::string dtype) {
+             return std::make_shared<Program>(parse_dtype(dtype), nq, ops);
+           }), py::arg("num_qubits"), py::arg("operations"), py::kw_only(),
+           py::arg("dtype") = "complex128")
+      .def("run", &Program::run, py::arg("state_vector"), py::kw_only(), py::arg("stream") = nullptr)
+      .def("stats", &Program::stats)
+      .def("adjoint", &Program::adjoint, py::arg("state_vector"), py::arg("work0"),
+           py::arg("work1"), py::arg("checkpoints"), py::arg("param_map"),
+           py::arg("num_params"), py::arg("paulis"), py::arg("qubits"), py::arg("coeffs"),
+           py::kw_only(), py::arg("stream") = nullptr);
+
+  // core (core/runtime.cc:19-103) and core.cuda (core/cuda/runtime.cu:4-25)
+  auto m_core = m.def_submodule("core");
+  m_core.def("mem_info", []() {
+    struct sysinfo si;
+    if (sysinfo(&si) != 0) throw std::runtime_error("sysinfo failed");
+    return std::make_pair((size_t)si.freeram * si.mem_unit, (size_t)si.totalram * si.mem_unit);
+  });
+  m_core.def("is_initialized_fs", []() { return false; });
+  m_core.def("initialize_fs", [](size_t, size_t, const std::vector<std::string> &) {
+    throw std::invalid_argument("the storage-offload tier is not part of this engine");
+  });
+  auto m_cuda = m_core.def_submodule("cuda");
+  m_cuda.def("mem_info", []() {
+    size_t f, t;
+    check(sq_mem_info(&f, &t));
+    return std::make_pair(f, t);
+  });
+  m_cuda.def("device_count", []() {
+    int c;
+    check(sq_device_count(&c));
+    return c;
+  });
+  m_cuda.def("get_device", []() {
+    int d;
+    check(sq_get_device(&d));
+    return d;
+  });
+  m_cuda.def("set_device", [](int d) { check(sq_set_device(d)); });
+  m_cuda.def("device_synchronize", []() { check(sq_device_synchronize()); });
+  m.attr("__version__") = sq_version();
+}

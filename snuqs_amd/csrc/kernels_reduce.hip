@@ -1,0 +1,352 @@
+// kernels_reduce.hip -- results taken from a state: probabilities, norms,
+// inner products, Pauli-sum expectations, amplitude gathers, scaling.
+//
+// The reference reads results on the host with numpy after a full
+// device->host copy (src/braket/snuqs/simulation.py:196-220); it has no
+// expectation, norm or inner-product code.  Every reduction here accumulates
+// in FP64 (also for complex64 states) with a fixed grid and a fixed
+// combination order, so results are bitwise reproducible run to run.
+#include "kernels.hpp"
+#include "kernels_reduce.hpp"
+
+namespace sq {
+
+__device__ __forceinline__ double2 ld2(const float2 *p, uint64_t i) {
+  const float2 v = p[i];
+  return make_double2((double)v.x, (double)v.y);
+}
+__device__ __forceinline__ double2 ld2(const double2 *p, uint64_t i) {
+  return p[i];
+}
+
+// Block-wide sum of two doubles per thread; result valid in thread 0.
+__device__ __forceinline__ void block_sum2(double &a, double &b) {
+  __shared__ double sa[kBlock], sb[kBlock];
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sa[threadIdx.x] += sa[threadIdx.x + w];
+      sb[threadIdx.x] += sb[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  a = sa[0];
+  b = sb[0];
+}
+
+// ---- probabilities: p[i] = re^2 + im^2 in float64, no contraction --------
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_probabilities(const T *__restrict__ sv, double *__restrict__ p, uint64_t n) {
+#pragma clang fp contract(off)
+  uint64_t i = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < 4; ++u, i += kBlock) {
+    if (i < n) {
+      const double2 v = ld2(sv, i);
+      p[i] = v.x * v.x + v.y * v.y;
+    }
+  }
+}
+
+// ---- norm^2 and <a|b>: per-block partial sums -----------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_partial_norm2(const T *__restrict__ sv, uint64_t n, double *partial) {
+  double acc = 0.0, dummy = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double2 v = ld2(sv, i);
+    acc += v.x * v.x + v.y * v.y;
+  }
+  block_sum2(acc, dummy);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_partial_inner(const T *__restrict__ a, const T *__restrict__ b, uint64_t n,
+                    double *partial /* [2][gridDim.x] */) {
+  double re = 0.0, im = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double2 x = ld2(a, i), y = ld2(b, i);
+    re += x.x * y.x + x.y * y.y;  // conj(x) * y
+    im += x.x * y.y - x.y * y.x;
+  }
+  block_sum2(re, im);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = re;
+    partial[gridDim.x + blockIdx.x] = im;
+  }
+}
+
+// ---- Pauli strings ---------------------------------------------------------
+// <P> for P|i> = i^ny (-1)^popcount(i & z) |i ^ x>.  With x != 0 the sum runs
+// over pairs {i, i^x} (i has the top bit h of x clear):
+//   contribution = s_i (conj(psi_j) psi_i + (-1)^ny conj(psi_i) psi_j),
+// so ny even accumulates s_i Re(conj(psi_j) psi_i) and ny odd s_i Im(...);
+// the finaliser applies 2 * i^ny.  x == 0 accumulates s_i |psi_i|^2.
+struct PauliTerm {
+  uint64_t x, z;
+  uint64_t hmask;  // 2^h - 1 for the top bit h of x (0 if x == 0)
+  int ny;
+  int pad;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_partial_pauli(const T *__restrict__ sv, uint64_t n,
+                    const PauliTerm *__restrict__ terms, double *partial) {
+  const PauliTerm pt = terms[blockIdx.y];
+  double acc = 0.0, dummy = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  if (pt.x == 0) {
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+      const double2 v = ld2(sv, i);
+      const double w = v.x * v.x + v.y * v.y;
+      acc += (__popcll(i & pt.z) & 1) ? -w : w;
+    }
+  } else {
+    const uint64_t half = n >> 1;
+    const bool odd = pt.ny & 1;
+    for (uint64_t f = (uint64_t)blockIdx.x * kBlock + threadIdx.x; f < half; f += stride) {
+      const uint64_t lo = f & pt.hmask;
+      const uint64_t i = ((f ^ lo) << 1) | lo;
+      const uint64_t j = i ^ pt.x;
+      const double2 a = ld2(sv, i), b = ld2(sv, j);
+      // conj(b) * a
+      const double re = b.x * a.x + b.y * a.y;
+      const double im = b.x * a.y - b.y * a.x;
+      const double v = odd ? im : re;
+      acc += (__popcll(i & pt.z) & 1) ? -v : v;
+    }
+  }
+  block_sum2(acc, dummy);
+  if (threadIdx.x == 0) partial[(uint64_t)blockIdx.y * gridDim.x + blockIdx.x] = acc;
+}
+
+// out_j = sum_t c'_t (-1)^popcount((j^x_t) & z_t) in_{j^x_t},  c'_t = c_t i^ny_t
+struct PauliApplyTerm {
+  uint64_t x, z;
+  double cr, ci;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_apply_pauli_sum(const T *__restrict__ in, T *__restrict__ out, uint64_t n,
+                      const PauliApplyTerm *__restrict__ terms, int nterms) {
+  const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  double yr = 0.0, yi = 0.0;
+  for (int t = 0; t < nterms; ++t) {
+    const PauliApplyTerm pt = terms[t];
+    const uint64_t i = j ^ pt.x;
+    const double2 v = ld2(in, i);
+    double cr = pt.cr, ci = pt.ci;
+    if (__popcll(i & pt.z) & 1) {
+      cr = -cr;
+      ci = -ci;
+    }
+    yr += cr * v.x - ci * v.y;
+    yi += cr * v.y + ci * v.x;
+  }
+  T o;
+  o.x = yr;
+  o.y = yi;
+  out[j] = o;
+}
+
+// ---- finaliser: out[v] = beta*out[v] + alpha[v] * sum_b partial[v][b] -------
+__global__ void __launch_bounds__(kBlock)
+    k_finalize(const double *__restrict__ partial, int nblocks,
+               const double *__restrict__ alpha, double beta, double *out) {
+  const int v = blockIdx.x;
+  double acc = 0.0, dummy = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) acc += partial[(uint64_t)v * nblocks + b];
+  block_sum2(acc, dummy);
+  if (threadIdx.x == 0) {
+    const double a = alpha ? alpha[v] : 1.0;
+    out[v] = (beta == 0.0 ? 0.0 : beta * out[v]) + a * acc;
+  }
+}
+
+// ---- scale / gather --------------------------------------------------------
+template <typename T, typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_scale(T *__restrict__ sv, uint64_t n, R sr, R si) {
+  uint64_t i = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < 4; ++u, i += kBlock) {
+    if (i < n) {
+      const T v = sv[i];
+      T o;
+      o.x = sr * v.x - si * v.y;
+      o.y = sr * v.y + si * v.x;
+      sv[i] = o;
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_gather(const T *__restrict__ sv, const uint64_t *__restrict__ idx,
+             uint64_t count, double2 *__restrict__ out) {
+  const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j < count) out[j] = ld2(sv, idx[j]);
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static dim3 grid1(uint64_t work, int per_block) {
+  return dim3((unsigned)((work + per_block - 1) / per_block));
+}
+
+int launch_probabilities(const void *sv, sq_dtype_t dt, uint64_t n, double *p,
+                         hipStream_t st) {
+  if (dt == SQ_COMPLEX64)
+    hipLaunchKernelGGL(k_probabilities<float2>, grid1(n, kBlock * 4), dim3(kBlock), 0, st,
+                       (const float2 *)sv, p, n);
+  else
+    hipLaunchKernelGGL(k_probabilities<double2>, grid1(n, kBlock * 4), dim3(kBlock), 0, st,
+                       (const double2 *)sv, p, n);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+static int red_blocks(uint64_t work) {
+  uint64_t b = (work + kBlock * 8 - 1) / (kBlock * 8);
+  if (b > kRedBlocks) b = kRedBlocks;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+int launch_norm2(const void *sv, sq_dtype_t dt, uint64_t n, double *partial,
+                 double *out_dev, hipStream_t st) {
+  const int nb = red_blocks(n);
+  if (dt == SQ_COMPLEX64)
+    hipLaunchKernelGGL(k_partial_norm2<float2>, dim3(nb), dim3(kBlock), 0, st,
+                       (const float2 *)sv, n, partial);
+  else
+    hipLaunchKernelGGL(k_partial_norm2<double2>, dim3(nb), dim3(kBlock), 0, st,
+                       (const double2 *)sv, n, partial);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, st, partial, nb,
+                     (const double *)nullptr, 0.0, out_dev);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+int launch_inner(const void *a, const void *b, sq_dtype_t dt, uint64_t n,
+                 double *partial, double *out_dev, const double *alpha_dev,
+                 double beta, hipStream_t st) {
+  const int nb = red_blocks(n);
+  if (dt == SQ_COMPLEX64)
+    hipLaunchKernelGGL(k_partial_inner<float2>, dim3(nb), dim3(kBlock), 0, st,
+                       (const float2 *)a, (const float2 *)b, n, partial);
+  else
+    hipLaunchKernelGGL(k_partial_inner<double2>, dim3(nb), dim3(kBlock), 0, st,
+                       (const double2 *)a, (const double2 *)b, n, partial);
+  hipLaunchKernelGGL(k_finalize, dim3(2), dim3(kBlock), 0, st, partial, nb,
+                     alpha_dev, beta, out_dev);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+// terms_dev: PauliTerm[nterms] on the device; alpha_dev: per-term factors
+// (2*i^ny folded in); out_dev[t] = term value.
+int launch_pauli_terms(const void *sv, sq_dtype_t dt, uint64_t n, int nterms,
+                       const void *terms_dev, const double *alpha_dev,
+                       double *partial, double *out_dev, hipStream_t st) {
+  const int nb = red_blocks(n / 2);
+  if (dt == SQ_COMPLEX64)
+    hipLaunchKernelGGL(k_partial_pauli<float2>, dim3(nb, nterms), dim3(kBlock), 0, st,
+                       (const float2 *)sv, n, (const PauliTerm *)terms_dev, partial);
+  else
+    hipLaunchKernelGGL(k_partial_pauli<double2>, dim3(nb, nterms), dim3(kBlock), 0, st,
+                       (const double2 *)sv, n, (const PauliTerm *)terms_dev, partial);
+  hipLaunchKernelGGL(k_finalize, dim3(nterms), dim3(kBlock), 0, st, partial, nb,
+                     alpha_dev, 0.0, out_dev);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+size_t pauli_term_bytes() { return sizeof(PauliTerm); }
+size_t pauli_apply_term_bytes() { return sizeof(PauliApplyTerm); }
+
+void fill_pauli_terms(int nterms, const uint64_t *x, const uint64_t *z,
+                      const uint32_t *ny, const double *coeffs, void *terms_host,
+                      double *alpha_host) {
+  PauliTerm *t = (PauliTerm *)terms_host;
+  for (int i = 0; i < nterms; ++i) {
+    t[i].x = x[i];
+    t[i].z = z[i];
+    t[i].ny = (int)ny[i];
+    t[i].pad = 0;
+    t[i].hmask = x[i] ? ((1ull << (63 - __builtin_clzll(x[i]))) - 1) : 0;
+    double f;
+    if (x[i] == 0) {
+      f = 1.0;
+    } else if ((ny[i] & 1) == 0) {
+      f = ((ny[i] / 2) & 1) ? -2.0 : 2.0;  // 2 * i^ny, ny even
+    } else {
+      f = (((ny[i] - 1) / 2) & 1) ? 2.0 : -2.0;  // 2i * i^ny, ny odd
+    }
+    alpha_host[i] = f * (coeffs ? coeffs[i] : 1.0);
+  }
+}
+
+void fill_pauli_apply_terms(int nterms, const uint64_t *x, const uint64_t *z,
+                            const uint32_t *ny, const double *coeffs,
+                            void *terms_host) {
+  PauliApplyTerm *t = (PauliApplyTerm *)terms_host;
+  static const double ipr[4] = {1, 0, -1, 0}, ipi[4] = {0, 1, 0, -1};
+  for (int i = 0; i < nterms; ++i) {
+    t[i].x = x[i];
+    t[i].z = z[i];
+    t[i].cr = coeffs[i] * ipr[ny[i] & 3];
+    t[i].ci = coeffs[i] * ipi[ny[i] & 3];
+  }
+}
+
+int launch_apply_pauli_sum(const void *in, void *out, sq_dtype_t dt, uint64_t n,
+                           int nterms, const void *terms_dev, hipStream_t st) {
+  if (dt == SQ_COMPLEX64)
+    hipLaunchKernelGGL(k_apply_pauli_sum<float2>, grid1(n, kBlock), dim3(kBlock), 0, st,
+                       (const float2 *)in, (float2 *)out, n,
+                       (const PauliApplyTerm *)terms_dev, nterms);
+  else
+    hipLaunchKernelGGL(k_apply_pauli_sum<double2>, grid1(n, kBlock), dim3(kBlock), 0, st,
+                       (const double2 *)in, (double2 *)out, n,
+                       (const PauliApplyTerm *)terms_dev, nterms);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+int launch_scale(void *sv, sq_dtype_t dt, uint64_t n, double re, double im,
+                 hipStream_t st) {
+  if (dt == SQ_COMPLEX64)
+    hipLaunchKernelGGL((k_scale<float2, float>), grid1(n, kBlock * 4), dim3(kBlock), 0, st,
+                       (float2 *)sv, n, (float)re, (float)im);
+  else
+    hipLaunchKernelGGL((k_scale<double2, double>), grid1(n, kBlock * 4), dim3(kBlock), 0,
+                       st, (double2 *)sv, n, re, im);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+int launch_gather(const void *sv, sq_dtype_t dt, const uint64_t *idx_dev,
+                  uint64_t count, double *out_dev, hipStream_t st) {
+  if (dt == SQ_COMPLEX64)
+    hipLaunchKernelGGL(k_gather<float2>, grid1(count, kBlock), dim3(kBlock), 0, st,
+                       (const float2 *)sv, idx_dev, count, (double2 *)out_dev);
+  else
+    hipLaunchKernelGGL(k_gather<double2>, grid1(count, kBlock), dim3(kBlock), 0, st,
+                       (const double2 *)sv, idx_dev, count, (double2 *)out_dev);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+}  // namespace sq

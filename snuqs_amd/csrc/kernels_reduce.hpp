@@ -1,0 +1,46 @@
+// kernels_reduce.hpp -- launchers of the result kernels (kernels_reduce.hip)
+// and the sampler (kernels_sample.hip).
+#pragma once
+
+#include "sq_common.hpp"
+
+namespace sq {
+
+constexpr int kRedBlocks = 2048;  // partial sums per reduced value
+
+int launch_probabilities(const void *sv, sq_dtype_t dt, uint64_t n, double *p,
+                         hipStream_t st);
+int launch_norm2(const void *sv, sq_dtype_t dt, uint64_t n, double *partial,
+                 double *out_dev, hipStream_t st);
+// out_dev[0..1] = beta*out_dev + alpha_dev[0..1] * (re, im) of <a|b>
+int launch_inner(const void *a, const void *b, sq_dtype_t dt, uint64_t n,
+                 double *partial, double *out_dev, const double *alpha_dev,
+                 double beta, hipStream_t st);
+int launch_pauli_terms(const void *sv, sq_dtype_t dt, uint64_t n, int nterms,
+                       const void *terms_dev, const double *alpha_dev,
+                       double *partial, double *out_dev, hipStream_t st);
+int launch_apply_pauli_sum(const void *in, void *out, sq_dtype_t dt, uint64_t n,
+                           int nterms, const void *terms_dev, hipStream_t st);
+int launch_scale(void *sv, sq_dtype_t dt, uint64_t n, double re, double im,
+                 hipStream_t st);
+int launch_gather(const void *sv, sq_dtype_t dt, const uint64_t *idx_dev,
+                  uint64_t count, double *out_dev, hipStream_t st);
+
+size_t pauli_term_bytes();
+size_t pauli_apply_term_bytes();
+void fill_pauli_terms(int nterms, const uint64_t *x, const uint64_t *z,
+                      const uint32_t *ny, const double *coeffs, void *terms_host,
+                      double *alpha_host);
+void fill_pauli_apply_terms(int nterms, const uint64_t *x, const uint64_t *z,
+                            const uint32_t *ny, const double *coeffs,
+                            void *terms_host);
+
+// Sampling.  cdf (device, n doubles) receives cumsum(p); exact selects the
+// bit-exact sequential-order cumsum.  u_dev / out_dev: shots entries.
+int sample_cdf(const double *p, uint64_t n, double *cdf, bool exact,
+               void *scratch, size_t scratch_bytes, hipStream_t st);
+size_t sample_scratch_bytes(uint64_t n, bool exact);
+int launch_search(const double *cdf, uint64_t n, const double *u_dev,
+                  uint64_t shots, int64_t *out_dev, hipStream_t st);
+
+}  // namespace sq

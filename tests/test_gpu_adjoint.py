@@ -1,0 +1,102 @@
+"""Adjoint gradients vs central finite differences and parameter shift, both
+computed with the FP64 CPU oracle."""
+import numpy as np
+import pytest
+
+from helpers import device_state
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_energy(n, ops, paulis, qubits, coeffs):
+    st = O.basis_state(n)
+    for op in ops:
+        O.apply(st, np.array(op.matrix()), op.targets, n)
+    e = 0.0
+    for p, qs, c in zip(paulis, qubits, coeffs):
+        e += c * O.expect_pauli(st, p, qs, n).real
+    return e
+
+
+def rebuild(ops_fn, thetas):
+    return ops_fn(thetas)
+
+
+def _adjoint(C, n, ops, pmap, nparams, obs, dtype, checkpoints=0):
+    prog = C.functionals.Program(n, ops, dtype=dtype)
+    sv = device_state(C, n, dtype)
+    w0 = device_state(C, n, dtype)
+    w1 = device_state(C, n, dtype)
+    cps = [device_state(C, n, dtype) for _ in range(checkpoints)]
+    return prog.adjoint(sv, w0, w1, cps, pmap, nparams, *obs)
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-8), ("complex64", 1e-5)])
+@pytest.mark.parametrize("checkpoints", [0, 3])
+def test_hea_adjoint_vs_finite_differences(C, dtype, tol, checkpoints):
+    from snuqs_amd.circuits import hea_ansatz, random_pauli_sum
+
+    n, layers = 8, 3
+    rng = np.random.default_rng(24)
+    thetas = rng.uniform(0, 2 * np.pi, 2 * n * layers)
+    obs = random_pauli_sum(n, 12, 3, rng)
+    ops, pmap = hea_ansatz(n, layers, thetas)
+    e, grads = _adjoint(C, n, ops, pmap, thetas.size, obs, dtype, checkpoints)
+    e_ref = oracle_energy(n, ops, *obs)
+    assert abs(e - e_ref) < (1e-12 if dtype == "complex128" else 1e-5)
+    h = 1e-5
+    fd = np.empty_like(thetas)
+    for j in range(thetas.size):
+        tp, tm = thetas.copy(), thetas.copy()
+        tp[j] += h
+        tm[j] -= h
+        fd[j] = (oracle_energy(n, hea_ansatz(n, layers, tp)[0], *obs)
+                 - oracle_energy(n, hea_ansatz(n, layers, tm)[0], *obs)) / (2 * h)
+    # parameter shift (exact for exp(-i theta/2 P) gates)
+    ps = np.empty_like(thetas)
+    for j in range(thetas.size):
+        tp, tm = thetas.copy(), thetas.copy()
+        tp[j] += np.pi / 2
+        tm[j] -= np.pi / 2
+        ps[j] = 0.5 * (oracle_energy(n, hea_ansatz(n, layers, tp)[0], *obs)
+                       - oracle_energy(n, hea_ansatz(n, layers, tm)[0], *obs))
+    assert np.abs(ps - fd).max() < 1e-7  # the two oracles agree
+    assert np.abs(grads - ps).max() < tol
+
+
+def test_adjoint_all_parametric_gate_kinds(C):
+    """Every parametric gate (including U's three angles and shared params)."""
+    g = C.operation.gate_operations
+    n = 5
+    rng = np.random.default_rng(6)
+    th = rng.uniform(0, 2 * np.pi, 7)
+
+    def build(t):
+        ops = [g.Hadamard([q]) for q in range(n)]
+        pm = [[] for _ in range(n)]
+        spec = [
+            (g.RotX([0], t[0]), [(0, 0)]), (g.PhaseShift([1], t[1]), [(0, 1)]),
+            (g.CPhaseShift([1, 2], t[2]), [(0, 2)]), (g.CPhaseShift01([3, 2], t[2]), [(0, 2)]),
+            (g.PSwap([0, 4], t[3]), [(0, 3)]), (g.XY([2, 4], t[4]), [(0, 4)]),
+            (g.XX([1, 3], t[0]), [(0, 0)]), (g.YY([0, 2], t[1]), [(0, 1)]),
+            (g.ZZ([3, 4], t[4]), [(0, 4)]), (g.U([2], t[5], t[6], t[3]), [(0, 5), (1, 6), (2, 3)]),
+            (g.CPhaseShift10([4, 0], t[6]), [(0, 6)]), (g.CPhaseShift00([1, 4], t[5]), [(0, 5)]),
+            (g.RotY([3], t[2]), [(0, 2)]), (g.RotZ([1], t[1]), [(0, 1)]),
+        ]
+        for op, m in spec:
+            ops.append(op)
+            pm.append(m)
+        return ops, pm
+
+    obs = (["XZ", "YY", "Z", "XYZ"], [[0, 1], [2, 3], [4], [0, 2, 4]], [0.4, -0.7, 1.3, 0.2])
+    ops, pm = build(th)
+    e, grads = _adjoint(C, n, ops, pm, th.size, obs, "complex128")
+    h = 1e-6
+    fd = np.empty_like(th)
+    for j in range(th.size):
+        tp, tm = th.copy(), th.copy()
+        tp[j] += h
+        tm[j] -= h
+        fd[j] = (oracle_energy(n, build(tp)[0], *obs) - oracle_energy(n, build(tm)[0], *obs)) / (2 * h)
+    assert np.abs(grads - fd).max() < 1e-7

@@ -1,0 +1,188 @@
+"""Gate application on the GPU vs the CPU oracle (cpu::applyGate restatement).
+
+Bars: complex128 max-abs <= 1e-12; complex64 max-abs <= 1e-5 against the
+complex128 oracle run on the same (complex64-rounded) input.
+"""
+import struct
+
+import numpy as np
+import pytest
+
+from helpers import device_state, random_state, run_engine, run_oracle, to_np
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"complex128": 1e-12, "complex64": 1e-5}
+
+
+def _kinds(C):
+    from snuqs_amd.circuits import GATE_KINDS
+
+    return GATE_KINDS
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("n", [5, 11])
+def test_every_gate_kind_every_position(C, dtype, n):
+    """Each of the 33 reference gate kinds on several target sets (lowest,
+    highest and random bit positions, both target orders)."""
+    from snuqs_amd.circuits import make_gate
+
+    rng = np.random.default_rng(100 + n)
+    init = random_state(n, rng, np.complex64 if dtype == "complex64" else np.complex128)
+    worst = 0.0
+    for kind in _kinds(C):
+        k = kind[1]
+        choices = [list(range(k)), list(range(n - k, n))[::-1]] + [
+            list(rng.permutation(n)[:k]) for _ in range(3)]
+        for t in choices:
+            op = make_gate(kind, t, rng.random(3) * 2 * np.pi)
+            got = run_engine(C, n, [op], dtype, init)
+            ref = run_oracle(n, [op], init.astype(np.complex128))
+            err = np.abs(got - ref).max()
+            worst = max(worst, err)
+            assert err <= TOL[dtype], (op, t, err)
+    print(f"n={n} {dtype}: worst {worst:.2e}")
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("n", [7, 12])
+def test_unitary_k_targets(C, dtype, k, n):
+    """Dense Unitary on k targets: k<=3 VALU kernel, k=4,5 MFMA (n=12) or the
+    LDS kernel (small states)."""
+    from snuqs_amd.circuits import haar_unitary
+
+    g = C.operation.gate_operations
+    rng = np.random.default_rng(7 * k + n)
+    init = random_state(n, rng, np.complex64 if dtype == "complex64" else np.complex128)
+    for trial in range(4):
+        if trial == 0:
+            t = list(range(k))  # highest strides
+        elif trial == 1:
+            t = list(range(n - k, n))  # lowest strides, including bit 0
+        else:
+            t = [int(x) for x in rng.permutation(n)[:k]]
+        op = g.Unitary(t, haar_unitary(1 << k, rng))
+        got = run_engine(C, n, [op], dtype, init)
+        ref = run_oracle(n, [op], init.astype(np.complex128))
+        err = np.abs(got - ref).max()
+        assert err <= TOL[dtype], (t, err)
+
+
+def test_ghz20_closed_form_bitwise(C):
+    """GHZ-20 complex128: amplitudes 0 and 2^20-1 are bitwise M_SQRT1_2
+    (0x3FE6A09E667F3BCD), every other amplitude is exactly zero."""
+    from snuqs_amd.circuits import ghz
+
+    n = 20
+    got = run_engine(C, n, ghz(n), "complex128")
+    bits = struct.unpack("<Q", struct.pack("<d", got[0].real))[0]
+    assert bits == 0x3FE6A09E667F3BCD
+    assert got[-1] == got[0] and got[0].imag == 0.0
+    rest = got[1:-1]
+    assert np.count_nonzero(rest) == 0
+    p = C.functionals.probabilities(device_state(C, 1))  # smoke of the API on a tiny state
+    assert p.shape == (2,)
+
+
+def test_ghz20_matches_reference_cpu_path(C):
+    """GHZ-20 against the reference's own cpu::applyGate (oracle/_ref), bitwise."""
+    from snuqs_amd.circuits import ghz
+
+    if O.ref_lib() is None:
+        pytest.skip("reference CPU build absent")
+    n = 20
+    ops = ghz(n)
+    st = O.basis_state(n)
+    for op in ops:
+        O.ref_apply(st, np.array(op.matrix()), op.targets, n)
+    got = run_engine(C, n, ops, "complex128")
+    assert np.array_equal(got, st)
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_random_circuit(C, dtype):
+    from snuqs_amd.circuits import random_gate_list
+
+    n = 12
+    rng = np.random.default_rng(2024)
+    ops = random_gate_list(n, 400, rng, with_unitary=True)
+    got = run_engine(C, n, ops, dtype)
+    ref = run_oracle(n, ops)
+    assert np.abs(got - ref).max() <= TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_ctrl_and_power_modifiers(C, dtype):
+    g = C.operation.gate_operations
+    n = 6
+    rng = np.random.default_rng(3)
+    init = random_state(n, rng)
+    ops = [
+        g.PauliX([0, 3], ctrl_modifiers=[1]),
+        g.PauliX([1, 2, 4], ctrl_modifiers=[0, 1]),
+        g.RotY([5, 0], 0.7, ctrl_modifiers=[0]),
+        g.Hadamard([2], power=3),
+        g.T([4], power=-1),
+        g.U([1, 3], 0.3, 0.2, 0.1, ctrl_modifiers=[1]),
+        g.GPhase([0, 1], 0.4),
+        g.GPhase([], 0.9),
+    ]
+    init_d = init.astype(np.complex64 if dtype == "complex64" else np.complex128)
+    got = run_engine(C, n, ops, dtype, init_d)
+    ref = run_oracle(n, ops, init_d.astype(np.complex128))
+    assert np.abs(got - ref).max() <= TOL[dtype]
+
+
+def test_slice_with_larger_num_qubits(C):
+    """As in the reference, num_qubits may exceed log2(num_elems): index math
+    uses num_qubits, the sweep covers the slice (functionals.cc:15-28)."""
+    g = C.operation.gate_operations
+    n, m = 12, 9  # 2^9-amplitude slice of a 12-qubit index space
+    rng = np.random.default_rng(11)
+    full = C.result_types.StateVector(C.DeviceType.CUDA, m, dtype="complex128")
+    init = random_state(m, rng)
+    host = C.result_types.StateVector(m)
+    np.array(host, copy=False)[:] = init
+    full.copy(host)
+    ops = [g.Hadamard([5]), g.CX([3, 11]), g.RotZ([8], 0.3), g.CCNot([4, 6, 10])]
+    ref = init.copy()
+    for op in ops:
+        C.functionals.apply(full, op, n, op.targets)
+        O.apply(ref, np.array(op.matrix()), op.targets, n)
+    assert np.abs(to_np(C, full) - ref).max() <= 1e-12
+    with pytest.raises(ValueError):
+        C.functionals.apply(full, g.Hadamard([2]), n, [2])  # stride beyond the slice
+
+
+def test_invalid_targets_raise(C):
+    g = C.operation.gate_operations
+    sv = device_state(C, 4)
+    with pytest.raises(ValueError):
+        C.functionals.apply(sv, g.CX([0, 0]), 4, [0, 0])
+    with pytest.raises(ValueError):
+        C.functionals.apply(sv, g.Hadamard([4]), 4, [4])
+    with pytest.raises(ValueError):
+        C.functionals.apply(sv, g.CX([0, 1]), 4, [0])
+    host = C.result_types.StateVector(4)
+    with pytest.raises(Exception):
+        C.functionals.apply(host, g.Hadamard([0]), 4, [0])  # no CPU path
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_program_matches_per_gate_apply(C, dtype):
+    from snuqs_amd.circuits import random_gate_list
+
+    n = 13
+    rng = np.random.default_rng(8)
+    ops = random_gate_list(n, 300, rng, with_unitary=True)
+    prog = C.functionals.Program(n, ops, dtype=dtype)
+    sv = device_state(C, n, dtype)
+    prog.run(sv)
+    got = to_np(C, sv)
+    ref = run_oracle(n, ops)
+    assert np.abs(got - ref).max() <= TOL[dtype]
+    st = prog.stats()
+    assert st["launches"] <= len(ops)
