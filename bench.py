@@ -1,0 +1,287 @@
+#!/usr/bin/env python3
+"""Benchmark: HBM GB/s of gate sweeps on one MI355X, plus adjoint gradients/s
+against parameter shift.
+
+Workload (one "step"): the random layered circuit on 30 qubits, depth 30,
+complex64 -- each layer pairs the qubits by a seeded permutation, applies a
+Haar-random 4x4 Unitary to each pair and RotZ(U[0, 2pi)) to every qubit
+(1350 gates) -- applied to a device-resident |0...0> state (8 GiB).  The
+scheduler fuses the gates into <= 5-qubit dense blocks that run on the matrix
+cores; the unfused per-gate sweep is measured too.
+
+value = algorithmic HBM bytes of the launched kernels / wall time of the timed
+steps (2 x touched amplitudes x 8 B per kernel).  Per-kernel durations come
+from HIP events recorded on the launch stream around every kernel of every
+timed step.
+
+Also reported: the dominant kernel's roofline, a plain copy kernel's
+bandwidth, the 24-qubit hardware-efficient ansatz gradient by adjoint vs
+parameter shift on the same GPU, and the reference CPU path
+(cpu::applyGate) timed on a bounded sample on the host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "HBM GB/s of gate sweeps at 30q c64; adjoint grads/s vs param-shift"
+HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
+CLASS_NAMES = {0: "none", 1: "phase", 2: "diagonal", 3: "permutation", 4: "dense_1_3",
+               5: "dense_4_5_mfma", 6: "dense_4_5_valu"}
+KERNEL_NAMES = {1: "k_phase", 2: "k_diag", 3: "k_perm", 4: "k_dense", 5: "k_mfma_c64",
+                6: "k_dense_lds"}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def sweep_stats(C, prog, sv, steps, warmup):
+    """Warm up, then time `steps` runs with per-kernel events.  Returns wall
+    seconds of the timed steps, per-gate ms (summed over steps) and the
+    per-gate (class, bytes)."""
+    for _ in range(warmup):
+        prog.run(sv)
+    C.core.cuda.device_synchronize()
+    ng = len(prog)
+    acc = np.zeros(ng)
+    C.core.cuda.device_synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        acc += prog.run_timed(sv)
+        log(f"step {s + 1}/{steps} done")
+    C.core.cuda.device_synchronize()
+    wall = time.perf_counter() - t0
+    info = [prog.gate_info(g) for g in range(ng)]
+    return wall, acc, info
+
+
+def class_table(acc_ms, info, steps):
+    tab = {}
+    for ms, (cls, kk, b) in zip(acc_ms, info):
+        if cls == 0:
+            continue
+        key = (cls, kk)
+        t = tab.setdefault(key, {"launches": 0, "ms": 0.0, "bytes": 0.0})
+        t["launches"] += steps
+        t["ms"] += float(ms)
+        t["bytes"] += b * steps
+    return tab
+
+
+def run_adjoint_bench(C, n=24, layers=8, nterms=64, weight=4, shift_params=12):
+    from snuqs_amd.circuits import hea_ansatz, random_pauli_sum
+
+    rng = np.random.default_rng(2024)
+    thetas = rng.uniform(0, 2 * np.pi, 2 * n * layers)
+    obs = random_pauli_sum(n, nterms, weight, rng)
+    ops, pmap = hea_ansatz(n, layers, thetas)
+    res = {}
+    for dtype in ("complex128", "complex64"):
+        prog = C.functionals.Program(n, ops, dtype=dtype)
+        st = [C.result_types.StateVector(C.DeviceType.CUDA, n, dtype=dtype) for _ in range(3)]
+        cps = [C.result_types.StateVector(C.DeviceType.CUDA, n, dtype=dtype) for _ in range(4)]
+        prog.adjoint(st[0], st[1], st[2], cps, pmap, thetas.size, *obs)  # warm-up
+        C.core.cuda.device_synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e, gr = prog.adjoint(st[0], st[1], st[2], cps, pmap, thetas.size, *obs)
+        dt = (time.perf_counter() - t0) / reps
+        res[dtype] = {"energy": e, "grads": gr, "seconds": dt}
+        log(f"adjoint {dtype}: {dt * 1e3:.1f} ms per {thetas.size}-component gradient")
+    # parameter shift on the same GPU (complex64, same per-gate kernels):
+    # programs are built before the clock starts; each shifted circuit is run
+    # from |0..0> and its energy reduced on the device.
+    sub = np.random.default_rng(1).choice(thetas.size, shift_params, replace=False)
+    progs = []
+    for j in sub:
+        for sgn in (+1, -1):
+            th = thetas.copy()
+            th[j] += sgn * np.pi / 2
+            progs.append(C.functionals.Program(n, hea_ansatz(n, layers, th)[0], dtype="complex64"))
+    sv = C.result_types.StateVector(C.DeviceType.CUDA, n, dtype="complex64")
+    C.functionals.initialize_basis_z(sv)
+    progs[0].run(sv)
+    C.functionals.expectation(sv, *obs)
+    C.core.cuda.device_synchronize()
+    t0 = time.perf_counter()
+    es = []
+    for p in progs:
+        C.functionals.initialize_basis_z(sv)
+        p.run(sv)
+        es.append(C.functionals.expectation(sv, *obs)[0])
+    dt_ps = time.perf_counter() - t0
+    ps = 0.5 * (np.array(es[0::2]) - np.array(es[1::2]))
+    g64, g128 = res["complex64"]["grads"], res["complex128"]["grads"]
+    adj_rate = thetas.size / res["complex64"]["seconds"]
+    ps_rate = len(sub) / dt_ps
+    return {
+        "config": f"{n}-qubit hardware-efficient ansatz, {layers} layers of RotY+RotZ on every "
+                  f"qubit + CZ ring, {thetas.size} parameters; observable: {nterms} random "
+                  f"weight-{weight} Pauli strings with N(0,1) coefficients",
+        "adjoint_grads_per_s": adj_rate,
+        "adjoint_ms_per_gradient_c64": res["complex64"]["seconds"] * 1e3,
+        "adjoint_ms_per_gradient_c128": res["complex128"]["seconds"] * 1e3,
+        "param_shift_grads_per_s": ps_rate,
+        "param_shift_sample": f"{len(sub)} of {thetas.size} components ({2 * len(sub)} shifted "
+                              f"circuits), complex64",
+        "speedup": adj_rate / ps_rate,
+        "max_abs_grad_err_c64_vs_c128": float(np.abs(g64 - g128).max()),
+        "max_abs_param_shift_vs_adjoint_c128": float(np.abs(ps - g128[sub]).max()),
+        "energy_c128": res["complex128"]["energy"],
+    }
+
+
+def cpu_baseline(qubits, depth, seconds):
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--qubits", str(qubits), "--depth",
+           str(depth), "--seconds", str(seconds)]
+    log("cpu baseline:", " ".join(cmd))
+    p = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True, timeout=600)
+    if p.returncode != 0:
+        log("cpu baseline failed:", p.stderr[-2000:])
+        return None
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=30)
+    ap.add_argument("--dtype", default="complex64")
+    ap.add_argument("--skip-unfused", action="store_true")
+    ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the state-vector engine runs on a single MI355X "
+              "(no multi-GPU path is built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from snuqs_amd import _C as C
+    from snuqs_amd.circuits import random_layered
+    from snuqs_amd.transpile import fuse_operations
+
+    n, dtype = args.qubits, args.dtype
+    item = 8 if dtype == "complex64" else 16
+    log(f"building circuit: {n} qubits, depth {args.depth}")
+    ops = random_layered(n, args.depth, seed=1234)
+    fused = fuse_operations(ops, 5)
+    log(f"{len(ops)} gates fused into {len(fused)} blocks")
+    prog = C.functionals.Program(n, fused, dtype=dtype)
+    sv = C.result_types.StateVector(C.DeviceType.CUDA, n, dtype=dtype)
+    C.functionals.initialize_basis_z(sv)
+
+    wall, acc_ms, info = sweep_stats(C, prog, sv, args.steps, args.warmup)
+    step_bytes = sum(b for _, _, b in info)
+    value = step_bytes * args.steps / wall / 1e9
+    kernel_ms = float(acc_ms.sum())
+    tab = class_table(acc_ms, info, args.steps)
+    dom = max(tab.items(), key=lambda kv: kv[1]["ms"])
+    (dcls, dkk), d = dom
+    per_launch_bytes = d["bytes"] / d["launches"]
+    avg_ms = d["ms"] / d["launches"]
+    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    # MFMA utilisation of the dominant fused kernel: 8 * 2^k real flops per
+    # amplitude (complex 2^k x 2^k matrix-vector product)
+    flops = 8.0 * (1 << dkk) * (1 << n) if dcls == 5 else 0.0
+    norm = C.functionals.norm2(sv)
+    log(f"fused: {wall / args.steps * 1e3:.1f} ms/step, {value:.0f} GB/s, norm^2 {norm:.12f}")
+
+    # copy-kernel ceiling on the same state size
+    tmp = C.result_types.StateVector(C.DeviceType.CUDA, n, dtype=dtype)
+    C.functionals.copy_kernel(tmp, sv)
+    C.core.cuda.device_synchronize()
+    ev0, ev1 = C.event.Event.create(), C.event.Event.create()
+    ev0.record()
+    for _ in range(5):
+        C.functionals.copy_kernel(tmp, sv)
+    ev1.record()
+    ev1.synchronize()
+    copy_gbs = 5 * 2 * (1 << n) * item / (ev0.elapsed_ms(ev1) * 1e-3) / 1e9
+    del tmp
+
+    unfused = None
+    if not args.skip_unfused:
+        prog_u = C.functionals.Program(n, ops, dtype=dtype)
+        C.functionals.initialize_basis_z(sv)
+        wall_u, acc_u, info_u = sweep_stats(C, prog_u, sv, 1, 0)
+        tab_u = class_table(acc_u, info_u, 1)
+        bytes_u = sum(b for _, _, b in info_u)
+        unfused = {
+            "ms_per_step": wall_u * 1e3,
+            "GBps": bytes_u / wall_u / 1e9,
+            "launches": int(sum(1 for c, _, _ in info_u if c)),
+            "kernels": {f"{CLASS_NAMES[c]}_k{k}": {
+                "launches": t["launches"], "avg_ms": t["ms"] / t["launches"],
+                "GBps": t["bytes"] / (t["ms"] * 1e-3) / 1e9} for (c, k), t in tab_u.items()},
+        }
+        log(f"unfused: {wall_u * 1e3:.0f} ms/step, {unfused['GBps']:.0f} GB/s")
+        del prog_u
+    del sv, prog
+
+    adj = None if args.skip_adjoint else run_adjoint_bench(C)
+    cpu = None if args.skip_cpu else cpu_baseline(n, args.depth, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64" if dtype == "complex64" else "c128",
+        "data": "synthetic",
+        "config": {
+            "workload": f"random layered circuit, {n} qubits, depth {args.depth}, {dtype}: "
+                        f"seeded qubit pairing, Haar-random 4x4 Unitary per pair, RotZ U[0,2pi) "
+                        f"on every qubit; {len(ops)} gates fused into {len(fused)} <=5-qubit blocks",
+            "qubits": n, "depth": args.depth, "gates": len(ops), "kernels_per_step": len(fused),
+            "algorithmic_bytes_per_step": step_bytes, "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": KERNEL_NAMES.get(dcls, "?") + f"<{dkk}>",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_ms,
+            "launches": d["launches"],
+            "mfma_tflops": flops / (avg_ms * 1e-3) / 1e12 if flops else None,
+            "mfma_frac_fp32": flops / (avg_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TFLOPS if flops else None,
+        },
+        "kernel_ms_per_step": kernel_ms / args.steps,
+        "copy_kernel_GBps": copy_gbs,
+        "frac_of_copy": value / copy_gbs,
+        "kernels": {f"{CLASS_NAMES[c]}_k{k}": {
+            "launches": t["launches"], "avg_ms": t["ms"] / t["launches"],
+            "GBps": t["bytes"] / (t["ms"] * 1e-3) / 1e9} for (c, k), t in tab.items()},
+        "unfused": unfused,
+        "adjoint": adj,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -65,6 +65,10 @@ int sq_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
 int sq_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
 int sq_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 
+/* Plain device-to-device copy kernel (16-byte loads/stores); the HBM
+ * bandwidth ceiling used to calibrate the gate kernels' roofline. */
+int sq_copy(void *dst, const void *src, size_t bytes, void *stream);
+
 /* ---- streams and events: stream/stream.cc:6-34, event/event.cc:5-26 ---- */
 int sq_stream_create(void **stream, int nonblocking);
 int sq_stream_destroy(void *stream);
@@ -113,6 +117,18 @@ int sq_program_destroy(void *program);
 int sq_program_stats(void *program, uint64_t *num_launches,
                      double *algorithmic_bytes);
 int sq_program_run(void *program, void *sv, uint64_t num_elems, void *stream);
+/* Measurement variant of sq_program_run: records a HIP event before every
+ * launch and one after the last on `stream`, waits for completion and writes
+ * each gate's elapsed time to ms_per_gate[num_gates] (0 for gates that
+ * launch nothing). */
+int sq_program_run_timed(void *program, void *sv, uint64_t num_elems,
+                         void *stream, float *ms_per_gate);
+/* Kernel class of gate g (0 none, 1 phase, 2 diagonal, 3 permutation,
+ * 4 dense 1-3 targets, 5 dense 4-5 targets on MFMA, 6 dense 4-5 targets on
+ * VALU), its target count after control extraction and its algorithmic HBM
+ * bytes (2 x touched amplitudes x itemsize). */
+int sq_program_gate_info(void *program, uint32_t gate, int *kernel_class,
+                         int *num_targets, double *algorithmic_bytes);
 
 /* ---- results: simulation.py:196-220 (state_vector, probabilities,
  *      retrieve_samples) and the expectation/inner-product reductions that

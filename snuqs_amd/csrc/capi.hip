@@ -138,6 +138,12 @@ int sq_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream) {
   return SQ_OK;
 }
 
+int sq_copy(void *dst, const void *src, size_t bytes, void *stream) {
+  if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+    return fail(SQ_ERR_INVALID, "sq_copy needs 16-byte aligned pointers and sizes");
+  return launch_copy(dst, src, bytes, (hipStream_t)stream);
+}
+
 // ---- streams / events (default priority only) ------------------------------
 int sq_stream_create(void **stream, int nonblocking) {
   hipStream_t s;
@@ -337,6 +343,63 @@ int sq_program_destroy(void *program) {
   if (p->dev_mats) SQ_HIP(hipFree(p->dev_mats));
   delete p;
   return SQ_OK;
+}
+
+static int gate_class(const GatePlan &p, uint64_t n, sq_dtype_t dt) {
+  switch (p.kind) {
+    case Kind::NOOP: return 0;
+    case Kind::PHASE: return 1;
+    case Kind::DIAG: return 2;
+    case Kind::PERM: return 3;
+    case Kind::DENSE: {
+      if (p.kk <= 3) return 4;
+      const uint64_t nfree = n >> (p.kk + __builtin_popcountll(p.fixed_mask));
+      const uint64_t tg = dt == SQ_COMPLEX64 ? 32 : 16;
+      return (nfree >= tg && nfree % tg == 0) ? 5 : 6;
+    }
+  }
+  return 0;
+}
+
+int sq_program_gate_info(void *program, uint32_t gate, int *kernel_class, int *num_targets,
+                         double *algorithmic_bytes) {
+  Program *p = (Program *)program;
+  if (!p) return fail(SQ_ERR_INVALID, "null program");
+  if (gate >= p->fwd.size()) return fail(SQ_ERR_INVALID, "gate index out of range");
+  const GatePlan &g = p->fwd[gate];
+  if (kernel_class) *kernel_class = gate_class(g, 1ull << p->nq, p->dt);
+  if (num_targets) *num_targets = g.kk;
+  if (algorithmic_bytes)
+    *algorithmic_bytes = 2.0 * g.touched * std::ldexp(1.0, p->nq) * itemsize(p->dt);
+  return SQ_OK;
+}
+
+int sq_program_run_timed(void *program, void *sv, uint64_t n, void *stream, float *ms) {
+  Program *p = (Program *)program;
+  if (!p) return fail(SQ_ERR_INVALID, "null program");
+  SQ_TRY(check_state(sv, p->dt, n));
+  if (n != (1ull << p->nq)) return fail(SQ_ERR_INVALID, "state size does not match program");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t ng = p->fwd.size();
+  std::vector<hipEvent_t> ev(ng + 1);
+  for (auto &e : ev) SQ_HIP(hipEventCreate(&e));
+  int rc = SQ_OK;
+  for (size_t g = 0; g < ng && rc == SQ_OK; ++g) {
+    if (hipEventRecord(ev[g], st) != hipSuccess) rc = fail(SQ_ERR_HIP, "hipEventRecord failed");
+    if (rc == SQ_OK) rc = launch_plan(p->fwd[g], sv, p->dt, n, st, p->fwd_dev[g]);
+  }
+  if (rc == SQ_OK && hipEventRecord(ev[ng], st) != hipSuccess)
+    rc = fail(SQ_ERR_HIP, "hipEventRecord failed");
+  if (rc == SQ_OK && hipEventSynchronize(ev[ng]) != hipSuccess)
+    rc = fail(SQ_ERR_HIP, "hipEventSynchronize failed");
+  for (size_t g = 0; g < ng && rc == SQ_OK; ++g) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, ev[g], ev[g + 1]) != hipSuccess)
+      rc = fail(SQ_ERR_HIP, "hipEventElapsedTime failed");
+    ms[g] = p->fwd[g].kind == Kind::NOOP ? 0.f : t;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  return rc;
 }
 
 int sq_program_stats(void *program, uint64_t *num_launches, double *bytes) {

@@ -338,6 +338,24 @@ class Program {
     py::gil_scoped_release nogil;
     check(sq_program_run(prog_, sv.ptr(), sv.num_elems(), sraw(stream)));
   }
+  py::array_t<float> run_timed(StateVector &sv, std::shared_ptr<Stream> stream) {
+    sv.require_device("Program.run_timed");
+    if (sv.dtype() != dt_) throw py::value_error("state dtype differs from the program's");
+    py::array_t<float> ms(ops_.size());
+    float *m = ms.mutable_data();
+    {
+      py::gil_scoped_release nogil;
+      check(sq_program_run_timed(prog_, sv.ptr(), sv.num_elems(), sraw(stream), m));
+    }
+    return ms;
+  }
+  py::tuple gate_info(size_t g) const {
+    int cls, kk;
+    double b;
+    check(sq_program_gate_info(prog_, (uint32_t)g, &cls, &kk, &b));
+    return py::make_tuple(cls, kk, b);
+  }
+  size_t num_gates() const { return ops_.size(); }
   py::dict stats() const {
     uint64_t l;
     double b;
@@ -609,6 +627,13 @@ PYBIND11_MODULE(_C, m) {
            py::arg("stream") = nullptr);
   m_fn.def("sample", &sample, py::arg("state_vector"), py::arg("uniforms"), py::kw_only(),
            py::arg("exact") = true, py::arg("stream") = nullptr);
+  m_fn.def("copy_kernel", [](StateVector &dst, StateVector &src, std::shared_ptr<Stream> s) {
+    dst.require_device("copy_kernel");
+    src.require_device("copy_kernel");
+    if (dst.num_elems() != src.num_elems() || dst.dtype() != src.dtype())
+      throw py::value_error("copy_kernel: states differ in size or dtype");
+    check(sq_copy(dst.ptr(), src.ptr(), dst.bytes(), sraw(s)));
+  }, py::arg("dst"), py::arg("src"), py::kw_only(), py::arg("stream") = nullptr);
   m_fn.def("norm2", [](StateVector &sv, std::shared_ptr<Stream> s) {
     sv.require_device("norm2");
     double out;
@@ -668,6 +693,10 @@ PYBIND11_MODULE(_C, m) {
            }), py::arg("num_qubits"), py::arg("operations"), py::kw_only(),
            py::arg("dtype") = "complex128")
       .def("run", &Program::run, py::arg("state_vector"), py::kw_only(), py::arg("stream") = nullptr)
+      .def("run_timed", &Program::run_timed, py::arg("state_vector"), py::kw_only(),
+           py::arg("stream") = nullptr)
+      .def("gate_info", &Program::gate_info)
+      .def("__len__", &Program::num_gates)
       .def("stats", &Program::stats)
       .def("adjoint", &Program::adjoint, py::arg("state_vector"), py::arg("work0"),
            py::arg("work1"), py::arg("checkpoints"), py::arg("param_map"),

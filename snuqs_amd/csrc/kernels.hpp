@@ -64,4 +64,7 @@ std::vector<double> fused_real_block(const GatePlan &p);
 int launch_init(void *sv, sq_dtype_t dt, uint64_t nelems, bool basis,
                 hipStream_t stream);
 
+// Plain 16-byte-vector copy (bandwidth ceiling for the roofline).
+int launch_copy(void *dst, const void *src, size_t bytes, hipStream_t stream);
+
 }  // namespace sq

@@ -1,0 +1,129 @@
+"""Gate scheduling before execution (the position of src/braket/snuqs/transpile.py).
+
+* ``pseudo_sort_operations_descending`` -- the reordering the reference applies
+  on its CPU/CUDA path (transpile.py:33-89, 184-192): an insertion sort whose
+  comparison puts zero-target gates first, keeps gates that share a qubit in
+  program order and otherwise puts the gate with the larger min(targets)
+  first.  The comparison is not a strict weak order, so the insertion
+  procedure itself is kept to reproduce the reference's order exactly.
+* ``fuse_operations`` -- new (the reference's optimize_operations is a stub,
+  transpile.py:233-234): greedy fusion into dense blocks of at most
+  ``max_qubits`` qubits.  A gate joins an existing block when every block
+  created after it is disjoint from the gate's qubits (so moving the gate
+  there only commutes it past disjoint gates) and the union stays within
+  ``max_qubits``; among those blocks the one that grows least is chosen.
+  Block matrices are accumulated in complex128.  A block holding a single
+  gate is emitted as that gate, so cheap kernel classes (phase, diagonal,
+  permutation) are kept; multi-gate blocks become Unitary gates on the
+  block's qubits in ascending order.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._C.operation import GateOperation
+from ._C.operation import gate_operations as g
+
+
+def _gt(op1, i, op2, j) -> bool:
+    t1, t2 = op1.targets, op2.targets
+    if not t1 and not t2:
+        return i < j
+    if not t1:
+        return True
+    if not t2:
+        return False
+    if set(t1) & set(t2):
+        return i < j
+    s1, s2 = op1.sliceable(), op2.sliceable()
+    if s1 and s2:
+        return i < j
+    if s1 != s2:
+        return s1
+    return min(t1) > min(t2)
+
+
+def pseudo_sort_operations_descending(operations):
+    ops = list(operations)
+    for i in range(1, len(ops)):
+        cur = ops[i]
+        j = i - 1
+        while j >= 0 and _gt(cur, i, ops[j], j):
+            ops[j + 1] = ops[j]
+            j -= 1
+        ops[j + 1] = cur
+    return ops
+
+
+def _apply_to_block(mat: np.ndarray, qubits, gate_mat: np.ndarray, targets) -> np.ndarray:
+    """Left-multiply the block matrix (local index MSB = qubits[0]) by a gate
+    acting on `targets` (subset of `qubits`)."""
+    k, t = len(qubits), len(targets)
+    pos = [qubits.index(q) for q in targets]
+    m = mat.reshape([2] * k + [1 << k])
+    gm = gate_mat.reshape([2] * (2 * t))
+    out = np.tensordot(gm, m, axes=(list(range(t, 2 * t)), pos))
+    rest = [a for a in range(k) if a not in pos]
+    # out axes: gate outputs (for pos order), then the remaining row axes, then columns
+    order = [0] * (k + 1)
+    for a_out, p in enumerate(pos):
+        order[p] = a_out
+    for i, a in enumerate(rest):
+        order[a] = t + i
+    order[k] = k
+    return np.transpose(out, order).reshape(1 << k, 1 << k)
+
+
+@dataclass
+class _Block:
+    qubits: list
+    gates: list = field(default_factory=list)
+
+
+def fuse_operations(operations, max_qubits: int = 5):
+    """Fuse gates into blocks of at most max_qubits qubits (see module doc)."""
+    blocks: list[_Block] = []
+    last: dict[int, int] = {}
+    phases = []
+    for op in operations:
+        q = list(op.targets)
+        if not q:
+            phases.append(op)
+            continue
+        qs = set(q)
+        lo = max((last[x] for x in qs if x in last), default=0)
+        best, best_grow = None, None
+        for b in range(len(blocks) - 1, lo - 1, -1):
+            union = qs | set(blocks[b].qubits)
+            if len(union) > max_qubits:
+                continue
+            grow = len(union) - len(blocks[b].qubits)
+            if best is None or grow < best_grow:
+                best, best_grow = b, grow
+            if grow == 0:
+                break
+        if best is None:
+            blocks.append(_Block(sorted(qs), [op]))
+            best = len(blocks) - 1
+        else:
+            blk = blocks[best]
+            blk.qubits = sorted(set(blk.qubits) | qs)
+            blk.gates.append(op)
+        for x in qs:
+            last[x] = max(last.get(x, -1), best)
+    out = []
+    for blk in blocks:
+        if len(blk.gates) == 1:
+            out.append(blk.gates[0])
+            continue
+        k = len(blk.qubits)
+        mat = np.eye(1 << k, dtype=np.complex128)
+        for op in blk.gates:
+            mat = _apply_to_block(mat, blk.qubits, np.array(op.matrix()), list(op.targets))
+        out.append(g.Unitary(blk.qubits, mat))
+    if phases:
+        ph = complex(np.prod([np.array(op.matrix())[0, 0] for op in phases]))
+        out.append(g.GPhase([], float(np.angle(ph))))
+    return out

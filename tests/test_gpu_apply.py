@@ -186,3 +186,35 @@ def test_program_matches_per_gate_apply(C, dtype):
     assert np.abs(got - ref).max() <= TOL[dtype]
     st = prog.stats()
     assert st["launches"] <= len(ops)
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_fused_layered_circuit(C, dtype):
+    """The benchmark circuit family at 16 qubits: fused 5-qubit blocks on the
+    matrix cores vs the unfused oracle run."""
+    from snuqs_amd.circuits import random_layered
+    from snuqs_amd.transpile import fuse_operations
+
+    n = 16
+    ops = random_layered(n, 8, seed=1234)
+    fused = fuse_operations(ops, 5)
+    prog = C.functionals.Program(n, fused, dtype=dtype)
+    sv = device_state(C, n, dtype)
+    prog.run(sv)
+    got = to_np(C, sv)
+    ref = run_oracle(n, ops)
+    assert np.abs(got - ref).max() <= TOL[dtype]
+    classes = {prog.gate_info(i)[0] for i in range(len(prog))}
+    assert 5 in classes  # MFMA kernel used
+    ms = prog.run_timed(sv)
+    assert ms.shape == (len(fused),) and (ms >= 0).all()
+
+
+def test_copy_kernel(C):
+    rng = np.random.default_rng(0)
+    n = 15
+    v = random_state(n, rng)
+    a = device_state(C, n, "complex128", v)
+    b = device_state(C, n, "complex128")
+    C.functionals.copy_kernel(b, a)
+    assert np.array_equal(to_np(C, b), v)

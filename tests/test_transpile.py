@@ -1,0 +1,46 @@
+"""Scheduler (CPU): pseudo_sort reproduces the reference order; fusion
+preserves the circuit's unitary action (checked with the CPU oracle)."""
+import numpy as np
+import pytest
+
+from helpers import run_oracle
+from oracle import oracle as O
+
+
+def test_pseudo_sort_matches_reference_order():
+    from snuqs_amd.circuits import random_gate_list
+    from snuqs_amd.transpile import pseudo_sort_operations_descending
+
+    rng = np.random.default_rng(0)
+    for n in (3, 7, 12):
+        ops = random_gate_list(n, 150, rng, with_unitary=True)
+        got = pseudo_sort_operations_descending(ops)
+        want = O.pseudo_sort_operations_descending(ops)
+        assert all(a is b for a, b in zip(got, want))
+        # reordering changes evaluation order only, not the state
+        assert np.abs(run_oracle(n, got) - run_oracle(n, ops)).max() < 1e-13
+
+
+@pytest.mark.parametrize("maxq", [2, 3, 4, 5])
+def test_fusion_preserves_state(maxq):
+    from snuqs_amd.circuits import random_gate_list
+    from snuqs_amd.transpile import fuse_operations
+
+    rng = np.random.default_rng(maxq)
+    n = 9
+    ops = random_gate_list(n, 250, rng, with_unitary=True)
+    fused = fuse_operations(ops, maxq)
+    assert len(fused) < len(ops)
+    assert all(len(op.targets) <= max(maxq, 3) for op in fused)
+    assert np.abs(run_oracle(n, fused) - run_oracle(n, ops)).max() < 1e-12
+
+
+def test_fusion_of_layered_circuit_uses_five_qubit_blocks():
+    from snuqs_amd.circuits import random_layered
+    from snuqs_amd.transpile import fuse_operations
+
+    n = 10
+    ops = random_layered(n, 6, seed=3)
+    fused = fuse_operations(ops, 5)
+    assert max(len(op.targets) for op in fused) == 5
+    assert np.abs(run_oracle(n, fused) - run_oracle(n, ops)).max() < 1e-12
