@@ -69,6 +69,14 @@ int sq_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
  * bandwidth ceiling used to calibrate the gate kernels' roofline. */
 int sq_copy(void *dst, const void *src, size_t bytes, void *stream);
 
+/* Launch-shape parameters of the kernels (set by the autotuner,
+ * snuqs_amd/autotune.py; process-wide).  Keys: "mfma_blocks_per_cu" (1-64,
+ * persistent grid of the 4-5 target MFMA kernels, default 16) and
+ * "mfma5_pipeline" (0/1, prefetch the next tile in the 5-target MFMA kernels,
+ * default 1).  No reference counterpart. */
+int sq_tune_set(const char *key, int value);
+int sq_tune_get(const char *key, int *value);
+
 /* ---- streams and events: stream/stream.cc:6-34, event/event.cc:5-26 ---- */
 int sq_stream_create(void **stream, int nonblocking);
 int sq_stream_destroy(void *stream);

@@ -17,7 +17,9 @@
  *   sqo_init_zero / sqo_init_basis_z
  *       csrc/operation/initialization_impl_cpu.cc:7-22.
  *   sqo_probabilities
- *       simulation.py:208-214  (np.abs(state_vector) ** 2: hypot, then square).
+ *       simulation.py:208-214  (np.abs(state_vector) ** 2: hypot, then square;
+ *       numpy's complex abs is its own routine, so results agree with numpy to
+ *       a few ulp, not bitwise).
  *   sqo_cumsum / sqo_choice
  *       simulation.py:217-220  (np.random.choice(len(sv), p=probabilities,
  *       size=shots)).  numpy's legacy RandomState.choice with p draws

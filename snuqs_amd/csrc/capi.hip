@@ -144,6 +144,27 @@ int sq_copy(void *dst, const void *src, size_t bytes, void *stream) {
   return launch_copy(dst, src, bytes, (hipStream_t)stream);
 }
 
+int sq_tune_set(const char *key, int value) {
+  const std::string k = key ? key : "";
+  if (k == "mfma_blocks_per_cu") {
+    if (value < 1 || value > 64) return fail(SQ_ERR_INVALID, "mfma_blocks_per_cu in [1, 64]");
+    tuning().mfma_blocks_per_cu = value;
+  } else if (k == "mfma5_pipeline") {
+    tuning().mfma5_pipeline = value ? 1 : 0;
+  } else {
+    return fail(SQ_ERR_INVALID, "unknown tuning key '" + k + "'");
+  }
+  return SQ_OK;
+}
+
+int sq_tune_get(const char *key, int *value) {
+  const std::string k = key ? key : "";
+  if (k == "mfma_blocks_per_cu") *value = tuning().mfma_blocks_per_cu;
+  else if (k == "mfma5_pipeline") *value = tuning().mfma5_pipeline;
+  else return fail(SQ_ERR_INVALID, "unknown tuning key '" + k + "'");
+  return SQ_OK;
+}
+
 // ---- streams / events (default priority only) ------------------------------
 int sq_stream_create(void **stream, int nonblocking) {
   hipStream_t s;
@@ -214,7 +235,7 @@ int sq_apply(void *sv, sq_dtype_t dt, uint64_t n, uint32_t num_qubits,
     int dev;
     SQ_TRY(current_device(&dev));
     std::lock_guard<std::mutex> lk(g_ws_mutex[dev]);
-    std::vector<double> g = fused_real_block(p);
+    std::vector<double> g = fused_block_3m(p);
     const size_t esz = dt == SQ_COMPLEX64 ? 4 : 8;
     const size_t bytes = g.size() * esz;
     Workspace *w;
@@ -272,7 +293,7 @@ static int upload_blocks(const std::vector<const GatePlan *> &plans, sq_dtype_t 
   std::vector<char> host(total);
   for (size_t i = 0; i < plans.size(); ++i) {
     if (off[i] == (size_t)-1) continue;
-    std::vector<double> g = fused_real_block(*plans[i]);
+    std::vector<double> g = fused_block_3m(*plans[i]);
     if (dt == SQ_COMPLEX64) {
       float *h = (float *)(host.data() + off[i]);
       for (size_t e = 0; e < g.size(); ++e) h[e] = (float)g[e];

@@ -732,5 +732,11 @@ PYBIND11_MODULE(_C, m) {
   });
   m_cuda.def("set_device", [](int d) { check(sq_set_device(d)); });
   m_cuda.def("device_synchronize", []() { check(sq_device_synchronize()); });
+  m_core.def("tune_set", [](const std::string &k, int v) { check(sq_tune_set(k.c_str(), v)); });
+  m_core.def("tune_get", [](const std::string &k) {
+    int v;
+    check(sq_tune_get(k.c_str(), &v));
+    return v;
+  });
   m.attr("__version__") = sq_version();
 }

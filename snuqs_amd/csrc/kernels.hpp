@@ -38,6 +38,13 @@ __device__ __forceinline__ uint64_t deposit(uint64_t f, const IdxMap &m) {
   return f | m.fixed_val;
 }
 
+// Launch-shape parameters chosen by the autotuner (sq_tune_set).
+struct Tuning {
+  int mfma_blocks_per_cu = 16;  // persistent grid of the MFMA kernels
+  int mfma5_pipeline = 1;       // 5-target MFMA: prefetch the next tile
+};
+Tuning &tuning();
+
 // Host: build the IdxMap that inserts `ins_mask` bits (ascending).
 inline IdxMap make_idxmap(uint64_t ins_mask, uint64_t fixed_val) {
   IdxMap m{};
@@ -48,18 +55,15 @@ inline IdxMap make_idxmap(uint64_t ins_mask, uint64_t fixed_val) {
   return m;
 }
 
-// Launch one plan on a state.  `dev_matrix` is the device copy of the real
-// block form of a DENSE plan with kk >= 4 (see fused_real_block); unused
-// otherwise.
+// Launch one plan on a state.  `dev_matrix` is the device copy of a DENSE
+// plan with kk >= 4 in the layout of fused_block_3m; unused otherwise.
 int launch_plan(const GatePlan &p, void *sv, sq_dtype_t dt, uint64_t nelems,
                 hipStream_t stream, const void *dev_matrix);
 
-// Real block form [[Re, -Im], [Im, Re]] of the canonical 2^kk x 2^kk matrix,
-// interleaved so that real row 2r+c / column 2o+c' correspond to output
-// amplitude r component c and input amplitude o component c', stored
-// column-major (element (i, kcol) at kcol * D + i), D = 2^(kk+1).  Returned
-// as doubles; the c64 launcher converts.
-std::vector<double> fused_real_block(const GatePlan &p);
+// Device layout of a DENSE plan with kk >= 4: entry (o, r) of the canonical
+// 2^kk x 2^kk matrix m[r][o] stored as {Re, Im, Re + Im, 0} at index
+// o * 2^kk + r.  Returned as doubles; the complex64 path converts to float.
+std::vector<double> fused_block_3m(const GatePlan &p);
 
 int launch_init(void *sv, sq_dtype_t dt, uint64_t nelems, bool basis,
                 hipStream_t stream);

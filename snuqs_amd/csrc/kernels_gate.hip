@@ -12,7 +12,8 @@
 //   k_diag   : DIAG plans (RotZ, ZZ, ...)
 //   k_perm   : PERM plans (X, Y, CX, CY, Swap, ISwap, PSwap, CCNot, CSwap)
 //   k_dense  : DENSE plans on 1-3 targets (H, V, RotX/Y, U, XX, YY, XY, Unitary)
-//   k_mfma_* : DENSE plans on 4-5 targets (fused blocks) on the matrix cores
+//   k_mfma4_*, k_mfma5_* : DENSE plans on 4 / 5 targets (fused blocks) on the
+//              matrix cores
 //   k_dense_lds : DENSE plans on 4-5 targets for small states (VALU)
 #include <vector>
 
@@ -50,6 +51,11 @@ int launch_init(void *sv, sq_dtype_t dt, uint64_t n, bool basis,
                        (double2 *)sv, n, (int)basis);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
+}
+
+Tuning &tuning() {
+  static Tuning t;
+  return t;
 }
 
 // Plain copy, 16 bytes per lane per access, 4 accesses in flight per lane.
@@ -240,22 +246,38 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------------------------
-// DENSE, 4-5 targets, VALU: matrix staged in LDS (broadcast reads).  Used for
-// states too small to fill MFMA tiles.
+// Dense 4-5 target matrices live in device memory in one layout shared by the
+// kernels below: entry (o, r) = {Re m[r][o], Im m[r][o], Re + Im, 0} at
+// index o * 2^K + r (column-major in the complex matrix; see fused_block_3m).
 // ---------------------------------------------------------------------------
+template <typename R>
+struct Q4;
+template <>
+struct Q4<float> {
+  using T = float4;
+};
+template <>
+struct Q4<double> {
+  using T = HIP_vector_type<double, 4>;
+};
+
 struct OffArgs {
   uint64_t o[1 << kMaxK];
 };
 
+// DENSE, 4-5 targets, VALU: matrix staged in LDS (broadcast reads).  Used for
+// states too small to fill MFMA tiles.
 template <typename R, int K>
 __global__ void __launch_bounds__(kBlock)
     k_dense_lds(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
-                OffArgs off, const R *__restrict__ G) {
+                OffArgs off, const typename Q4<R>::T *__restrict__ G) {
   using T = typename CV<R>::T;
   constexpr int D = 1 << K;
-  constexpr int RD = 2 * D;  // real block dimension
-  __shared__ R g[RD * RD];
-  for (int e = threadIdx.x; e < RD * RD; e += kBlock) g[e] = G[e];
+  __shared__ R gr[D * D], gi[D * D];  // [o][r]
+  for (int e = threadIdx.x; e < D * D; e += kBlock) {
+    gr[e] = G[e].x;
+    gi[e] = G[e].y;
+  }
   __syncthreads();
   const uint64_t f = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   if (f >= nfree) return;
@@ -263,14 +285,12 @@ __global__ void __launch_bounds__(kBlock)
   T x[D];
 #pragma unroll
   for (int c = 0; c < D; ++c) x[c] = sv[base + off.o[c]];
-  // g is column-major real block: element (i, kcol) at kcol*RD + i;
-  // Re m[r][c] = g(2r, 2c), Im m[r][c] = g(2r+1, 2c).
 #pragma unroll
   for (int r = 0; r < D; ++r) {
     R yr = 0, yi = 0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      const R mr = g[(2 * c) * RD + 2 * r], mi = g[(2 * c) * RD + 2 * r + 1];
+      const R mr = gr[c * D + r], mi = gi[c * D + r];
       yr += mr * x[c].x - mi * x[c].y;
       yi += mr * x[c].y + mi * x[c].x;
     }
@@ -281,39 +301,41 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// ---------------------------------------------------------------------------
-// DENSE, 4-5 targets on the matrix cores.
-//
-// Y = G X for a tile of groups, in real block form: the 2^K x 2^K complex G is
-// the D x D real matrix Gr (D = 2^(K+1)), a group's 2^K amplitudes the real
-// D-vector (re0, im0, re1, im1, ...).  complex64 uses
-// v_mfma_f32_32x32x2f32 with 32 groups per wave tile, complex128 uses
-// v_mfma_f64_16x16x4f64 with 16 groups per wave tile.  The B operand of one
-// MFMA step is loaded straight from HBM: lane l takes one float (double) of one
-// amplitude, so a step's 64 lanes read the re and im words of 32 (16)
-// consecutive groups.  The A operand (Gr) is staged in LDS once per workgroup.
-// ---------------------------------------------------------------------------
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-template <int K>
+// ---------------------------------------------------------------------------
+// DENSE, 4 targets on the matrix cores, real block form: the 16x16 complex G
+// is the 32x32 real matrix Gr = [[Re, -Im], [Im, Re]] interleaved so real row
+// 2r+c / column 2o+c' are amplitude r component c / amplitude o component c'
+// (staged in LDS column-major from the shared layout).  complex64:
+// v_mfma_f32_32x32x2f32, 32 groups per wave tile; complex128:
+// v_mfma_f64_16x16x4f64, 16 groups per wave tile.  Lane l takes one float
+// (double) of one amplitude per MFMA step.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ void stage_real_block16(R *As, const typename Q4<R>::T *G) {
+  constexpr int RD = 32;
+  for (int e = threadIdx.x; e < RD * RD; e += kBlock) {
+    const int kcol = e / RD, i = e % RD;
+    const int r = i >> 1, c = i & 1, o = kcol >> 1, cp = kcol & 1;
+    const auto z = G[o * 16 + r];
+    As[e] = (c == cp) ? z.x : (c ? z.y : -z.y);
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
-    k_mfma_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im,
-               OffArgs off, const float *__restrict__ G) {
-  constexpr int D = 2 << K;   // 32 or 64
-  constexpr int NT = D / 32;  // output row tiles
-  constexpr int NS = D / 2;   // MFMA k-steps
+    k_mfma4_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, OffArgs off,
+                const float4 *__restrict__ G) {
+  constexpr int D = 32, NS = 16;
   __shared__ float As[D * D];
-  for (int e = threadIdx.x; e < D * D; e += kBlock) As[e] = G[e];
+  stage_real_block16<float>(As, G);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 31, half = lane >> 5;
-  float a[NT][NS];
+  float a[NS];
 #pragma unroll
-  for (int t = 0; t < NT; ++t)
-#pragma unroll
-    for (int s = 0; s < NS; ++s) a[t][s] = As[(2 * s + half) * D + 32 * t + col];
-
+  for (int s = 0; s < NS; ++s) a[s] = As[(2 * s + half) * D + col];
   for (uint64_t tile = (uint64_t)blockIdx.x * 4 + wave; tile < ntiles;
        tile += (uint64_t)gridDim.x * 4) {
     const uint64_t base = deposit(tile * 32 + col, im);
@@ -321,44 +343,33 @@ __global__ void __launch_bounds__(kBlock)
     float b[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) b[s] = src[2 * off.o[s]];
-    floatx16 acc[NT];
+    floatx16 acc;
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
 #pragma unroll
     for (int s = 0; s < NS; ++s)
-#pragma unroll
-      for (int t = 0; t < NT; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[t][s], b[s], acc[t], 0,
-                                                       0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s], b[s], acc, 0, 0, 0);
     // C layout: lane l, register v -> column l%32, row 8(v/4) + 4(l/32) + v%4.
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = 16 * t + 4 * q;  // + 2*half, +1
-        const uint64_t o0 = half ? off.o[r + 2] : off.o[r];
-        const uint64_t o1 = half ? off.o[r + 3] : off.o[r + 1];
-        sv[base + o0] = make_float2(acc[t][4 * q + 0], acc[t][4 * q + 1]);
-        sv[base + o1] = make_float2(acc[t][4 * q + 2], acc[t][4 * q + 3]);
-      }
+    for (int q = 0; q < 4; ++q) {
+      const int r = 4 * q;  // + 2*half, +1
+      const uint64_t o0 = half ? off.o[r + 2] : off.o[r];
+      const uint64_t o1 = half ? off.o[r + 3] : off.o[r + 1];
+      sv[base + o0] = make_float2(acc[4 * q + 0], acc[4 * q + 1]);
+      sv[base + o1] = make_float2(acc[4 * q + 2], acc[4 * q + 3]);
+    }
   }
 }
 
-template <int K>
 __global__ void __launch_bounds__(kBlock)
-    k_mfma_c128(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im,
-                OffArgs off, const double *__restrict__ G) {
-  constexpr int D = 2 << K;   // 32 or 64
-  constexpr int NT = D / 16;  // output row tiles
-  constexpr int NS = D / 4;   // MFMA k-steps
+    k_mfma4_c128(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, OffArgs off,
+                 const typename Q4<double>::T *__restrict__ G) {
+  constexpr int D = 32, NT = 2, NS = 8;
   __shared__ double As[D * D];
-  for (int e = threadIdx.x; e < D * D; e += kBlock) As[e] = G[e];
+  stage_real_block16<double>(As, G);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, q = lane >> 4;
-
   for (uint64_t tile = (uint64_t)blockIdx.x * 4 + wave; tile < ntiles;
        tile += (uint64_t)gridDim.x * 4) {
     const uint64_t base = deposit(tile * 16 + col, im);
@@ -382,9 +393,7 @@ __global__ void __launch_bounds__(kBlock)
         acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[s], acc[t], 0, 0, 0);
       }
     // C layout (v_mfma_f64_16x16x4f64): lane l, register v -> column l%16,
-    // row 4v + l/16.  Row 16t + 4v + q is component q%2 of output amplitude
-    // 8t + 2v + q/2, so each register is one double of one amplitude; lanes
-    // with q = 0,1 (2,3) write the re/im halves of one amplitude per group.
+    // row 4v + l/16: component q%2 of output amplitude 8t + 2v + q/2.
     double *dst = reinterpret_cast<double *>(sv + base) + (q & 1);
 #pragma unroll
     for (int t = 0; t < NT; ++t)
@@ -398,19 +407,156 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------------------------
+// DENSE, 5 targets on the matrix cores: three real GEMMs per complex product
+// (T1 = Gr Xr, T2 = Gi Xi, T3 = (Gr + Gi)(Xr + Xi); Yr = T1 - T2,
+// Yi = T3 - T1 - T2), i.e. 48 instead of 64 MFMAs per wave tile, with whole
+// complex amplitudes as operands: the B operand of a step is lane l's
+// amplitude o of group l%32 (l%16 for complex128), loaded as one float2
+// (double2), and each output register pair is one amplitude of one group.
+// The next tile's amplitudes are loaded before the current tile's MFMAs, so
+// every wave keeps a tile of loads in flight while its matrix cores run.
+// A operands {Re, Im, Re + Im} are read from LDS per step (ds_read_b128).
+// ---------------------------------------------------------------------------
+template <bool PIPE>
+__global__ void __launch_bounds__(kBlock)
+    k_mfma5_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, OffArgs off,
+                const float4 *__restrict__ G) {
+  constexpr int D = 32, NS = 16;
+  __shared__ float4 As[D * D];  // [o][r]
+  for (int e = threadIdx.x; e < D * D; e += kBlock) As[e] = G[e];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 31, half = lane >> 5;
+  uint64_t tile = (uint64_t)blockIdx.x * 4 + wave;
+  const uint64_t step = (uint64_t)gridDim.x * 4;
+  if (tile >= ntiles) return;
+  // Lane half h reads input amplitude o = 2s + h (local bit 0 = h) and writes
+  // output amplitudes r = 8(v/4) + 4h + v%4 (local bit 2 = h): both are a
+  // uniform offset plus a per-lane constant.
+  const uint64_t lin = half ? off.o[1] : 0;   // stride of local bit 0
+  const uint64_t lout = half ? off.o[4] : 0;  // stride of local bit 2
+  float2 cur[NS], nxt[NS];
+  uint64_t base = deposit(tile * 32 + col, im);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) cur[s] = sv[base + lin + off.o[2 * s]];
+  while (true) {
+    const uint64_t ntile = tile + step;
+    const bool more = ntile < ntiles;
+    uint64_t nbase = 0;
+    if (PIPE && more) {
+      nbase = deposit(ntile * 32 + col, im);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) nxt[s] = sv[nbase + lin + off.o[2 * s]];
+    }
+    floatx16 t1, t2, t3;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) t1[v] = t2[v] = t3[v] = 0.f;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const float4 a = As[(2 * s + half) * D + col];
+      t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, cur[s].x, t1, 0, 0, 0);
+      t2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, cur[s].y, t2, 0, 0, 0);
+      t3 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, cur[s].x + cur[s].y, t3, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+      sv[base + lout + off.o[8 * (v / 4) + (v % 4)]] =
+          make_float2(t1[v] - t2[v], t3[v] - t1[v] - t2[v]);
+    if (!more) break;
+    tile = ntile;
+    if (PIPE) {
+      base = nbase;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) cur[s] = nxt[s];
+    } else {
+      base = deposit(tile * 32 + col, im);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) cur[s] = sv[base + lin + off.o[2 * s]];
+    }
+  }
+}
+
+template <bool PIPE>
+__global__ void __launch_bounds__(kBlock)
+    k_mfma5_c128(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, OffArgs off,
+                 const typename Q4<double>::T *__restrict__ G) {
+  using D4 = typename Q4<double>::T;
+  constexpr int D = 32, NS = 8, NT = 2;
+  __shared__ D4 As[D * D];  // [o][r], 32 KB
+  for (int e = threadIdx.x; e < D * D; e += kBlock) As[e] = G[e];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, q = lane >> 4;
+  uint64_t tile = (uint64_t)blockIdx.x * 4 + wave;
+  const uint64_t step = (uint64_t)gridDim.x * 4;
+  if (tile >= ntiles) return;
+  // Lane quarter q reads input amplitude o = 4s + q and writes output
+  // amplitudes r = 16t + 4v + q: q is local bits 0-1 in both, so one per-lane
+  // constant plus uniform offsets addresses everything.
+  const uint64_t lq = ((q & 1) ? off.o[1] : 0) + ((q & 2) ? off.o[2] : 0);
+  double2 cur[NS], nxt[NS];
+  uint64_t base = deposit(tile * 16 + col, im);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) cur[s] = sv[base + lq + off.o[4 * s]];
+  while (true) {
+    const uint64_t ntile = tile + step;
+    const bool more = ntile < ntiles;
+    uint64_t nbase = 0;
+    if (PIPE && more) {
+      nbase = deposit(ntile * 16 + col, im);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) nxt[s] = sv[nbase + lq + off.o[4 * s]];
+    }
+    doublex4 t1[NT], t2[NT], t3[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) t1[t][v] = t2[t][v] = t3[t][v] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const double xs = cur[s].x + cur[s].y;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const D4 a = As[(4 * s + q) * D + 16 * t + col];
+        t1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, cur[s].x, t1[t], 0, 0, 0);
+        t2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, cur[s].y, t2[t], 0, 0, 0);
+        t3[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.z, xs, t3[t], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        sv[base + lq + off.o[16 * t + 4 * v]] =
+            make_double2(t1[t][v] - t2[t][v], t3[t][v] - t1[t][v] - t2[t][v]);
+    if (!more) break;
+    tile = ntile;
+    if (PIPE) {
+      base = nbase;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) cur[s] = nxt[s];
+    } else {
+      base = deposit(tile * 16 + col, im);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) cur[s] = sv[base + lq + off.o[4 * s]];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-std::vector<double> fused_real_block(const GatePlan &p) {
-  const int d = 1 << p.kk, D = 2 * d;
-  std::vector<double> g(D * D);
+std::vector<double> fused_block_3m(const GatePlan &p) {
+  const int d = 1 << p.kk;
+  std::vector<double> g(4 * d * d);
   for (int r = 0; r < d; ++r)
     for (int o = 0; o < d; ++o) {
       const cd z = p.m[r * d + o];
-      // element (i, kcol) at kcol * D + i
-      g[(2 * o + 0) * D + 2 * r + 0] = z.real();
-      g[(2 * o + 1) * D + 2 * r + 0] = -z.imag();
-      g[(2 * o + 0) * D + 2 * r + 1] = z.imag();
-      g[(2 * o + 1) * D + 2 * r + 1] = z.real();
+      double *e = &g[4 * (o * d + r)];
+      e[0] = z.real();
+      e[1] = z.imag();
+      e[2] = z.real() + z.imag();
+      e[3] = 0.0;
     }
   return g;
 }
@@ -518,30 +664,35 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
         if (mfma) {
           const uint64_t ntiles = nfree / tile_groups;
           uint64_t blocks = (ntiles + 3) / 4;
-          const uint64_t cap = 256ull * 16;  // persistent: 16 workgroups per CU
+          const uint64_t cap = 256ull * (uint64_t)tuning().mfma_blocks_per_cu;
           if (blocks > cap) blocks = cap;
+          const bool pipe = tuning().mfma5_pipeline != 0;
+          using Q = typename Q4<R>::T;
           if constexpr (c64) {
             if (p.kk == 4)
-              hipLaunchKernelGGL((k_mfma_c64<4>), dim3(blocks), dim3(kBlock), 0, st,
-                                 (float2 *)sv, ntiles, im, off, (const float *)gdev);
+              hipLaunchKernelGGL(k_mfma4_c64, dim3(blocks), dim3(kBlock), 0, st,
+                                 (float2 *)sv, ntiles, im, off, (const Q *)gdev);
             else
-              hipLaunchKernelGGL((k_mfma_c64<5>), dim3(blocks), dim3(kBlock), 0, st,
-                                 (float2 *)sv, ntiles, im, off, (const float *)gdev);
+              hipLaunchKernelGGL(pipe ? k_mfma5_c64<true> : k_mfma5_c64<false>, dim3(blocks),
+                                 dim3(kBlock), 0, st, (float2 *)sv, ntiles, im, off,
+                                 (const Q *)gdev);
           } else {
             if (p.kk == 4)
-              hipLaunchKernelGGL((k_mfma_c128<4>), dim3(blocks), dim3(kBlock), 0, st,
-                                 (double2 *)sv, ntiles, im, off, (const double *)gdev);
+              hipLaunchKernelGGL(k_mfma4_c128, dim3(blocks), dim3(kBlock), 0, st,
+                                 (double2 *)sv, ntiles, im, off, (const Q *)gdev);
             else
-              hipLaunchKernelGGL((k_mfma_c128<5>), dim3(blocks), dim3(kBlock), 0, st,
-                                 (double2 *)sv, ntiles, im, off, (const double *)gdev);
+              hipLaunchKernelGGL(pipe ? k_mfma5_c128<true> : k_mfma5_c128<false>,
+                                 dim3(blocks), dim3(kBlock), 0, st, (double2 *)sv, ntiles,
+                                 im, off, (const Q *)gdev);
           }
         } else {
+          using Q = typename Q4<R>::T;
           if (p.kk == 4)
             hipLaunchKernelGGL((k_dense_lds<R, 4>), grid_for(nfree, kBlock), dim3(kBlock),
-                               0, st, sv, nfree, im, off, gdev);
+                               0, st, sv, nfree, im, off, (const Q *)gdev);
           else
             hipLaunchKernelGGL((k_dense_lds<R, 5>), grid_for(nfree, kBlock), dim3(kBlock),
-                               0, st, sv, nfree, im, off, gdev);
+                               0, st, sv, nfree, im, off, (const Q *)gdev);
         }
       }
       break;

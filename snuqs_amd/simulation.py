@@ -1,0 +1,117 @@
+"""State-vector simulation driven by the HIP engine.
+
+Mirrors src/braket/snuqs/simulation.py of the reference (Simulation ABC,
+StateVectorSimulation with state_vector / probabilities / retrieve_samples /
+evolve) for the single-GPU tier.  Differences, by design:
+
+* the state lives in HBM (``DeviceType.CUDA`` = the HIP device) in complex64
+  or complex128; the reference's CPU / hybrid / offload tiers
+  (simulation.py:113-194, 286-542) collapse into one GPU's memory;
+* ``evolve`` uploads the whole gate list once (``_C.functionals.Program``) and
+  runs it in one call instead of one pybind call per gate
+  (simulation.py:274-284), after optional fusion into <= 5-qubit blocks;
+* ``retrieve_samples`` draws the uniforms from numpy's global RandomState as
+  np.random.choice does (simulation.py:220) and searches the cumulative
+  distribution on the GPU: with ``exact_sampling`` the indices are identical
+  to np.random.choice(len(sv), p=probabilities, size=shots);
+* ``probabilities`` is computed on the GPU in float64.
+"""
+from __future__ import annotations
+
+from abc import ABC, abstractmethod
+
+import numpy as np
+
+from ._C import DeviceType
+from ._C.functionals import Program, initialize_basis_z
+from ._C.result_types import StateVector
+from . import _C
+from .subcircuit import Subcircuit
+from .transpile import fuse_operations
+from .types import AcceleratorType, OffloadType
+
+
+class Simulation(ABC):
+    """Tracks the evolution of a quantum system with ``qubit_count`` qubits."""
+
+    def __init__(self, qubit_count: int, shots: int):
+        self._qubit_count = qubit_count
+        self._shots = shots
+
+    @property
+    def qubit_count(self) -> int:
+        return self._qubit_count
+
+    @property
+    def shots(self) -> int:
+        return self._shots
+
+    @abstractmethod
+    def retrieve_samples(self) -> np.ndarray:
+        """Sampled basis-state indices (one per shot)."""
+
+    @property
+    @abstractmethod
+    def probabilities(self) -> np.ndarray:
+        """Probability of each computational basis state."""
+
+    @abstractmethod
+    def evolve(self, subcircuit: Subcircuit) -> None:
+        """Apply the subcircuit's gates to the state."""
+
+
+class StateVectorSimulation(Simulation):
+    def __init__(self, qubit_count: int, shots: int, subcircuit: Subcircuit,
+                 *, precision: str = "complex128", fuse: int = 5,
+                 exact_sampling: bool = True):
+        super().__init__(qubit_count=qubit_count, shots=shots)
+        if subcircuit.offload not in (OffloadType.NONE, None):
+            raise TypeError("offload tiers are not part of the single-GPU engine; "
+                            "use offload=None")
+        if subcircuit.accelerator not in (AcceleratorType.CUDA, AcceleratorType.HIP):
+            raise TypeError(f"accelerator {subcircuit.accelerator} is not supported: the "
+                            "engine runs on the GPU (accelerator='cuda' or 'hip')")
+        self._precision = precision
+        self._fuse = fuse
+        self._exact = exact_sampling
+        self._state_vector = StateVector(DeviceType.CUDA, qubit_count, dtype=precision)
+        initialize_basis_z(self._state_vector)
+
+    @property
+    def precision(self) -> str:
+        return self._precision
+
+    @property
+    def device_state(self) -> StateVector:
+        """The device-resident state (for the engine's functionals)."""
+        return self._state_vector
+
+    def evolve(self, subcircuit: Subcircuit) -> None:
+        ops = list(subcircuit.operations)
+        if not ops:
+            return
+        if self._fuse and self._fuse >= 2:
+            ops = fuse_operations(ops, self._fuse)
+        Program(self._qubit_count, ops, dtype=self._precision).run(self._state_vector)
+
+    @property
+    def state_vector(self) -> np.ndarray:
+        """Host copy of the state (complex128 for a complex128 simulation)."""
+        return np.array(self._state_vector.cpu(), copy=True)
+
+    @property
+    def probabilities(self) -> np.ndarray:
+        return _C.functionals.probabilities(self._state_vector)
+
+    def retrieve_samples(self) -> np.ndarray:
+        if self._shots == 0:
+            return np.array([], dtype=np.int64)
+        u = np.random.random_sample(self._shots)
+        return _C.functionals.sample(self._state_vector, u, exact=self._exact)
+
+    def amplitudes(self, indices) -> np.ndarray:
+        return _C.functionals.amplitudes(self._state_vector, [int(i) for i in indices])
+
+    def expectation(self, paulis, qubits, coeffs) -> float:
+        return _C.functionals.expectation(self._state_vector, list(paulis),
+                                          [list(q) for q in qubits], list(coeffs))[0]

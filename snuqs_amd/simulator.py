@@ -1,0 +1,235 @@
+"""Braket local-simulator entry point (src/braket/snuqs/simulator.py of the reference).
+
+``StateVectorSimulator`` keeps the reference's device id ("snuqs"), its
+``run(ir, *, shots, qubit_count_cpu, qubit_count_cuda, qubit_count_slice,
+accelerator, prefetch, offload, path, block_count)`` signature
+(simulator.py:152-155, 450-461) and its steps: contiguous-qubit remap
+(:278-307), transpile, simulation.evolve, result assembly (:309-381).  The
+Braket SDK and schemas are optional: without them ``run`` takes a JAQCD
+program as a JSON dict (or string) and returns a ``TaskResult`` with the same
+fields as GateModelTaskResult.  Extra keyword arguments: ``precision``
+("complex64" / "complex128"), ``fuse`` (largest fused block, 0 = off) and
+``exact_sampling``.
+
+Differences from the reference, by design: the engine runs on the GPU only
+(accelerator None / "cuda" / "hip"; "cpu" / "hybrid" and the offload tiers
+raise), and besides StateVector the result types Amplitude, Probability and
+Expectation (Pauli-product observables) are computed on the GPU.  OpenQASM
+source needs the reference's ANTLR interpreter and is not supported.
+"""
+from __future__ import annotations
+
+import json
+import math
+import uuid
+from dataclasses import dataclass, field
+from typing import Any, Optional
+
+import numpy as np
+
+from . import _C
+from .gate_operations import BRAKET_GATES, from_braket_instruction
+from .simulation import StateVectorSimulation
+from .subcircuit import Subcircuit
+from .transpile import pseudo_sort_operations_descending
+from .types import AcceleratorType, OffloadType, PrefetchType
+
+
+@dataclass
+class TaskResult:
+    """GateModelTaskResult's fields, for use without the Braket schemas."""
+    measurements: list
+    measuredQubits: list
+    resultTypes: list
+    taskMetadata: dict = field(default_factory=dict)
+    additionalMetadata: dict = field(default_factory=dict)
+
+
+def _get(x, name, default=None):
+    if isinstance(x, dict):
+        return x.get(name, default)
+    return getattr(x, name, default)
+
+
+class BaseSimulator:
+    DEVICE_ID = "snuqs"
+
+    @staticmethod
+    def _validate_simulation_type(accelerator: Optional[str]) -> AcceleratorType:
+        supported = {None: AcceleratorType.CUDA, "cuda": AcceleratorType.CUDA,
+                     "hip": AcceleratorType.HIP, "gpu": AcceleratorType.CUDA}
+        if accelerator in ("cpu", "hybrid"):
+            raise TypeError(f"Accelerator {accelerator} is not supported: this engine runs "
+                            "on the GPU (accelerator='cuda' or 'hip')")
+        if accelerator not in supported:
+            raise TypeError(f"Accelerator {accelerator} is not supported")
+        return supported[accelerator]
+
+    @staticmethod
+    def _validate_offload_type(offload: Optional[str]) -> OffloadType:
+        if offload in (None, "none"):
+            return OffloadType.NONE
+        if offload in ("cpu", "storage"):
+            raise TypeError(f"Offload {offload} is not supported: the state lives in one "
+                            "GPU's HBM (up to 33 qubits complex128, 34 qubits complex64)")
+        raise TypeError(f"Offload {offload} is not supported")
+
+    @staticmethod
+    def _validate_prefetch_type(prefetch: Optional[str]) -> PrefetchType:
+        table = {None: PrefetchType.NONE, "none": PrefetchType.NONE,
+                 "cuda": PrefetchType.CUDA, "storage": PrefetchType.STORAGE}
+        if prefetch not in table:
+            raise TypeError(f"Prefetch {prefetch} is not supported")
+        return table[prefetch]
+
+    @staticmethod
+    def _validate_qubit_count(qubit_count: int, precision: str) -> None:
+        free, _ = _C.core.cuda.mem_info()
+        need = (1 << qubit_count) * (8 if precision == "complex64" else 16)
+        if need > free:
+            raise MemoryError(f"{qubit_count} qubits in {precision} need {need} bytes; "
+                              f"{free} bytes of device memory are free")
+
+    @staticmethod
+    def _contiguous_qubit_mapping(qubits) -> dict:
+        return {q: i for i, q in enumerate(sorted(qubits))}
+
+    @staticmethod
+    def _formatted_measurements(samples: np.ndarray, qubit_count: int,
+                                measured_qubits=None) -> list:
+        """Sample index -> list of bit characters, qubit q = character q
+        (simulator.py:309-343)."""
+        bits = ((samples[:, None] >> (qubit_count - 1 - np.arange(qubit_count))) & 1)
+        if measured_qubits:
+            mq = np.array(measured_qubits)
+            inside = mq[mq < qubit_count]
+            bits = bits[:, inside]
+            pad = int(np.count_nonzero(mq >= qubit_count))
+            if pad:
+                bits = np.pad(bits, ((0, 0), (0, pad)))
+        return [[str(b) for b in row] for row in bits.tolist()]
+
+    @staticmethod
+    def _parse_program(ir):
+        if isinstance(ir, (str, bytes)):
+            ir = json.loads(ir)
+        if isinstance(ir, dict):
+            return ir.get("instructions", []), ir.get("results") or [], \
+                ir.get("basis_rotation_instructions") or []
+        src = _get(ir, "source")
+        if src is not None:
+            raise NotImplementedError("OpenQASM programs need the reference's OpenQASM "
+                                      "interpreter; submit the circuit as JAQCD")
+        return (_get(ir, "instructions", []), _get(ir, "results") or [],
+                _get(ir, "basis_rotation_instructions") or [])
+
+    def run(self, ir, *args, **kwargs):
+        return self.run_jaqcd(ir, *args, **kwargs)
+
+    def run_jaqcd(self, ir, *, shots: int = 0, qubit_count_cpu: Optional[int] = None,
+                  qubit_count_cuda: Optional[int] = None,
+                  qubit_count_slice: Optional[int] = None, accelerator: Optional[str] = None,
+                  prefetch: Optional[str] = None, offload: Optional[str] = None,
+                  path=None, block_count: Optional[int] = None,
+                  precision: str = "complex128", fuse: int = 5,
+                  exact_sampling: bool = True):
+        instructions, results, basis = self._parse_program(ir)
+        ops = [from_braket_instruction(i) for i in instructions]
+        if shots > 0 and basis:
+            ops += [from_braket_instruction(i) for i in basis]
+        used = {q for op in ops for q in op.targets}
+        for r in results:
+            for q in (_get(r, "targets") or []):
+                used.add(int(q))
+        qmap = self._contiguous_qubit_mapping(used)
+        for op in ops:
+            op.targets = [qmap[q] for q in op.targets]
+        qubit_count = max(len(qmap), 1)
+        acc = self._validate_simulation_type(accelerator)
+        self._validate_prefetch_type(prefetch)
+        off = self._validate_offload_type(offload)
+        self._validate_qubit_count(qubit_count, precision)
+        ops = pseudo_sort_operations_descending(ops)
+        sub = Subcircuit(qubit_count=qubit_count, accelerator=acc, offload=off, operations=ops)
+        sim = StateVectorSimulation(qubit_count, shots, sub, precision=precision, fuse=fuse,
+                                    exact_sampling=exact_sampling)
+        sim.evolve(sub)
+        values = []
+        if not shots:
+            for r in results:
+                values.append({"type": r, "value": self._calculate(r, sim, qmap, qubit_count)})
+        measurements = []
+        if shots:
+            measurements = self._formatted_measurements(sim.retrieve_samples(), qubit_count)
+        meta = {"id": str(uuid.uuid4()), "shots": shots, "deviceId": self.DEVICE_ID}
+        try:  # the Braket schemas, when installed
+            from braket.task_result import (AdditionalMetadata, GateModelTaskResult,
+                                            ResultTypeValue, TaskMetadata)
+
+            return GateModelTaskResult.construct(
+                taskMetadata=TaskMetadata(**meta),
+                additionalMetadata=AdditionalMetadata(action=ir),
+                resultTypes=[ResultTypeValue.construct(type=v["type"], value=v["value"])
+                             for v in values],
+                measurements=measurements, measuredQubits=list(range(qubit_count)))
+        except ImportError:
+            return TaskResult(measurements=measurements, measuredQubits=list(range(qubit_count)),
+                              resultTypes=values, taskMetadata=meta)
+
+    @staticmethod
+    def _calculate(r, sim: StateVectorSimulation, qmap, n) -> Any:
+        t = str(getattr(_get(r, "type"), "value", _get(r, "type"))).lower()
+        targets = [qmap[int(q)] for q in (_get(r, "targets") or [])]
+        if t == "statevector":
+            return sim.state_vector
+        if t == "amplitude":
+            states = _get(r, "states")
+            amps = sim.amplitudes([int(s, 2) for s in states])
+            return {s: complex(a) for s, a in zip(states, amps)}
+        if t == "probability":
+            p = sim.probabilities
+            if not targets:
+                return p
+            keep = [n - 1 - q for q in targets]
+            idx = np.arange(p.size)
+            sub = np.zeros(idx.size, dtype=np.int64)
+            for j, b in enumerate(keep):
+                sub |= ((idx >> b) & 1) << (len(keep) - 1 - j)
+            return np.bincount(sub, weights=p, minlength=1 << len(keep))
+        if t == "expectation":
+            obs = _get(r, "observable")
+            letters = [str(o).upper() for o in obs]
+            if len(letters) == 1 and not targets:
+                raise ValueError("an expectation over all qubits needs targets")
+            if len(letters) == 1 and len(targets) > 1:
+                letters = letters * len(targets)
+            if any(L not in ("X", "Y", "Z", "I") for L in letters):
+                raise NotImplementedError("only Pauli-product observables are supported")
+            return sim.expectation(["".join(letters)], [targets], [1.0])
+        raise NotImplementedError(f"Result type {t} not supported")
+
+
+class StateVectorSimulator(BaseSimulator):
+    DEVICE_ID = "snuqs"
+
+    def __init__(self):
+        self.max_qubits = 34
+
+    @property
+    def properties(self) -> dict:
+        return {
+            "service": {"shotsRange": [0, 1 << 20]},
+            "action": {"braket.ir.jaqcd.program": {
+                "actionType": "braket.ir.jaqcd.program", "version": ["1"],
+                "supportedOperations": sorted(k for k in BRAKET_GATES
+                                              if k not in ("U", "gphase")),
+                "supportedResultTypes": [{"name": n} for n in
+                                         ("StateVector", "Amplitude", "Probability",
+                                          "Expectation")],
+            }},
+            "paradigm": {"qubitCount": self.max_qubits},
+        }
+
+    def initialize_simulation(self, *, qubit_count: int, shots: int, subcircuit: Subcircuit,
+                              **kw) -> StateVectorSimulation:
+        return StateVectorSimulation(qubit_count, shots, subcircuit, **kw)
