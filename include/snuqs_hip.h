@@ -70,10 +70,11 @@ int sq_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 int sq_copy(void *dst, const void *src, size_t bytes, void *stream);
 
 /* Launch-shape parameters of the kernels (set by the autotuner,
- * snuqs_amd/autotune.py; process-wide).  Keys: "mfma_blocks_per_cu" (1-64,
- * persistent grid of the 4-5 target MFMA kernels, default 16) and
- * "mfma5_pipeline" (0/1, prefetch the next tile in the 5-target MFMA kernels,
- * default 1).  No reference counterpart. */
+ * snuqs_amd/autotune.py; process-wide), one value per dtype: keys
+ * "mfma_blocks_per_cu_c64" / "_c128" (1-64, persistent grid of the 4-5 target
+ * MFMA kernels, default 16) and "mfma5_pipeline_c64" / "_c128" (0/1, prefetch
+ * the next tile in the 5-target MFMA kernels, default 1).  No reference
+ * counterpart. */
 int sq_tune_set(const char *key, int value);
 int sq_tune_get(const char *key, int *value);
 
@@ -208,6 +209,27 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs,
                         const uint64_t *xmask, const uint64_t *zmask,
                         const uint32_t *ny, const double *coeffs,
                         double *energy, double *grads, void *stream);
+
+/* ---- batched QAOA MaxCut (new; no reference counterpart).
+ * num_instances circuits of uneven qubit counts num_qubits[j] (2..30) on
+ * graphs with num_edges[j] edges (qubit pairs, back to back in `edges`):
+ * H on every qubit, then p layers of ZZ(2 gamma_l) on every edge and
+ * RX(2 beta_l) on every qubit.  A run computes for every instance the energy
+ * <C> = sum_edges (1 - <Z_a Z_b>) / 2 and its adjoint gradient.  Every launch
+ * serves all instances (one pass of each); the launch sequence is captured
+ * once into a hipGraph and replayed by later runs (use_graph != 0).
+ * out: num_instances x (1 + 2p) host doubles: E, dE/dgamma_0..p-1,
+ * dE/dbeta_0..p-1.  gammas/betas: num_instances x p host doubles.
+ * complex64 states with FP64 reductions (complex128 also accepted). */
+int sq_qaoa_create(void **batch, sq_dtype_t dtype, uint32_t num_instances,
+                   const uint32_t *num_qubits, const uint32_t *num_edges,
+                   const uint32_t *edges, uint32_t p);
+int sq_qaoa_run(void *batch, const double *gammas, const double *betas,
+                int use_graph, double *out, void *stream);
+/* Kernel launches per run, bytes of all states, HBM bytes one run moves. */
+int sq_qaoa_stats(void *batch, uint64_t *launches, double *state_bytes,
+                  double *traffic_bytes);
+int sq_qaoa_destroy(void *batch);
 
 #ifdef __cplusplus
 }

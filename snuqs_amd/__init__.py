@@ -17,4 +17,8 @@ except ImportError as e:  # fail loudly: no fallback path exists
         "`python -c 'import __graft_entry__ as g; g.build()'` from the repository root"
     ) from e
 
+from . import autotune as _autotune  # noqa: E402
+
+_autotune.load()  # launch-shape knobs measured on the target GPU, if present
+
 __all__ = ["_C"]

@@ -28,6 +28,7 @@ HIP_SOURCES = [
     "kernels_gate.hip",
     "kernels_reduce.hip",
     "kernels_sample.hip",
+    "kernels_qaoa.hip",
     "capi.hip",
 ]
 HOST_SOURCES = ["host/pymodule.cc", "host/gates.cc"]

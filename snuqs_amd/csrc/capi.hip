@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "kernels_qaoa.hpp"
 #include "kernels_reduce.hpp"
 
 namespace sq {
@@ -144,24 +145,37 @@ int sq_copy(void *dst, const void *src, size_t bytes, void *stream) {
   return launch_copy(dst, src, bytes, (hipStream_t)stream);
 }
 
+// Tuning keys: "<knob>_c64" / "<knob>_c128".
+static int *tune_slot(const std::string &k) {
+  static const struct {
+    const char *name;
+    int *(*slot)();
+  } table[] = {
+      {"mfma_blocks_per_cu_c64", [] { return &tuning().mfma_blocks_per_cu[0]; }},
+      {"mfma_blocks_per_cu_c128", [] { return &tuning().mfma_blocks_per_cu[1]; }},
+      {"mfma5_pipeline_c64", [] { return &tuning().mfma5_pipeline[0]; }},
+      {"mfma5_pipeline_c128", [] { return &tuning().mfma5_pipeline[1]; }},
+  };
+  for (auto &e : table)
+    if (k == e.name) return e.slot();
+  return nullptr;
+}
+
 int sq_tune_set(const char *key, int value) {
   const std::string k = key ? key : "";
-  if (k == "mfma_blocks_per_cu") {
-    if (value < 1 || value > 64) return fail(SQ_ERR_INVALID, "mfma_blocks_per_cu in [1, 64]");
-    tuning().mfma_blocks_per_cu = value;
-  } else if (k == "mfma5_pipeline") {
-    tuning().mfma5_pipeline = value ? 1 : 0;
-  } else {
-    return fail(SQ_ERR_INVALID, "unknown tuning key '" + k + "'");
-  }
+  int *slot = tune_slot(k);
+  if (!slot) return fail(SQ_ERR_INVALID, "unknown tuning key '" + k + "'");
+  if (k.rfind("mfma_blocks_per_cu", 0) == 0 && (value < 1 || value > 64))
+    return fail(SQ_ERR_INVALID, "mfma_blocks_per_cu must be in [1, 64]");
+  *slot = k.rfind("mfma5_pipeline", 0) == 0 ? (value ? 1 : 0) : value;
   return SQ_OK;
 }
 
 int sq_tune_get(const char *key, int *value) {
   const std::string k = key ? key : "";
-  if (k == "mfma_blocks_per_cu") *value = tuning().mfma_blocks_per_cu;
-  else if (k == "mfma5_pipeline") *value = tuning().mfma5_pipeline;
-  else return fail(SQ_ERR_INVALID, "unknown tuning key '" + k + "'");
+  int *slot = tune_slot(k);
+  if (!slot) return fail(SQ_ERR_INVALID, "unknown tuning key '" + k + "'");
+  *value = *slot;
   return SQ_OK;
 }
 
@@ -781,5 +795,24 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   *energy = hg[2 * num_params + 2];
   return SQ_OK;
 }
+
+// ---- batched QAOA --------------------------------------------------------------
+int sq_qaoa_create(void **batch, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq,
+                   const uint32_t *nedges, const uint32_t *edges, uint32_t p) {
+  if (!batch || !nq || !nedges) return fail(SQ_ERR_INVALID, "null argument");
+  *batch = nullptr;
+  if (!valid_dtype(dt)) return fail(SQ_ERR_INVALID, "bad dtype");
+  return qaoa_create(batch, dt, ninst, nq, nedges, edges, p);
+}
+int sq_qaoa_run(void *batch, const double *gammas, const double *betas, int use_graph,
+                double *out, void *stream) {
+  if (!batch || !gammas || !betas || !out) return fail(SQ_ERR_INVALID, "null argument");
+  return qaoa_run(batch, gammas, betas, use_graph, out, (hipStream_t)stream);
+}
+int sq_qaoa_stats(void *batch, uint64_t *launches, double *state_bytes, double *traffic_bytes) {
+  if (!batch) return fail(SQ_ERR_INVALID, "null batch");
+  return qaoa_stats(batch, launches, state_bytes, traffic_bytes);
+}
+int sq_qaoa_destroy(void *batch) { return qaoa_destroy(batch); }
 
 }  // extern "C"

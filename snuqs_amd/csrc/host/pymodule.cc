@@ -413,6 +413,57 @@ class Program {
   void *prog_ = nullptr;
 };
 
+// Batched QAOA MaxCut (sq_qaoa_*).
+class QaoaBatch {
+ public:
+  QaoaBatch(const std::vector<uint32_t> &nq, const std::vector<std::vector<std::pair<uint32_t, uint32_t>>> &graphs,
+            uint32_t p, sq_dtype_t dt)
+      : n_(nq.size()), p_(p) {
+    if (graphs.size() != nq.size()) throw py::value_error("one graph per instance");
+    std::vector<uint32_t> ne, ed;
+    for (auto &gph : graphs) {
+      ne.push_back((uint32_t)gph.size());
+      for (auto &e : gph) {
+        ed.push_back(e.first);
+        ed.push_back(e.second);
+      }
+    }
+    check(sq_qaoa_create(&h_, dt, (uint32_t)nq.size(), nq.data(), ne.data(), ed.data(), p));
+  }
+  ~QaoaBatch() {
+    if (h_) sq_qaoa_destroy(h_);
+  }
+  py::array_t<double> run(py::array_t<double, py::array::c_style | py::array::forcecast> gammas,
+                          py::array_t<double, py::array::c_style | py::array::forcecast> betas,
+                          bool use_graph, std::shared_ptr<Stream> stream) {
+    if ((size_t)gammas.size() != n_ * p_ || (size_t)betas.size() != n_ * p_)
+      throw py::value_error("gammas and betas must have num_instances x p entries");
+    py::array_t<double> out({n_, (size_t)(1 + 2 * p_)});
+    const double *g = gammas.data(), *b = betas.data();
+    double *o = out.mutable_data();
+    {
+      py::gil_scoped_release nogil;
+      check(sq_qaoa_run(h_, g, b, use_graph ? 1 : 0, o, sraw(stream)));
+    }
+    return out;
+  }
+  py::dict stats() const {
+    uint64_t l;
+    double sb, tb;
+    check(sq_qaoa_stats(h_, &l, &sb, &tb));
+    py::dict d;
+    d["launches"] = l;
+    d["state_bytes"] = sb;
+    d["traffic_bytes"] = tb;
+    return d;
+  }
+
+ private:
+  size_t n_;
+  uint32_t p_;
+  void *h_ = nullptr;
+};
+
 py::buffer_info gate_buffer(GateOperation &g) {
   return py::buffer_info(g.data(), sizeof(cplx), py::format_descriptor<cplx>::format(), 2,
                          g.shape(), g.stride());
@@ -702,6 +753,17 @@ PYBIND11_MODULE(_C, m) {
            py::arg("work1"), py::arg("checkpoints"), py::arg("param_map"),
            py::arg("num_params"), py::arg("paulis"), py::arg("qubits"), py::arg("coeffs"),
            py::kw_only(), py::arg("stream") = nullptr);
+
+  py::class_<QaoaBatch, std::shared_ptr<QaoaBatch>>(m_fn, "QaoaBatch")
+      .def(py::init([](const std::vector<uint32_t> &nq,
+                       const std::vector<std::vector<std::pair<uint32_t, uint32_t>>> &graphs,
+                       uint32_t p, std::string dtype) {
+             return std::make_shared<QaoaBatch>(nq, graphs, p, parse_dtype(dtype));
+           }), py::arg("num_qubits"), py::arg("graphs"), py::arg("p"), py::kw_only(),
+           py::arg("dtype") = "complex64")
+      .def("run", &QaoaBatch::run, py::arg("gammas"), py::arg("betas"), py::kw_only(),
+           py::arg("use_graph") = true, py::arg("stream") = nullptr)
+      .def("stats", &QaoaBatch::stats);
 
   // core (core/runtime.cc:19-103) and core.cuda (core/cuda/runtime.cu:4-25)
   auto m_core = m.def_submodule("core");

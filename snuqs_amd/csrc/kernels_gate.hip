@@ -664,9 +664,10 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
         if (mfma) {
           const uint64_t ntiles = nfree / tile_groups;
           uint64_t blocks = (ntiles + 3) / 4;
-          const uint64_t cap = 256ull * (uint64_t)tuning().mfma_blocks_per_cu;
+          const int di = c64 ? 0 : 1;
+          const uint64_t cap = 256ull * (uint64_t)tuning().mfma_blocks_per_cu[di];
           if (blocks > cap) blocks = cap;
-          const bool pipe = tuning().mfma5_pipeline != 0;
+          const bool pipe = tuning().mfma5_pipeline[di] != 0;
           using Q = typename Q4<R>::T;
           if constexpr (c64) {
             if (p.kk == 4)

@@ -1,0 +1,599 @@
+// kernels_qaoa.hip -- batched QAOA MaxCut: energies and adjoint gradients of
+// many instances with uneven qubit counts per launch.
+//
+// No reference counterpart (the reference has no batching, expectation or
+// gradient code).  Per instance: |+>^n, then p layers of
+//   cost   U_C(g) = prod_edges ZZ(2g) = exp(-i g S),  S_ii = s(i) = |E| - 2 cut(i)
+//   mixer  U_B(b) = prod_q RX(2b)     = exp(-i b sum_q X_q)
+// energy E = <psi| C |psi> with C_ii = cut(i) (the number of cut edges), and
+//   dE/dg_l = 2 Im <lambda| S |psi>,  dE/db_l = 2 Im <lambda| sum_q X_q |psi>
+// evaluated on the adjoint sweep where lambda = C psi is carried backwards.
+//
+// Work unit: a chunk of 2^L amplitudes (L = min(12, n)); one workgroup per
+// (instance, chunk).  Qubits on bits below L ("low") are applied inside the
+// chunk in LDS; every higher bit h is one pair pass over the state.  Each
+// launch runs the same pass for every instance that has it, so one sequence
+// of launches serves the whole batch (and is captured in one hipGraph).
+// Reductions write one FP64 partial per (pass, chunk) and are summed in a
+// fixed order by the finaliser: results are bitwise reproducible.
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "kernels.hpp"
+#include "kernels_qaoa.hpp"
+
+namespace sq {
+
+constexpr int kQL = 12;            // log2 chunk size
+constexpr int kQChunk = 1 << kQL;  // amplitudes per chunk
+constexpr int kQMaxEdges = 128;
+
+struct QInstDev {
+  uint64_t psi, lam;  // amplitude offsets into the batch state buffer
+  uint64_t slot;      // first partial slot
+  int n, nedges, edge_off, lc;  // lc = min(L, n)
+  int nchunks, pad0, pad1, pad2;
+};
+
+// Launch table entry: instance and its first block index in this launch.
+struct QEntry {
+  int inst;
+  int pad;
+  uint64_t first_block;
+};
+
+__device__ __forceinline__ int find_entry(const QEntry *tab, int count, uint64_t b) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].first_block <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int cut_of(uint64_t i, const uint8_t *e, int ne) {
+  int c = 0;
+  for (int k = 0; k < ne; ++k) c += (int)(((i >> e[2 * k]) ^ (i >> e[2 * k + 1])) & 1);
+  return c;
+}
+
+template <typename R>
+__device__ __forceinline__ void cmul(R &x, R &y, R cr, R ci) {
+  const R t = x * cr - y * ci;
+  y = x * ci + y * cr;
+  x = t;
+}
+
+// Block-wide FP64 sum; thread 0 returns it.
+__device__ double qblock_sum(double v) {
+  __shared__ double s[kBlock];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double r = s[0];
+  __syncthreads();
+  return r;
+}
+
+// Slot layout of instance j (nch = nchunks): energy [nch], then per layer l:
+// gamma [nch], beta [(n - lc + 1) * nch] (low pass first, then high bits).
+__device__ __forceinline__ uint64_t slot_energy(const QInstDev &q) { return q.slot; }
+__device__ __forceinline__ uint64_t slot_gamma(const QInstDev &q, int l) {
+  return q.slot + (uint64_t)q.nchunks * (1 + (uint64_t)l * (2 + q.n - q.lc));
+}
+__device__ __forceinline__ uint64_t slot_beta(const QInstDev &q, int l, int part) {
+  return slot_gamma(q, l) + (uint64_t)q.nchunks * (1 + part);
+}
+
+// ---- low pass, forward: (init |+>) cost(l), RX on every low bit ------------
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_qaoa_low_fwd(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
+                   const uint8_t *__restrict__ edges, const double *__restrict__ gam,
+                   const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
+                   int count, int layer) {
+  using T = typename CV<R>::T;
+  __shared__ R xr[kQChunk], xi[kQChunk];
+  __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
+  const int ei = find_entry(tab, count, blockIdx.x);
+  const QInstDev q = inst[tab[ei].inst];
+  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  const int m = 1 << q.lc;
+  const uint64_t base = chunk << q.lc;
+  const uint8_t *e = edges + 2 * q.edge_off;
+  const double g = gam[tab[ei].inst * p + layer], b = bet[tab[ei].inst * p + layer];
+  // phases exp(-i g s) for s = |E| - 2c, c = 0..|E|, in double
+  for (int c = threadIdx.x; c <= q.nedges; c += kBlock) {
+    double sn, cs;
+    sincos(-g * (double)(q.nedges - 2 * c), &sn, &cs);
+    ph_r[c] = (R)cs;
+    ph_i[c] = (R)sn;
+  }
+  __syncthreads();
+  const R amp0 = (R)(1.0 / sqrt((double)(1ull << q.n)));
+  for (int j = threadIdx.x; j < m; j += kBlock) {
+    R x, y;
+    if (layer == 0) {
+      x = amp0;
+      y = 0;
+    } else {
+      const T v = st[q.psi + base + j];
+      x = v.x;
+      y = v.y;
+    }
+    const int c = cut_of(base + j, e, q.nedges);
+    cmul(x, y, ph_r[c], ph_i[c]);
+    xr[j] = x;
+    xi[j] = y;
+  }
+  __syncthreads();
+  double sn, cs;
+  sincos(b, &sn, &cs);
+  const R c0 = (R)cs, s0 = (R)sn;  // RX(2b) = [[c, -i s], [-i s, c]]
+  for (int bit = 0; bit < q.lc; ++bit) {
+    for (int f = threadIdx.x; f < m / 2; f += kBlock) {
+      const int lo = f & ((1 << bit) - 1);
+      const int i0 = ((f ^ lo) << 1) | lo, i1 = i0 | (1 << bit);
+      const R ar = xr[i0], ai = xi[i0], br = xr[i1], bi = xi[i1];
+      xr[i0] = c0 * ar + s0 * bi;
+      xi[i0] = c0 * ai - s0 * br;
+      xr[i1] = c0 * br + s0 * ai;
+      xi[i1] = c0 * bi - s0 * ar;
+    }
+    __syncthreads();
+  }
+  for (int j = threadIdx.x; j < m; j += kBlock) {
+    T o;
+    o.x = xr[j];
+    o.y = xi[j];
+    st[q.psi + base + j] = o;
+  }
+}
+
+// ---- high pass, forward: RX(2b) on bit h -----------------------------------
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_qaoa_high_fwd(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
+                    const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
+                    int count, int layer, int h) {
+  using T = typename CV<R>::T;
+  const int ei = find_entry(tab, count, blockIdx.x);
+  const QInstDev q = inst[tab[ei].inst];
+  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  double sn, cs;
+  sincos(bet[tab[ei].inst * p + layer], &sn, &cs);
+  const R c0 = (R)cs, s0 = (R)sn;
+  const uint64_t hmask = (1ull << h) - 1;
+  const uint64_t npair = 1ull << (q.lc - 1);
+  for (uint64_t t = threadIdx.x; t < npair; t += kBlock) {
+    const uint64_t f = chunk * npair + t;
+    const uint64_t lo = f & hmask;
+    const uint64_t i0 = ((f ^ lo) << 1) | lo, i1 = i0 | (1ull << h);
+    const T a = st[q.psi + i0], bb = st[q.psi + i1];
+    T o0, o1;
+    o0.x = c0 * a.x + s0 * bb.y;
+    o0.y = c0 * a.y - s0 * bb.x;
+    o1.x = c0 * bb.x + s0 * a.y;
+    o1.y = c0 * bb.y - s0 * a.x;
+    st[q.psi + i0] = o0;
+    st[q.psi + i1] = o1;
+  }
+}
+
+// ---- energy: lambda = C psi, partial sum of cut(i) |psi_i|^2 ----------------
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_qaoa_energy(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
+                  const uint8_t *__restrict__ edges, const QEntry *__restrict__ tab, int count,
+                  double *__restrict__ partial) {
+  using T = typename CV<R>::T;
+  const int ei = find_entry(tab, count, blockIdx.x);
+  const QInstDev q = inst[tab[ei].inst];
+  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  const int m = 1 << q.lc;
+  const uint64_t base = chunk << q.lc;
+  const uint8_t *e = edges + 2 * q.edge_off;
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < m; j += kBlock) {
+    const T v = st[q.psi + base + j];
+    const int c = cut_of(base + j, e, q.nedges);
+    acc += (double)c * ((double)v.x * v.x + (double)v.y * v.y);
+    T o;
+    o.x = (R)c * v.x;
+    o.y = (R)c * v.y;
+    st[q.lam + base + j] = o;
+  }
+  acc = qblock_sum(acc);
+  if (threadIdx.x == 0) partial[slot_energy(q) + chunk] = acc;
+}
+
+// ---- high pass, reverse: beta partial on bit h, undo RX on psi and lambda ---
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_qaoa_high_rev(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
+                    const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
+                    int count, int layer, int h, double *__restrict__ partial) {
+  using T = typename CV<R>::T;
+  const int ei = find_entry(tab, count, blockIdx.x);
+  const QInstDev q = inst[tab[ei].inst];
+  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  double sn, cs;
+  sincos(bet[tab[ei].inst * p + layer], &sn, &cs);
+  const R c0 = (R)cs, s0 = (R)sn;
+  const uint64_t hmask = (1ull << h) - 1;
+  const uint64_t npair = 1ull << (q.lc - 1);
+  double acc = 0.0;
+  for (uint64_t t = threadIdx.x; t < npair; t += kBlock) {
+    const uint64_t f = chunk * npair + t;
+    const uint64_t lo = f & hmask;
+    const uint64_t i0 = ((f ^ lo) << 1) | lo, i1 = i0 | (1ull << h);
+    const T a = st[q.psi + i0], bb = st[q.psi + i1];
+    const T la = st[q.lam + i0], lb = st[q.lam + i1];
+    // Im(conj(la) b + conj(lb) a)
+    acc += (double)la.x * bb.y - (double)la.y * bb.x + (double)lb.x * a.y - (double)lb.y * a.x;
+    // RX(-2b) = [[c, i s], [i s, c]]
+    T o0, o1, m0, m1;
+    o0.x = c0 * a.x - s0 * bb.y;
+    o0.y = c0 * a.y + s0 * bb.x;
+    o1.x = c0 * bb.x - s0 * a.y;
+    o1.y = c0 * bb.y + s0 * a.x;
+    m0.x = c0 * la.x - s0 * lb.y;
+    m0.y = c0 * la.y + s0 * lb.x;
+    m1.x = c0 * lb.x - s0 * la.y;
+    m1.y = c0 * lb.y + s0 * la.x;
+    st[q.psi + i0] = o0;
+    st[q.psi + i1] = o1;
+    st[q.lam + i0] = m0;
+    st[q.lam + i1] = m1;
+  }
+  acc = qblock_sum(acc);
+  if (threadIdx.x == 0) partial[slot_beta(q, layer, 1 + (h - q.lc)) + chunk] = acc;
+}
+
+// ---- low pass, reverse: beta partials and undo RX on low bits, gamma partial
+// and undo cost, on psi and lambda --------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_qaoa_low_rev(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
+                   const uint8_t *__restrict__ edges, const double *__restrict__ gam,
+                   const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
+                   int count, int layer, double *__restrict__ partial) {
+  using T = typename CV<R>::T;
+  __shared__ R xr[kQChunk], xi[kQChunk], yr[kQChunk], yi[kQChunk];
+  __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
+  const int ei = find_entry(tab, count, blockIdx.x);
+  const QInstDev q = inst[tab[ei].inst];
+  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  const int m = 1 << q.lc;
+  const uint64_t base = chunk << q.lc;
+  const uint8_t *e = edges + 2 * q.edge_off;
+  const double g = gam[tab[ei].inst * p + layer], b = bet[tab[ei].inst * p + layer];
+  for (int c = threadIdx.x; c <= q.nedges; c += kBlock) {
+    double sn, cs;
+    sincos(g * (double)(q.nedges - 2 * c), &sn, &cs);  // undo: exp(+i g s)
+    ph_r[c] = (R)cs;
+    ph_i[c] = (R)sn;
+  }
+  for (int j = threadIdx.x; j < m; j += kBlock) {
+    const T v = st[q.psi + base + j], w = st[q.lam + base + j];
+    xr[j] = v.x;
+    xi[j] = v.y;
+    yr[j] = w.x;
+    yi[j] = w.y;
+  }
+  __syncthreads();
+  double sn, cs;
+  sincos(b, &sn, &cs);
+  const R c0 = (R)cs, s0 = (R)sn;
+  double accb = 0.0;
+  for (int bit = 0; bit < q.lc; ++bit) {
+    for (int f = threadIdx.x; f < m / 2; f += kBlock) {
+      const int lo = f & ((1 << bit) - 1);
+      const int i0 = ((f ^ lo) << 1) | lo, i1 = i0 | (1 << bit);
+      const R ar = xr[i0], ai = xi[i0], br = xr[i1], bi = xi[i1];
+      const R lar = yr[i0], lai = yi[i0], lbr = yr[i1], lbi = yi[i1];
+      accb += (double)lar * bi - (double)lai * br + (double)lbr * ai - (double)lbi * ar;
+      xr[i0] = c0 * ar - s0 * bi;
+      xi[i0] = c0 * ai + s0 * br;
+      xr[i1] = c0 * br - s0 * ai;
+      xi[i1] = c0 * bi + s0 * ar;
+      yr[i0] = c0 * lar - s0 * lbi;
+      yi[i0] = c0 * lai + s0 * lbr;
+      yr[i1] = c0 * lbr - s0 * lai;
+      yi[i1] = c0 * lbi + s0 * lar;
+    }
+    __syncthreads();
+  }
+  double accg = 0.0;
+  for (int j = threadIdx.x; j < m; j += kBlock) {
+    const int c = cut_of(base + j, e, q.nedges);
+    const double s = (double)(q.nedges - 2 * c);
+    R x = xr[j], y = xi[j], u = yr[j], w = yi[j];
+    // Im(conj(lambda) s psi)
+    accg += s * ((double)u * y - (double)w * x);
+    cmul(x, y, ph_r[c], ph_i[c]);
+    cmul(u, w, ph_r[c], ph_i[c]);
+    T o, l;
+    o.x = x;
+    o.y = y;
+    l.x = u;
+    l.y = w;
+    st[q.psi + base + j] = o;
+    st[q.lam + base + j] = l;
+  }
+  accb = qblock_sum(accb);
+  if (threadIdx.x == 0) partial[slot_beta(q, layer, 0) + chunk] = accb;
+  accg = qblock_sum(accg);
+  if (threadIdx.x == 0) partial[slot_gamma(q, layer) + chunk] = accg;
+}
+
+// ---- finaliser: out[j] = {E, dE/dg_0..p-1, dE/db_0..p-1} -------------------
+__global__ void __launch_bounds__(kBlock)
+    k_qaoa_finalize(const QInstDev *__restrict__ inst, int ninst, int p,
+                    const double *__restrict__ partial, double *__restrict__ out) {
+  const int j = blockIdx.x;
+  if (j >= ninst) return;
+  const QInstDev q = inst[j];
+  const uint64_t nch = q.nchunks;
+  for (int v = 0; v < 1 + 2 * p; ++v) {
+    uint64_t first, len;
+    double scale;
+    if (v == 0) {
+      first = slot_energy(q);
+      len = nch;
+      scale = 1.0;
+    } else if (v <= p) {
+      first = slot_gamma(q, v - 1);
+      len = nch;
+      scale = 2.0;
+    } else {
+      first = slot_beta(q, v - 1 - p, 0);
+      len = nch * (uint64_t)(q.n - q.lc + 1);
+      scale = 2.0;
+    }
+    double acc = 0.0;
+    for (uint64_t k = threadIdx.x; k < len; k += kBlock) acc += partial[first + k];
+    acc = qblock_sum(acc);
+    if (threadIdx.x == 0) out[(uint64_t)j * (1 + 2 * p) + v] = scale * acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct QaoaBatch {
+  sq_dtype_t dt;
+  int ninst, p, maxn;
+  std::vector<QInstDev> host_inst;
+  void *states = nullptr;
+  QInstDev *inst = nullptr;
+  uint8_t *edges = nullptr;
+  double *params = nullptr;  // gam [ninst*p], bet [ninst*p]
+  double *partial = nullptr;
+  double *out = nullptr;
+  QEntry *tables = nullptr;
+  struct Launch {
+    int kind, layer, bit;  // kind: 0 low_fwd, 1 high_fwd, 2 energy, 3 high_rev, 4 low_rev, 5 fin
+    size_t tab_off;
+    int count;
+    uint64_t blocks;
+  };
+  std::vector<Launch> launches;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  double state_bytes = 0, traffic_bytes = 0;
+};
+
+static void free_batch(QaoaBatch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (void *ptr : {(void *)b->states, (void *)b->inst, (void *)b->edges, (void *)b->params,
+                    (void *)b->partial, (void *)b->out, (void *)b->tables})
+    if (ptr) (void)hipFree(ptr);
+}
+
+int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq,
+                const uint32_t *nedges, const uint32_t *edges, uint32_t p) {
+  if (ninst == 0 || p == 0) return fail(SQ_ERR_INVALID, "qaoa: empty batch");
+  auto B = std::make_unique<QaoaBatch>();
+  B->dt = dt;
+  B->ninst = (int)ninst;
+  B->p = (int)p;
+  B->maxn = 0;
+  const size_t item = itemsize(dt);
+  uint64_t amp = 0, slot = 0;
+  size_t eoff = 0;
+  std::vector<uint8_t> hedges;
+  for (uint32_t j = 0; j < ninst; ++j) {
+    const int n = (int)nq[j];
+    if (n < 2 || n > 30) return fail(SQ_ERR_INVALID, "qaoa: qubit counts must be in [2, 30]");
+    if (nedges[j] > (uint32_t)kQMaxEdges) return fail(SQ_ERR_INVALID, "qaoa: too many edges");
+    QInstDev q{};
+    q.n = n;
+    q.lc = std::min(n, kQL);
+    q.nchunks = 1 << (n - q.lc);
+    q.nedges = (int)nedges[j];
+    q.edge_off = (int)(hedges.size() / 2);
+    for (uint32_t k = 0; k < nedges[j]; ++k) {
+      const uint32_t a = edges[2 * (eoff + k)], b = edges[2 * (eoff + k) + 1];
+      if (a >= (uint32_t)n || b >= (uint32_t)n || a == b)
+        return fail(SQ_ERR_INVALID, "qaoa: bad edge");
+      hedges.push_back((uint8_t)(n - 1 - a));  // qubit -> bit position
+      hedges.push_back((uint8_t)(n - 1 - b));
+    }
+    eoff += nedges[j];
+    q.psi = amp;
+    q.lam = amp + (1ull << n);
+    amp += 2ull << n;
+    q.slot = slot;
+    slot += (uint64_t)q.nchunks * (1 + (uint64_t)p * (2 + n - q.lc));
+    B->host_inst.push_back(q);
+    B->maxn = std::max(B->maxn, n);
+    B->state_bytes += 2.0 * std::ldexp(1.0, n) * item;
+  }
+  if (hedges.empty()) hedges.push_back(0);
+  auto dmalloc = [](void **ptr, size_t bytes) -> int {
+    if (hipMalloc(ptr, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      *ptr = nullptr;
+      return fail(SQ_ERR_OOM, "qaoa: device allocation of " + std::to_string(bytes) + " failed");
+    }
+    return SQ_OK;
+  };
+  int rc = SQ_OK;
+  if (!rc) rc = dmalloc(&B->states, amp * item);
+  if (!rc) rc = dmalloc((void **)&B->inst, ninst * sizeof(QInstDev));
+  if (!rc) rc = dmalloc((void **)&B->edges, hedges.size());
+  if (!rc) rc = dmalloc((void **)&B->params, 2 * ninst * p * sizeof(double));
+  if (!rc) rc = dmalloc((void **)&B->partial, std::max<uint64_t>(slot, 1) * sizeof(double));
+  if (!rc) rc = dmalloc((void **)&B->out, ninst * (1 + 2 * p) * sizeof(double));
+  // launch sequence and tables
+  std::vector<QEntry> tabs;
+  auto add = [&](int kind, int layer, int bit, auto pred) {
+    QaoaBatch::Launch L{kind, layer, bit, tabs.size(), 0, 0};
+    for (int j = 0; j < (int)ninst; ++j) {
+      const QInstDev &q = B->host_inst[j];
+      if (!pred(q)) continue;
+      tabs.push_back(QEntry{j, 0, L.blocks});
+      L.blocks += (uint64_t)q.nchunks;
+      L.count++;
+    }
+    if (L.count) B->launches.push_back(L);
+  };
+  auto all = [](const QInstDev &) { return true; };
+  for (int l = 0; l < (int)p; ++l) {
+    add(0, l, 0, all);
+    for (int h = kQL; h < B->maxn; ++h)
+      add(1, l, h, [h](const QInstDev &q) { return q.n > h; });
+  }
+  add(2, 0, 0, all);
+  for (int l = (int)p - 1; l >= 0; --l) {
+    for (int h = B->maxn - 1; h >= kQL; --h)
+      add(3, l, h, [h](const QInstDev &q) { return q.n > h; });
+    add(4, l, 0, all);
+  }
+  if (!rc) rc = dmalloc((void **)&B->tables, tabs.size() * sizeof(QEntry));
+  if (rc) {
+    free_batch(B.get());
+    return rc;
+  }
+  if (hipMemcpy(B->inst, B->host_inst.data(), ninst * sizeof(QInstDev), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(B->edges, hedges.data(), hedges.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(B->tables, tabs.data(), tabs.size() * sizeof(QEntry), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    free_batch(B.get());
+    return fail(SQ_ERR_HIP, "qaoa: upload failed");
+  }
+  // HBM traffic of one run (reads + writes of the states)
+  for (auto &L : B->launches) {
+    const double per = (L.kind == 0 || L.kind == 1) ? 2.0 : (L.kind == 2 ? 3.0 : 4.0);
+    for (int k = 0; k < L.count; ++k) {
+      const QInstDev &q = B->host_inst[tabs[L.tab_off + k].inst];
+      const double amps = std::ldexp(1.0, q.n);
+      B->traffic_bytes += per * amps * item * ((L.kind == 0 && L.layer == 0) ? 0.5 : 1.0);
+    }
+  }
+  *handle = B.release();
+  return SQ_OK;
+}
+
+template <typename R>
+static int enqueue_t(QaoaBatch *B, hipStream_t st) {
+  using T = typename CV<R>::T;
+  T *s = (T *)B->states;
+  const double *gam = B->params, *bet = B->params + (size_t)B->ninst * B->p;
+  for (auto &L : B->launches) {
+    const QEntry *tab = B->tables + L.tab_off;
+    const dim3 g((unsigned)L.blocks), bl(kBlock);
+    switch (L.kind) {
+      case 0:
+        hipLaunchKernelGGL(k_qaoa_low_fwd<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet, B->p,
+                           tab, L.count, L.layer);
+        break;
+      case 1:
+        hipLaunchKernelGGL(k_qaoa_high_fwd<R>, g, bl, 0, st, s, B->inst, bet, B->p, tab, L.count,
+                           L.layer, L.bit);
+        break;
+      case 2:
+        hipLaunchKernelGGL(k_qaoa_energy<R>, g, bl, 0, st, s, B->inst, B->edges, tab, L.count,
+                           B->partial);
+        break;
+      case 3:
+        hipLaunchKernelGGL(k_qaoa_high_rev<R>, g, bl, 0, st, s, B->inst, bet, B->p, tab, L.count,
+                           L.layer, L.bit, B->partial);
+        break;
+      case 4:
+        hipLaunchKernelGGL(k_qaoa_low_rev<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet, B->p,
+                           tab, L.count, L.layer, B->partial);
+        break;
+    }
+  }
+  hipLaunchKernelGGL(k_qaoa_finalize, dim3(B->ninst), dim3(kBlock), 0, st, B->inst, B->ninst,
+                     B->p, B->partial, B->out);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+static int enqueue(QaoaBatch *B, hipStream_t st) {
+  return B->dt == SQ_COMPLEX64 ? enqueue_t<float>(B, st) : enqueue_t<double>(B, st);
+}
+
+int qaoa_run(void *handle, const double *gammas, const double *betas, int use_graph,
+             double *out_host, hipStream_t st) {
+  QaoaBatch *B = (QaoaBatch *)handle;
+  const size_t np = (size_t)B->ninst * B->p;
+  SQ_HIP(hipMemcpyAsync(B->params, gammas, np * sizeof(double), hipMemcpyHostToDevice, st));
+  SQ_HIP(hipMemcpyAsync(B->params + np, betas, np * sizeof(double), hipMemcpyHostToDevice, st));
+  if (use_graph) {
+    if (!B->exec) {
+      // capture the launch sequence once; parameters live in device memory
+      hipStream_t cs;
+      SQ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+      int rc = SQ_OK;
+      if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        rc = fail(SQ_ERR_HIP, "qaoa: hipStreamBeginCapture failed");
+      if (!rc) rc = enqueue(B, cs);
+      hipGraph_t g = nullptr;
+      if (hipStreamEndCapture(cs, &g) != hipSuccess && !rc)
+        rc = fail(SQ_ERR_HIP, "qaoa: hipStreamEndCapture failed");
+      (void)hipStreamDestroy(cs);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      B->graph = g;
+      SQ_HIP(hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0));
+    }
+    SQ_HIP(hipGraphLaunch(B->exec, st));
+  } else {
+    SQ_TRY(enqueue(B, st));
+  }
+  SQ_HIP(hipMemcpyAsync(out_host, B->out, (size_t)B->ninst * (1 + 2 * B->p) * sizeof(double),
+                        hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  return SQ_OK;
+}
+
+int qaoa_destroy(void *handle) {
+  QaoaBatch *B = (QaoaBatch *)handle;
+  if (!B) return SQ_OK;
+  free_batch(B);
+  delete B;
+  return SQ_OK;
+}
+
+int qaoa_stats(void *handle, uint64_t *launches, double *state_bytes, double *traffic_bytes) {
+  QaoaBatch *B = (QaoaBatch *)handle;
+  if (launches) *launches = B->launches.size() + 1;
+  if (state_bytes) *state_bytes = B->state_bytes;
+  if (traffic_bytes) *traffic_bytes = B->traffic_bytes;
+  return SQ_OK;
+}
+
+}  // namespace sq
