@@ -146,6 +146,38 @@ def run_adjoint_bench(C, n=24, layers=8, nterms=64, weight=4, shift_params=12):
     }
 
 
+def run_qaoa_bench(C, reps=2):
+    """Config 4: 256 QAOA MaxCut instances, qubits uniform over the even values
+    12..26 (3-regular graphs need an even count), p = 4, complex64 with FP64
+    reductions; energies + adjoint gradients of the whole batch per run."""
+    from snuqs_amd.qaoa import make_batch, qaoa_batch
+
+    nq, graphs, gam, bet = make_batch(256, p=4, seed=7)
+    batch = qaoa_batch(nq, graphs, 4, dtype="complex64")
+    t0 = time.perf_counter()
+    out = batch.run(gam, bet, use_graph=True)  # captures the graph
+    first = time.perf_counter() - t0
+    C.core.cuda.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = batch.run(gam, bet, use_graph=True)
+    dt = (time.perf_counter() - t0) / reps
+    st = batch.stats()
+    log(f"qaoa batch: {dt * 1e3:.0f} ms per run ({first * 1e3:.0f} ms first run incl. capture)")
+    res = {
+        "config": "256 QAOA MaxCut instances, qubits uniform over {12,14,...,26}, random "
+                  "3-regular graphs, p=4, complex64, energies + 8-parameter adjoint gradients",
+        "ms_per_batch": dt * 1e3,
+        "instances_per_s": 256 / dt,
+        "launches_per_batch": st["launches"],
+        "hbm_GBps": st["traffic_bytes"] / dt / 1e9,
+        "state_GiB": st["state_bytes"] / 2**30,
+        "energy_mean": float(out[:, 0].mean()),
+    }
+    del batch
+    return res
+
+
 def cpu_baseline(qubits, depth, seconds):
     cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--qubits", str(qubits), "--depth",
            str(depth), "--seconds", str(seconds)]
@@ -168,6 +200,7 @@ def main():
     ap.add_argument("--skip-unfused", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-qaoa", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -239,6 +272,7 @@ def main():
     del sv, prog
 
     adj = None if args.skip_adjoint else run_adjoint_bench(C)
+    qaoa = None if args.skip_qaoa else run_qaoa_bench(C)
     cpu = None if args.skip_cpu else cpu_baseline(n, args.depth, args.cpu_seconds)
 
     out = {
@@ -278,6 +312,7 @@ def main():
             "GBps": t["bytes"] / (t["ms"] * 1e-3) / 1e9} for (c, k), t in tab.items()},
         "unfused": unfused,
         "adjoint": adj,
+        "qaoa_batch": qaoa,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -168,7 +168,7 @@ int make_plan(const double *matrix, const uint32_t *targets, int k,
       if (!is_one(p.m[r])) ++changed_n;
     }
     p.touched = sub * changed_n / dd;
-  } else if (mono) {
+  } else if (mono && kk <= 3) {  // larger permutations run as dense blocks
     p.kind = Kind::PERM;
     p.m.resize(dd);
     int changed_n = 0;

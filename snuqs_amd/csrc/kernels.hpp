@@ -22,6 +22,16 @@ struct CV<double> {
 
 constexpr int kBlock = 256;  // threads per workgroup for the sweep kernels
 
+// A dispatch holds fewer than 2^32 work-items; sweep kernels are grid-stride
+// loops over at most kMaxBlocks workgroups.
+constexpr uint64_t kMaxBlocks = 1ull << 20;
+inline dim3 sweep_grid(uint64_t work, uint64_t per_block) {
+  uint64_t b = (work + per_block - 1) / per_block;
+  if (b > kMaxBlocks) b = kMaxBlocks;
+  if (b == 0) b = 1;
+  return dim3((unsigned)b);
+}
+
 // Insert zero bits at ascending positions (lowmask[j] = 2^p_j - 1) into a free
 // index, then set the fixed (control) bits.
 struct IdxMap {
@@ -40,7 +50,7 @@ __device__ __forceinline__ uint64_t deposit(uint64_t f, const IdxMap &m) {
 
 // Launch-shape parameters chosen by the autotuner (sq_tune_set).
 struct Tuning {
-  int mfma_blocks_per_cu[2] = {16, 16};  // persistent grid of the MFMA kernels [c64, c128]
+  int mfma_blocks_per_cu[2] = {32, 32};  // persistent grid of the MFMA kernels [c64, c128]
   int mfma5_pipeline[2] = {1, 1};        // 5-target MFMA: prefetch the next tile
 };
 Tuning &tuning();

@@ -28,26 +28,29 @@ namespace sq {
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_init(T *__restrict__ sv, uint64_t n, int basis) {
-  uint64_t i = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x; i0 < n;
+       i0 += (uint64_t)gridDim.x * (kBlock * 4)) {
+    uint64_t i = i0;
 #pragma unroll
-  for (int u = 0; u < 4; ++u, i += kBlock) {
-    if (i < n) {
-      T z;
-      z.x = (basis && i == 0) ? 1 : 0;
-      z.y = 0;
-      sv[i] = z;
+    for (int u = 0; u < 4; ++u, i += kBlock) {
+      if (i < n) {
+        T z;
+        z.x = (basis && i == 0) ? 1 : 0;
+        z.y = 0;
+        sv[i] = z;
+      }
     }
   }
 }
 
 int launch_init(void *sv, sq_dtype_t dt, uint64_t n, bool basis,
                 hipStream_t stream) {
-  uint64_t blocks = (n + kBlock * 4 - 1) / (kBlock * 4);
+  const dim3 blocks = sweep_grid(n, kBlock * 4);
   if (dt == SQ_COMPLEX64)
-    hipLaunchKernelGGL(k_init<float2>, dim3(blocks), dim3(kBlock), 0, stream,
+    hipLaunchKernelGGL(k_init<float2>, blocks, dim3(kBlock), 0, stream,
                        (float2 *)sv, n, (int)basis);
   else
-    hipLaunchKernelGGL(k_init<double2>, dim3(blocks), dim3(kBlock), 0, stream,
+    hipLaunchKernelGGL(k_init<double2>, blocks, dim3(kBlock), 0, stream,
                        (double2 *)sv, n, (int)basis);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
@@ -61,21 +64,23 @@ Tuning &tuning() {
 // Plain copy, 16 bytes per lane per access, 4 accesses in flight per lane.
 __global__ void __launch_bounds__(kBlock)
     k_copy(uint4 *__restrict__ dst, const uint4 *__restrict__ src, uint64_t n16) {
-  uint64_t i = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x;
-  uint4 v[4];
+  for (uint64_t i = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x; i < n16;
+       i += (uint64_t)gridDim.x * (kBlock * 4)) {
+    uint4 v[4];
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
-    if (i + (uint64_t)u * kBlock < n16) v[u] = src[i + (uint64_t)u * kBlock];
+    for (int u = 0; u < 4; ++u)
+      if (i + (uint64_t)u * kBlock < n16) v[u] = src[i + (uint64_t)u * kBlock];
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
-    if (i + (uint64_t)u * kBlock < n16) dst[i + (uint64_t)u * kBlock] = v[u];
+    for (int u = 0; u < 4; ++u)
+      if (i + (uint64_t)u * kBlock < n16) dst[i + (uint64_t)u * kBlock] = v[u];
+  }
 }
 
 int launch_copy(void *dst, const void *src, size_t bytes, hipStream_t stream) {
   const uint64_t n16 = bytes / 16;
   if (n16 == 0) return SQ_OK;
-  hipLaunchKernelGGL(k_copy, dim3((unsigned)((n16 + kBlock * 4 - 1) / (kBlock * 4))),
-                     dim3(kBlock), 0, stream, (uint4 *)dst, (const uint4 *)src, n16);
+  hipLaunchKernelGGL(k_copy, sweep_grid(n16, kBlock * 4), dim3(kBlock), 0, stream,
+                     (uint4 *)dst, (const uint4 *)src, n16);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
 }
@@ -88,22 +93,24 @@ __global__ void __launch_bounds__(kBlock)
     k_phase(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
             R pr, R pi) {
   using T = typename CV<R>::T;
-  const uint64_t f0 = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x;
-  T v[U];
-  uint64_t idx[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    uint64_t f = f0 + (uint64_t)u * kBlock;
-    idx[u] = deposit(f, im);
-    if (f < nfree) v[u] = sv[idx[u]];
-  }
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    if (f0 + (uint64_t)u * kBlock < nfree) {
-      T o;
-      o.x = pr * v[u].x - pi * v[u].y;
-      o.y = pr * v[u].y + pi * v[u].x;
-      sv[idx[u]] = o;
+  for (uint64_t f0 = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x; f0 < nfree;
+       f0 += (uint64_t)gridDim.x * (kBlock * U)) {
+    T v[U];
+    uint64_t idx[U];
+  #pragma unroll
+    for (int u = 0; u < U; ++u) {
+      uint64_t f = f0 + (uint64_t)u * kBlock;
+      idx[u] = deposit(f, im);
+      if (f < nfree) v[u] = sv[idx[u]];
+    }
+  #pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (f0 + (uint64_t)u * kBlock < nfree) {
+        T o;
+        o.x = pr * v[u].x - pi * v[u].y;
+        o.y = pr * v[u].y + pi * v[u].x;
+        sv[idx[u]] = o;
+      }
     }
   }
 }
@@ -129,25 +136,27 @@ __global__ void __launch_bounds__(kBlock)
     sdi[threadIdx.x] = a.di[threadIdx.x];
   }
   __syncthreads();
-  const uint64_t f0 = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x;
-  T v[U];
-  uint64_t idx[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    uint64_t f = f0 + (uint64_t)u * kBlock;
-    idx[u] = deposit(f, im);
-    if (f < nfree) v[u] = sv[idx[u]];
-  }
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    if (f0 + (uint64_t)u * kBlock < nfree) {
-      int r = 0;
-      for (int j = 0; j < a.kk; ++j) r |= (int)((idx[u] >> a.tbit[j]) & 1) << j;
-      const R pr = sdr[r], pi = sdi[r];
-      T o;
-      o.x = pr * v[u].x - pi * v[u].y;
-      o.y = pr * v[u].y + pi * v[u].x;
-      sv[idx[u]] = o;
+  for (uint64_t f0 = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x; f0 < nfree;
+       f0 += (uint64_t)gridDim.x * (kBlock * U)) {
+    T v[U];
+    uint64_t idx[U];
+  #pragma unroll
+    for (int u = 0; u < U; ++u) {
+      uint64_t f = f0 + (uint64_t)u * kBlock;
+      idx[u] = deposit(f, im);
+      if (f < nfree) v[u] = sv[idx[u]];
+    }
+  #pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (f0 + (uint64_t)u * kBlock < nfree) {
+        int r = 0;
+        for (int j = 0; j < a.kk; ++j) r |= (int)((idx[u] >> a.tbit[j]) & 1) << j;
+        const R pr = sdr[r], pi = sdi[r];
+        T o;
+        o.x = pr * v[u].x - pi * v[u].y;
+        o.y = pr * v[u].y + pi * v[u].x;
+        sv[idx[u]] = o;
+      }
     }
   }
 }
@@ -168,29 +177,31 @@ __global__ void __launch_bounds__(kBlock)
     k_perm(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
            PermArgs<R> a) {
   using T = typename CV<R>::T;
-  const uint64_t f0 = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x;
-  T v[U][8];
-  uint64_t base[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    uint64_t f = f0 + (uint64_t)u * kBlock;
-    base[u] = deposit(f, im);
-    if (f < nfree) {
-#pragma unroll
-      for (int r = 0; r < 8; ++r)
-        if (r < a.count) v[u][r] = sv[base[u] + a.src[r]];
+  for (uint64_t f0 = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x; f0 < nfree;
+       f0 += (uint64_t)gridDim.x * (kBlock * U)) {
+    T v[U][8];
+    uint64_t base[U];
+  #pragma unroll
+    for (int u = 0; u < U; ++u) {
+      uint64_t f = f0 + (uint64_t)u * kBlock;
+      base[u] = deposit(f, im);
+      if (f < nfree) {
+  #pragma unroll
+        for (int r = 0; r < 8; ++r)
+          if (r < a.count) v[u][r] = sv[base[u] + a.src[r]];
+      }
     }
-  }
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    if (f0 + (uint64_t)u * kBlock < nfree) {
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        if (r < a.count) {
-          T o;
-          o.x = a.pr[r] * v[u][r].x - a.pi[r] * v[u][r].y;
-          o.y = a.pr[r] * v[u][r].y + a.pi[r] * v[u][r].x;
-          sv[base[u] + a.dst[r]] = o;
+  #pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (f0 + (uint64_t)u * kBlock < nfree) {
+  #pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          if (r < a.count) {
+            T o;
+            o.x = a.pr[r] * v[u][r].x - a.pi[r] * v[u][r].y;
+            o.y = a.pr[r] * v[u][r].y + a.pi[r] * v[u][r].x;
+            sv[base[u] + a.dst[r]] = o;
+          }
         }
       }
     }
@@ -212,34 +223,36 @@ __global__ void __launch_bounds__(kBlock)
             DenseArgs<R, K> a) {
   using T = typename CV<R>::T;
   constexpr int D = 1 << K;
-  const uint64_t f0 = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x;
-  T x[U][D];
-  uint64_t base[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    uint64_t f = f0 + (uint64_t)u * kBlock;
-    base[u] = deposit(f, im);
-    if (f < nfree) {
-#pragma unroll
-      for (int c = 0; c < D; ++c) x[u][c] = sv[base[u] + a.off[c]];
+  for (uint64_t f0 = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x; f0 < nfree;
+       f0 += (uint64_t)gridDim.x * (kBlock * U)) {
+    T x[U][D];
+    uint64_t base[U];
+  #pragma unroll
+    for (int u = 0; u < U; ++u) {
+      uint64_t f = f0 + (uint64_t)u * kBlock;
+      base[u] = deposit(f, im);
+      if (f < nfree) {
+  #pragma unroll
+        for (int c = 0; c < D; ++c) x[u][c] = sv[base[u] + a.off[c]];
+      }
     }
-  }
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    if (f0 + (uint64_t)u * kBlock < nfree) {
-#pragma unroll
-      for (int r = 0; r < D; ++r) {
-        R yr = a.mr[r * D] * x[u][0].x - a.mi[r * D] * x[u][0].y;
-        R yi = a.mr[r * D] * x[u][0].y + a.mi[r * D] * x[u][0].x;
-#pragma unroll
-        for (int c = 1; c < D; ++c) {
-          yr += a.mr[r * D + c] * x[u][c].x - a.mi[r * D + c] * x[u][c].y;
-          yi += a.mr[r * D + c] * x[u][c].y + a.mi[r * D + c] * x[u][c].x;
+  #pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (f0 + (uint64_t)u * kBlock < nfree) {
+  #pragma unroll
+        for (int r = 0; r < D; ++r) {
+          R yr = a.mr[r * D] * x[u][0].x - a.mi[r * D] * x[u][0].y;
+          R yi = a.mr[r * D] * x[u][0].y + a.mi[r * D] * x[u][0].x;
+  #pragma unroll
+          for (int c = 1; c < D; ++c) {
+            yr += a.mr[r * D + c] * x[u][c].x - a.mi[r * D + c] * x[u][c].y;
+            yi += a.mr[r * D + c] * x[u][c].y + a.mi[r * D + c] * x[u][c].x;
+          }
+          T o;
+          o.x = yr;
+          o.y = yi;
+          sv[base[u] + a.off[r]] = o;
         }
-        T o;
-        o.x = yr;
-        o.y = yi;
-        sv[base[u] + a.off[r]] = o;
       }
     }
   }
@@ -279,25 +292,26 @@ __global__ void __launch_bounds__(kBlock)
     gi[e] = G[e].y;
   }
   __syncthreads();
-  const uint64_t f = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (f >= nfree) return;
-  const uint64_t base = deposit(f, im);
-  T x[D];
+  for (uint64_t f = (uint64_t)blockIdx.x * kBlock + threadIdx.x; f < nfree;
+       f += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t base = deposit(f, im);
+    T x[D];
 #pragma unroll
-  for (int c = 0; c < D; ++c) x[c] = sv[base + off.o[c]];
+    for (int c = 0; c < D; ++c) x[c] = sv[base + off.o[c]];
 #pragma unroll
-  for (int r = 0; r < D; ++r) {
-    R yr = 0, yi = 0;
+    for (int r = 0; r < D; ++r) {
+      R yr = 0, yi = 0;
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-      const R mr = gr[c * D + r], mi = gi[c * D + r];
-      yr += mr * x[c].x - mi * x[c].y;
-      yi += mr * x[c].y + mi * x[c].x;
+      for (int c = 0; c < D; ++c) {
+        const R mr = gr[c * D + r], mi = gi[c * D + r];
+        yr += mr * x[c].x - mi * x[c].y;
+        yi += mr * x[c].y + mi * x[c].x;
+      }
+      T o;
+      o.x = yr;
+      o.y = yi;
+      sv[base + off.o[r]] = o;
     }
-    T o;
-    o.x = yr;
-    o.y = yi;
-    sv[base + off.o[r]] = o;
   }
 }
 
@@ -575,7 +589,7 @@ static uint64_t target_mask(const GatePlan &p) {
 }
 
 static inline dim3 grid_for(uint64_t work, int per_block) {
-  return dim3((unsigned)((work + per_block - 1) / per_block));
+  return sweep_grid(work, (uint64_t)per_block);
 }
 
 template <typename R>

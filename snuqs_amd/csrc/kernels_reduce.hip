@@ -41,12 +41,15 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_probabilities(const T *__restrict__ sv, double *__restrict__ p, uint64_t n) {
 #pragma clang fp contract(off)
-  uint64_t i = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x; i0 < n;
+       i0 += (uint64_t)gridDim.x * (kBlock * 4)) {
+    uint64_t i = i0;
 #pragma unroll
-  for (int u = 0; u < 4; ++u, i += kBlock) {
-    if (i < n) {
-      const double2 v = ld2(sv, i);
-      p[i] = v.x * v.x + v.y * v.y;
+    for (int u = 0; u < 4; ++u, i += kBlock) {
+      if (i < n) {
+        const double2 v = ld2(sv, i);
+        p[i] = v.x * v.x + v.y * v.y;
+      }
     }
   }
 }
@@ -138,25 +141,26 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_apply_pauli_sum(const T *__restrict__ in, T *__restrict__ out, uint64_t n,
                       const PauliApplyTerm *__restrict__ terms, int nterms) {
-  const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (j >= n) return;
-  double yr = 0.0, yi = 0.0;
-  for (int t = 0; t < nterms; ++t) {
-    const PauliApplyTerm pt = terms[t];
-    const uint64_t i = j ^ pt.x;
-    const double2 v = ld2(in, i);
-    double cr = pt.cr, ci = pt.ci;
-    if (__popcll(i & pt.z) & 1) {
-      cr = -cr;
-      ci = -ci;
+  for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < n;
+       j += (uint64_t)gridDim.x * kBlock) {
+    double yr = 0.0, yi = 0.0;
+    for (int t = 0; t < nterms; ++t) {
+      const PauliApplyTerm pt = terms[t];
+      const uint64_t i = j ^ pt.x;
+      const double2 v = ld2(in, i);
+      double cr = pt.cr, ci = pt.ci;
+      if (__popcll(i & pt.z) & 1) {
+        cr = -cr;
+        ci = -ci;
+      }
+      yr += cr * v.x - ci * v.y;
+      yi += cr * v.y + ci * v.x;
     }
-    yr += cr * v.x - ci * v.y;
-    yi += cr * v.y + ci * v.x;
+    T o;
+    o.x = yr;
+    o.y = yi;
+    out[j] = o;
   }
-  T o;
-  o.x = yr;
-  o.y = yi;
-  out[j] = o;
 }
 
 // ---- finaliser: out[v] = beta*out[v] + alpha[v] * sum_b partial[v][b] -------
@@ -177,15 +181,18 @@ __global__ void __launch_bounds__(kBlock)
 template <typename T, typename R>
 __global__ void __launch_bounds__(kBlock)
     k_scale(T *__restrict__ sv, uint64_t n, R sr, R si) {
-  uint64_t i = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x; i0 < n;
+       i0 += (uint64_t)gridDim.x * (kBlock * 4)) {
+    uint64_t i = i0;
 #pragma unroll
-  for (int u = 0; u < 4; ++u, i += kBlock) {
-    if (i < n) {
-      const T v = sv[i];
-      T o;
-      o.x = sr * v.x - si * v.y;
-      o.y = sr * v.y + si * v.x;
-      sv[i] = o;
+    for (int u = 0; u < 4; ++u, i += kBlock) {
+      if (i < n) {
+        const T v = sv[i];
+        T o;
+        o.x = sr * v.x - si * v.y;
+        o.y = sr * v.y + si * v.x;
+        sv[i] = o;
+      }
     }
   }
 }
@@ -194,16 +201,15 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_gather(const T *__restrict__ sv, const uint64_t *__restrict__ idx,
              uint64_t count, double2 *__restrict__ out) {
-  const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (j < count) out[j] = ld2(sv, idx[j]);
+  for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < count;
+       j += (uint64_t)gridDim.x * kBlock)
+    out[j] = ld2(sv, idx[j]);
 }
 
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-static dim3 grid1(uint64_t work, int per_block) {
-  return dim3((unsigned)((work + per_block - 1) / per_block));
-}
+static dim3 grid1(uint64_t work, int per_block) { return sweep_grid(work, per_block); }
 
 int launch_probabilities(const void *sv, sq_dtype_t dt, uint64_t n, double *p,
                          hipStream_t st) {

@@ -147,23 +147,24 @@ static int scan_inplace(T *data, uint64_t n, T *scratch, hipStream_t st) {
 __global__ void __launch_bounds__(kBlock)
     k_increments(const double *__restrict__ p, uint64_t a, uint64_t len,
                  int shift, int64_t *__restrict__ d, int32_t *__restrict__ tie) {
-  const uint64_t m = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (m >= len) return;
-  const double q = ldexp(p[a + m], shift);  // exact power-of-two scaling (inf
-                                            // on overflow, flagged below)
-  int64_t di;
-  int32_t t = 0;
-  if (q >= 9007199254740992.0) {  // 2^53: leaves the binade on its own
-    di = (int64_t)1 << 53;
-  } else {
-    const double k = floor(q);
-    const double fr = q - k;  // exact
-    di = (int64_t)k;
-    if (fr > 0.5) di += 1;
-    else if (fr == 0.5) t = 1;
+  for (uint64_t m = (uint64_t)blockIdx.x * kBlock + threadIdx.x; m < len;
+       m += (uint64_t)gridDim.x * kBlock) {
+    const double q = ldexp(p[a + m], shift);  // exact power-of-two scaling
+                                              // (inf on overflow, flagged below)
+    int64_t di;
+    int32_t t = 0;
+    if (q >= 9007199254740992.0) {  // 2^53: leaves the binade on its own
+      di = (int64_t)1 << 53;
+    } else {
+      const double k = floor(q);
+      const double fr = q - k;  // exact
+      di = (int64_t)k;
+      if (fr > 0.5) di += 1;
+      else if (fr == 0.5) t = 1;
+    }
+    d[m] = di;
+    tie[m] = t;
   }
-  d[m] = di;
-  tie[m] = t;
 }
 
 // Resolve ties in order (one thread).  tie holds the inclusive scan of the
@@ -171,10 +172,11 @@ __global__ void __launch_bounds__(kBlock)
 // adj[r] = number of ties among 0..r that rounded up.
 __global__ void k_compact_ties(const int32_t *__restrict__ tie_scan, uint64_t len,
                                uint64_t *tiepos) {
-  const uint64_t m = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (m >= len) return;
-  const int32_t prev = m ? tie_scan[m - 1] : 0;
-  if (tie_scan[m] != prev) tiepos[prev] = m;
+  for (uint64_t m = (uint64_t)blockIdx.x * kBlock + threadIdx.x; m < len;
+       m += (uint64_t)gridDim.x * kBlock) {
+    const int32_t prev = m ? tie_scan[m - 1] : 0;
+    if (tie_scan[m] != prev) tiepos[prev] = m;
+  }
 }
 
 __global__ void k_resolve_ties(const int64_t *__restrict__ dscan,
@@ -197,16 +199,17 @@ __global__ void __launch_bounds__(kBlock)
     k_epoch_values(const int64_t *__restrict__ dscan, uint64_t len, int64_t c0,
                    const uint64_t *__restrict__ tiepos, const int32_t *__restrict__ adj,
                    int ntie, int64_t *__restrict__ cval, unsigned long long *cross) {
-  const uint64_t m = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (m >= len) return;
-  int lo = 0, hi = ntie;  // number of ties with position <= m
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (tiepos[mid] <= m) lo = mid + 1; else hi = mid;
+  for (uint64_t m = (uint64_t)blockIdx.x * kBlock + threadIdx.x; m < len;
+       m += (uint64_t)gridDim.x * kBlock) {
+    int lo = 0, hi = ntie;  // number of ties with position <= m
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (tiepos[mid] <= m) lo = mid + 1; else hi = mid;
+    }
+    const int64_t cm = c0 + dscan[m] + (lo ? adj[lo - 1] : 0);
+    cval[m] = cm;
+    if (cm >= ((int64_t)1 << 53)) atomicMin(cross, (unsigned long long)m);
   }
-  const int64_t cm = c0 + dscan[m] + (lo ? adj[lo - 1] : 0);
-  cval[m] = cm;
-  if (cm >= ((int64_t)1 << 53)) atomicMin(cross, (unsigned long long)m);
 }
 
 // cdf[a + m] = C_m * u for m < stop; the crossing element (if any) is added
@@ -214,8 +217,9 @@ __global__ void __launch_bounds__(kBlock)
 __global__ void __launch_bounds__(kBlock)
     k_epoch_write(const int64_t *__restrict__ cval, uint64_t a, uint64_t stop,
                   double u, double *__restrict__ cdf) {
-  const uint64_t m = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (m < stop) cdf[a + m] = (double)cval[m] * u;  // exact: C_m < 2^53
+  for (uint64_t m = (uint64_t)blockIdx.x * kBlock + threadIdx.x; m < stop;
+       m += (uint64_t)gridDim.x * kBlock)
+    cdf[a + m] = (double)cval[m] * u;  // exact: C_m < 2^53
 }
 
 __global__ void k_epoch_cross(const double *__restrict__ p, double *cdf, uint64_t pos,
@@ -230,23 +234,24 @@ __global__ void k_epoch_cross(const double *__restrict__ p, double *cdf, uint64_
 __global__ void __launch_bounds__(kBlock)
     k_search(const double *__restrict__ cdf, uint64_t n, const double *__restrict__ u,
              uint64_t shots, int64_t *__restrict__ out) {
-  const uint64_t s = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (s >= shots) return;
   const double total = cdf[n - 1];
-  const double v = u[s];
-  uint64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint64_t mid = lo + ((hi - lo) >> 1);
-    if (cdf[mid] / total <= v) lo = mid + 1; else hi = mid;
+  for (uint64_t s = (uint64_t)blockIdx.x * kBlock + threadIdx.x; s < shots;
+       s += (uint64_t)gridDim.x * kBlock) {
+    const double v = u[s];
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const uint64_t mid = lo + ((hi - lo) >> 1);
+      if (cdf[mid] / total <= v) lo = mid + 1; else hi = mid;
+    }
+    out[s] = (int64_t)lo;
   }
-  out[s] = (int64_t)lo;
 }
 
 int launch_search(const double *cdf, uint64_t n, const double *u_dev, uint64_t shots,
                   int64_t *out_dev, hipStream_t st) {
   if (shots == 0) return SQ_OK;
-  hipLaunchKernelGGL(k_search, dim3((unsigned)((shots + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, st, cdf, n, u_dev, shots, out_dev);
+  hipLaunchKernelGGL(k_search, sweep_grid(shots, kBlock), dim3(kBlock), 0, st, cdf, n, u_dev,
+                     shots, out_dev);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
 }
@@ -342,8 +347,8 @@ int sample_cdf(const double *p, uint64_t n, double *cdf, bool exact, void *scrat
       if (hi <= a) hi = std::min<uint64_t>(n, a + 1);
     }
     const uint64_t len = hi - a;
-    const unsigned g = (unsigned)((len + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_increments, dim3(g), dim3(kBlock), 0, st, p, a, len,
+    const dim3 g = sweep_grid(len, kBlock);
+    hipLaunchKernelGGL(k_increments, g, dim3(kBlock), 0, st, p, a, len,
                        52 - e, dinc, tie);
     SQ_HIP(hipGetLastError());
     SQ_TRY(scan_inplace<int64_t>(dinc, len, iscr, st));
@@ -358,7 +363,7 @@ int sample_cdf(const double *p, uint64_t n, double *cdf, bool exact, void *scrat
         (size_t)ntie * sizeof(int32_t) > scan_scratch<int32_t>(n))
       return fail(SQ_ERR_UNSUPPORTED, "exact sampler: too many rounding ties");
     if (ntie > 0) {
-      hipLaunchKernelGGL(k_compact_ties, dim3(g), dim3(kBlock), 0, st,
+      hipLaunchKernelGGL(k_compact_ties, g, dim3(kBlock), 0, st,
                          (const int32_t *)tie, len, tpos);
       hipLaunchKernelGGL(k_resolve_ties, dim3(1), dim3(64), 0, st,
                          (const int64_t *)dinc, (const uint64_t *)tpos, (int)ntie, c0, adj);
@@ -366,7 +371,7 @@ int sample_cdf(const double *p, uint64_t n, double *cdf, bool exact, void *scrat
     }
     const unsigned long long none = ~0ull;
     SQ_HIP(hipMemcpyAsync(dev_cross, &none, sizeof(none), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_epoch_values, dim3(g), dim3(kBlock), 0, st, (const int64_t *)dinc,
+    hipLaunchKernelGGL(k_epoch_values, g, dim3(kBlock), 0, st, (const int64_t *)dinc,
                        len, c0, (const uint64_t *)tpos, (const int32_t *)adj, (int)ntie,
                        cval, dev_cross);
     SQ_HIP(hipGetLastError());
@@ -375,8 +380,8 @@ int sample_cdf(const double *p, uint64_t n, double *cdf, bool exact, void *scrat
     SQ_HIP(hipStreamSynchronize(st));
     const uint64_t stop = (cross == none) ? len : (uint64_t)cross;
     if (stop > 0) {
-      hipLaunchKernelGGL(k_epoch_write, dim3((unsigned)((stop + kBlock - 1) / kBlock)),
-                         dim3(kBlock), 0, st, (const int64_t *)cval, a, stop, u, cdf);
+      hipLaunchKernelGGL(k_epoch_write, sweep_grid(stop, kBlock), dim3(kBlock), 0, st,
+                         (const int64_t *)cval, a, stop, u, cdf);
       SQ_HIP(hipGetLastError());
     }
     if (cross == none) {

@@ -7,8 +7,9 @@ from oracle import oracle as O
 
 
 def to_np(C, sv):
-    """Device state -> complex128 numpy copy."""
-    return np.array(sv.cpu(), copy=True).astype(np.complex128)
+    """Device state -> complex128 numpy array (host copy)."""
+    a = np.asarray(sv.cpu())
+    return a if a.dtype == np.complex128 else a.astype(np.complex128)
 
 
 def device_state(C, n, dtype="complex128", init=None):

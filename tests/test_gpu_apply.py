@@ -218,3 +218,21 @@ def test_copy_kernel(C):
     b = device_state(C, n, "complex128")
     C.functionals.copy_kernel(b, a)
     assert np.array_equal(to_np(C, b), v)
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_fused_permutation_blocks(C, dtype):
+    """Blocks that are pure permutations on 4-5 qubits (fused CX / Swap / CCNot)."""
+    g = C.operation.gate_operations
+    from snuqs_amd.transpile import fuse_operations
+
+    n = 12
+    rng = np.random.default_rng(4)
+    init = random_state(n, rng, np.complex64 if dtype == "complex64" else np.complex128)
+    ops = [g.CX([0, 1]), g.CX([2, 3]), g.Swap([1, 2]), g.CCNot([4, 5, 6]), g.CX([6, 7]),
+           g.CSwap([8, 9, 10]), g.CX([10, 11])]
+    fused = fuse_operations(ops, 5)
+    assert max(len(op.targets) for op in fused) >= 4
+    got = run_engine(C, n, fused, dtype, init)
+    ref = run_oracle(n, ops, init.astype(np.complex128))
+    assert np.abs(got - ref).max() <= TOL[dtype]
