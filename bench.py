@@ -41,8 +41,25 @@ HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
 CLASS_NAMES = {0: "none", 1: "phase", 2: "diagonal", 3: "permutation", 4: "dense_1_3",
                5: "dense_4_5_mfma", 6: "dense_4_5_valu"}
-KERNEL_NAMES = {1: "k_phase", 2: "k_diag", 3: "k_perm", 4: "k_dense", 5: "k_mfma_c64",
+KERNEL_NAMES = {1: "k_phase", 2: "k_diag", 3: "k_perm", 4: "k_dense", 5: "k_mfma{k}_{d}",
                 6: "k_dense_lds"}
+
+
+def kernel_name(cls, kk, dtype):
+    return KERNEL_NAMES.get(cls, "?").format(k=kk, d="c64" if dtype == "complex64" else "c128")
+
+
+def pmc_traffic(name):
+    """HBM bytes per launch of `name` from the committed PMC summary
+    (profiles/pmc_traffic.json: FETCH_SIZE and WRITE_SIZE runs, scaled by the
+    copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(name)
+        return None if k is None else k["read_bytes"] + k["write_bytes"]
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def log(*a):
@@ -296,9 +313,10 @@ def main():
             "algorithmic_bytes_per_step": step_bytes, "parallelism": "single GPU",
         },
         "roofline": {
-            "bound": "hbm", "kernel": KERNEL_NAMES.get(dcls, "?") + f"<{dkk}>",
+            "bound": "hbm", "kernel": kernel_name(dcls, dkk, dtype),
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": pmc_traffic(kernel_name(dcls, dkk, dtype)) if n == 30 else None,
             "algorithmic_bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_ms,
             "launches": d["launches"],
             "mfma_tflops": flops / (avg_ms * 1e-3) / 1e12 if flops else None,

@@ -71,7 +71,7 @@ int sq_copy(void *dst, const void *src, size_t bytes, void *stream);
 
 /* Launch-shape parameters of the kernels (set by the autotuner,
  * snuqs_amd/autotune.py; process-wide), one value per dtype: keys
- * "mfma_blocks_per_cu_c64" / "_c128" (1-64, persistent grid of the 4-5 target
+ * "mfma_blocks_per_cu_c64" / "_c128" (1-4096, persistent grid of the 4-5 target
  * MFMA kernels, default 32) and "mfma5_pipeline_c64" / "_c128" (0/1, prefetch
  * the next tile in the 5-target MFMA kernels, default 1).  No reference
  * counterpart. */

@@ -19,7 +19,7 @@ from . import _C
 
 DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 SPACE = {
-    "mfma_blocks_per_cu": [8, 16, 32, 64],
+    "mfma_blocks_per_cu": [16, 32, 64, 128, 256, 1024],
     "mfma5_pipeline": [0, 1],
 }
 SUFFIX = {"complex64": "c64", "complex128": "c128"}

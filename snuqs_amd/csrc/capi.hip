@@ -165,8 +165,8 @@ int sq_tune_set(const char *key, int value) {
   const std::string k = key ? key : "";
   int *slot = tune_slot(k);
   if (!slot) return fail(SQ_ERR_INVALID, "unknown tuning key '" + k + "'");
-  if (k.rfind("mfma_blocks_per_cu", 0) == 0 && (value < 1 || value > 64))
-    return fail(SQ_ERR_INVALID, "mfma_blocks_per_cu must be in [1, 64]");
+  if (k.rfind("mfma_blocks_per_cu", 0) == 0 && (value < 1 || value > 4096))
+    return fail(SQ_ERR_INVALID, "mfma_blocks_per_cu must be in [1, 4096]");
   *slot = k.rfind("mfma5_pipeline", 0) == 0 ? (value ? 1 : 0) : value;
   return SQ_OK;
 }

@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(kBlock)
 int launch_copy(void *dst, const void *src, size_t bytes, hipStream_t stream) {
   const uint64_t n16 = bytes / 16;
   if (n16 == 0) return SQ_OK;
-  hipLaunchKernelGGL(k_copy, sweep_grid(n16, kBlock * 4), dim3(kBlock), 0, stream,
+  hipLaunchKernelGGL(k_copy, sweep_grid(n16, kBlock), dim3(kBlock), 0, stream,
                      (uint4 *)dst, (const uint4 *)src, n16);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
