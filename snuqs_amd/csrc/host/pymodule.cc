@@ -12,6 +12,7 @@
 #include <pybind11/stl.h>
 #include <sys/sysinfo.h>
 
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -691,6 +692,15 @@ PYBIND11_MODULE(_C, m) {
     check(sq_norm2(sv.ptr(), sv.dtype(), sv.num_elems(), &out, sraw(s)));
     return out;
   }, py::arg("state_vector"), py::kw_only(), py::arg("stream") = nullptr);
+  m_fn.def("renormalize", [](StateVector &sv, std::shared_ptr<Stream> s) {
+    sv.require_device("renormalize");
+    double n2;
+    check(sq_norm2(sv.ptr(), sv.dtype(), sv.num_elems(), &n2, sraw(s)));
+    if (!(n2 > 0.0)) throw py::value_error("cannot renormalize a zero state");
+    check(sq_scale(sv.ptr(), sv.dtype(), sv.num_elems(), 1.0 / std::sqrt(n2), 0.0, sraw(s)));
+    return n2;
+  }, py::arg("state_vector"), py::kw_only(), py::arg("stream") = nullptr,
+     "Scale the state to unit norm (FP64 norm); returns the norm^2 before scaling.");
   m_fn.def("scale", [](StateVector &sv, cplx f, std::shared_ptr<Stream> s) {
     sv.require_device("scale");
     check(sq_scale(sv.ptr(), sv.dtype(), sv.num_elems(), f.real(), f.imag(), sraw(s)));

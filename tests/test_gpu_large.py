@@ -93,3 +93,39 @@ def test_config2_full_30q_first_layer_vs_cpu_restatement(C):
     del got, ref
     gc.collect()
     assert err <= 1e-12, err
+
+
+def test_config2_30q_c64_million_samples_exact(C):
+    """Config 2 at 30 qubits complex64 (fused): 1e6 samples from the exact GPU
+    sampler are identical to numpy's RandomState.choice on the same
+    probabilities and seed; fast mode's mismatch count is reported."""
+    import time
+
+    from snuqs_amd.circuits import random_layered
+    from snuqs_amd.transpile import fuse_operations
+
+    n = 30
+    if _free_gb(C) < 40:
+        pytest.skip("needs 40 GiB of free device memory")
+    sv = device_state(C, n, "complex64")
+    C.functionals.Program(n, fuse_operations(random_layered(n, 30, seed=1234), 5),
+                          dtype="complex64").run(sv)
+    n2 = C.functionals.renormalize(sv)  # FP32 drift of the norm, restored in FP64
+    assert abs(n2 - 1) < 1e-5
+    p = C.functionals.probabilities(sv)
+    u = np.random.RandomState(1234).random_sample(10**6)
+    t0 = time.perf_counter()
+    got = C.functionals.sample(sv, u, exact=True)
+    t_exact = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fast = C.functionals.sample(sv, u, exact=False)
+    t_fast = time.perf_counter() - t0
+    # numpy's choice rejects probabilities whose sum is off by more than
+    # sqrt(eps); the device normalises by cumsum(p)[-1] exactly as choice does
+    assert abs(p.sum() - 1) < 1.4e-8, p.sum()
+    want = np.random.RandomState(1234).choice(p.size, p=p, size=10**6)
+    print(f"exact {t_exact:.2f} s, fast {t_fast:.2f} s, fast-mode mismatches "
+          f"{int(np.count_nonzero(fast != want))} of 1e6")
+    assert np.array_equal(got, want)
+    del sv
+    gc.collect()

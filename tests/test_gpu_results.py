@@ -45,6 +45,25 @@ def test_inner_and_scale(C, dtype):
 
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_renormalize(C, dtype):
+    """renormalize scales by 1/sqrt(FP64 norm^2), the product formed in FP64 and
+    rounded once: bitwise equal to numpy's complex128 product cast back."""
+    rng = np.random.default_rng(3)
+    n = 12
+    npdt = np.complex64 if dtype == "complex64" else np.complex128
+    v = (random_state(n, rng) * 1.37).astype(npdt)
+    sv = device_state(C, n, dtype, v)
+    n2 = C.functionals.renormalize(sv)
+    want_n2 = np.vdot(v.astype(np.complex128), v.astype(np.complex128)).real
+    assert abs(n2 - want_n2) < 1e-13 * want_n2
+    f = 1.0 / np.sqrt(n2)
+    want = (v.astype(np.complex128) * f).astype(npdt)
+    assert np.array_equal(np.asarray(sv.cpu()), want)
+    p = C.functionals.probabilities(sv)
+    assert abs(p.sum() - 1) < (1e-8 if dtype == "complex64" else 1e-14)
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
 def test_pauli_expectation(C, dtype):
     from snuqs_amd.circuits import random_pauli_sum
 
