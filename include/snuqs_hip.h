@@ -72,7 +72,7 @@ int sq_copy(void *dst, const void *src, size_t bytes, void *stream);
 /* Launch-shape parameters of the kernels (set by the autotuner,
  * snuqs_amd/autotune.py; process-wide), one value per dtype: keys
  * "mfma_blocks_per_cu_c64" / "_c128" (1-4096, persistent grid of the 4-5 target
- * MFMA kernels, default 32) and "mfma5_pipeline_c64" / "_c128" (0/1, prefetch
+ * MFMA kernels, default 256) and "mfma5_pipeline_c64" / "_c128" (0/1, prefetch
  * the next tile in the 5-target MFMA kernels, default 1).  No reference
  * counterpart. */
 int sq_tune_set(const char *key, int value);
@@ -147,6 +147,14 @@ int sq_program_gate_info(void *program, uint32_t gate, int *kernel_class,
 /* probs[i] = |sv[i]|^2 in float64, device -> device (np.abs(sv)**2). */
 int sq_probabilities(const void *sv, sq_dtype_t dtype, uint64_t num_elems,
                      double *probs, void *stream);
+/* probs[i] = |sv[i]|^2 / sum_j |sv[j]|^2, all in FP64 (device -> device):
+ * the distribution of a complex64 state whose FP32 norm has drifted, summing
+ * to 1 within FP64 rounding (np.abs(sv)**2 of a normalised state,
+ * simulation.py:208-211; numpy's choice needs the sum within sqrt(eps)).
+ * Returns when probs is written. */
+int sq_probabilities_normalized(const void *sv, sq_dtype_t dtype,
+                                uint64_t num_elems, double *probs,
+                                void *stream);
 /* sum_i |sv[i]|^2 in FP64; returns a host value. */
 int sq_norm2(const void *sv, sq_dtype_t dtype, uint64_t num_elems,
              double *out, void *stream);

@@ -465,6 +465,24 @@ int sq_probabilities(const void *sv, sq_dtype_t dt, uint64_t n, double *probs, v
   return launch_probabilities(sv, dt, n, probs, (hipStream_t)stream);
 }
 
+int sq_probabilities_normalized(const void *sv, sq_dtype_t dt, uint64_t n, double *probs,
+                                void *stream) {
+  SQ_TRY(check_state(sv, dt, n));
+  if (!probs) return fail(SQ_ERR_INVALID, "null output");
+  int dev;
+  SQ_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mutex[dev]);
+  Workspace *w;
+  SQ_TRY(ws_reserve(dev, (kRedBlocks + 64) * sizeof(double), 64, &w));
+  hipStream_t st = (hipStream_t)stream;
+  double *res = (double *)w->dev;
+  SQ_TRY(launch_norm2(sv, dt, n, res + 32, res, st));
+  SQ_TRY(launch_probabilities_div(sv, dt, n, probs, res, st));
+  // the workspace is reused by the next call: finish before the lock drops
+  SQ_HIP(hipStreamSynchronize(st));
+  return SQ_OK;
+}
+
 int sq_norm2(const void *sv, sq_dtype_t dt, uint64_t n, double *out, void *stream) {
   SQ_TRY(check_state(sv, dt, n));
   int dev;

@@ -275,7 +275,12 @@ PauliSum make_pauli_sum(size_t nq, const std::vector<std::string> &paulis,
   return s;
 }
 
-py::array_t<double> probabilities(StateVector &sv, std::shared_ptr<Stream> stream) {
+int probs_into(StateVector &sv, double *d, bool normalize, std::shared_ptr<Stream> stream) {
+  return normalize ? sq_probabilities_normalized(sv.ptr(), sv.dtype(), sv.num_elems(), d, sraw(stream))
+                   : sq_probabilities(sv.ptr(), sv.dtype(), sv.num_elems(), d, sraw(stream));
+}
+
+py::array_t<double> probabilities(StateVector &sv, bool normalize, std::shared_ptr<Stream> stream) {
   sv.require_device("probabilities");
   const size_t n = sv.num_elems();
   void *d;
@@ -284,7 +289,7 @@ py::array_t<double> probabilities(StateVector &sv, std::shared_ptr<Stream> strea
   py::array_t<double> out(n);
   {
     py::gil_scoped_release nogil;
-    check(sq_probabilities(sv.ptr(), sv.dtype(), n, (double *)d, sraw(stream)));
+    check(probs_into(sv, (double *)d, normalize, stream));
     check(sq_memcpy_d2h(out.mutable_data(), d, n * sizeof(double), sraw(stream)));
     check(sq_stream_synchronize(sraw(stream)));
   }
@@ -292,7 +297,7 @@ py::array_t<double> probabilities(StateVector &sv, std::shared_ptr<Stream> strea
 }
 
 py::array_t<int64_t> sample(StateVector &sv, py::array_t<double, py::array::c_style> uniforms,
-                            bool exact, std::shared_ptr<Stream> stream) {
+                            bool exact, bool normalize, std::shared_ptr<Stream> stream) {
   sv.require_device("sample");
   const size_t n = sv.num_elems();
   void *d;
@@ -304,7 +309,7 @@ py::array_t<int64_t> sample(StateVector &sv, py::array_t<double, py::array::c_st
   const size_t shots = uniforms.size();
   {
     py::gil_scoped_release nogil;
-    check(sq_probabilities(sv.ptr(), sv.dtype(), n, (double *)d, sraw(stream)));
+    check(probs_into(sv, (double *)d, normalize, stream));
     check(sq_sample((const double *)d, n, u, shots, exact ? 1 : 0, o, sraw(stream)));
   }
   return out;
@@ -676,9 +681,11 @@ PYBIND11_MODULE(_C, m) {
   m_fn.def("initialize_basis_z", &initialize_basis_z, py::arg("state_vector"), py::kw_only(),
            py::arg("stream") = nullptr);
   m_fn.def("probabilities", &probabilities, py::arg("state_vector"), py::kw_only(),
-           py::arg("stream") = nullptr);
+           py::arg("normalize") = false, py::arg("stream") = nullptr,
+           "|sv|^2 in float64; normalize=True divides by the FP64 norm^2.");
   m_fn.def("sample", &sample, py::arg("state_vector"), py::arg("uniforms"), py::kw_only(),
-           py::arg("exact") = true, py::arg("stream") = nullptr);
+           py::arg("exact") = true, py::arg("normalize") = false, py::arg("stream") = nullptr,
+           "Indices drawn from probabilities(state_vector, normalize=normalize).");
   m_fn.def("copy_kernel", [](StateVector &dst, StateVector &src, std::shared_ptr<Stream> s) {
     dst.require_device("copy_kernel");
     src.require_device("copy_kernel");

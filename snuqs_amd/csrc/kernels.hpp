@@ -50,7 +50,7 @@ __device__ __forceinline__ uint64_t deposit(uint64_t f, const IdxMap &m) {
 
 // Launch-shape parameters chosen by the autotuner (sq_tune_set).
 struct Tuning {
-  int mfma_blocks_per_cu[2] = {32, 32};  // persistent grid of the MFMA kernels [c64, c128]
+  int mfma_blocks_per_cu[2] = {256, 256};  // grid cap of the MFMA kernels [c64, c128]
   int mfma5_pipeline[2] = {1, 1};        // 5-target MFMA: prefetch the next tile
 };
 Tuning &tuning();

@@ -287,18 +287,22 @@ __global__ void __launch_bounds__(kBlock)
   using T = typename CV<R>::T;
   constexpr int D = 1 << K;
   __shared__ R gr[D * D], gi[D * D];  // [o][r]
+  __shared__ uint64_t offs[D];
   for (int e = threadIdx.x; e < D * D; e += kBlock) {
     gr[e] = G[e].x;
     gi[e] = G[e].y;
   }
+  if (threadIdx.x < D) offs[threadIdx.x] = off.o[threadIdx.x];
   __syncthreads();
   for (uint64_t f = (uint64_t)blockIdx.x * kBlock + threadIdx.x; f < nfree;
        f += (uint64_t)gridDim.x * kBlock) {
     const uint64_t base = deposit(f, im);
     T x[D];
 #pragma unroll
-    for (int c = 0; c < D; ++c) x[c] = sv[base + off.o[c]];
-#pragma unroll
+    for (int c = 0; c < D; ++c) x[c] = sv[base + offs[c]];
+    // rows one at a time (not unrolled): the D inputs stay in registers and
+    // the row's matrix entries are LDS broadcasts
+#pragma unroll 1
     for (int r = 0; r < D; ++r) {
       R yr = 0, yi = 0;
 #pragma unroll
@@ -310,7 +314,7 @@ __global__ void __launch_bounds__(kBlock)
       T o;
       o.x = yr;
       o.y = yi;
-      sv[base + off.o[r]] = o;
+      sv[base + offs[r]] = o;
     }
   }
 }
@@ -424,102 +428,190 @@ __global__ void __launch_bounds__(kBlock)
 // DENSE, 5 targets on the matrix cores: three real GEMMs per complex product
 // (T1 = Gr Xr, T2 = Gi Xi, T3 = (Gr + Gi)(Xr + Xi); Yr = T1 - T2,
 // Yi = T3 - T1 - T2), i.e. 48 instead of 64 MFMAs per wave tile, with whole
-// complex amplitudes as operands: the B operand of a step is lane l's
-// amplitude o of group l%32 (l%16 for complex128), loaded as one float2
-// (double2), and each output register pair is one amplitude of one group.
-// The next tile's amplitudes are loaded before the current tile's MFMAs, so
-// every wave keeps a tile of loads in flight while its matrix cores run.
-// A operands {Re, Im, Re + Im} are read from LDS per step (ds_read_b128).
+// complex amplitudes as operands.
+//
+// A wave tile is G groups (consecutive free indices: G = 32 for complex64,
+// 16 for complex128) x the 32 amplitudes of the block, i.e. the amplitudes
+// whose index bits at the 5 targets and the lowest log2(G) free positions
+// vary.  Those "tile bits" sorted by position give the tile's amplitudes in
+// address order j; with no control below them the lowest tile bits are the
+// lowest address bits, so every run of consecutive j is contiguous in memory
+// (at least 32 complex64 / 16 complex128 amplitudes = 256 B).  The tile is
+// moved through LDS: lane L loads and stores 16 B at j = 2L + 128k
+// (complex64: two amplitudes) or j = L + 64k (complex128), so each global
+// access is a full run of 16 lanes x 16 B whatever the targets; the MFMA
+// operands are then read from LDS in (group, amplitude) order.  The LDS slot
+// of element j is linear over GF(2) in j's bits (per-bit weights XORed; an
+// XOR swizzle keeps both access orders free of bank conflicts), so lanes
+// need one address constant plus uniform per-k terms.  Loads of the next
+// tile are issued before the current tile's MFMAs (PIPE).
 // ---------------------------------------------------------------------------
+struct StageArgs {
+  uint64_t a[10];  // address stride (amplitudes) of tile bit i
+  uint32_t w[10];  // LDS slot weight of tile bit i (slots = XOR of weights)
+};
+
+// Keep LDS writes and reads of one wave in program order across lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// complex64: element (group c, amplitude r) at slot 64(r>>1) + 2c + (r&1),
+// swizzled by XOR ((r>>1) & 15) << 1.
+__host__ __device__ constexpr uint32_t swz_c64(uint32_t x) {
+  return x ^ (((x >> 6) & 15u) << 1);
+}
+// complex128: element (c, r) at slot 16r + c, swizzled by XOR (r & 15).
+__host__ __device__ constexpr uint32_t swz_c128(uint32_t x) { return x ^ ((x >> 4) & 15u); }
+
 template <bool PIPE>
 __global__ void __launch_bounds__(kBlock)
-    k_mfma5_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, OffArgs off,
+    k_mfma5_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
                 const float4 *__restrict__ G) {
   constexpr int D = 32, NS = 16;
-  __shared__ float4 As[D * D];  // [o][r]
+  __shared__ float4 As[D * D];      // [o][r], 16 KB
+  __shared__ float2 Xs[4][D * 32];  // one tile per wave, 32 KB
   for (int e = threadIdx.x; e < D * D; e += kBlock) As[e] = G[e];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 31, half = lane >> 5;
+  float2 *X = Xs[wave];
   uint64_t tile = (uint64_t)blockIdx.x * 4 + wave;
   const uint64_t step = (uint64_t)gridDim.x * 4;
   if (tile >= ntiles) return;
-  // Lane half h reads input amplitude o = 2s + h (local bit 0 = h) and writes
-  // output amplitudes r = 8(v/4) + 4h + v%4 (local bit 2 = h): both are a
-  // uniform offset plus a per-lane constant.
-  const uint64_t lin = half ? off.o[1] : 0;   // stride of local bit 0
-  const uint64_t lout = half ? off.o[4] : 0;  // stride of local bit 2
-  float2 cur[NS], nxt[NS];
-  uint64_t base = deposit(tile * 32 + col, im);
+  // j = 2 lane + 128 k + e: lane bits are tile bits 1-6, k bits tile bits 7-9
+  uint64_t la = 0;
+  uint32_t lw = 0;
 #pragma unroll
-  for (int s = 0; s < NS; ++s) cur[s] = sv[base + lin + off.o[2 * s]];
+  for (int i = 0; i < 6; ++i)
+    if ((lane >> i) & 1) {
+      la += sa.a[i + 1];
+      lw ^= sa.w[i + 1];
+    }
+#define SQ_KA(k) \
+  ((((k) & 1) ? sa.a[7] : 0) + (((k) & 2) ? sa.a[8] : 0) + (((k) & 4) ? sa.a[9] : 0))
+#define SQ_KW(k) \
+  ((((k) & 1) ? sa.w[7] : 0u) ^ (((k) & 2) ? sa.w[8] : 0u) ^ (((k) & 4) ? sa.w[9] : 0u))
+  const uint32_t w0 = sa.w[0];
+  float4 cur[8], nxt[8];
+  uint64_t base = deposit(tile * 32, im) + la;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) cur[k] = *reinterpret_cast<const float4 *>(sv + base + SQ_KA(k));
   while (true) {
     const uint64_t ntile = tile + step;
     const bool more = ntile < ntiles;
     uint64_t nbase = 0;
     if (PIPE && more) {
-      nbase = deposit(ntile * 32 + col, im);
+      nbase = deposit(ntile * 32, im) + la;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) nxt[s] = sv[nbase + lin + off.o[2 * s]];
+      for (int k = 0; k < 8; ++k)
+        nxt[k] = *reinterpret_cast<const float4 *>(sv + nbase + SQ_KA(k));
     }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      X[lw ^ SQ_KW(k)] = make_float2(cur[k].x, cur[k].y);
+      X[lw ^ SQ_KW(k) ^ w0] = make_float2(cur[k].z, cur[k].w);
+    }
+    wave_lds_sync();
+    // B operand of step s: lane (col, half) takes amplitude 2s + half of group col
+    float2 b[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) b[s] = X[64 * s + 2 * (col ^ s) + half];
     floatx16 t1, t2, t3;
 #pragma unroll
     for (int v = 0; v < 16; ++v) t1[v] = t2[v] = t3[v] = 0.f;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const float4 a = As[(2 * s + half) * D + col];
-      t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, cur[s].x, t1, 0, 0, 0);
-      t2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, cur[s].y, t2, 0, 0, 0);
-      t3 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, cur[s].x + cur[s].y, t3, 0, 0, 0);
+      t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b[s].x, t1, 0, 0, 0);
+      t2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b[s].y, t2, 0, 0, 0);
+      t3 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b[s].x + b[s].y, t3, 0, 0, 0);
     }
+    wave_lds_sync();
+    // C layout: lane l, register v -> group l%32, amplitude 8(v/4) + 4(l/32) + v%4
 #pragma unroll
-    for (int v = 0; v < 16; ++v)
-      sv[base + lout + off.o[8 * (v / 4) + (v % 4)]] =
+    for (int v = 0; v < 16; ++v) {
+      const int r = 8 * (v / 4) + 4 * half + (v % 4), rs = r >> 1;
+      X[64 * rs + 2 * (col ^ (rs & 15)) + (r & 1)] =
           make_float2(t1[v] - t2[v], t3[v] - t1[v] - t2[v]);
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float2 y0 = X[lw ^ SQ_KW(k)], y1 = X[lw ^ SQ_KW(k) ^ w0];
+      *reinterpret_cast<float4 *>(sv + base + SQ_KA(k)) = make_float4(y0.x, y0.y, y1.x, y1.y);
+    }
+    wave_lds_sync();
     if (!more) break;
     tile = ntile;
     if (PIPE) {
       base = nbase;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) cur[s] = nxt[s];
+      for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
     } else {
-      base = deposit(tile * 32 + col, im);
+      base = deposit(tile * 32, im) + la;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) cur[s] = sv[base + lin + off.o[2 * s]];
+      for (int k = 0; k < 8; ++k)
+        cur[k] = *reinterpret_cast<const float4 *>(sv + base + SQ_KA(k));
     }
   }
+#undef SQ_KA
+#undef SQ_KW
 }
 
 template <bool PIPE>
 __global__ void __launch_bounds__(kBlock)
-    k_mfma5_c128(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, OffArgs off,
+    k_mfma5_c128(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
                  const typename Q4<double>::T *__restrict__ G) {
   using D4 = typename Q4<double>::T;
   constexpr int D = 32, NS = 8, NT = 2;
-  __shared__ D4 As[D * D];  // [o][r], 32 KB
+  __shared__ D4 As[D * D];           // [o][r], 32 KB
+  __shared__ double2 Xs[4][D * 16];  // one tile per wave, 32 KB
   for (int e = threadIdx.x; e < D * D; e += kBlock) As[e] = G[e];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, q = lane >> 4;
+  double2 *X = Xs[wave];
   uint64_t tile = (uint64_t)blockIdx.x * 4 + wave;
   const uint64_t step = (uint64_t)gridDim.x * 4;
   if (tile >= ntiles) return;
-  // Lane quarter q reads input amplitude o = 4s + q and writes output
-  // amplitudes r = 16t + 4v + q: q is local bits 0-1 in both, so one per-lane
-  // constant plus uniform offsets addresses everything.
-  const uint64_t lq = ((q & 1) ? off.o[1] : 0) + ((q & 2) ? off.o[2] : 0);
-  double2 cur[NS], nxt[NS];
-  uint64_t base = deposit(tile * 16 + col, im);
+  // j = lane + 64 k: lane bits are tile bits 0-5, k bits tile bits 6-8
+  uint64_t la = 0;
+  uint32_t lw = 0;
 #pragma unroll
-  for (int s = 0; s < NS; ++s) cur[s] = sv[base + lq + off.o[4 * s]];
+  for (int i = 0; i < 6; ++i)
+    if ((lane >> i) & 1) {
+      la += sa.a[i];
+      lw ^= sa.w[i];
+    }
+#define SQ_KA(k) \
+  ((((k) & 1) ? sa.a[6] : 0) + (((k) & 2) ? sa.a[7] : 0) + (((k) & 4) ? sa.a[8] : 0))
+#define SQ_KW(k) \
+  ((((k) & 1) ? sa.w[6] : 0u) ^ (((k) & 2) ? sa.w[7] : 0u) ^ (((k) & 4) ? sa.w[8] : 0u))
+  double2 cur[8], nxt[8];
+  uint64_t base = deposit(tile * 16, im) + la;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) cur[k] = sv[base + SQ_KA(k)];
   while (true) {
     const uint64_t ntile = tile + step;
     const bool more = ntile < ntiles;
     uint64_t nbase = 0;
     if (PIPE && more) {
-      nbase = deposit(ntile * 16 + col, im);
+      nbase = deposit(ntile * 16, im) + la;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) nxt[s] = sv[nbase + lq + off.o[4 * s]];
+      for (int k = 0; k < 8; ++k) nxt[k] = sv[nbase + SQ_KA(k)];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) X[lw ^ SQ_KW(k)] = make_double2(cur[k].x, cur[k].y);
+    wave_lds_sync();
+    // B operand of step s: lane (col, q) takes amplitude 4s + q of group col
+    double2 b[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int o = 4 * s + q;
+      b[s] = X[16 * o + (col ^ (o & 15))];
     }
     doublex4 t1[NT], t2[NT], t3[NT];
 #pragma unroll
@@ -528,33 +620,47 @@ __global__ void __launch_bounds__(kBlock)
       for (int v = 0; v < 4; ++v) t1[t][v] = t2[t][v] = t3[t][v] = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      const double xs = cur[s].x + cur[s].y;
+      const double xs = b[s].x + b[s].y;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const D4 a = As[(4 * s + q) * D + 16 * t + col];
-        t1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, cur[s].x, t1[t], 0, 0, 0);
-        t2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, cur[s].y, t2[t], 0, 0, 0);
+        t1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b[s].x, t1[t], 0, 0, 0);
+        t2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b[s].y, t2[t], 0, 0, 0);
         t3[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.z, xs, t3[t], 0, 0, 0);
       }
     }
+    wave_lds_sync();
+    // C layout (v_mfma_f64_16x16x4f64): lane l, register v of block t ->
+    // group l%16, amplitude 16t + 4v + l/16
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        sv[base + lq + off.o[16 * t + 4 * v]] =
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * t + 4 * v + q;
+        X[16 * r + (col ^ (r & 15))] =
             make_double2(t1[t][v] - t2[t][v], t3[t][v] - t1[t][v] - t2[t][v]);
+      }
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double2 y = X[lw ^ SQ_KW(k)];
+      sv[base + SQ_KA(k)] = make_double2(y.x, y.y);
+    }
+    wave_lds_sync();
     if (!more) break;
     tile = ntile;
     if (PIPE) {
       base = nbase;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) cur[s] = nxt[s];
+      for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
     } else {
-      base = deposit(tile * 16 + col, im);
+      base = deposit(tile * 16, im) + la;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) cur[s] = sv[base + lq + off.o[4 * s]];
+      for (int k = 0; k < 8; ++k) cur[k] = sv[base + SQ_KA(k)];
     }
   }
+#undef SQ_KA
+#undef SQ_KW
 }
 
 // ---------------------------------------------------------------------------
@@ -586,6 +692,35 @@ static uint64_t target_mask(const GatePlan &p) {
   uint64_t m = 0;
   for (int j = 0; j < p.kk; ++j) m |= 1ull << p.tbit[j];
   return m;
+}
+
+// Tile geometry of the 5-target MFMA kernels: the 5 targets and the lowest
+// `nfb` free positions (5 for complex64, 4 for complex128) in ascending order,
+// each with its address stride and its LDS slot weight (see StageArgs).
+// False when a control sits below the tile (complex64 then lacks the
+// contiguous amplitude pairs its 16 B accesses need).
+static bool make_stage(const GatePlan &p, bool c64, StageArgs *sa) {
+  const int nfb = c64 ? 5 : 4;
+  const uint64_t tmask = target_mask(p);
+  int i = 0, nf = 0;
+  for (int b = 0; b < 64 && i < nfb + p.kk; ++b) {
+    const uint64_t bit = 1ull << b;
+    if (p.fixed_mask & bit) continue;
+    uint32_t slot;
+    if (tmask & bit) {
+      int q = 0;
+      while (p.tbit[q] != b) ++q;
+      slot = c64 ? (q == 0 ? 1u : 1u << (q + 5)) : 16u << q;
+    } else {
+      if (nf == nfb) continue;
+      slot = c64 ? 2u << nf : 1u << nf;
+      ++nf;
+    }
+    sa->a[i] = bit;
+    sa->w[i] = c64 ? swz_c64(slot) : swz_c128(slot);
+    ++i;
+  }
+  return i == nfb + p.kk && (!c64 || sa->a[0] == 1);
 }
 
 static inline dim3 grid_for(uint64_t work, int per_block) {
@@ -674,7 +809,12 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
         for (int r = 0; r < d; ++r) off.o[r] = local_offset(p, r);
         constexpr bool c64 = sizeof(R) == 4;
         const uint64_t tile_groups = c64 ? 32 : 16;
-        const bool mfma = nfree >= tile_groups && (nfree % tile_groups) == 0;
+        StageArgs sa{};
+        // complex64 tiles move 16 B (two amplitudes) per access: the state must
+        // start on a 16 B boundary (a slice at an odd amplitude does not)
+        const bool aligned = !c64 || ((uintptr_t)sv & 15) == 0;
+        const bool mfma = nfree >= tile_groups && (nfree % tile_groups) == 0 &&
+                          (p.kk == 4 || (aligned && make_stage(p, c64, &sa)));
         if (mfma) {
           const uint64_t ntiles = nfree / tile_groups;
           uint64_t blocks = (ntiles + 3) / 4;
@@ -689,7 +829,7 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
                                  (float2 *)sv, ntiles, im, off, (const Q *)gdev);
             else
               hipLaunchKernelGGL(pipe ? k_mfma5_c64<true> : k_mfma5_c64<false>, dim3(blocks),
-                                 dim3(kBlock), 0, st, (float2 *)sv, ntiles, im, off,
+                                 dim3(kBlock), 0, st, (float2 *)sv, ntiles, im, sa,
                                  (const Q *)gdev);
           } else {
             if (p.kk == 4)
@@ -698,7 +838,7 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
             else
               hipLaunchKernelGGL(pipe ? k_mfma5_c128<true> : k_mfma5_c128<false>,
                                  dim3(blocks), dim3(kBlock), 0, st, (double2 *)sv, ntiles,
-                                 im, off, (const Q *)gdev);
+                                 im, sa, (const Q *)gdev);
           }
         } else {
           using Q = typename Q4<R>::T;

@@ -54,6 +54,27 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// p[i] = |sv[i]|^2 / norm2, norm2 read from device memory (the FP64 sum of the
+// same |sv[i]|^2 by the fixed-order reduction), division correctly rounded.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_probabilities_div(const T *__restrict__ sv, double *__restrict__ p, uint64_t n,
+                        const double *__restrict__ norm2) {
+#pragma clang fp contract(off)
+  const double d = *norm2;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x; i0 < n;
+       i0 += (uint64_t)gridDim.x * (kBlock * 4)) {
+    uint64_t i = i0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u, i += kBlock) {
+      if (i < n) {
+        const double2 v = ld2(sv, i);
+        p[i] = (v.x * v.x + v.y * v.y) / d;
+      }
+    }
+  }
+}
+
 // ---- norm^2 and <a|b>: per-block partial sums -----------------------------
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
@@ -223,6 +244,18 @@ int launch_probabilities(const void *sv, sq_dtype_t dt, uint64_t n, double *p,
   else
     hipLaunchKernelGGL(k_probabilities<double2>, grid1(n, kBlock * 4), dim3(kBlock), 0, st,
                        (const double2 *)sv, p, n);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+int launch_probabilities_div(const void *sv, sq_dtype_t dt, uint64_t n, double *p,
+                             const double *norm2_dev, hipStream_t st) {
+  if (dt == SQ_COMPLEX64)
+    hipLaunchKernelGGL(k_probabilities_div<float2>, grid1(n, kBlock * 4), dim3(kBlock), 0,
+                       st, (const float2 *)sv, p, n, norm2_dev);
+  else
+    hipLaunchKernelGGL(k_probabilities_div<double2>, grid1(n, kBlock * 4), dim3(kBlock), 0,
+                       st, (const double2 *)sv, p, n, norm2_dev);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
 }

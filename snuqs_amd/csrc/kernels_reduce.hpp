@@ -10,6 +10,8 @@ constexpr int kRedBlocks = 2048;  // partial sums per reduced value
 
 int launch_probabilities(const void *sv, sq_dtype_t dt, uint64_t n, double *p,
                          hipStream_t st);
+int launch_probabilities_div(const void *sv, sq_dtype_t dt, uint64_t n, double *p,
+                             const double *norm2_dev, hipStream_t st);
 int launch_norm2(const void *sv, sq_dtype_t dt, uint64_t n, double *partial,
                  double *out_dev, hipStream_t st);
 // out_dev[0..1] = beta*out_dev + alpha_dev[0..1] * (re, im) of <a|b>

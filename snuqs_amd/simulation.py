@@ -14,7 +14,8 @@ evolve) for the single-GPU tier.  Differences, by design:
   np.random.choice does (simulation.py:220) and searches the cumulative
   distribution on the GPU: with ``exact_sampling`` the indices are identical
   to np.random.choice(len(sv), p=probabilities, size=shots);
-* ``probabilities`` is computed on the GPU in float64.
+* ``probabilities`` is computed on the GPU in float64 (for complex64 divided
+  by the FP64 norm^2, so it sums to 1 within FP64 rounding).
 """
 from __future__ import annotations
 
@@ -93,10 +94,6 @@ class StateVectorSimulation(Simulation):
         if self._fuse and self._fuse >= 2:
             ops = fuse_operations(ops, self._fuse)
         Program(self._qubit_count, ops, dtype=self._precision).run(self._state_vector)
-        if self._precision == "complex64":
-            # FP32 rounding drifts the norm by ~1e-7 over deep circuits; restore
-            # it in FP64 so probabilities sum to 1 within numpy's tolerance
-            _C.functionals.renormalize(self._state_vector)
 
     @property
     def state_vector(self) -> np.ndarray:
@@ -104,14 +101,24 @@ class StateVectorSimulation(Simulation):
         return np.array(self._state_vector.cpu(), copy=True)
 
     @property
+    def _normalize(self) -> bool:
+        # FP32 rounding drifts a complex64 state's norm by ~1e-7 over deep
+        # circuits, beyond the sqrt(eps) = 1.5e-8 that numpy's choice accepts
+        # in sum(p); the FP32 state cannot be rescaled that finely (the
+        # correction is below half an ulp), so its probabilities are divided
+        # by the FP64 norm instead
+        return self._precision == "complex64"
+
+    @property
     def probabilities(self) -> np.ndarray:
-        return _C.functionals.probabilities(self._state_vector)
+        return _C.functionals.probabilities(self._state_vector, normalize=self._normalize)
 
     def retrieve_samples(self) -> np.ndarray:
         if self._shots == 0:
             return np.array([], dtype=np.int64)
         u = np.random.random_sample(self._shots)
-        return _C.functionals.sample(self._state_vector, u, exact=self._exact)
+        return _C.functionals.sample(self._state_vector, u, exact=self._exact,
+                                     normalize=self._normalize)
 
     def amplitudes(self, indices) -> np.ndarray:
         return _C.functionals.amplitudes(self._state_vector, [int(i) for i in indices])

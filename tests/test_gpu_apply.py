@@ -71,6 +71,38 @@ def test_unitary_k_targets(C, dtype, k, n):
         assert err <= TOL[dtype], (t, err)
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("grid,pipe", [(256, 1), (1, 1), (1, 0)])
+def test_unitary_5_target_tiles(C, dtype, grid, pipe):
+    """5-target MFMA kernels (LDS-staged tiles) on target sets that put the
+    tile's bits everywhere: the lowest five address bits, the highest, spread
+    and given out of order; a 1-block-per-CU grid cap makes every wave loop over
+    several tiles with and without the prefetch."""
+    from snuqs_amd.circuits import haar_unitary
+
+    g = C.operation.gate_operations
+    n = 21
+    rng = np.random.default_rng(55)
+    init = random_state(n, rng, np.complex64 if dtype == "complex64" else np.complex128)
+    ref0 = init.astype(np.complex128)
+    suffix = "c64" if dtype == "complex64" else "c128"
+    keys = (f"mfma_blocks_per_cu_{suffix}", f"mfma5_pipeline_{suffix}")
+    saved = [C.core.tune_get(k) for k in keys]
+    try:
+        C.core.tune_set(keys[0], grid)
+        C.core.tune_set(keys[1], pipe)
+        for t in ([16, 17, 18, 19, 20], [0, 1, 2, 3, 4], [20, 0, 13, 6, 17],
+                  [18, 11, 5, 19, 2], [20, 18, 16, 14, 12]):
+            op = g.Unitary(t, haar_unitary(32, rng))
+            got = run_engine(C, n, [op], dtype, init)
+            ref = run_oracle(n, [op], ref0)
+            err = np.abs(got - ref).max()
+            assert err <= TOL[dtype], (t, err)
+    finally:
+        for k, v in zip(keys, saved):
+            C.core.tune_set(k, v)
+
+
 def test_ghz20_closed_form_bitwise(C):
     """GHZ-20 complex128: amplitudes 0 and 2^20-1 are bitwise M_SQRT1_2
     (0x3FE6A09E667F3BCD), every other amplitude is exactly zero."""

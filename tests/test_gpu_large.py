@@ -110,19 +110,18 @@ def test_config2_30q_c64_million_samples_exact(C):
     sv = device_state(C, n, "complex64")
     C.functionals.Program(n, fuse_operations(random_layered(n, 30, seed=1234), 5),
                           dtype="complex64").run(sv)
-    n2 = C.functionals.renormalize(sv)  # FP32 drift of the norm, restored in FP64
-    assert abs(n2 - 1) < 1e-5
-    p = C.functionals.probabilities(sv)
+    assert abs(C.functionals.norm2(sv) - 1) < 1e-5  # FP32 drift of the norm
+    p = C.functionals.probabilities(sv, normalize=True)
     u = np.random.RandomState(1234).random_sample(10**6)
     t0 = time.perf_counter()
-    got = C.functionals.sample(sv, u, exact=True)
+    got = C.functionals.sample(sv, u, exact=True, normalize=True)
     t_exact = time.perf_counter() - t0
     t0 = time.perf_counter()
-    fast = C.functionals.sample(sv, u, exact=False)
+    fast = C.functionals.sample(sv, u, exact=False, normalize=True)
     t_fast = time.perf_counter() - t0
     # numpy's choice rejects probabilities whose sum is off by more than
     # sqrt(eps); the device normalises by cumsum(p)[-1] exactly as choice does
-    assert abs(p.sum() - 1) < 1.4e-8, p.sum()
+    assert abs(p.sum() - 1) < 1e-12, p.sum()
     want = np.random.RandomState(1234).choice(p.size, p=p, size=10**6)
     print(f"exact {t_exact:.2f} s, fast {t_fast:.2f} s, fast-mode mismatches "
           f"{int(np.count_nonzero(fast != want))} of 1e6")

@@ -64,6 +64,25 @@ def test_renormalize(C, dtype):
 
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_probabilities_normalized(C, dtype):
+    """normalize=True: |a|^2 / norm2 in FP64 with norm2 the device's FP64 norm,
+    division correctly rounded (bitwise equal to numpy's), summing to 1."""
+    rng = np.random.default_rng(6)
+    n = 13
+    npdt = np.complex64 if dtype == "complex64" else np.complex128
+    v = (random_state(n, rng) * 0.93).astype(npdt)
+    sv = device_state(C, n, dtype, v)
+    n2 = C.functionals.norm2(sv)
+    p = C.functionals.probabilities(sv, normalize=True)
+    w = v.astype(np.complex128)
+    assert np.array_equal(p, (w.real * w.real + w.imag * w.imag) / n2)
+    assert abs(p.sum() - 1) < 1e-14
+    u = np.random.RandomState(8).random_sample(5000)
+    assert np.array_equal(C.functionals.sample(sv, u, exact=True, normalize=True),
+                          np.random.RandomState(8).choice(p.size, p=p, size=5000))
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
 def test_pauli_expectation(C, dtype):
     from snuqs_amd.circuits import random_pauli_sum
 
