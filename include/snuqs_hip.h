@@ -72,9 +72,11 @@ int sq_copy(void *dst, const void *src, size_t bytes, void *stream);
 /* Launch-shape parameters of the kernels (set by the autotuner,
  * snuqs_amd/autotune.py; process-wide), one value per dtype: keys
  * "mfma_blocks_per_cu_c64" / "_c128" (1-4096, persistent grid of the 4-5 target
- * MFMA kernels, default 256) and "mfma5_pipeline_c64" / "_c128" (0/1, prefetch
- * the next tile in the 5-target MFMA kernels, default 1).  No reference
- * counterpart. */
+ * MFMA kernels, default 256), "mfma5_pipeline_c64" / "_c128" (0/1, prefetch
+ * the next tile in the 5-target MFMA kernels, default 1) and
+ * "mfma5_order_c64" / "_c128" (0 tiles dealt cyclically over the grid,
+ * 1 contiguous tile ranges per workgroup; default 0).  A gate tuned by
+ * sq_program_tune uses its own shape instead.  No reference counterpart. */
 int sq_tune_set(const char *key, int value);
 int sq_tune_get(const char *key, int *value);
 
@@ -132,6 +134,16 @@ int sq_program_run(void *program, void *sv, uint64_t num_elems, void *stream);
  * launch nothing). */
 int sq_program_run_timed(void *program, void *sv, uint64_t num_elems,
                          void *stream, float *ms_per_gate);
+/* Per-gate autotuner (no reference counterpart): for every gate that runs
+ * on the 5-target MFMA kernels, time each candidate launch shape (tile order
+ * x grid size x prefetch) inside `reps` whole-program runs on sv and keep each
+ * gate's fastest for the gate and its inverse.  Choices are cached process-wide by dtype, state
+ * size and target/control bits, so later programs with the same gates reuse
+ * them without timing.  Results do not depend on the shape (the same
+ * arithmetic runs in every shape).  sv is overwritten: the gates are applied
+ * to it many times.  *num_tuned (may be NULL) = gates given a shape. */
+int sq_program_tune(void *program, void *sv, uint64_t num_elems, int reps,
+                    void *stream, uint32_t *num_tuned);
 /* Kernel class of gate g (0 none, 1 phase, 2 diagonal, 3 permutation,
  * 4 dense 1-3 targets, 5 dense 4-5 targets on MFMA, 6 dense 4-5 targets on
  * VALU), its target count after control extraction and its algorithmic HBM

@@ -10,7 +10,9 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "kernels.hpp"
@@ -155,6 +157,8 @@ static int *tune_slot(const std::string &k) {
       {"mfma_blocks_per_cu_c128", [] { return &tuning().mfma_blocks_per_cu[1]; }},
       {"mfma5_pipeline_c64", [] { return &tuning().mfma5_pipeline[0]; }},
       {"mfma5_pipeline_c128", [] { return &tuning().mfma5_pipeline[1]; }},
+      {"mfma5_order_c64", [] { return &tuning().mfma5_order[0]; }},
+      {"mfma5_order_c128", [] { return &tuning().mfma5_order[1]; }},
   };
   for (auto &e : table)
     if (k == e.name) return e.slot();
@@ -167,7 +171,8 @@ int sq_tune_set(const char *key, int value) {
   if (!slot) return fail(SQ_ERR_INVALID, "unknown tuning key '" + k + "'");
   if (k.rfind("mfma_blocks_per_cu", 0) == 0 && (value < 1 || value > 4096))
     return fail(SQ_ERR_INVALID, "mfma_blocks_per_cu must be in [1, 4096]");
-  *slot = k.rfind("mfma5_pipeline", 0) == 0 ? (value ? 1 : 0) : value;
+  const bool flag = k.rfind("mfma5_pipeline", 0) == 0 || k.rfind("mfma5_order", 0) == 0;
+  *slot = flag ? (value ? 1 : 0) : value;
   return SQ_OK;
 }
 
@@ -435,6 +440,106 @@ int sq_program_run_timed(void *program, void *sv, uint64_t n, void *stream, floa
   }
   for (auto &e : ev) (void)hipEventDestroy(e);
   return rc;
+}
+
+// Launch shapes chosen by sq_program_tune, keyed by what decides the memory
+// pattern: dtype, state size, target bits, control bits and values.
+namespace {
+struct ShapeKey {
+  int dt, nbits;
+  uint64_t tmask, fmask, fval;
+  bool operator<(const ShapeKey &o) const {
+    return std::tie(dt, nbits, tmask, fmask, fval) <
+           std::tie(o.dt, o.nbits, o.tmask, o.fmask, o.fval);
+  }
+};
+std::mutex g_shape_mutex;
+std::map<ShapeKey, GatePlan::Shape> g_shape_cache;
+}  // namespace
+
+// Runs the program once with an event before every gate and adds each gate's
+// elapsed time to ms[g].
+static int run_timed_into(Program *p, void *sv, uint64_t n, hipStream_t st,
+                          std::vector<hipEvent_t> &ev, double *ms) {
+  const size_t ng = p->fwd.size();
+  for (size_t g = 0; g < ng; ++g) {
+    SQ_HIP(hipEventRecord(ev[g], st));
+    SQ_TRY(launch_plan(p->fwd[g], sv, p->dt, n, st, p->fwd_dev[g]));
+  }
+  SQ_HIP(hipEventRecord(ev[ng], st));
+  SQ_HIP(hipEventSynchronize(ev[ng]));
+  for (size_t g = 0; g < ng; ++g) {
+    float t = 0.f;
+    SQ_HIP(hipEventElapsedTime(&t, ev[g], ev[g + 1]));
+    ms[g] += t;
+  }
+  return SQ_OK;
+}
+
+int sq_program_tune(void *program, void *sv, uint64_t n, int reps, void *stream,
+                    uint32_t *num_tuned) {
+  Program *p = (Program *)program;
+  if (!p) return fail(SQ_ERR_INVALID, "null program");
+  SQ_TRY(check_state(sv, p->dt, n));
+  if (n != (1ull << p->nq)) return fail(SQ_ERR_INVALID, "state size does not match program");
+  if (reps < 1 || reps > 100) return fail(SQ_ERR_INVALID, "reps must be in [1, 100]");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t ng = p->fwd.size();
+  // gates to tune; those whose key is cached take the cached shape
+  std::vector<size_t> todo;
+  std::vector<ShapeKey> keys(ng);
+  uint32_t tuned = 0;
+  for (size_t g = 0; g < ng; ++g) {
+    GatePlan &f = p->fwd[g];
+    if (gate_class(f, n, p->dt) != 5 || f.kk != 5) continue;
+    uint64_t tmask = 0;
+    for (int j = 0; j < f.kk; ++j) tmask |= 1ull << f.tbit[j];
+    keys[g] = ShapeKey{(int)p->dt, p->nq, tmask, f.fixed_mask, f.fixed_val};
+    std::lock_guard<std::mutex> lk(g_shape_mutex);
+    auto it = g_shape_cache.find(keys[g]);
+    if (it != g_shape_cache.end()) {
+      f.shape = p->inv[g].shape = it->second;
+      ++tuned;
+    } else {
+      todo.push_back(g);
+    }
+  }
+  if (todo.empty()) {
+    if (num_tuned) *num_tuned = tuned;
+    return SQ_OK;
+  }
+  // Each candidate shape is timed inside whole-program runs (what a gate
+  // costs depends on the launches around it: timing a gate alone, launch
+  // after launch, mispredicts it); every gate then keeps its fastest shape.
+  std::vector<GatePlan::Shape> cands;
+  for (int order = 0; order < 2; ++order)
+    for (int bpc : {64, 256, 1024}) cands.push_back({bpc, order, 1});
+  cands.push_back({256, 0, 0});
+  std::vector<hipEvent_t> ev(ng + 1);
+  for (auto &e : ev) SQ_HIP(hipEventCreate(&e));
+  std::vector<std::vector<double>> ms(cands.size(), std::vector<double>(ng, 0.0));
+  int rc = SQ_OK;
+  for (size_t c = 0; c < cands.size() && rc == SQ_OK; ++c) {
+    for (size_t g : todo) p->fwd[g].shape = cands[c];
+    for (int r = 0; r < reps && rc == SQ_OK; ++r)
+      rc = run_timed_into(p, sv, n, st, ev, ms[c].data());
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (rc != SQ_OK) {
+    for (size_t g : todo) p->fwd[g].shape = GatePlan::Shape{};
+    return rc;
+  }
+  std::lock_guard<std::mutex> lk(g_shape_mutex);
+  for (size_t g : todo) {
+    size_t best = 0;
+    for (size_t c = 1; c < cands.size(); ++c)
+      if (ms[c][g] < ms[best][g]) best = c;
+    p->fwd[g].shape = p->inv[g].shape = cands[best];
+    g_shape_cache[keys[g]] = cands[best];
+    ++tuned;
+  }
+  if (num_tuned) *num_tuned = tuned;
+  return SQ_OK;
 }
 
 int sq_program_stats(void *program, uint64_t *num_launches, double *bytes) {

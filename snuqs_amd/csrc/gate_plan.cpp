@@ -187,6 +187,30 @@ int make_plan(const double *matrix, const uint32_t *targets, int k,
     p.kind = Kind::DENSE;
     p.m = M;
     p.touched = sub;
+    // A 4-target block on a large state runs as a 5-target block with an
+    // identity factor on the lowest free bit (I (x) M in that bit's place):
+    // the same amplitudes change, and the 5-target MFMA kernel's staged
+    // tiles sweep HBM faster than the 4-target kernel does.  Needs bit 0
+    // free of controls (the complex64 tiles' 16 B accesses) and enough free
+    // positions left for full tiles.
+    const int nfixed = __builtin_popcountll(p.fixed_mask);
+    if (kk == 4 && !(p.fixed_mask & 1) && nbits - nfixed - 5 >= 10) {
+      uint64_t used = p.fixed_mask;
+      for (int j = 0; j < kk; ++j) used |= 1ull << tb[j];
+      int nb = 0;
+      while ((used >> nb) & 1) ++nb;
+      int j0 = 0;  // local position of the new bit (tbit stays ascending)
+      while (j0 < kk && tb[j0] < nb) ++j0;
+      const int d5 = 32;
+      std::vector<cd> W(d5 * d5, cd(0.0, 0.0));
+      for (int r = 0; r < d5; ++r)
+        for (int c = 0; c < d5; ++c)
+          if (((r >> j0) & 1) == ((c >> j0) & 1)) W[r * d5 + c] = M[squeeze(r, j0) * dd + squeeze(c, j0)];
+      tb.insert(tb.begin() + j0, nb);
+      p.kk = 5;
+      for (int j = 0; j < 5; ++j) p.tbit[j] = tb[j];
+      p.m.swap(W);
+    }
   }
   *plan = std::move(p);
   return SQ_OK;

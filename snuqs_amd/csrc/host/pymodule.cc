@@ -355,6 +355,16 @@ class Program {
     }
     return ms;
   }
+  uint32_t tune(StateVector &sv, int reps, std::shared_ptr<Stream> stream) {
+    sv.require_device("Program.tune");
+    if (sv.dtype() != dt_) throw py::value_error("state dtype differs from the program's");
+    uint32_t tuned = 0;
+    {
+      py::gil_scoped_release nogil;
+      check(sq_program_tune(prog_, sv.ptr(), sv.num_elems(), reps, sraw(stream), &tuned));
+    }
+    return tuned;
+  }
   py::tuple gate_info(size_t g) const {
     int cls, kk;
     double b;
@@ -761,6 +771,10 @@ PYBIND11_MODULE(_C, m) {
            }), py::arg("num_qubits"), py::arg("operations"), py::kw_only(),
            py::arg("dtype") = "complex128")
       .def("run", &Program::run, py::arg("state_vector"), py::kw_only(), py::arg("stream") = nullptr)
+      .def("tune", &Program::tune, py::arg("state_vector"), py::kw_only(), py::arg("reps") = 2,
+           py::arg("stream") = nullptr,
+           "Pick each 5-target gate's fastest launch shape by timing it on state_vector "
+           "(overwritten); returns the number of gates tuned.")
       .def("run_timed", &Program::run_timed, py::arg("state_vector"), py::kw_only(),
            py::arg("stream") = nullptr)
       .def("gate_info", &Program::gate_info)

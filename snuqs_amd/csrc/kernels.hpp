@@ -48,10 +48,56 @@ __device__ __forceinline__ uint64_t deposit(uint64_t f, const IdxMap &m) {
   return f | m.fixed_val;
 }
 
+// Non-temporal global access (streaming: the sweep kernels touch each
+// amplitude once per launch).  Loads and stores are both marked: on MI355X
+// an in-place streaming sweep moves ~10% more bytes per second than with
+// ordinary accesses, while marking only one side gains nothing.
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+typedef unsigned int nt_u4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float ld_nt(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ double ld_nt(const double *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float2 ld_nt(const float2 *p) {
+  const nt_f2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f2 *>(p));
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ double2 ld_nt(const double2 *p) {
+  const nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2 *>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ float4 ld_nt(const float4 *p) {
+  const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4 *>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 ld_nt(const uint4 *p) {
+  const nt_u4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_u4 *>(p));
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(float2 *p, float2 v) {
+  const nt_f2 w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<nt_f2 *>(p));
+}
+__device__ __forceinline__ void st_nt(double2 *p, double2 v) {
+  const nt_d2 w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<nt_d2 *>(p));
+}
+__device__ __forceinline__ void st_nt(float4 *p, float4 v) {
+  const nt_f4 w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, reinterpret_cast<nt_f4 *>(p));
+}
+__device__ __forceinline__ void st_nt(uint4 *p, uint4 v) {
+  const nt_u4 w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, reinterpret_cast<nt_u4 *>(p));
+}
+
 // Launch-shape parameters chosen by the autotuner (sq_tune_set).
 struct Tuning {
   int mfma_blocks_per_cu[2] = {256, 256};  // grid cap of the MFMA kernels [c64, c128]
   int mfma5_pipeline[2] = {1, 1};        // 5-target MFMA: prefetch the next tile
+  int mfma5_order[2] = {0, 0};           // 5-target MFMA: 0 cyclic tiles, 1 blocked
 };
 Tuning &tuning();
 

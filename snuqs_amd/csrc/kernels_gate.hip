@@ -37,7 +37,7 @@ __global__ void __launch_bounds__(kBlock)
         T z;
         z.x = (basis && i == 0) ? 1 : 0;
         z.y = 0;
-        sv[i] = z;
+        st_nt(sv + i, z);
       }
     }
   }
@@ -69,10 +69,10 @@ __global__ void __launch_bounds__(kBlock)
     uint4 v[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (i + (uint64_t)u * kBlock < n16) v[u] = src[i + (uint64_t)u * kBlock];
+      if (i + (uint64_t)u * kBlock < n16) v[u] = ld_nt(src + i + (uint64_t)u * kBlock);
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (i + (uint64_t)u * kBlock < n16) dst[i + (uint64_t)u * kBlock] = v[u];
+      if (i + (uint64_t)u * kBlock < n16) st_nt(dst + i + (uint64_t)u * kBlock, v[u]);
   }
 }
 
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int u = 0; u < U; ++u) {
       uint64_t f = f0 + (uint64_t)u * kBlock;
       idx[u] = deposit(f, im);
-      if (f < nfree) v[u] = sv[idx[u]];
+      if (f < nfree) v[u] = ld_nt(sv + idx[u]);
     }
   #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(kBlock)
         T o;
         o.x = pr * v[u].x - pi * v[u].y;
         o.y = pr * v[u].y + pi * v[u].x;
-        sv[idx[u]] = o;
+        st_nt(sv + idx[u], o);
       }
     }
   }
@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int u = 0; u < U; ++u) {
       uint64_t f = f0 + (uint64_t)u * kBlock;
       idx[u] = deposit(f, im);
-      if (f < nfree) v[u] = sv[idx[u]];
+      if (f < nfree) v[u] = ld_nt(sv + idx[u]);
     }
   #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(kBlock)
         T o;
         o.x = pr * v[u].x - pi * v[u].y;
         o.y = pr * v[u].y + pi * v[u].x;
-        sv[idx[u]] = o;
+        st_nt(sv + idx[u], o);
       }
     }
   }
@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(kBlock)
       if (f < nfree) {
   #pragma unroll
         for (int r = 0; r < 8; ++r)
-          if (r < a.count) v[u][r] = sv[base[u] + a.src[r]];
+          if (r < a.count) v[u][r] = ld_nt(sv + base[u] + a.src[r]);
       }
     }
   #pragma unroll
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(kBlock)
             T o;
             o.x = a.pr[r] * v[u][r].x - a.pi[r] * v[u][r].y;
             o.y = a.pr[r] * v[u][r].y + a.pi[r] * v[u][r].x;
-            sv[base[u] + a.dst[r]] = o;
+            st_nt(sv + base[u] + a.dst[r], o);
           }
         }
       }
@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(kBlock)
       base[u] = deposit(f, im);
       if (f < nfree) {
   #pragma unroll
-        for (int c = 0; c < D; ++c) x[u][c] = sv[base[u] + a.off[c]];
+        for (int c = 0; c < D; ++c) x[u][c] = ld_nt(sv + base[u] + a.off[c]);
       }
     }
   #pragma unroll
@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(kBlock)
           T o;
           o.x = yr;
           o.y = yi;
-          sv[base[u] + a.off[r]] = o;
+          st_nt(sv + base[u] + a.off[r], o);
         }
       }
     }
@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(kBlock)
     const float *src = reinterpret_cast<const float *>(sv + base) + half;
     float b[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) b[s] = src[2 * off.o[s]];
+    for (int s = 0; s < NS; ++s) b[s] = ld_nt(src + 2 * off.o[s]);
     floatx16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
@@ -373,8 +373,8 @@ __global__ void __launch_bounds__(kBlock)
       const int r = 4 * q;  // + 2*half, +1
       const uint64_t o0 = half ? off.o[r + 2] : off.o[r];
       const uint64_t o1 = half ? off.o[r + 3] : off.o[r + 1];
-      sv[base + o0] = make_float2(acc[4 * q + 0], acc[4 * q + 1]);
-      sv[base + o1] = make_float2(acc[4 * q + 2], acc[4 * q + 3]);
+      st_nt(sv + base + o0, make_float2(acc[4 * q + 0], acc[4 * q + 1]));
+      st_nt(sv + base + o1, make_float2(acc[4 * q + 2], acc[4 * q + 3]));
     }
   }
 }
@@ -396,7 +396,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const uint64_t o = (q >> 1) ? off.o[2 * s + 1] : off.o[2 * s];
-      b[s] = src[2 * o];
+      b[s] = ld_nt(src + 2 * o);
     }
     doublex4 acc[NT];
 #pragma unroll
@@ -419,7 +419,7 @@ __global__ void __launch_bounds__(kBlock)
       for (int v = 0; v < 4; ++v) {
         const int r = 8 * t + 2 * v;
         const uint64_t o = (q >> 1) ? off.o[r + 1] : off.o[r];
-        dst[2 * o] = acc[t][v];
+        st_nt(dst + 2 * o, acc[t][v]);
       }
   }
 }
@@ -446,6 +446,30 @@ __global__ void __launch_bounds__(kBlock)
 // need one address constant plus uniform per-k terms.  Loads of the next
 // tile are issued before the current tile's MFMAs (PIPE).
 // ---------------------------------------------------------------------------
+// Tiles of wave `wave` of this workgroup: cyclic (tile 4b + w, then + 4 x grid)
+// or blocked (workgroup b owns the contiguous range [b per, (b + 1) per),
+// per a multiple of 4, its waves interleaved).  Which order keeps HBM busier
+// depends on the target bits; sq_program_tune picks per gate.
+struct TileRange {
+  uint64_t first, end, step;
+};
+template <bool BLK>
+__device__ __forceinline__ TileRange tile_range(uint64_t ntiles, int wave) {
+  TileRange r;
+  if (BLK) {
+    const uint64_t per = ((ntiles + gridDim.x - 1) / gridDim.x + 3) / 4 * 4;
+    r.first = (uint64_t)blockIdx.x * per + wave;
+    const uint64_t e = (uint64_t)(blockIdx.x + 1) * per;
+    r.end = e < ntiles ? e : ntiles;
+    r.step = 4;
+  } else {
+    r.first = (uint64_t)blockIdx.x * 4 + wave;
+    r.end = ntiles;
+    r.step = (uint64_t)gridDim.x * 4;
+  }
+  return r;
+}
+
 struct StageArgs {
   uint64_t a[10];  // address stride (amplitudes) of tile bit i
   uint32_t w[10];  // LDS slot weight of tile bit i (slots = XOR of weights)
@@ -466,7 +490,7 @@ __host__ __device__ constexpr uint32_t swz_c64(uint32_t x) {
 // complex128: element (c, r) at slot 16r + c, swizzled by XOR (r & 15).
 __host__ __device__ constexpr uint32_t swz_c128(uint32_t x) { return x ^ ((x >> 4) & 15u); }
 
-template <bool PIPE>
+template <bool PIPE, bool BLK>
 __global__ void __launch_bounds__(kBlock)
     k_mfma5_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
                 const float4 *__restrict__ G) {
@@ -478,9 +502,9 @@ __global__ void __launch_bounds__(kBlock)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 31, half = lane >> 5;
   float2 *X = Xs[wave];
-  uint64_t tile = (uint64_t)blockIdx.x * 4 + wave;
-  const uint64_t step = (uint64_t)gridDim.x * 4;
-  if (tile >= ntiles) return;
+  TileRange tr = tile_range<BLK>(ntiles, wave);
+  uint64_t tile = tr.first;
+  if (tile >= tr.end) return;
   // j = 2 lane + 128 k + e: lane bits are tile bits 1-6, k bits tile bits 7-9
   uint64_t la = 0;
   uint32_t lw = 0;
@@ -498,16 +522,16 @@ __global__ void __launch_bounds__(kBlock)
   float4 cur[8], nxt[8];
   uint64_t base = deposit(tile * 32, im) + la;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) cur[k] = *reinterpret_cast<const float4 *>(sv + base + SQ_KA(k));
+  for (int k = 0; k < 8; ++k) cur[k] = ld_nt(reinterpret_cast<const float4 *>(sv + base + SQ_KA(k)));
   while (true) {
-    const uint64_t ntile = tile + step;
-    const bool more = ntile < ntiles;
+    const uint64_t ntile = tile + tr.step;
+    const bool more = ntile < tr.end;
     uint64_t nbase = 0;
     if (PIPE && more) {
       nbase = deposit(ntile * 32, im) + la;
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        nxt[k] = *reinterpret_cast<const float4 *>(sv + nbase + SQ_KA(k));
+        nxt[k] = ld_nt(reinterpret_cast<const float4 *>(sv + nbase + SQ_KA(k)));
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -541,7 +565,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const float2 y0 = X[lw ^ SQ_KW(k)], y1 = X[lw ^ SQ_KW(k) ^ w0];
-      *reinterpret_cast<float4 *>(sv + base + SQ_KA(k)) = make_float4(y0.x, y0.y, y1.x, y1.y);
+      st_nt(reinterpret_cast<float4 *>(sv + base + SQ_KA(k)), make_float4(y0.x, y0.y, y1.x, y1.y));
     }
     wave_lds_sync();
     if (!more) break;
@@ -554,14 +578,14 @@ __global__ void __launch_bounds__(kBlock)
       base = deposit(tile * 32, im) + la;
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        cur[k] = *reinterpret_cast<const float4 *>(sv + base + SQ_KA(k));
+        cur[k] = ld_nt(reinterpret_cast<const float4 *>(sv + base + SQ_KA(k)));
     }
   }
 #undef SQ_KA
 #undef SQ_KW
 }
 
-template <bool PIPE>
+template <bool PIPE, bool BLK>
 __global__ void __launch_bounds__(kBlock)
     k_mfma5_c128(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
                  const typename Q4<double>::T *__restrict__ G) {
@@ -574,9 +598,9 @@ __global__ void __launch_bounds__(kBlock)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, q = lane >> 4;
   double2 *X = Xs[wave];
-  uint64_t tile = (uint64_t)blockIdx.x * 4 + wave;
-  const uint64_t step = (uint64_t)gridDim.x * 4;
-  if (tile >= ntiles) return;
+  TileRange tr = tile_range<BLK>(ntiles, wave);
+  uint64_t tile = tr.first;
+  if (tile >= tr.end) return;
   // j = lane + 64 k: lane bits are tile bits 0-5, k bits tile bits 6-8
   uint64_t la = 0;
   uint32_t lw = 0;
@@ -593,15 +617,15 @@ __global__ void __launch_bounds__(kBlock)
   double2 cur[8], nxt[8];
   uint64_t base = deposit(tile * 16, im) + la;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) cur[k] = sv[base + SQ_KA(k)];
+  for (int k = 0; k < 8; ++k) cur[k] = ld_nt(sv + base + SQ_KA(k));
   while (true) {
-    const uint64_t ntile = tile + step;
-    const bool more = ntile < ntiles;
+    const uint64_t ntile = tile + tr.step;
+    const bool more = ntile < tr.end;
     uint64_t nbase = 0;
     if (PIPE && more) {
       nbase = deposit(ntile * 16, im) + la;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) nxt[k] = sv[nbase + SQ_KA(k)];
+      for (int k = 0; k < 8; ++k) nxt[k] = ld_nt(sv + nbase + SQ_KA(k));
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) X[lw ^ SQ_KW(k)] = make_double2(cur[k].x, cur[k].y);
@@ -644,7 +668,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const double2 y = X[lw ^ SQ_KW(k)];
-      sv[base + SQ_KA(k)] = make_double2(y.x, y.y);
+      st_nt(sv + base + SQ_KA(k), make_double2(y.x, y.y));
     }
     wave_lds_sync();
     if (!more) break;
@@ -656,7 +680,7 @@ __global__ void __launch_bounds__(kBlock)
     } else {
       base = deposit(tile * 16, im) + la;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) cur[k] = sv[base + SQ_KA(k)];
+      for (int k = 0; k < 8; ++k) cur[k] = ld_nt(sv + base + SQ_KA(k));
     }
   }
 #undef SQ_KA
@@ -819,26 +843,34 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
           const uint64_t ntiles = nfree / tile_groups;
           uint64_t blocks = (ntiles + 3) / 4;
           const int di = c64 ? 0 : 1;
-          const uint64_t cap = 256ull * (uint64_t)tuning().mfma_blocks_per_cu[di];
+          const Tuning &tu = tuning();
+          const int bpc = p.shape.blocks_per_cu > 0 ? p.shape.blocks_per_cu
+                                                    : tu.mfma_blocks_per_cu[di];
+          const bool pipe = (p.shape.pipe >= 0 ? p.shape.pipe : tu.mfma5_pipeline[di]) != 0;
+          const bool blk = (p.shape.order >= 0 ? p.shape.order : tu.mfma5_order[di]) != 0;
+          const uint64_t cap = 256ull * (uint64_t)bpc;
           if (blocks > cap) blocks = cap;
-          const bool pipe = tuning().mfma5_pipeline[di] != 0;
           using Q = typename Q4<R>::T;
           if constexpr (c64) {
-            if (p.kk == 4)
+            if (p.kk == 4) {
               hipLaunchKernelGGL(k_mfma4_c64, dim3(blocks), dim3(kBlock), 0, st,
                                  (float2 *)sv, ntiles, im, off, (const Q *)gdev);
-            else
-              hipLaunchKernelGGL(pipe ? k_mfma5_c64<true> : k_mfma5_c64<false>, dim3(blocks),
-                                 dim3(kBlock), 0, st, (float2 *)sv, ntiles, im, sa,
-                                 (const Q *)gdev);
+            } else {
+              auto k = pipe ? (blk ? k_mfma5_c64<true, true> : k_mfma5_c64<true, false>)
+                            : (blk ? k_mfma5_c64<false, true> : k_mfma5_c64<false, false>);
+              hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), 0, st, (float2 *)sv, ntiles, im,
+                                 sa, (const Q *)gdev);
+            }
           } else {
-            if (p.kk == 4)
+            if (p.kk == 4) {
               hipLaunchKernelGGL(k_mfma4_c128, dim3(blocks), dim3(kBlock), 0, st,
                                  (double2 *)sv, ntiles, im, off, (const Q *)gdev);
-            else
-              hipLaunchKernelGGL(pipe ? k_mfma5_c128<true> : k_mfma5_c128<false>,
-                                 dim3(blocks), dim3(kBlock), 0, st, (double2 *)sv, ntiles,
+            } else {
+              auto k = pipe ? (blk ? k_mfma5_c128<true, true> : k_mfma5_c128<true, false>)
+                            : (blk ? k_mfma5_c128<false, true> : k_mfma5_c128<false, false>);
+              hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), 0, st, (double2 *)sv, ntiles,
                                  im, sa, (const Q *)gdev);
+            }
           }
         } else {
           using Q = typename Q4<R>::T;

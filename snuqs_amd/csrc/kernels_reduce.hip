@@ -18,6 +18,11 @@ __device__ __forceinline__ double2 ld2(const float2 *p, uint64_t i) {
 __device__ __forceinline__ double2 ld2(const double2 *p, uint64_t i) {
   return p[i];
 }
+__device__ __forceinline__ double2 ld2_nt(const float2 *p, uint64_t i) {
+  const float2 v = ld_nt(p + i);
+  return make_double2((double)v.x, (double)v.y);
+}
+__device__ __forceinline__ double2 ld2_nt(const double2 *p, uint64_t i) { return ld_nt(p + i); }
 
 // Block-wide sum of two doubles per thread; result valid in thread 0.
 __device__ __forceinline__ void block_sum2(double &a, double &b) {
@@ -47,8 +52,8 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int u = 0; u < 4; ++u, i += kBlock) {
       if (i < n) {
-        const double2 v = ld2(sv, i);
-        p[i] = v.x * v.x + v.y * v.y;
+        const double2 v = ld2_nt(sv, i);
+        st_nt(p + i, v.x * v.x + v.y * v.y);
       }
     }
   }
@@ -68,8 +73,8 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int u = 0; u < 4; ++u, i += kBlock) {
       if (i < n) {
-        const double2 v = ld2(sv, i);
-        p[i] = (v.x * v.x + v.y * v.y) / d;
+        const double2 v = ld2_nt(sv, i);
+        st_nt(p + i, (v.x * v.x + v.y * v.y) / d);
       }
     }
   }
@@ -211,12 +216,12 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int u = 0; u < 4; ++u, i += kBlock) {
       if (i < n) {
-        const T v = sv[i];
+        const T v = ld_nt(sv + i);
         const double vx = v.x, vy = v.y;
         T o;
         o.x = sr * vx - si * vy;
         o.y = sr * vy + si * vx;
-        sv[i] = o;
+        st_nt(sv + i, o);
       }
     }
   }

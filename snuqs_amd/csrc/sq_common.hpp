@@ -66,6 +66,13 @@ struct GatePlan {
   int perm[1 << kMaxK] = {0}; // PERM: source local index of each output
   uint32_t moved = 0;         // PERM: local indices whose value changes
   double touched = 0.0;       // fraction of amplitudes whose value changes
+  // Launch shape of the 5-target MFMA kernels chosen for this gate by
+  // sq_program_tune; a negative field defers to the process-wide tuning.
+  struct Shape {
+    int blocks_per_cu = -1;  // grid cap / 256 CUs
+    int order = -1;          // 0: tiles dealt cyclically, 1: contiguous per block
+    int pipe = -1;           // prefetch the next tile
+  } shape;
 };
 
 // Build the plan of `matrix` (row-major, reference local order: targets[0] is

@@ -72,12 +72,14 @@ def test_unitary_k_targets(C, dtype, k, n):
 
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
-@pytest.mark.parametrize("grid,pipe", [(256, 1), (1, 1), (1, 0)])
-def test_unitary_5_target_tiles(C, dtype, grid, pipe):
+@pytest.mark.parametrize("grid,pipe,order", [(256, 1, 0), (1, 1, 0), (1, 0, 0), (1, 1, 1),
+                                             (3, 0, 1), (1024, 1, 1)])
+def test_unitary_5_target_tiles(C, dtype, grid, pipe, order):
     """5-target MFMA kernels (LDS-staged tiles) on target sets that put the
     tile's bits everywhere: the lowest five address bits, the highest, spread
-    and given out of order; a 1-block-per-CU grid cap makes every wave loop over
-    several tiles with and without the prefetch."""
+    and given out of order, plus 4-target and controlled 4-target blocks; a
+    1-block-per-CU grid cap makes every wave loop over several tiles with and
+    without the prefetch."""
     from snuqs_amd.circuits import haar_unitary
 
     g = C.operation.gate_operations
@@ -86,14 +88,27 @@ def test_unitary_5_target_tiles(C, dtype, grid, pipe):
     init = random_state(n, rng, np.complex64 if dtype == "complex64" else np.complex128)
     ref0 = init.astype(np.complex128)
     suffix = "c64" if dtype == "complex64" else "c128"
-    keys = (f"mfma_blocks_per_cu_{suffix}", f"mfma5_pipeline_{suffix}")
+    keys = (f"mfma_blocks_per_cu_{suffix}", f"mfma5_pipeline_{suffix}",
+            f"mfma5_order_{suffix}")
     saved = [C.core.tune_get(k) for k in keys]
     try:
         C.core.tune_set(keys[0], grid)
         C.core.tune_set(keys[1], pipe)
-        for t in ([16, 17, 18, 19, 20], [0, 1, 2, 3, 4], [20, 0, 13, 6, 17],
-                  [18, 11, 5, 19, 2], [20, 18, 16, 14, 12]):
-            op = g.Unitary(t, haar_unitary(32, rng))
+        C.core.tune_set(keys[2], order)
+        ops = [g.Unitary(t, haar_unitary(32, rng))
+               for t in ([16, 17, 18, 19, 20], [0, 1, 2, 3, 4], [20, 0, 13, 6, 17],
+                         [18, 11, 5, 19, 2], [20, 18, 16, 14, 12])]
+        # 4-target blocks (widened to 5 targets with an identity factor on the
+        # lowest free bit) and controlled 4-target blocks: control on the
+        # lowest bit (not widened) and on a middle bit (widened)
+        ops += [g.Unitary(t, haar_unitary(16, rng))
+                for t in ([17, 18, 19, 20], [0, 7, 19, 3], [1, 2, 3, 20])]
+        for t in ([20, 0, 5, 11, 16], [9, 20, 19, 2, 14]):
+            m = np.eye(32, dtype=complex)
+            m[16:, 16:] = haar_unitary(16, rng)  # targets[0] is the control
+            ops.append(g.Unitary(t, m))
+        for op in ops:
+            t = list(op.targets)
             got = run_engine(C, n, [op], dtype, init)
             ref = run_oracle(n, [op], ref0)
             err = np.abs(got - ref).max()
@@ -240,6 +255,30 @@ def test_fused_layered_circuit(C, dtype):
     assert 5 in classes  # MFMA kernel used
     ms = prog.run_timed(sv)
     assert ms.shape == (len(fused),) and (ms >= 0).all()
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_program_tune_keeps_results_bitwise(C, dtype):
+    """Program.tune picks a launch shape per 5-target gate; the arithmetic is
+    the same in every shape, so the tuned program's output is bitwise equal to
+    the untuned one's."""
+    from snuqs_amd.circuits import random_layered
+    from snuqs_amd.transpile import fuse_operations
+
+    n = 21
+    fused = fuse_operations(random_layered(n, 6, seed=99), 5)
+    prog = C.functionals.Program(n, fused, dtype=dtype)
+    sv = device_state(C, n, dtype)
+    prog.run(sv)
+    before = np.asarray(sv.cpu()).copy()
+    scratch = device_state(C, n, dtype)
+    tuned = prog.tune(scratch, reps=1)
+    assert tuned == sum(1 for i in range(len(prog)) if prog.gate_info(i)[:2] == (5, 5))
+    assert tuned > 0
+    assert prog.tune(scratch, reps=1) == tuned  # second call: cached shapes
+    sv2 = device_state(C, n, dtype)
+    prog.run(sv2)
+    assert np.array_equal(np.asarray(sv2.cpu()), before)
 
 
 def test_copy_kernel(C):
