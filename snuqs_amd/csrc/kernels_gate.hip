@@ -688,6 +688,205 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------------------------
+// DENSE (1-3 targets) and PERM gates whose targets or controls sit on the
+// lowest address bits.  Lane-per-group kernels would read 8-16 B per lane at
+// a stride there (a group's amplitudes are neighbours), which wastes most of
+// every cache line each load instruction touches.  Instead a wave moves a
+// tile of 2^(k+6) amplitudes -- the k targets plus the six lowest other bit
+// positions (controls or free) -- through LDS in address order with 16 B per
+// lane (complex64: two amplitudes; every run of consecutive tile amplitudes
+// is contiguous in memory, at least 64 amplitudes long), then each of up to
+// 64 lanes applies the gate to one group in LDS.  Amplitudes whose control
+// bits do not match pass through unchanged.  LDS slots are tile-local indices
+// (address order), so the global<->LDS copies are contiguous.
+// ---------------------------------------------------------------------------
+struct SmallTile {
+  uint64_t a[9];       // address stride (amplitudes) of tile bit i
+  uint64_t fval;       // values of the controls outside the tile
+  uint32_t cw[6];      // compute phase: tile-local weight of lane bit b
+  uint32_t cfix;       // tile-local index bits of the controls inside the tile
+  uint32_t off[8];     // tile-local offset of target-local index r
+  int nact;            // lanes with a group in the compute phase
+  int shift;           // tile positions 0..shift-1: inserted by a shift
+};
+
+// Tile base: t's bits spread over the positions outside the tile and the
+// outside controls (zeros at those), shifted up past the low tile prefix.
+__device__ __forceinline__ uint64_t small_tile_base(uint64_t t, const IdxMap &im,
+                                                    const SmallTile &st) {
+  return (deposit(t, im) << st.shift) | st.fval;
+}
+
+template <typename R, int TB>
+__device__ __forceinline__ void small_tile_load(const typename CV<R>::T *sv, uint64_t base,
+                                                typename CV<R>::T *X, int lane,
+                                                const SmallTile &st) {
+  if constexpr (sizeof(R) == 4) {
+    // chunk ch = lane + 64 k holds tile amplitudes 2ch, 2ch + 1
+    constexpr int CH = 1 << (TB - 7);
+    uint64_t la = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+      if ((lane >> i) & 1) la += st.a[i + 1];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      uint64_t ka = 0;
+#pragma unroll
+      for (int b = 0; b < TB - 7; ++b)
+        if ((k >> b) & 1) ka += st.a[7 + b];
+      const float4 v = ld_nt(reinterpret_cast<const float4 *>(sv + base + la + ka));
+      *reinterpret_cast<float4 *>(X + 2 * (lane + 64 * k)) = v;
+    }
+  } else {
+    constexpr int CH = 1 << (TB - 6);
+    uint64_t la = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+      if ((lane >> i) & 1) la += st.a[i];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      uint64_t ka = 0;
+#pragma unroll
+      for (int b = 0; b < TB - 6; ++b)
+        if ((k >> b) & 1) ka += st.a[6 + b];
+      const double2 v = ld_nt(sv + base + la + ka);
+      X[lane + 64 * k] = make_double2(v.x, v.y);
+    }
+  }
+}
+
+template <typename R, int TB>
+__device__ __forceinline__ void small_tile_store(typename CV<R>::T *sv, uint64_t base,
+                                                 const typename CV<R>::T *X, int lane,
+                                                 const SmallTile &st) {
+  if constexpr (sizeof(R) == 4) {
+    constexpr int CH = 1 << (TB - 7);
+    uint64_t la = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+      if ((lane >> i) & 1) la += st.a[i + 1];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      uint64_t ka = 0;
+#pragma unroll
+      for (int b = 0; b < TB - 7; ++b)
+        if ((k >> b) & 1) ka += st.a[7 + b];
+      const float4 v = *reinterpret_cast<const float4 *>(X + 2 * (lane + 64 * k));
+      st_nt(reinterpret_cast<float4 *>(sv + base + la + ka), v);
+    }
+  } else {
+    constexpr int CH = 1 << (TB - 6);
+    uint64_t la = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+      if ((lane >> i) & 1) la += st.a[i];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      uint64_t ka = 0;
+#pragma unroll
+      for (int b = 0; b < TB - 6; ++b)
+        if ((k >> b) & 1) ka += st.a[6 + b];
+      const double2 v = X[lane + 64 * k];
+      st_nt(sv + base + la + ka, make_double2(v.x, v.y));
+    }
+  }
+}
+
+// Tile-local index of lane's group (compute phase).
+__device__ __forceinline__ uint32_t small_group(int lane, const SmallTile &st) {
+  uint32_t c = st.cfix;
+#pragma unroll
+  for (int b = 0; b < 6; ++b)
+    if ((lane >> b) & 1) c |= st.cw[b];
+  return c;
+}
+
+template <typename R, int K>
+__global__ void __launch_bounds__(kBlock)
+    k_small_dense(typename CV<R>::T *__restrict__ sv, uint64_t ntiles, IdxMap im,
+                  SmallTile st, DenseArgs<R, K> a) {
+  using T = typename CV<R>::T;
+  constexpr int TB = K + 6, D = 1 << K;
+  __shared__ T Xs[4][1 << TB];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T *X = Xs[wave];
+  const TileRange tr = tile_range<false>(ntiles, wave);
+  const uint32_t cb = small_group(lane, st);
+  for (uint64_t t = tr.first; t < tr.end; t += tr.step) {
+    const uint64_t base = small_tile_base(t, im, st);
+    small_tile_load<R, TB>(sv, base, X, lane, st);
+    wave_lds_sync();
+    if (lane < st.nact) {
+      T x[D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) x[c] = X[cb + st.off[c]];
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        R yr = a.mr[r * D] * x[0].x - a.mi[r * D] * x[0].y;
+        R yi = a.mr[r * D] * x[0].y + a.mi[r * D] * x[0].x;
+#pragma unroll
+        for (int c = 1; c < D; ++c) {
+          yr += a.mr[r * D + c] * x[c].x - a.mi[r * D + c] * x[c].y;
+          yi += a.mr[r * D + c] * x[c].y + a.mi[r * D + c] * x[c].x;
+        }
+        T o;
+        o.x = yr;
+        o.y = yi;
+        X[cb + st.off[r]] = o;
+      }
+    }
+    wave_lds_sync();
+    small_tile_store<R, TB>(sv, base, X, lane, st);
+    wave_lds_sync();
+  }
+}
+
+// PERM on the tile: out[dst[i]] = ph[i] * in[src[i]] for the moved local
+// indices; src/dst are tile-local offsets from the group's index.
+template <typename R>
+struct SmallPermArgs {
+  uint32_t src[8], dst[8];
+  R pr[8], pi[8];
+  int count;
+};
+
+template <typename R, int K>
+__global__ void __launch_bounds__(kBlock)
+    k_small_perm(typename CV<R>::T *__restrict__ sv, uint64_t ntiles, IdxMap im,
+                 SmallTile st, SmallPermArgs<R> a) {
+  using T = typename CV<R>::T;
+  constexpr int TB = K + 6, D = 1 << K;
+  __shared__ T Xs[4][1 << TB];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T *X = Xs[wave];
+  const TileRange tr = tile_range<false>(ntiles, wave);
+  const uint32_t cb = small_group(lane, st);
+  for (uint64_t t = tr.first; t < tr.end; t += tr.step) {
+    const uint64_t base = small_tile_base(t, im, st);
+    small_tile_load<R, TB>(sv, base, X, lane, st);
+    wave_lds_sync();
+    if (lane < st.nact) {
+      T v[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+        if (i < a.count) v[i] = X[cb + a.src[i]];
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        if (i < a.count) {
+          T o;
+          o.x = a.pr[i] * v[i].x - a.pi[i] * v[i].y;
+          o.y = a.pr[i] * v[i].y + a.pi[i] * v[i].x;
+          X[cb + a.dst[i]] = o;
+        }
+      }
+    }
+    wave_lds_sync();
+    small_tile_store<R, TB>(sv, base, X, lane, st);
+    wave_lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 std::vector<double> fused_block_3m(const GatePlan &p) {
@@ -747,6 +946,59 @@ static bool make_stage(const GatePlan &p, bool c64, StageArgs *sa) {
   return i == nfb + p.kk && (!c64 || sa->a[0] == 1);
 }
 
+// LDS-tiled path of a <= 3-target DENSE / PERM plan: taken when a target or
+// control sits below bit 4 (complex64) / 3 (complex128), where lane-strided
+// accesses would touch runs shorter than 128 B.  Tile = the targets plus the
+// lowest kk + 6 - kk other positions (controls there stay in the tile and
+// pass through).  False when the state is too small for it.
+static bool make_small_tile(const GatePlan &p, uint64_t nelems, bool c64, SmallTile *st,
+                            IdxMap *im, uint64_t *ntiles) {
+  if (p.kk < 1 || p.kk > 3 || nelems == 0 || (nelems & (nelems - 1))) return false;
+  const int nbits = __builtin_ctzll(nelems);
+  const uint64_t tmask = target_mask(p);
+  const int lowest = __builtin_ctzll(tmask | p.fixed_mask);
+  if (lowest >= (c64 ? 4 : 3)) return false;
+  const int TB = p.kk + 6;
+  uint64_t tile = tmask;
+  for (int b = 0; b < nbits && __builtin_popcountll(tile) < TB; ++b) tile |= 1ull << b;
+  if (__builtin_popcountll(tile) != TB) return false;
+  const uint64_t fin = p.fixed_mask & tile, fout = p.fixed_mask & ~tile;
+  const uint64_t ins = tile | fout;
+  const int nins = __builtin_popcountll(ins);
+  if (nbits < nins + 2) return false;  // at least 4 tiles
+  int m = 0;
+  while ((ins >> m) & 1) ++m;
+  if (__builtin_popcountll(ins >> m) > 8) return false;
+  *st = SmallTile{};
+  int idx[64];
+  int i = 0, f = 0;
+  for (int b = 0; b < 64; ++b) {
+    if (!((tile >> b) & 1)) continue;
+    idx[b] = i;
+    st->a[i] = 1ull << b;
+    if (!((tmask >> b) & 1)) {
+      if ((fin >> b) & 1) {
+        if ((p.fixed_val >> b) & 1) st->cfix |= 1u << i;
+      } else {
+        st->cw[f++] = 1u << i;
+      }
+    }
+    ++i;
+  }
+  st->nact = 1 << f;
+  for (int r = 0; r < (1 << p.kk); ++r) {
+    uint32_t o = 0;
+    for (int q = 0; q < p.kk; ++q)
+      if ((r >> q) & 1) o |= 1u << idx[p.tbit[q]];
+    st->off[r] = o;
+  }
+  st->shift = m;
+  st->fval = p.fixed_val & fout;
+  *im = make_idxmap(ins >> m, 0);
+  *ntiles = nelems >> nins;
+  return true;
+}
+
 static inline dim3 grid_for(uint64_t work, int per_block) {
   return sweep_grid(work, (uint64_t)per_block);
 }
@@ -784,6 +1036,28 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
     }
     case Kind::PERM: {
       if (p.kk > 3) return fail(SQ_ERR_UNSUPPORTED, "internal: PERM plan on >3 targets");
+      SmallTile stile;
+      IdxMap sim;
+      uint64_t ntiles;
+      if (make_small_tile(p, nelems, sizeof(R) == 4, &stile, &sim, &ntiles)) {
+        SmallPermArgs<R> a{};
+        for (int r = 0; r < (1 << p.kk); ++r) {
+          if (!((p.moved >> r) & 1)) continue;
+          a.src[a.count] = stile.off[p.perm[r]];
+          a.dst[a.count] = stile.off[r];
+          a.pr[a.count] = (R)p.m[r].real();
+          a.pi[a.count] = (R)p.m[r].imag();
+          ++a.count;
+        }
+        const dim3 grid = sweep_grid(ntiles, 4);
+        if (p.kk == 1)
+          hipLaunchKernelGGL((k_small_perm<R, 1>), grid, dim3(kBlock), 0, st, sv, ntiles, sim, stile, a);
+        else if (p.kk == 2)
+          hipLaunchKernelGGL((k_small_perm<R, 2>), grid, dim3(kBlock), 0, st, sv, ntiles, sim, stile, a);
+        else
+          hipLaunchKernelGGL((k_small_perm<R, 3>), grid, dim3(kBlock), 0, st, sv, ntiles, sim, stile, a);
+        break;
+      }
       constexpr int U = 2;
       const uint64_t nfree = nelems >> (nfixed + p.kk);
       IdxMap im = make_idxmap(p.fixed_mask | target_mask(p), p.fixed_val);
@@ -806,17 +1080,25 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
       IdxMap im = make_idxmap(p.fixed_mask | target_mask(p), p.fixed_val);
       const int d = 1 << p.kk;
       if (p.kk <= 3) {
-#define SQ_DENSE_CASE(KV, UV)                                               \
-  case KV: {                                                                \
-    DenseArgs<R, KV> a{};                                                   \
-    for (int i = 0; i < d * d; ++i) {                                       \
-      a.mr[i] = (R)p.m[i].real();                                           \
-      a.mi[i] = (R)p.m[i].imag();                                           \
-    }                                                                       \
-    for (int r = 0; r < d; ++r) a.off[r] = local_offset(p, r);              \
-    hipLaunchKernelGGL((k_dense<R, KV, UV>), grid_for(nfree, kBlock * UV),  \
-                       dim3(kBlock), 0, st, sv, nfree, im, a);              \
-    break;                                                                  \
+        SmallTile stile;
+        IdxMap sim;
+        uint64_t ntiles = 0;
+        const bool small = make_small_tile(p, nelems, sizeof(R) == 4, &stile, &sim, &ntiles);
+#define SQ_DENSE_CASE(KV, UV)                                                   \
+  case KV: {                                                                    \
+    DenseArgs<R, KV> a{};                                                       \
+    for (int i = 0; i < d * d; ++i) {                                           \
+      a.mr[i] = (R)p.m[i].real();                                               \
+      a.mi[i] = (R)p.m[i].imag();                                               \
+    }                                                                           \
+    for (int r = 0; r < d; ++r) a.off[r] = local_offset(p, r);                  \
+    if (small)                                                                  \
+      hipLaunchKernelGGL((k_small_dense<R, KV>), sweep_grid(ntiles, 4),         \
+                         dim3(kBlock), 0, st, sv, ntiles, sim, stile, a);       \
+    else                                                                        \
+      hipLaunchKernelGGL((k_dense<R, KV, UV>), grid_for(nfree, kBlock * UV),    \
+                         dim3(kBlock), 0, st, sv, nfree, im, a);                \
+    break;                                                                      \
   }
         switch (p.kk) {
           SQ_DENSE_CASE(1, 4)

@@ -23,14 +23,18 @@ def _kinds(C):
 
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
-@pytest.mark.parametrize("n", [5, 11])
+@pytest.mark.parametrize("n", [5, 11, 16])
 def test_every_gate_kind_every_position(C, dtype, n):
     """Each of the 33 reference gate kinds on several target sets (lowest,
-    highest and random bit positions, both target orders)."""
+    highest and random bit positions, both target orders).  Targets or
+    controls on the lowest bits take the LDS-tiled kernels at n = 11 and 16;
+    permutations stay exact moves there (checked bitwise below)."""
     from snuqs_amd.circuits import make_gate
 
     rng = np.random.default_rng(100 + n)
     init = random_state(n, rng, np.complex64 if dtype == "complex64" else np.complex128)
+    g = C.operation.gate_operations
+    perms = (g.CX, g.Swap, g.CCNot, g.CSwap, g.PauliX)
     worst = 0.0
     for kind in _kinds(C):
         k = kind[1]
@@ -43,6 +47,9 @@ def test_every_gate_kind_every_position(C, dtype, n):
             err = np.abs(got - ref).max()
             worst = max(worst, err)
             assert err <= TOL[dtype], (op, t, err)
+            if kind[0] in perms:
+                # pure permutations: every amplitude moves unchanged
+                assert np.array_equal(got, ref.astype(got.dtype)), (op, t)
     print(f"n={n} {dtype}: worst {worst:.2e}")
 
 
