@@ -512,9 +512,10 @@ int sq_program_tune(void *program, void *sv, uint64_t n, int reps, void *stream,
   // costs depends on the launches around it: timing a gate alone, launch
   // after launch, mispredicts it); every gate then keeps its fastest shape.
   std::vector<GatePlan::Shape> cands;
-  for (int order = 0; order < 2; ++order)
+  for (int order = 0; order < 2; ++order) {
     for (int bpc : {64, 256, 1024}) cands.push_back({bpc, order, 1});
-  cands.push_back({256, 0, 0});
+    for (int bpc : {256, 1024}) cands.push_back({bpc, order, 0});
+  }
   std::vector<hipEvent_t> ev(ng + 1);
   for (auto &e : ev) SQ_HIP(hipEventCreate(&e));
   std::vector<std::vector<double>> ms(cands.size(), std::vector<double>(ng, 0.0));

@@ -24,7 +24,7 @@ constexpr int kBlock = 256;  // threads per workgroup for the sweep kernels
 
 // A dispatch holds fewer than 2^32 work-items; sweep kernels are grid-stride
 // loops over at most kMaxBlocks workgroups.
-constexpr uint64_t kMaxBlocks = 1ull << 20;
+constexpr uint64_t kMaxBlocks = 1ull << 22;
 inline dim3 sweep_grid(uint64_t work, uint64_t per_block) {
   uint64_t b = (work + per_block - 1) / per_block;
   if (b > kMaxBlocks) b = kMaxBlocks;
