@@ -11,7 +11,12 @@
 //
 // Work unit: a chunk of 2^L amplitudes (L = min(12, n)); one workgroup per
 // (instance, chunk).  Qubits on bits below L ("low") are applied inside the
-// chunk in LDS; every higher bit h is one pair pass over the state.  Each
+// chunk in LDS; the higher bits go in groups of up to kQGF (forward) / kQGR
+// (reverse, which holds psi and lambda) bits per pass over
+// the state (each lane holds the 2^g amplitudes one group of bits spans).
+// Grouping is exact for the gradients too: every X_q commutes with the
+// whole mixer layer, so <lambda| X_q |psi> has the same value anywhere inside
+// it and a group's partial can be taken before the group is undone.  Each
 // launch runs the same pass for every instance that has it, so one sequence
 // of launches serves the whole batch (and is captured in one hipGraph).
 // Reductions write one FP64 partial per (pass, chunk) and are summed in a
@@ -29,12 +34,15 @@ namespace sq {
 constexpr int kQL = 12;            // log2 chunk size
 constexpr int kQChunk = 1 << kQL;  // amplitudes per chunk
 constexpr int kQMaxEdges = 128;
+constexpr int kQGF = 4;            // high bits per forward pass
+constexpr int kQGR = 3;            // high bits per reverse pass (psi and lambda in registers)
 
 struct QInstDev {
   uint64_t psi, lam;  // amplitude offsets into the batch state buffer
   uint64_t slot;      // first partial slot
   int n, nedges, edge_off, lc;  // lc = min(L, n)
-  int nchunks, pad0, pad1, pad2;
+  int nchunks, ngroups;         // ngroups = ceil((n - lc) / kQGR) reverse high passes
+  int pad1, pad2;
 };
 
 // Launch table entry: instance and its first block index in this launch.
@@ -81,10 +89,10 @@ __device__ double qblock_sum(double v) {
 }
 
 // Slot layout of instance j (nch = nchunks): energy [nch], then per layer l:
-// gamma [nch], beta [(n - lc + 1) * nch] (low pass first, then high bits).
+// gamma [nch], beta [(1 + ngroups) * nch] (low pass first, then high groups).
 __device__ __forceinline__ uint64_t slot_energy(const QInstDev &q) { return q.slot; }
 __device__ __forceinline__ uint64_t slot_gamma(const QInstDev &q, int l) {
-  return q.slot + (uint64_t)q.nchunks * (1 + (uint64_t)l * (2 + q.n - q.lc));
+  return q.slot + (uint64_t)q.nchunks * (1 + (uint64_t)l * (2 + q.ngroups));
 }
 __device__ __forceinline__ uint64_t slot_beta(const QInstDev &q, int l, int part) {
   return slot_gamma(q, l) + (uint64_t)q.nchunks * (1 + part);
@@ -122,7 +130,7 @@ __global__ void __launch_bounds__(kBlock)
       x = amp0;
       y = 0;
     } else {
-      const T v = st[q.psi + base + j];
+      const T v = ld_nt(st + q.psi + base + j);
       x = v.x;
       y = v.y;
     }
@@ -151,37 +159,51 @@ __global__ void __launch_bounds__(kBlock)
     T o;
     o.x = xr[j];
     o.y = xi[j];
-    st[q.psi + base + j] = o;
+    st_nt(st + q.psi + base + j, o);
   }
 }
 
-// ---- high pass, forward: RX(2b) on bit h -----------------------------------
-template <typename R>
+// ---- high pass, forward: RX(2b) on bits h0 .. h0+G-1 -----------------------
+// Lane t takes one group: the 2^G amplitudes f with bits h0.. set in every
+// combination (consecutive lanes: consecutive low bits, coalesced).
+template <int G>
+__device__ __forceinline__ uint64_t qgroup_base(uint64_t f, int h0) {
+  const uint64_t lo = f & ((1ull << h0) - 1);
+  return ((f ^ lo) << G) | lo;
+}
+
+template <typename R, int G>
 __global__ void __launch_bounds__(kBlock)
     k_qaoa_high_fwd(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
                     const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
-                    int count, int layer, int h) {
+                    int count, int layer, int h0) {
   using T = typename CV<R>::T;
+  constexpr int D = 1 << G;
   const int ei = find_entry(tab, count, blockIdx.x);
   const QInstDev q = inst[tab[ei].inst];
   const uint64_t chunk = blockIdx.x - tab[ei].first_block;
   double sn, cs;
   sincos(bet[tab[ei].inst * p + layer], &sn, &cs);
-  const R c0 = (R)cs, s0 = (R)sn;
-  const uint64_t hmask = (1ull << h) - 1;
-  const uint64_t npair = 1ull << (q.lc - 1);
-  for (uint64_t t = threadIdx.x; t < npair; t += kBlock) {
-    const uint64_t f = chunk * npair + t;
-    const uint64_t lo = f & hmask;
-    const uint64_t i0 = ((f ^ lo) << 1) | lo, i1 = i0 | (1ull << h);
-    const T a = st[q.psi + i0], bb = st[q.psi + i1];
-    T o0, o1;
-    o0.x = c0 * a.x + s0 * bb.y;
-    o0.y = c0 * a.y - s0 * bb.x;
-    o1.x = c0 * bb.x + s0 * a.y;
-    o1.y = c0 * bb.y - s0 * a.x;
-    st[q.psi + i0] = o0;
-    st[q.psi + i1] = o1;
+  const R c0 = (R)cs, s0 = (R)sn;  // RX(2b) = [[c, -i s], [-i s, c]]
+  const uint64_t ngrp = 1ull << (q.lc - G);  // groups per chunk-sized share
+  for (uint64_t t = threadIdx.x; t < ngrp; t += kBlock) {
+    const uint64_t i0 = qgroup_base<G>(chunk * ngrp + t, h0);
+    T a[D];
+#pragma unroll
+    for (int r = 0; r < D; ++r) a[r] = ld_nt(st + q.psi + i0 + ((uint64_t)r << h0));
+#pragma unroll
+    for (int b = 0; b < G; ++b)
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        if (r & (1 << b)) continue;
+        const T x = a[r], y = a[r | (1 << b)];
+        a[r].x = c0 * x.x + s0 * y.y;
+        a[r].y = c0 * x.y - s0 * y.x;
+        a[r | (1 << b)].x = c0 * y.x + s0 * x.y;
+        a[r | (1 << b)].y = c0 * y.y - s0 * x.x;
+      }
+#pragma unroll
+    for (int r = 0; r < D; ++r) st_nt(st + q.psi + i0 + ((uint64_t)r << h0), a[r]);
   }
 }
 
@@ -200,59 +222,78 @@ __global__ void __launch_bounds__(kBlock)
   const uint8_t *e = edges + 2 * q.edge_off;
   double acc = 0.0;
   for (int j = threadIdx.x; j < m; j += kBlock) {
-    const T v = st[q.psi + base + j];
+    const T v = ld_nt(st + q.psi + base + j);
     const int c = cut_of(base + j, e, q.nedges);
     acc += (double)c * ((double)v.x * v.x + (double)v.y * v.y);
     T o;
     o.x = (R)c * v.x;
     o.y = (R)c * v.y;
-    st[q.lam + base + j] = o;
+    st_nt(st + q.lam + base + j, o);
   }
   acc = qblock_sum(acc);
   if (threadIdx.x == 0) partial[slot_energy(q) + chunk] = acc;
 }
 
-// ---- high pass, reverse: beta partial on bit h, undo RX on psi and lambda ---
-template <typename R>
+// ---- high pass, reverse: beta partial of bits h0.., undo their RX on psi and
+// lambda --------------------------------------------------------------------
+template <typename R, int G>
 __global__ void __launch_bounds__(kBlock)
     k_qaoa_high_rev(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
                     const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
-                    int count, int layer, int h, double *__restrict__ partial) {
+                    int count, int layer, int h0, double *__restrict__ partial) {
   using T = typename CV<R>::T;
+  constexpr int D = 1 << G;
   const int ei = find_entry(tab, count, blockIdx.x);
   const QInstDev q = inst[tab[ei].inst];
   const uint64_t chunk = blockIdx.x - tab[ei].first_block;
   double sn, cs;
   sincos(bet[tab[ei].inst * p + layer], &sn, &cs);
   const R c0 = (R)cs, s0 = (R)sn;
-  const uint64_t hmask = (1ull << h) - 1;
-  const uint64_t npair = 1ull << (q.lc - 1);
+  const uint64_t ngrp = 1ull << (q.lc - G);
   double acc = 0.0;
-  for (uint64_t t = threadIdx.x; t < npair; t += kBlock) {
-    const uint64_t f = chunk * npair + t;
-    const uint64_t lo = f & hmask;
-    const uint64_t i0 = ((f ^ lo) << 1) | lo, i1 = i0 | (1ull << h);
-    const T a = st[q.psi + i0], bb = st[q.psi + i1];
-    const T la = st[q.lam + i0], lb = st[q.lam + i1];
-    // Im(conj(la) b + conj(lb) a)
-    acc += (double)la.x * bb.y - (double)la.y * bb.x + (double)lb.x * a.y - (double)lb.y * a.x;
-    // RX(-2b) = [[c, i s], [i s, c]]
-    T o0, o1, m0, m1;
-    o0.x = c0 * a.x - s0 * bb.y;
-    o0.y = c0 * a.y + s0 * bb.x;
-    o1.x = c0 * bb.x - s0 * a.y;
-    o1.y = c0 * bb.y + s0 * a.x;
-    m0.x = c0 * la.x - s0 * lb.y;
-    m0.y = c0 * la.y + s0 * lb.x;
-    m1.x = c0 * lb.x - s0 * la.y;
-    m1.y = c0 * lb.y + s0 * la.x;
-    st[q.psi + i0] = o0;
-    st[q.psi + i1] = o1;
-    st[q.lam + i0] = m0;
-    st[q.lam + i1] = m1;
+  for (uint64_t t = threadIdx.x; t < ngrp; t += kBlock) {
+    const uint64_t i0 = qgroup_base<G>(chunk * ngrp + t, h0);
+    T a[D], l[D];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      a[r] = ld_nt(st + q.psi + i0 + ((uint64_t)r << h0));
+      l[r] = ld_nt(st + q.lam + i0 + ((uint64_t)r << h0));
+    }
+    // sum over the group's bits b and pairs (r, r^b): Im(conj(l_r) a_{r^b} + conj(l_{r^b}) a_r)
+#pragma unroll
+    for (int b = 0; b < G; ++b)
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        if (r & (1 << b)) continue;
+        const int u = r | (1 << b);
+        acc += (double)l[r].x * a[u].y - (double)l[r].y * a[u].x + (double)l[u].x * a[r].y -
+               (double)l[u].y * a[r].x;
+      }
+    // RX(-2b) = [[c, i s], [i s, c]] on every bit of the group
+#pragma unroll
+    for (int b = 0; b < G; ++b)
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        if (r & (1 << b)) continue;
+        const int u = r | (1 << b);
+        const T x = a[r], y = a[u], v = l[r], w = l[u];
+        a[r].x = c0 * x.x - s0 * y.y;
+        a[r].y = c0 * x.y + s0 * y.x;
+        a[u].x = c0 * y.x - s0 * x.y;
+        a[u].y = c0 * y.y + s0 * x.x;
+        l[r].x = c0 * v.x - s0 * w.y;
+        l[r].y = c0 * v.y + s0 * w.x;
+        l[u].x = c0 * w.x - s0 * v.y;
+        l[u].y = c0 * w.y + s0 * v.x;
+      }
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      st_nt(st + q.psi + i0 + ((uint64_t)r << h0), a[r]);
+      st_nt(st + q.lam + i0 + ((uint64_t)r << h0), l[r]);
+    }
   }
   acc = qblock_sum(acc);
-  if (threadIdx.x == 0) partial[slot_beta(q, layer, 1 + (h - q.lc)) + chunk] = acc;
+  if (threadIdx.x == 0) partial[slot_beta(q, layer, 1 + (h0 - q.lc) / kQGR) + chunk] = acc;
 }
 
 // ---- low pass, reverse: beta partials and undo RX on low bits, gamma partial
@@ -280,7 +321,7 @@ __global__ void __launch_bounds__(kBlock)
     ph_i[c] = (R)sn;
   }
   for (int j = threadIdx.x; j < m; j += kBlock) {
-    const T v = st[q.psi + base + j], w = st[q.lam + base + j];
+    const T v = ld_nt(st + q.psi + base + j), w = ld_nt(st + q.lam + base + j);
     xr[j] = v.x;
     xi[j] = v.y;
     yr[j] = w.x;
@@ -323,8 +364,8 @@ __global__ void __launch_bounds__(kBlock)
     o.y = y;
     l.x = u;
     l.y = w;
-    st[q.psi + base + j] = o;
-    st[q.lam + base + j] = l;
+    st_nt(st + q.psi + base + j, o);
+    st_nt(st + q.lam + base + j, l);
   }
   accb = qblock_sum(accb);
   if (threadIdx.x == 0) partial[slot_beta(q, layer, 0) + chunk] = accb;
@@ -353,7 +394,7 @@ __global__ void __launch_bounds__(kBlock)
       scale = 2.0;
     } else {
       first = slot_beta(q, v - 1 - p, 0);
-      len = nch * (uint64_t)(q.n - q.lc + 1);
+      len = nch * (uint64_t)(1 + q.ngroups);
       scale = 2.0;
     }
     double acc = 0.0;
@@ -378,7 +419,8 @@ struct QaoaBatch {
   double *out = nullptr;
   QEntry *tables = nullptr;
   struct Launch {
-    int kind, layer, bit;  // kind: 0 low_fwd, 1 high_fwd, 2 energy, 3 high_rev, 4 low_rev, 5 fin
+    int kind, layer, bit;  // kind: 0 low_fwd, 1 high_fwd, 2 energy, 3 high_rev, 4 low_rev
+    int nb;                // high passes: bits bit .. bit + nb - 1
     size_t tab_off;
     int count;
     uint64_t blocks;
@@ -417,6 +459,7 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
     q.n = n;
     q.lc = std::min(n, kQL);
     q.nchunks = 1 << (n - q.lc);
+    q.ngroups = (n - q.lc + kQGR - 1) / kQGR;
     q.nedges = (int)nedges[j];
     q.edge_off = (int)(hedges.size() / 2);
     for (uint32_t k = 0; k < nedges[j]; ++k) {
@@ -431,7 +474,7 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
     q.lam = amp + (1ull << n);
     amp += 2ull << n;
     q.slot = slot;
-    slot += (uint64_t)q.nchunks * (1 + (uint64_t)p * (2 + n - q.lc));
+    slot += (uint64_t)q.nchunks * (1 + (uint64_t)p * (2 + q.ngroups));
     B->host_inst.push_back(q);
     B->maxn = std::max(B->maxn, n);
     B->state_bytes += 2.0 * std::ldexp(1.0, n) * item;
@@ -454,8 +497,8 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
   if (!rc) rc = dmalloc((void **)&B->out, ninst * (1 + 2 * p) * sizeof(double));
   // launch sequence and tables
   std::vector<QEntry> tabs;
-  auto add = [&](int kind, int layer, int bit, auto pred) {
-    QaoaBatch::Launch L{kind, layer, bit, tabs.size(), 0, 0};
+  auto add = [&](int kind, int layer, int bit, int nb, auto pred) {
+    QaoaBatch::Launch L{kind, layer, bit, nb, tabs.size(), 0, 0};
     for (int j = 0; j < (int)ninst; ++j) {
       const QInstDev &q = B->host_inst[j];
       if (!pred(q)) continue;
@@ -466,16 +509,23 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
     if (L.count) B->launches.push_back(L);
   };
   auto all = [](const QInstDev &) { return true; };
+  // high group of up to gs bits starting at h0 with nb bits: the instances
+  // whose group there has exactly nb bits (a smaller last group for
+  // n - h0 < gs)
+  auto has_group = [](int h0, int nb, int gs) {
+    return [h0, nb, gs](const QInstDev &q) { return q.n > h0 && std::min(gs, q.n - h0) == nb; };
+  };
   for (int l = 0; l < (int)p; ++l) {
-    add(0, l, 0, all);
-    for (int h = kQL; h < B->maxn; ++h)
-      add(1, l, h, [h](const QInstDev &q) { return q.n > h; });
+    add(0, l, 0, 0, all);
+    for (int h0 = kQL; h0 < B->maxn; h0 += kQGF)
+      for (int nb = kQGF; nb >= 1; --nb) add(1, l, h0, nb, has_group(h0, nb, kQGF));
   }
-  add(2, 0, 0, all);
+  add(2, 0, 0, 0, all);
+  const int last = kQL + ((B->maxn - kQL - 1) / kQGR) * kQGR;
   for (int l = (int)p - 1; l >= 0; --l) {
-    for (int h = B->maxn - 1; h >= kQL; --h)
-      add(3, l, h, [h](const QInstDev &q) { return q.n > h; });
-    add(4, l, 0, all);
+    for (int h0 = last; h0 >= kQL; h0 -= kQGR)
+      for (int nb = kQGR; nb >= 1; --nb) add(3, l, h0, nb, has_group(h0, nb, kQGR));
+    add(4, l, 0, 0, all);
   }
   if (!rc) rc = dmalloc((void **)&B->tables, tabs.size() * sizeof(QEntry));
   if (rc) {
@@ -516,18 +566,24 @@ static int enqueue_t(QaoaBatch *B, hipStream_t st) {
         hipLaunchKernelGGL(k_qaoa_low_fwd<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet, B->p,
                            tab, L.count, L.layer);
         break;
-      case 1:
-        hipLaunchKernelGGL(k_qaoa_high_fwd<R>, g, bl, 0, st, s, B->inst, bet, B->p, tab, L.count,
-                           L.layer, L.bit);
+      case 1: {
+        auto k = L.nb == 4 ? k_qaoa_high_fwd<R, 4>
+                 : L.nb == 3 ? k_qaoa_high_fwd<R, 3>
+                 : L.nb == 2 ? k_qaoa_high_fwd<R, 2> : k_qaoa_high_fwd<R, 1>;
+        hipLaunchKernelGGL(k, g, bl, 0, st, s, B->inst, bet, B->p, tab, L.count, L.layer, L.bit);
         break;
+      }
       case 2:
         hipLaunchKernelGGL(k_qaoa_energy<R>, g, bl, 0, st, s, B->inst, B->edges, tab, L.count,
                            B->partial);
         break;
-      case 3:
-        hipLaunchKernelGGL(k_qaoa_high_rev<R>, g, bl, 0, st, s, B->inst, bet, B->p, tab, L.count,
-                           L.layer, L.bit, B->partial);
+      case 3: {
+        auto k = L.nb == 3 ? k_qaoa_high_rev<R, 3>
+                 : L.nb == 2 ? k_qaoa_high_rev<R, 2> : k_qaoa_high_rev<R, 1>;
+        hipLaunchKernelGGL(k, g, bl, 0, st, s, B->inst, bet, B->p, tab, L.count, L.layer, L.bit,
+                           B->partial);
         break;
+      }
       case 4:
         hipLaunchKernelGGL(k_qaoa_low_rev<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet, B->p,
                            tab, L.count, L.layer, B->partial);

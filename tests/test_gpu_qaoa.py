@@ -22,11 +22,23 @@ def oracle_energy(n, edges, gam, bet):
     return e
 
 
-def _batch(sizes, p, seed):
+def _graph(n, rng):
+    """Seeded 3-regular graph for even n; a ring plus n // 2 random chords for
+    odd n (no 3-regular graph exists on an odd vertex count)."""
     from snuqs_amd.circuits import random_3_regular
 
+    if n % 2 == 0:
+        return random_3_regular(n, rng)
+    edges = {(i, (i + 1) % n) if i < (i + 1) % n else ((i + 1) % n, i) for i in range(n)}
+    while len(edges) < n + n // 2:
+        a, b = sorted(int(x) for x in rng.choice(n, 2, replace=False))
+        edges.add((a, b))
+    return sorted(edges)
+
+
+def _batch(sizes, p, seed):
     rng = np.random.default_rng(seed)
-    graphs = [random_3_regular(n, rng) for n in sizes]
+    graphs = [_graph(n, rng) for n in sizes]
     gam = rng.uniform(0, np.pi, (len(sizes), p))
     bet = rng.uniform(0, np.pi / 2, (len(sizes), p))
     return graphs, gam, bet
@@ -37,7 +49,7 @@ def _batch(sizes, p, seed):
 def test_qaoa_batch_energies_and_gradients(C, dtype, etol, gtol):
     from snuqs_amd.qaoa import qaoa_batch
 
-    sizes = [4, 6, 12, 8, 14, 6, 16]  # below, at and above the 2^12 chunk
+    sizes = [4, 6, 12, 8, 14, 6, 16, 13, 15, 17]  # below, at and above the 2^12 chunk
     p = 3
     graphs, gam, bet = _batch(sizes, p, 1)
     batch = qaoa_batch(sizes, graphs, p, dtype=dtype)
@@ -66,11 +78,13 @@ def test_qaoa_batch_energies_and_gradients(C, dtype, etol, gtol):
 
 def test_qaoa_gradients_match_general_adjoint(C):
     """The batch kernels against the general engine's adjoint on the same
-    circuit (ZZ(2 gamma): dE/dgamma = 2 * sum of the ZZ-angle derivatives)."""
+    circuit (ZZ(2 gamma): dE/dgamma = 2 * sum of the ZZ-angle derivatives).
+    Sizes 13-20 give every high-bit group size of the forward (1-4 bits) and
+    reverse (1-3 bits) passes, and instances that stop inside a group."""
     from snuqs_amd.circuits import qaoa_maxcut
     from snuqs_amd.qaoa import qaoa_batch
 
-    sizes = [14, 10]
+    sizes = [14, 10, 17, 19, 13, 15, 20]
     p = 2
     graphs, gam, bet = _batch(sizes, p, 5)
     out = qaoa_batch(sizes, graphs, p, dtype="complex128").run(gam, bet, use_graph=False)
