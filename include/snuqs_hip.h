@@ -167,6 +167,14 @@ int sq_probabilities(const void *sv, sq_dtype_t dtype, uint64_t num_elems,
 int sq_probabilities_normalized(const void *sv, sq_dtype_t dtype,
                                 uint64_t num_elems, double *probs,
                                 void *stream);
+/* Marginal distribution over k (1..20) target qubits: out[r] (host, 2^k
+ * doubles) = sum of |sv[i]|^2 over the i whose target qubits read r, targets[0]
+ * the most significant bit of r (Braket's Probability result type with
+ * targets; the reference returns only the state vector,
+ * simulator.py:578-580).  FP64, fixed summation order. */
+int sq_marginal_probabilities(const void *sv, sq_dtype_t dtype, uint64_t num_elems,
+                              uint32_t num_qubits, const uint32_t *targets,
+                              uint32_t num_targets, double *out, void *stream);
 /* sum_i |sv[i]|^2 in FP64; returns a host value. */
 int sq_norm2(const void *sv, sq_dtype_t dtype, uint64_t num_elems,
              double *out, void *stream);

@@ -589,6 +589,40 @@ int sq_probabilities_normalized(const void *sv, sq_dtype_t dt, uint64_t n, doubl
   return SQ_OK;
 }
 
+int sq_marginal_probabilities(const void *sv, sq_dtype_t dt, uint64_t n, uint32_t num_qubits,
+                              const uint32_t *targets, uint32_t k, double *out, void *stream) {
+  SQ_TRY(check_state(sv, dt, n));
+  if (!out || (k && !targets)) return fail(SQ_ERR_INVALID, "null output or targets");
+  if (k < 1 || k > (uint32_t)kMaxMarginal)
+    return fail(SQ_ERR_INVALID, "marginal over 1.." + std::to_string(kMaxMarginal) +
+                                    " targets only (use sq_probabilities for the full distribution)");
+  const int nbits = log2u(n);
+  int tbit[kMaxMarginal];
+  uint64_t seen = 0;
+  for (uint32_t j = 0; j < k; ++j) {
+    if (targets[j] >= num_qubits) return fail(SQ_ERR_INVALID, "target out of range");
+    tbit[j] = (int)num_qubits - 1 - (int)targets[j];
+    if (tbit[j] >= nbits) return fail(SQ_ERR_INVALID, "target stride beyond the state");
+    if ((seen >> tbit[j]) & 1) return fail(SQ_ERR_INVALID, "duplicate target");
+    seen |= 1ull << tbit[j];
+  }
+  const int nb = std::max(1, std::min(kRedBlocks, 65536 >> k));
+  const uint64_t nout = 1ull << k;
+  int dev;
+  SQ_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mutex[dev]);
+  Workspace *w;
+  SQ_TRY(ws_reserve(dev, (nout * nb + nout) * sizeof(double), nout * sizeof(double), &w));
+  hipStream_t st = (hipStream_t)stream;
+  double *res = (double *)w->dev;
+  double *partial = res + nout;
+  SQ_TRY(launch_marginal(sv, dt, n, nbits, tbit, (int)k, partial, nb, res, st));
+  SQ_HIP(hipMemcpyAsync(w->host, res, nout * sizeof(double), hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  std::memcpy(out, w->host, nout * sizeof(double));
+  return SQ_OK;
+}
+
 int sq_norm2(const void *sv, sq_dtype_t dt, uint64_t n, double *out, void *stream) {
   SQ_TRY(check_state(sv, dt, n));
   int dev;

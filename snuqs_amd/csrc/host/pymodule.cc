@@ -703,6 +703,22 @@ PYBIND11_MODULE(_C, m) {
       throw py::value_error("copy_kernel: states differ in size or dtype");
     check(sq_copy(dst.ptr(), src.ptr(), dst.bytes(), sraw(s)));
   }, py::arg("dst"), py::arg("src"), py::kw_only(), py::arg("stream") = nullptr);
+  m_fn.def("marginal_probabilities", [](StateVector &sv, std::vector<uint32_t> targets,
+                                        std::shared_ptr<Stream> s) {
+    sv.require_device("marginal_probabilities");
+    if (targets.empty() || targets.size() > 20)
+      throw py::value_error("marginal_probabilities takes 1..20 targets");
+    py::array_t<double> out((size_t)1 << targets.size());
+    double *o = out.mutable_data();
+    {
+      py::gil_scoped_release nogil;
+      check(sq_marginal_probabilities(sv.ptr(), sv.dtype(), sv.num_elems(),
+                                      (uint32_t)sv.num_qubits(), targets.data(),
+                                      (uint32_t)targets.size(), o, sraw(s)));
+    }
+    return out;
+  }, py::arg("state_vector"), py::arg("targets"), py::kw_only(), py::arg("stream") = nullptr,
+     "Marginal distribution over `targets` (targets[0] = most significant bit).");
   m_fn.def("norm2", [](StateVector &sv, std::shared_ptr<Stream> s) {
     sv.require_device("norm2");
     double out;

@@ -98,7 +98,177 @@ __device__ __forceinline__ uint64_t slot_beta(const QInstDev &q, int l, int part
   return slot_gamma(q, l) + (uint64_t)q.nchunks * (1 + part);
 }
 
-// ---- low pass, forward: (init |+>) cost(l), RX on every low bit ------------
+// ---- full 4096-amplitude chunks (n >= 12): register phases ----------------
+// Thread t holds 16 amplitudes of the chunk in one of three layouts, each
+// putting 4 index bits in the register index r:
+//   L2: j = t + 256 r (bits 8-11 in r; the global load / store layout, coalesced)
+//   L1: j = (t & 15) | r << 4 | (t >> 4) << 8      (bits 4-7 in r)
+//   L0: j = r | t << 4                             (bits 0-3 in r)
+// RX on the r bits is applied in registers; layouts change through LDS with
+// slot swz(j), which keeps every one of the three access patterns free of
+// bank conflicts for 8 B and 16 B elements.
+__device__ __forceinline__ int qswz(int j) { return j ^ ((j >> 5) & 15) ^ (((j >> 8) & 1) << 4); }
+__device__ __forceinline__ int qlay(int layout, int t, int r) {
+  return layout == 2 ? t + 256 * r
+         : layout == 1 ? (t & 15) | (r << 4) | ((t >> 4) << 8)
+                       : r | (t << 4);
+}
+
+// Own 16 slots out in layout `from`, back in layout `to` (one barrier; the
+// caller separates successive uses of X by a barrier).
+template <typename T>
+__device__ __forceinline__ void qtranspose(T (&v)[16], T *X, int t, int from, int to) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) X[qswz(qlay(from, t, r))] = v[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = X[qswz(qlay(to, t, r))];
+}
+
+// RX(2b) (dir = +1) or RX(-2b) (dir = -1) on the 4 register-index bits.
+template <typename T, typename R>
+__device__ __forceinline__ void qrx16(T (&v)[16], R c0, R s0) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      if (r & (1 << b)) continue;
+      const T x = v[r], y = v[r | (1 << b)];
+      v[r].x = c0 * x.x + s0 * y.y;
+      v[r].y = c0 * x.y - s0 * y.x;
+      v[r | (1 << b)].x = c0 * y.x + s0 * x.y;
+      v[r | (1 << b)].y = c0 * y.y - s0 * x.x;
+    }
+}
+
+// sum over the 4 register bits b and pairs (r, r^b) of
+// Im(conj(l_r) a_{r^b} + conj(l_{r^b}) a_r)  (= Im <l| X_b |a> over the group),
+// products and sums in FP64 as two dependent FMA chains (few live registers)
+template <typename T>
+__device__ __forceinline__ double qxpart16(const T (&a)[16], const T (&l)[16]) {
+  double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      if (r & (1 << b)) continue;
+      const int u = r | (1 << b);
+      s0 = fma((double)l[r].x, (double)a[u].y, s0);
+      s0 = fma(-(double)l[r].y, (double)a[u].x, s0);
+      s1 = fma((double)l[u].x, (double)a[r].y, s1);
+      s1 = fma(-(double)l[u].y, (double)a[r].x, s1);
+    }
+  return s0 + s1;
+}
+
+// Cut counts of a full chunk at `base` for thread t's amplitudes j = t + 256 r:
+// cut = K + sum_k u_k r_k + rint[r], where the edges touching no bit 8-11
+// give K, an edge from bit 8+k to a known bit (of t or of base) gives u_k,
+// and rint[16] (shared) holds the edges inside bits 8-11.
+struct QCut {
+  int K, u0, u1, u2, u3;
+};
+
+__device__ __forceinline__ int qbit(uint64_t x, int b) { return (int)((x >> b) & 1); }
+
+__device__ __forceinline__ QCut qcut_setup(uint64_t base, int t, const uint8_t *e, int ne,
+                                           int *rint) {
+  QCut c{0, 0, 0, 0, 0};
+  for (int k = 0; k < ne; ++k) {
+    const int a = e[2 * k], b = e[2 * k + 1];
+    const bool ra = a >= 8 && a < 12, rb = b >= 8 && b < 12;
+    if (ra && rb) continue;
+    const int va = a < 8 ? qbit((uint64_t)t, a) : qbit(base, a);
+    const int vb = b < 8 ? qbit((uint64_t)t, b) : qbit(base, b);
+    if (!ra && !rb) {
+      c.K += va ^ vb;
+    } else {
+      const int kk = (ra ? a : b) - 8, v = ra ? vb : va, d = 1 - 2 * v;
+      c.K += v;
+      c.u0 += kk == 0 ? d : 0;
+      c.u1 += kk == 1 ? d : 0;
+      c.u2 += kk == 2 ? d : 0;
+      c.u3 += kk == 3 ? d : 0;
+    }
+  }
+  if (t < 16) {
+    int ci = 0;
+    for (int k = 0; k < ne; ++k) {
+      const int a = e[2 * k], b = e[2 * k + 1];
+      if (a >= 8 && a < 12 && b >= 8 && b < 12) ci += qbit((uint64_t)t, a - 8) ^ qbit((uint64_t)t, b - 8);
+    }
+    rint[t] = ci;
+  }
+  return c;
+}
+
+__device__ __forceinline__ int qcut_at(const QCut &c, const int *rint, int r) {
+  return c.K + (r & 1) * c.u0 + ((r >> 1) & 1) * c.u1 + ((r >> 2) & 1) * c.u2 +
+         ((r >> 3) & 1) * c.u3 + rint[r];
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_qaoa_low_fwd_full(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
+                        const uint8_t *__restrict__ edges, const double *__restrict__ gam,
+                        const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
+                        int count, int layer) {
+  using T = typename CV<R>::T;
+  __shared__ T X[kQChunk];
+  __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
+  __shared__ int rint[16];
+  const int ei = find_entry(tab, count, blockIdx.x);
+  const QInstDev q = inst[tab[ei].inst];
+  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  const uint64_t base = chunk << kQL;
+  const uint8_t *e = edges + 2 * q.edge_off;
+  const double g = gam[tab[ei].inst * p + layer], b = bet[tab[ei].inst * p + layer];
+  const int t = threadIdx.x;
+  for (int c = t; c <= q.nedges; c += kBlock) {
+    double sn, cs;
+    sincos(-g * (double)(q.nedges - 2 * c), &sn, &cs);
+    ph_r[c] = (R)cs;
+    ph_i[c] = (R)sn;
+  }
+  const QCut cc = qcut_setup(base, t, e, q.nedges, rint);
+  __syncthreads();
+  T v[16];
+  const R amp0 = (R)(1.0 / sqrt((double)(1ull << q.n)));
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    R x, y;
+    if (layer == 0) {
+      x = amp0;
+      y = 0;
+    } else {
+      const T w = ld_nt(st + q.psi + base + t + 256 * r);
+      x = w.x;
+      y = w.y;
+    }
+    const int c = qcut_at(cc, rint, r);
+    cmul(x, y, ph_r[c], ph_i[c]);
+    v[r].x = x;
+    v[r].y = y;
+  }
+  double sn, cs;
+  sincos(b, &sn, &cs);
+  const R c0 = (R)cs, s0 = (R)sn;  // RX(2b) = [[c, -i s], [-i s, c]]
+  qrx16(v, c0, s0);                // bits 8-11
+  qtranspose(v, X, t, 2, 1);
+  qrx16(v, c0, s0);                // bits 4-7
+#pragma unroll
+  for (int r = 0; r < 16; ++r) X[qswz(qlay(1, t, r))] = v[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = X[qswz(qlay(0, t, r))];
+  qrx16(v, c0, s0);                // bits 0-3
+  qtranspose(v, X, t, 0, 2);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) st_nt(st + q.psi + base + t + 256 * r, v[r]);
+}
+
+// ---- low pass, forward, chunks below 2^12 (n < 12): (init |+>) cost(l), RX on
+// every bit, butterflies in LDS --------------------------------------------
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
     k_qaoa_low_fwd(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
@@ -106,7 +276,7 @@ __global__ void __launch_bounds__(kBlock)
                    const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
                    int count, int layer) {
   using T = typename CV<R>::T;
-  __shared__ R xr[kQChunk], xi[kQChunk];
+  __shared__ R xr[kQChunk / 2], xi[kQChunk / 2];  // chunks below 2^12 (n < 12)
   __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
   const int ei = find_entry(tab, count, blockIdx.x);
   const QInstDev q = inst[tab[ei].inst];
@@ -214,16 +384,21 @@ __global__ void __launch_bounds__(kBlock)
                   const uint8_t *__restrict__ edges, const QEntry *__restrict__ tab, int count,
                   double *__restrict__ partial) {
   using T = typename CV<R>::T;
+  __shared__ int rint[16];
   const int ei = find_entry(tab, count, blockIdx.x);
   const QInstDev q = inst[tab[ei].inst];
   const uint64_t chunk = blockIdx.x - tab[ei].first_block;
   const int m = 1 << q.lc;
   const uint64_t base = chunk << q.lc;
   const uint8_t *e = edges + 2 * q.edge_off;
+  const bool full = q.lc == kQL;  // uniform per workgroup
+  QCut cc{0, 0, 0, 0, 0};
+  if (full) cc = qcut_setup(base, threadIdx.x, e, q.nedges, rint);
+  __syncthreads();
   double acc = 0.0;
   for (int j = threadIdx.x; j < m; j += kBlock) {
     const T v = ld_nt(st + q.psi + base + j);
-    const int c = cut_of(base + j, e, q.nedges);
+    const int c = full ? qcut_at(cc, rint, j >> 8) : cut_of(base + j, e, q.nedges);
     acc += (double)c * ((double)v.x * v.x + (double)v.y * v.y);
     T o;
     o.x = (R)c * v.x;
@@ -296,7 +471,86 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x == 0) partial[slot_beta(q, layer, 1 + (h0 - q.lc) / kQGR) + chunk] = acc;
 }
 
-// ---- low pass, reverse: beta partials and undo RX on low bits, gamma partial
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_qaoa_low_rev_full(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
+                        const uint8_t *__restrict__ edges, const double *__restrict__ gam,
+                        const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
+                        int count, int layer, double *__restrict__ partial) {
+  using T = typename CV<R>::T;
+  __shared__ T X[kQChunk];
+  __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
+  __shared__ int rint[16];
+  const int ei = find_entry(tab, count, blockIdx.x);
+  const QInstDev q = inst[tab[ei].inst];
+  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  const uint64_t base = chunk << kQL;
+  const uint8_t *e = edges + 2 * q.edge_off;
+  const double g = gam[tab[ei].inst * p + layer], b = bet[tab[ei].inst * p + layer];
+  const int t = threadIdx.x;
+  for (int c = t; c <= q.nedges; c += kBlock) {
+    double sn, cs;
+    sincos(g * (double)(q.nedges - 2 * c), &sn, &cs);  // undo: exp(+i g s)
+    ph_r[c] = (R)cs;
+    ph_i[c] = (R)sn;
+  }
+  const QCut cc = qcut_setup(base, t, e, q.nedges, rint);
+  T a[16], l[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    a[r] = ld_nt(st + q.psi + base + t + 256 * r);
+    l[r] = ld_nt(st + q.lam + base + t + 256 * r);
+  }
+  double sn, cs;
+  sincos(b, &sn, &cs);
+  const R c0 = (R)cs, s0 = -(R)sn;  // RX(-2b)
+  // beta partial (any point inside the mixer layer) and undo, 4 bits at a time
+  double accb = qxpart16(a, l);  // bits 8-11
+  qrx16(a, c0, s0);
+  qrx16(l, c0, s0);
+  qtranspose(a, X, t, 2, 1);
+  __syncthreads();
+  qtranspose(l, X, t, 2, 1);
+  accb += qxpart16(a, l);  // bits 4-7
+  qrx16(a, c0, s0);
+  qrx16(l, c0, s0);
+  __syncthreads();
+  qtranspose(a, X, t, 1, 0);
+  __syncthreads();
+  qtranspose(l, X, t, 1, 0);
+  accb += qxpart16(a, l);  // bits 0-3
+  qrx16(a, c0, s0);
+  qrx16(l, c0, s0);
+  __syncthreads();
+  qtranspose(a, X, t, 0, 2);
+  __syncthreads();
+  qtranspose(l, X, t, 0, 2);
+  // gamma partial Im(conj(lambda) s psi) and undo the cost layer
+  double accg = 0.0;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int c = qcut_at(cc, rint, r);
+    const double sv = (double)(q.nedges - 2 * c);
+    R x = a[r].x, y = a[r].y, u = l[r].x, w = l[r].y;
+    accg += sv * ((double)u * y - (double)w * x);
+    cmul(x, y, ph_r[c], ph_i[c]);
+    cmul(u, w, ph_r[c], ph_i[c]);
+    T o, m;
+    o.x = x;
+    o.y = y;
+    m.x = u;
+    m.y = w;
+    st_nt(st + q.psi + base + t + 256 * r, o);
+    st_nt(st + q.lam + base + t + 256 * r, m);
+  }
+  accb = qblock_sum(accb);
+  if (t == 0) partial[slot_beta(q, layer, 0) + chunk] = accb;
+  accg = qblock_sum(accg);
+  if (t == 0) partial[slot_gamma(q, layer) + chunk] = accg;
+}
+
+// ---- low pass, reverse, chunks below 2^12 (n < 12): beta partials and undo RX
+// on the bits, gamma partial
 // and undo cost, on psi and lambda --------------------------------------------
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
@@ -305,7 +559,7 @@ __global__ void __launch_bounds__(kBlock)
                    const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
                    int count, int layer, double *__restrict__ partial) {
   using T = typename CV<R>::T;
-  __shared__ R xr[kQChunk], xi[kQChunk], yr[kQChunk], yi[kQChunk];
+  __shared__ R xr[kQChunk / 2], xi[kQChunk / 2], yr[kQChunk / 2], yi[kQChunk / 2];  // n < 12
   __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
   const int ei = find_entry(tab, count, blockIdx.x);
   const QInstDev q = inst[tab[ei].inst];
@@ -515,8 +769,11 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
   auto has_group = [](int h0, int nb, int gs) {
     return [h0, nb, gs](const QInstDev &q) { return q.n > h0 && std::min(gs, q.n - h0) == nb; };
   };
+  auto full = [](const QInstDev &q) { return q.lc == kQL; };
+  auto small = [](const QInstDev &q) { return q.lc < kQL; };
   for (int l = 0; l < (int)p; ++l) {
-    add(0, l, 0, 0, all);
+    add(0, l, 0, 1, full);  // nb = 1: full-chunk kernel
+    add(0, l, 0, 0, small);
     for (int h0 = kQL; h0 < B->maxn; h0 += kQGF)
       for (int nb = kQGF; nb >= 1; --nb) add(1, l, h0, nb, has_group(h0, nb, kQGF));
   }
@@ -525,7 +782,8 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
   for (int l = (int)p - 1; l >= 0; --l) {
     for (int h0 = last; h0 >= kQL; h0 -= kQGR)
       for (int nb = kQGR; nb >= 1; --nb) add(3, l, h0, nb, has_group(h0, nb, kQGR));
-    add(4, l, 0, 0, all);
+    add(4, l, 0, 1, full);
+    add(4, l, 0, 0, small);
   }
   if (!rc) rc = dmalloc((void **)&B->tables, tabs.size() * sizeof(QEntry));
   if (rc) {
@@ -563,8 +821,12 @@ static int enqueue_t(QaoaBatch *B, hipStream_t st) {
     const dim3 g((unsigned)L.blocks), bl(kBlock);
     switch (L.kind) {
       case 0:
-        hipLaunchKernelGGL(k_qaoa_low_fwd<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet, B->p,
-                           tab, L.count, L.layer);
+        if (L.nb)
+          hipLaunchKernelGGL(k_qaoa_low_fwd_full<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet,
+                             B->p, tab, L.count, L.layer);
+        else
+          hipLaunchKernelGGL(k_qaoa_low_fwd<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet,
+                             B->p, tab, L.count, L.layer);
         break;
       case 1: {
         auto k = L.nb == 4 ? k_qaoa_high_fwd<R, 4>
@@ -585,8 +847,12 @@ static int enqueue_t(QaoaBatch *B, hipStream_t st) {
         break;
       }
       case 4:
-        hipLaunchKernelGGL(k_qaoa_low_rev<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet, B->p,
-                           tab, L.count, L.layer, B->partial);
+        if (L.nb)
+          hipLaunchKernelGGL(k_qaoa_low_rev_full<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet,
+                             B->p, tab, L.count, L.layer, B->partial);
+        else
+          hipLaunchKernelGGL(k_qaoa_low_rev<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet,
+                             B->p, tab, L.count, L.layer, B->partial);
         break;
     }
   }

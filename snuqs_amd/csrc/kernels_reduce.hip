@@ -189,6 +189,45 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// ---- marginal distribution over k targets ------------------------------------
+// out[r] = sum of |sv[i]|^2 over the i whose target bits spell r (target 0
+// the most significant bit of r).  Block b sums outcome r = b / nb over the
+// free indices f = b % nb (mod nb); partials are combined by k_finalize in a
+// fixed order (deterministic).  Free index -> address: the IdxMap when at
+// most 8 target bits are inserted, else a bit-by-bit deposit over the free
+// positions.
+struct MarginalArgs {
+  uint64_t tstride[kMaxMarginal];  // address stride of target j
+  uint8_t fpos[64];                // free bit positions, ascending
+  IdxMap im;
+  int k, nfb, use_im, pad;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_partial_marginal(const T *__restrict__ sv, uint64_t nfree, MarginalArgs a, int nb,
+                       double *__restrict__ partial) {
+  const uint64_t r = blockIdx.x / nb, bx = blockIdx.x % nb;
+  uint64_t off = 0;
+  for (int j = 0; j < a.k; ++j)
+    if ((r >> (a.k - 1 - j)) & 1) off += a.tstride[j];
+  double acc = 0.0, dummy = 0.0;
+  for (uint64_t f = bx * kBlock + threadIdx.x; f < nfree; f += (uint64_t)nb * kBlock) {
+    uint64_t i;
+    if (a.use_im) {
+      i = deposit(f, a.im);
+    } else {
+      i = 0;
+      for (int b = 0; b < a.nfb; ++b)
+        if ((f >> b) & 1) i |= 1ull << a.fpos[b];
+    }
+    const double2 v = ld2(sv, i + off);
+    acc += v.x * v.x + v.y * v.y;
+  }
+  block_sum2(acc, dummy);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
 // ---- finaliser: out[v] = beta*out[v] + alpha[v] * sum_b partial[v][b] -------
 __global__ void __launch_bounds__(kBlock)
     k_finalize(const double *__restrict__ partial, int nblocks,
@@ -261,6 +300,33 @@ int launch_probabilities_div(const void *sv, sq_dtype_t dt, uint64_t n, double *
   else
     hipLaunchKernelGGL(k_probabilities_div<double2>, grid1(n, kBlock * 4), dim3(kBlock), 0,
                        st, (const double2 *)sv, p, n, norm2_dev);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+int launch_marginal(const void *sv, sq_dtype_t dt, uint64_t n, int nbits, const int *tbit,
+                    int k, double *partial, int nb, double *out_dev, hipStream_t st) {
+  MarginalArgs a{};
+  uint64_t tmask = 0;
+  for (int j = 0; j < k; ++j) {
+    a.tstride[j] = 1ull << tbit[j];
+    tmask |= 1ull << tbit[j];
+  }
+  a.k = k;
+  for (int b = 0; b < nbits; ++b)
+    if (!((tmask >> b) & 1)) a.fpos[a.nfb++] = (uint8_t)b;
+  a.use_im = k <= 8;
+  if (a.use_im) a.im = make_idxmap(tmask, 0);
+  const uint64_t nfree = n >> k;
+  const uint64_t blocks = (1ull << k) * (uint64_t)nb;
+  if (dt == SQ_COMPLEX64)
+    hipLaunchKernelGGL(k_partial_marginal<float2>, dim3((unsigned)blocks), dim3(kBlock), 0, st,
+                       (const float2 *)sv, nfree, a, nb, partial);
+  else
+    hipLaunchKernelGGL(k_partial_marginal<double2>, dim3((unsigned)blocks), dim3(kBlock), 0, st,
+                       (const double2 *)sv, nfree, a, nb, partial);
+  hipLaunchKernelGGL(k_finalize, dim3((unsigned)(1ull << k)), dim3(kBlock), 0, st, partial, nb,
+                     nullptr, 0.0, out_dev);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
 }

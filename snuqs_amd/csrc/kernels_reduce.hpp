@@ -7,11 +7,17 @@
 namespace sq {
 
 constexpr int kRedBlocks = 2048;  // partial sums per reduced value
+constexpr int kMaxMarginal = 20;  // targets of a marginal distribution (2^20 outcomes)
 
 int launch_probabilities(const void *sv, sq_dtype_t dt, uint64_t n, double *p,
                          hipStream_t st);
 int launch_probabilities_div(const void *sv, sq_dtype_t dt, uint64_t n, double *p,
                              const double *norm2_dev, hipStream_t st);
+// out_dev[2^k] = marginal distribution of |sv|^2 over the target bits tbit[]
+// (tbit[0] the most significant bit of the outcome); partial holds
+// 2^k * nb doubles.
+int launch_marginal(const void *sv, sq_dtype_t dt, uint64_t n, int nbits, const int *tbit,
+                    int k, double *partial, int nb, double *out_dev, hipStream_t st);
 int launch_norm2(const void *sv, sq_dtype_t dt, uint64_t n, double *partial,
                  double *out_dev, hipStream_t st);
 // out_dev[0..1] = beta*out_dev + alpha_dev[0..1] * (re, im) of <a|b>

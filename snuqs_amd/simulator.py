@@ -13,8 +13,9 @@ fields as GateModelTaskResult.  Extra keyword arguments: ``precision``
 
 Differences from the reference, by design: the engine runs on the GPU only
 (accelerator None / "cuda" / "hip"; "cpu" / "hybrid" and the offload tiers
-raise), and besides StateVector the result types Amplitude, Probability and
-Expectation (Pauli-product observables) are computed on the GPU.  OpenQASM
+raise), and besides StateVector the result types Amplitude, Probability
+(marginals over up to 20 targets reduced on the GPU), Expectation and Variance
+(Pauli-product observables) are computed on the GPU.  OpenQASM
 source needs the reference's ANTLR interpreter and is not supported.
 """
 from __future__ import annotations
@@ -187,25 +188,24 @@ class BaseSimulator:
             amps = sim.amplitudes([int(s, 2) for s in states])
             return {s: complex(a) for s, a in zip(states, amps)}
         if t == "probability":
-            p = sim.probabilities
             if not targets:
-                return p
-            keep = [n - 1 - q for q in targets]
-            idx = np.arange(p.size)
-            sub = np.zeros(idx.size, dtype=np.int64)
-            for j, b in enumerate(keep):
-                sub |= ((idx >> b) & 1) << (len(keep) - 1 - j)
-            return np.bincount(sub, weights=p, minlength=1 << len(keep))
-        if t == "expectation":
+                return sim.probabilities
+            if len(targets) > 20 and sorted(targets) != list(range(n)):
+                raise ValueError("Probability over more than 20 of the qubits: request all "
+                                 "qubits or at most 20")
+            return sim.marginal_probabilities(targets)
+        if t in ("expectation", "variance"):
             obs = _get(r, "observable")
             letters = [str(o).upper() for o in obs]
             if len(letters) == 1 and not targets:
-                raise ValueError("an expectation over all qubits needs targets")
+                raise ValueError(f"a {t} over all qubits needs targets")
             if len(letters) == 1 and len(targets) > 1:
                 letters = letters * len(targets)
             if any(L not in ("X", "Y", "Z", "I") for L in letters):
                 raise NotImplementedError("only Pauli-product observables are supported")
-            return sim.expectation(["".join(letters)], [targets], [1.0])
+            e = sim.expectation(["".join(letters)], [targets], [1.0])
+            # a Pauli product squares to I: Var = <P^2> - <P>^2 = 1 - <P>^2
+            return e if t == "expectation" else 1.0 - e * e
         raise NotImplementedError(f"Result type {t} not supported")
 
 
@@ -225,7 +225,7 @@ class StateVectorSimulator(BaseSimulator):
                                               if k not in ("U", "gphase")),
                 "supportedResultTypes": [{"name": n} for n in
                                          ("StateVector", "Amplitude", "Probability",
-                                          "Expectation")],
+                                          "Expectation", "Variance")],
             }},
             "paradigm": {"qubitCount": self.max_qubits},
         }

@@ -64,6 +64,34 @@ def test_renormalize(C, dtype):
 
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("n,targets", [(14, [3]), (14, [13, 0]), (14, [0, 1, 2, 3, 4, 5, 6, 7]),
+                                       (14, [12, 13, 11, 10, 2, 9, 8, 0, 4]),
+                                       (16, list(range(15, 3, -1))), (21, list(range(1, 21)))])
+def test_marginal_probabilities(C, dtype, n, targets):
+    """Marginal distribution over target qubits (targets[0] = most significant
+    bit of the outcome) against numpy on the same amplitudes: <= 8 targets use
+    the inserted-bit index map, more use the bit-by-bit deposit; 20 targets of
+    21 qubits leave one block per outcome."""
+    rng = np.random.default_rng(len(targets) + n)
+    v, sv = _state(C, n, rng, dtype)
+    got = C.functionals.marginal_probabilities(sv, targets)
+    p = (v.real * v.real + v.imag * v.imag).reshape([2] * n)  # axis q = qubit q
+    rest = tuple(q for q in range(n) if q not in targets)
+    want = p.sum(axis=rest).transpose(np.argsort(np.argsort(targets))).reshape(-1)
+    assert got.shape == (1 << len(targets),)
+    assert np.abs(got - want).max() <= 1e-13 * max(1.0, want.max())
+    assert abs(got.sum() - p.sum()) < 1e-12
+
+
+def test_marginal_rejects_bad_targets(C):
+    rng = np.random.default_rng(0)
+    _, sv = _state(C, 6, rng)
+    for bad in ([], [6], [1, 1], list(range(6)) * 4):
+        with pytest.raises(ValueError):
+            C.functionals.marginal_probabilities(sv, bad)
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
 def test_probabilities_normalized(C, dtype):
     """normalize=True: |a|^2 / norm2 in FP64 with norm2 the device's FP64 norm,
     division correctly rounded (bitwise equal to numpy's), summing to 1."""

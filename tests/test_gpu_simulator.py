@@ -48,7 +48,10 @@ def test_run_mixed_program_against_oracle(precision):
     ]
     results = [{"type": "statevector"}, {"type": "probability", "targets": [10]},
                {"type": "amplitude", "states": ["000", "101"]},
-               {"type": "expectation", "observable": ["z", "x"], "targets": [3, 10]}]
+               {"type": "expectation", "observable": ["z", "x"], "targets": [3, 10]},
+               {"type": "variance", "observable": ["y", "z"], "targets": [7, 3]},
+               {"type": "probability", "targets": [10, 3]},
+               {"type": "probability", "targets": [7, 10, 3]}]
     res = StateVectorSimulator().run({"instructions": ins, "results": results},
                                      precision=precision)
     qmap = {3: 0, 7: 1, 10: 2}
@@ -66,6 +69,13 @@ def test_run_mixed_program_against_oracle(precision):
     assert abs(vals[2]["101"] - ref[5]) < tol and abs(vals[2]["000"] - ref[0]) < tol
     want = O.expect_pauli(ref, "ZX", [0, 2], 3).real
     assert abs(vals[3] - want) < tol
+    e_yz = O.expect_pauli(ref, "YZ", [1, 0], 3).real
+    assert abs(vals[4] - (1.0 - e_yz ** 2)) < 4 * tol
+    # marginal over (q2, q0): outcome bit 1 = qubit 2, bit 0 = qubit 0
+    pm = p.reshape(2, 2, 2)  # axes: qubit 0, 1, 2 (qubit 0 = MSB of the index)
+    assert np.abs(vals[5] - pm.sum(axis=1).T.reshape(-1)).max() < tol
+    # a permutation of all qubits reorders the distribution
+    assert np.abs(vals[6] - pm.transpose(1, 2, 0).reshape(-1)).max() < tol
 
 
 def test_shots_match_numpy_choice_with_global_seed():
