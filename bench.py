@@ -289,6 +289,8 @@ def main():
         unfused = {
             "ms_per_step": wall_u * 1e3,
             "GBps": bytes_u / wall_u / 1e9,
+            # full-sweep convention (2 N s per launched kernel, touched or not)
+            "GBps_full_sweep": sum(2.0 * (1 << n) * item for c, _, _ in info_u if c) / wall_u / 1e9,
             "launches": int(sum(1 for c, _, _ in info_u if c)),
             "kernels": {f"{CLASS_NAMES[c]}_k{k}": {
                 "launches": t["launches"], "avg_ms": t["ms"] / t["launches"],

@@ -128,6 +128,12 @@ int sq_program_destroy(void *program);
 int sq_program_stats(void *program, uint64_t *num_launches,
                      double *algorithmic_bytes);
 int sq_program_run(void *program, void *sv, uint64_t num_elems, void *stream);
+/* sq_program_run through a hipGraph (no reference counterpart; the reference
+ * launches one kernel per Python call, simulation.py:274-284): the launch
+ * sequence is captured on the first call for a state and replayed on later
+ * calls with the same state; another state (or sq_program_tune) recaptures.
+ * Asynchronous on `stream` like sq_program_run. */
+int sq_program_run_graph(void *program, void *sv, uint64_t num_elems, void *stream);
 /* Measurement variant of sq_program_run: records a HIP event before every
  * launch and one after the last on `stream`, waits for completion and writes
  * each gate's elapsed time to ms_per_gate[num_gates] (0 for gates that

@@ -290,7 +290,19 @@ struct Program {
   void *dev_mats = nullptr;
   uint64_t launches = 0;
   double bytes = 0.0;
+  // hipGraph of the forward launch sequence on one state (sq_program_run_graph)
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  void *graph_sv = nullptr;
 };
+
+static void drop_graph(Program *p) {
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->gexec = nullptr;
+  p->graph = nullptr;
+  p->graph_sv = nullptr;
+}
 
 // Upload the real-block matrices of the dense 4-5 target plans in `plans`.
 static int upload_blocks(const std::vector<const GatePlan *> &plans, sq_dtype_t dt,
@@ -380,6 +392,7 @@ int sq_program_create(void **program, sq_dtype_t dt, uint32_t nq, uint32_t ng,
 int sq_program_destroy(void *program) {
   Program *p = (Program *)program;
   if (!p) return SQ_OK;
+  drop_graph(p);
   if (p->dev_mats) SQ_HIP(hipFree(p->dev_mats));
   delete p;
   return SQ_OK;
@@ -485,6 +498,7 @@ int sq_program_tune(void *program, void *sv, uint64_t n, int reps, void *stream,
   if (reps < 1 || reps > 100) return fail(SQ_ERR_INVALID, "reps must be in [1, 100]");
   hipStream_t st = (hipStream_t)stream;
   const size_t ng = p->fwd.size();
+  drop_graph(p);  // launch shapes change: a captured graph is stale
   // gates to tune; those whose key is cached take the cached shape
   std::vector<size_t> todo;
   std::vector<ShapeKey> keys(ng);
@@ -540,6 +554,40 @@ int sq_program_tune(void *program, void *sv, uint64_t n, int reps, void *stream,
     ++tuned;
   }
   if (num_tuned) *num_tuned = tuned;
+  return SQ_OK;
+}
+
+int sq_program_run_graph(void *program, void *sv, uint64_t n, void *stream) {
+  Program *p = (Program *)program;
+  if (!p) return fail(SQ_ERR_INVALID, "null program");
+  SQ_TRY(check_state(sv, p->dt, n));
+  if (n != (1ull << p->nq)) return fail(SQ_ERR_INVALID, "state size does not match program");
+  if (p->graph_sv != sv) {
+    // capture on a private stream (thread-local mode: other threads' work is
+    // not pulled in); the graph holds only kernel launches
+    drop_graph(p);
+    hipStream_t cs;
+    SQ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    int rc = SQ_OK;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      rc = fail(SQ_ERR_HIP, "hipStreamBeginCapture failed");
+    for (size_t g = 0; g < p->fwd.size() && rc == SQ_OK; ++g)
+      rc = launch_plan(p->fwd[g], sv, p->dt, n, cs, p->fwd_dev[g]);
+    hipGraph_t gr = nullptr;
+    if (hipStreamEndCapture(cs, &gr) != hipSuccess && rc == SQ_OK)
+      rc = fail(SQ_ERR_HIP, "hipStreamEndCapture failed");
+    (void)hipStreamDestroy(cs);
+    if (rc == SQ_OK && hipGraphInstantiate(&p->gexec, gr, nullptr, nullptr, 0) != hipSuccess)
+      rc = fail(SQ_ERR_HIP, "hipGraphInstantiate failed");
+    if (rc != SQ_OK) {
+      if (gr) (void)hipGraphDestroy(gr);
+      p->gexec = nullptr;
+      return rc;
+    }
+    p->graph = gr;
+    p->graph_sv = sv;
+  }
+  SQ_HIP(hipGraphLaunch(p->gexec, (hipStream_t)stream));
   return SQ_OK;
 }
 

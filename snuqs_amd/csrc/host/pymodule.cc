@@ -338,11 +338,14 @@ class Program {
   ~Program() {
     if (prog_) sq_program_destroy(prog_);
   }
-  void run(StateVector &sv, std::shared_ptr<Stream> stream) {
+  void run(StateVector &sv, bool use_graph, std::shared_ptr<Stream> stream) {
     sv.require_device("Program.run");
     if (sv.dtype() != dt_) throw py::value_error("state dtype differs from the program's");
     py::gil_scoped_release nogil;
-    check(sq_program_run(prog_, sv.ptr(), sv.num_elems(), sraw(stream)));
+    if (use_graph)
+      check(sq_program_run_graph(prog_, sv.ptr(), sv.num_elems(), sraw(stream)));
+    else
+      check(sq_program_run(prog_, sv.ptr(), sv.num_elems(), sraw(stream)));
   }
   py::array_t<float> run_timed(StateVector &sv, std::shared_ptr<Stream> stream) {
     sv.require_device("Program.run_timed");
@@ -786,7 +789,9 @@ PYBIND11_MODULE(_C, m) {
              return std::make_shared<Program>(parse_dtype(dtype), nq, ops);
            }), py::arg("num_qubits"), py::arg("operations"), py::kw_only(),
            py::arg("dtype") = "complex128")
-      .def("run", &Program::run, py::arg("state_vector"), py::kw_only(), py::arg("stream") = nullptr)
+      .def("run", &Program::run, py::arg("state_vector"), py::kw_only(),
+           py::arg("use_graph") = false, py::arg("stream") = nullptr,
+           "Apply the gates in order; use_graph replays a captured hipGraph of the launches.")
       .def("tune", &Program::tune, py::arg("state_vector"), py::kw_only(), py::arg("reps") = 2,
            py::arg("stream") = nullptr,
            "Pick each 5-target gate's fastest launch shape by timing it on state_vector "

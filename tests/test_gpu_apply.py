@@ -288,6 +288,39 @@ def test_program_tune_keeps_results_bitwise(C, dtype):
     assert np.array_equal(np.asarray(sv2.cpu()), before)
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_program_graph_replays_match_eager(C, dtype):
+    """Program.run(use_graph=True) captures the launch sequence once per state
+    and replays it: every replay (with eager work and syncs between) is
+    bitwise equal to the eager run; another state or a re-tune recaptures."""
+    from snuqs_amd.circuits import random_gate_list, random_layered
+    from snuqs_amd.transpile import fuse_operations
+
+    n = 18
+    rng = np.random.default_rng(77)
+    ops = fuse_operations(random_layered(n, 3, seed=5), 5) + random_gate_list(n, 12, rng)
+    prog = C.functionals.Program(n, ops, dtype=dtype)
+    eager = device_state(C, n, dtype)
+    prog.run(eager)
+    want = np.asarray(eager.cpu()).copy()
+    sv = device_state(C, n, dtype)
+    other = device_state(C, n, dtype)
+    for rep in range(4):
+        C.functionals.initialize_basis_z(sv)  # eager kernel between replays
+        prog.run(sv, use_graph=True)
+        C.core.cuda.device_synchronize()
+        assert np.array_equal(np.asarray(sv.cpu()), want), rep
+        prog.run(other)  # eager launches of the same program in between
+        assert C.functionals.norm2(other) > 0
+    C.functionals.initialize_basis_z(other)
+    prog.run(other, use_graph=True)  # another state: recaptured
+    assert np.array_equal(np.asarray(other.cpu()), want)
+    prog.tune(other, reps=1)  # drops the graph
+    C.functionals.initialize_basis_z(sv)
+    prog.run(sv, use_graph=True)
+    assert np.array_equal(np.asarray(sv.cpu()), want)
+
+
 def test_copy_kernel(C):
     rng = np.random.default_rng(0)
     n = 15
