@@ -286,6 +286,8 @@ struct Program {
   std::vector<int> nt;                 // targets per gate
   std::vector<size_t> toff;            // offset of a gate's targets
   std::vector<uint32_t> targets;
+  std::vector<double> mats;            // the gates' complex128 matrices (as given)
+  std::vector<size_t> moffs;           // offset of a gate's matrix in mats
   std::vector<const void *> fwd_dev, inv_dev;  // real-block matrices (kk >= 4)
   void *dev_mats = nullptr;
   uint64_t launches = 0;
@@ -371,6 +373,8 @@ int sq_program_create(void **program, sq_dtype_t dt, uint32_t nq, uint32_t ng,
     prog->nt.push_back(k);
     prog->toff.push_back(toff);
     prog->targets.insert(prog->targets.end(), t, t + k);
+    prog->moffs.push_back(prog->mats.size());
+    prog->mats.insert(prog->mats.end(), m, m + (2ull << (2 * k)));
     toff += k;
     moff += 2ull << (2 * k);
     if (prog->fwd[g].kind != Kind::NOOP) {
@@ -909,8 +913,10 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   // derivative plans, grouped by gate
   std::vector<GatePlan> dplans(num_derivs);
   std::vector<std::vector<uint32_t>> by_gate(ng);
+  std::vector<size_t> doffs(num_derivs);
   size_t moff = 0;
   for (uint32_t d = 0; d < num_derivs; ++d) {
+    doffs[d] = moff;
     const uint32_t g = deriv_gate[d];
     if (g >= ng) return fail(SQ_ERR_INVALID, "deriv_gate out of range");
     if (deriv_param[d] >= num_params) return fail(SQ_ERR_INVALID, "deriv_param out of range");
@@ -979,12 +985,29 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   // reverse sweep
   int cidx = (int)num_checkpoints - 1;
   for (int g = (int)ng - 1; g >= 0; --g) {
+    bool restored = false;
     if (cidx >= 0 && bound[cidx] == (uint32_t)g) {
       SQ_HIP(hipMemcpyAsync(sv, checkpoints[cidx], n * itemsize(dt), hipMemcpyDeviceToDevice, st));
       while (cidx >= 0 && bound[cidx] == (uint32_t)g) --cidx;
-    } else {
-      SQ_TRY(launch_plan(p->inv[g], sv, dt, n, st, p->inv_dev[g]));
+      restored = true;
     }
+    if (p->nt[g] == 1 && by_gate[g].size() == 1) {
+      // one pass: undo G on psi and lambda, derivative inner product between
+      const uint32_t d = by_gate[g][0];
+      const double *G = p->mats.data() + p->moffs[g];
+      double gi[8];
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c) {  // G^dagger[r][c] = conj(G[c][r])
+          gi[2 * (2 * r + c)] = G[2 * (2 * c + r)];
+          gi[2 * (2 * r + c) + 1] = -G[2 * (2 * c + r) + 1];
+        }
+      const size_t doff = doffs[d];
+      const int bit = p->nq - 1 - (int)p->targets[p->toff[g]];
+      SQ_TRY(launch_adjoint_step1(sv, work0, dt, n, bit, gi, dmatrices + doff, restored ? 0 : 1,
+                                  partial, gslots + 2 * deriv_param[d], two, st));
+      continue;
+    }
+    if (!restored) SQ_TRY(launch_plan(p->inv[g], sv, dt, n, st, p->inv_dev[g]));
     for (uint32_t d : by_gate[g]) {
       SQ_HIP(hipMemcpyAsync(work1, sv, n * itemsize(dt), hipMemcpyDeviceToDevice, st));
       SQ_TRY(launch_plan(dplans[d], work1, dt, n, st, dmat_ptr[d]));

@@ -189,6 +189,58 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// ---- fused adjoint step of a 1-target gate with one parameter ---------------
+// At reverse step g (psi = the state after gate g, lam = the co-state after
+// it): psi <- G^dagger psi (skipped when psi was just reloaded from a
+// checkpoint), partial += Re <lam| dG |psi>, lam <- G^dagger lam.  One pass
+// over both states instead of separate undo, copy, derivative and inner
+// product sweeps.  Gate arithmetic in the state's precision, the inner
+// product in FP64.
+template <typename R>
+struct Mat2 {
+  R r[4], i[4];  // row-major 2x2
+};
+
+template <typename T, typename R>
+__device__ __forceinline__ void mat2_apply(const Mat2<R> &m, const T &x0, const T &x1, T &y0,
+                                           T &y1) {
+  y0.x = m.r[0] * x0.x - m.i[0] * x0.y + m.r[1] * x1.x - m.i[1] * x1.y;
+  y0.y = m.r[0] * x0.y + m.i[0] * x0.x + m.r[1] * x1.y + m.i[1] * x1.x;
+  y1.x = m.r[2] * x0.x - m.i[2] * x0.y + m.r[3] * x1.x - m.i[3] * x1.y;
+  y1.y = m.r[2] * x0.y + m.i[2] * x0.x + m.r[3] * x1.y + m.i[3] * x1.x;
+}
+
+template <typename T, typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_adjoint_step1(T *__restrict__ psi, T *__restrict__ lam, uint64_t npair, int bit,
+                    Mat2<R> gi, Mat2<R> dg, int undo_psi, double *__restrict__ partial) {
+  double acc = 0.0, dummy = 0.0;
+  const uint64_t lowm = (1ull << bit) - 1;
+  for (uint64_t f = (uint64_t)blockIdx.x * kBlock + threadIdx.x; f < npair;
+       f += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t lo = f & lowm;
+    const uint64_t i0 = ((f ^ lo) << 1) | lo, i1 = i0 | (1ull << bit);
+    const T a0 = ld_nt(psi + i0), a1 = ld_nt(psi + i1);
+    const T l0 = ld_nt(lam + i0), l1 = ld_nt(lam + i1);
+    T p0 = a0, p1 = a1;
+    if (undo_psi) mat2_apply(gi, a0, a1, p0, p1);
+    T d0, d1;
+    mat2_apply(dg, p0, p1, d0, d1);
+    acc += (double)l0.x * d0.x + (double)l0.y * d0.y + (double)l1.x * d1.x +
+           (double)l1.y * d1.y;
+    T q0, q1;
+    mat2_apply(gi, l0, l1, q0, q1);
+    if (undo_psi) {
+      st_nt(psi + i0, p0);
+      st_nt(psi + i1, p1);
+    }
+    st_nt(lam + i0, q0);
+    st_nt(lam + i1, q1);
+  }
+  block_sum2(acc, dummy);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
 // ---- marginal distribution over k targets ------------------------------------
 // out[r] = sum of |sv[i]|^2 over the i whose target bits spell r (target 0
 // the most significant bit of r).  Block b sums outcome r = b / nb over the
@@ -327,6 +379,40 @@ int launch_marginal(const void *sv, sq_dtype_t dt, uint64_t n, int nbits, const 
                        (const double2 *)sv, nfree, a, nb, partial);
   hipLaunchKernelGGL(k_finalize, dim3((unsigned)(1ull << k)), dim3(kBlock), 0, st, partial, nb,
                      nullptr, 0.0, out_dev);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+static int red_blocks(uint64_t work);
+
+int launch_adjoint_step1(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int bit,
+                         const double *ginv, const double *dgm, int undo_psi, double *partial,
+                         double *out_dev, const double *alpha_dev, hipStream_t st) {
+  const uint64_t npair = n >> 1;
+  const int nb = red_blocks(npair);
+  if (dt == SQ_COMPLEX64) {
+    Mat2<float> gi, dg;
+    for (int e = 0; e < 4; ++e) {
+      gi.r[e] = (float)ginv[2 * e];
+      gi.i[e] = (float)ginv[2 * e + 1];
+      dg.r[e] = (float)dgm[2 * e];
+      dg.i[e] = (float)dgm[2 * e + 1];
+    }
+    hipLaunchKernelGGL((k_adjoint_step1<float2, float>), dim3(nb), dim3(kBlock), 0, st,
+                       (float2 *)psi, (float2 *)lam, npair, bit, gi, dg, undo_psi, partial);
+  } else {
+    Mat2<double> gi, dg;
+    for (int e = 0; e < 4; ++e) {
+      gi.r[e] = ginv[2 * e];
+      gi.i[e] = ginv[2 * e + 1];
+      dg.r[e] = dgm[2 * e];
+      dg.i[e] = dgm[2 * e + 1];
+    }
+    hipLaunchKernelGGL((k_adjoint_step1<double2, double>), dim3(nb), dim3(kBlock), 0, st,
+                       (double2 *)psi, (double2 *)lam, npair, bit, gi, dg, undo_psi, partial);
+  }
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, st, partial, nb, alpha_dev, 1.0,
+                     out_dev);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
 }

@@ -24,6 +24,12 @@ int launch_norm2(const void *sv, sq_dtype_t dt, uint64_t n, double *partial,
 int launch_inner(const void *a, const void *b, sq_dtype_t dt, uint64_t n,
                  double *partial, double *out_dev, const double *alpha_dev,
                  double beta, hipStream_t st);
+// Fused reverse step of a 1-target gate on amplitude bit `bit` (ginv = G^dagger,
+// dgm = dG, row-major complex128 2x2): psi <- ginv psi (if undo_psi),
+// out_dev[0] += alpha_dev[0] * Re <lam| dgm |psi>, lam <- ginv lam.
+int launch_adjoint_step1(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int bit,
+                         const double *ginv, const double *dgm, int undo_psi, double *partial,
+                         double *out_dev, const double *alpha_dev, hipStream_t st);
 int launch_pauli_terms(const void *sv, sq_dtype_t dt, uint64_t n, int nterms,
                        const void *terms_dev, const double *alpha_dev,
                        double *partial, double *out_dev, hipStream_t st);

@@ -100,3 +100,33 @@ def test_adjoint_all_parametric_gate_kinds(C):
         tm[j] -= h
         fd[j] = (oracle_energy(n, build(tp)[0], *obs) - oracle_energy(n, build(tm)[0], *obs)) / (2 * h)
     assert np.abs(grads - fd).max() < 1e-7
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-9), ("complex64", 2e-4)])
+def test_hea_adjoint_16q_vs_gpu_parameter_shift(C, dtype, tol):
+    """16 qubits (targets on every bit position, including the ones above the
+    lowest eight): adjoint gradients against parameter shift evaluated with the
+    engine itself in complex128 (exact for the exp(-i theta/2 P) rotations)."""
+    from snuqs_amd.circuits import hea_ansatz, random_pauli_sum
+
+    n, layers = 16, 2
+    rng = np.random.default_rng(16)
+    thetas = rng.uniform(0, 2 * np.pi, 2 * n * layers)
+    obs = random_pauli_sum(n, 10, 4, rng)
+    ops, pmap = hea_ansatz(n, layers, thetas)
+    e, grads = _adjoint(C, n, ops, pmap, thetas.size, obs, dtype, checkpoints=2)
+    sv = device_state(C, n, "complex128")
+
+    def energy(th):
+        C.functionals.initialize_basis_z(sv)
+        C.functionals.Program(n, hea_ansatz(n, layers, th)[0], dtype="complex128").run(sv)
+        return C.functionals.expectation(sv, *obs)[0]
+
+    assert abs(e - energy(thetas)) < (1e-12 if dtype == "complex128" else 1e-5)
+    ps = np.empty_like(thetas)
+    for j in range(thetas.size):
+        tp, tm = thetas.copy(), thetas.copy()
+        tp[j] += np.pi / 2
+        tm[j] -= np.pi / 2
+        ps[j] = 0.5 * (energy(tp) - energy(tm))
+    assert np.abs(grads - ps).max() < tol
