@@ -163,6 +163,68 @@ def run_adjoint_bench(C, n=24, layers=8, nterms=64, weight=4, shift_params=12):
     }
 
 
+def run_other_configs(C, depth):
+    """The remaining configurations, one timed run each after a warm-up:
+    config 2 in complex128 (30 qubits, 16 GiB), config 1 (GHZ-20 complex128,
+    closed-form check) and config 5 (34-qubit complex64 and 33-qubit
+    complex128 random U + CX circuits of 20 layers, fused, with the
+    probabilities of 1e4 seeded bitstrings)."""
+    import struct
+
+    from snuqs_amd.circuits import ghz, random_layered, random_u_cx
+    from snuqs_amd.transpile import fuse_operations
+
+    out = {}
+
+    def timed_program(n, ops, dtype, tune=True):
+        prog = C.functionals.Program(n, ops, dtype=dtype)
+        sv = C.result_types.StateVector(C.DeviceType.CUDA, n, dtype=dtype)
+        if tune:
+            prog.tune(sv, reps=1)
+        C.functionals.initialize_basis_z(sv)
+        C.core.cuda.device_synchronize()
+        t0 = time.perf_counter()
+        prog.run(sv)
+        C.core.cuda.device_synchronize()
+        dt = time.perf_counter() - t0
+        nbytes = prog.stats()["algorithmic_bytes"]
+        return prog, sv, dt, nbytes
+
+    # config 2, complex128
+    ops = fuse_operations(random_layered(30, depth, seed=1234), 5)
+    prog, sv, dt, nbytes = timed_program(30, ops, "complex128")
+    out["config2_c128_30q"] = {"ms": dt * 1e3, "GBps": nbytes / dt / 1e9, "kernels": len(ops)}
+    log(f"config 2 c128: {dt * 1e3:.0f} ms, {nbytes / dt / 1e9:.0f} GB/s")
+    del prog, sv
+    # config 1: GHZ-20 complex128
+    prog, sv, dt, nbytes = timed_program(20, ghz(20), "complex128", tune=False)
+    a = C.functionals.amplitudes(sv, [0, (1 << 20) - 1])
+    bits = {struct.pack("<d", float(z.real)).hex() for z in a}
+    out["config1_ghz20_c128"] = {"ms": dt * 1e3, "launches": 20,
+                                 "amplitudes_bitwise_M_SQRT1_2": bits == {struct.pack("<d", 0.5 ** 0.5).hex()}}
+    del prog, sv
+    # config 5: largest single-GPU states
+    for n, dtype in ((34, "complex64"), (33, "complex128")):
+        free = C.core.cuda.mem_info()[0]
+        need = (1 << n) * (8 if dtype == "complex64" else 16)
+        if free < need + (4 << 30):
+            out[f"config5_{dtype}_{n}q"] = "skipped: not enough device memory"
+            continue
+        ops = fuse_operations(random_u_cx(n, 20, seed=n), 5)
+        prog, sv, dt, nbytes = timed_program(n, ops, dtype)
+        q = np.random.default_rng(n).integers(0, 1 << n, 10**4, dtype=np.uint64)
+        t0 = time.perf_counter()
+        amps = C.functionals.amplitudes(sv, [int(x) for x in q])
+        tq = time.perf_counter() - t0
+        out[f"config5_{dtype}_{n}q"] = {
+            "ms": dt * 1e3, "GBps": nbytes / dt / 1e9, "kernels": len(ops),
+            "state_GiB": need / 2**30, "query_1e4_ms": tq * 1e3,
+            "sum_queried_probabilities": float(np.sum(np.abs(amps) ** 2))}
+        log(f"config 5 {dtype} {n} q: {dt * 1e3:.0f} ms, {nbytes / dt / 1e9:.0f} GB/s")
+        del prog, sv
+    return out
+
+
 def run_qaoa_bench(C, reps=2):
     """Config 4: 256 QAOA MaxCut instances, qubits uniform over the even values
     12..26 (3-regular graphs need an even count), p = 4, complex64 with FP64
@@ -220,6 +282,8 @@ def main():
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-qaoa", action="store_true")
+    ap.add_argument("--skip-configs", action="store_true",
+                    help="skip the config 1, config 2 complex128 and config 5 legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -301,6 +365,7 @@ def main():
     del sv, prog
 
     adj = None if args.skip_adjoint else run_adjoint_bench(C)
+    others = None if args.skip_configs else run_other_configs(C, args.depth)
     qaoa = None if args.skip_qaoa else run_qaoa_bench(C)
     cpu = None if args.skip_cpu else cpu_baseline(n, args.depth, args.cpu_seconds)
 
@@ -344,6 +409,7 @@ def main():
         "unfused": unfused,
         "adjoint": adj,
         "qaoa_batch": qaoa,
+        "other_configs": others,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
