@@ -211,7 +211,8 @@ def run_other_configs(C, depth):
             out[f"config5_{dtype}_{n}q"] = "skipped: not enough device memory"
             continue
         ops = fuse_operations(random_u_cx(n, 20, seed=n), 5)
-        prog, sv, dt, nbytes = timed_program(n, ops, dtype)
+        # default launch shapes: tuning at 128 GiB would take ~50 s per state
+        prog, sv, dt, nbytes = timed_program(n, ops, dtype, tune=False)
         q = np.random.default_rng(n).integers(0, 1 << n, 10**4, dtype=np.uint64)
         t0 = time.perf_counter()
         amps = C.functionals.amplitudes(sv, [int(x) for x in q])
