@@ -211,6 +211,37 @@ def test_slice_with_larger_num_qubits(C):
         C.functionals.apply(full, g.Hadamard([2]), n, [2])  # stride beyond the slice
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_large_slices_take_the_tiled_kernels(C, dtype):
+    """2^16-amplitude slices of a 20-qubit state, gates given with
+    num_qubits = 20 (targets t >= 4 fall inside the slice): the 5-target MFMA
+    kernel, the LDS-tiled low-bit kernels and the permutation path run on the
+    slice's own amplitudes and leave the other slices untouched."""
+    from snuqs_amd.circuits import haar_unitary
+
+    g = C.operation.gate_operations
+    n, m = 20, 16
+    rng = np.random.default_rng(12)
+    npdt = np.complex64 if dtype == "complex64" else np.complex128
+    init = random_state(n, rng, npdt)
+    full = device_state(C, n, dtype, init)
+    ops = [g.Unitary([19, 5, 12, 8, 17], haar_unitary(32, rng)),
+           g.Unitary([18, 19], haar_unitary(4, rng)), g.Hadamard([19]),
+           g.CX([19, 7]), g.CCNot([18, 19, 6]), g.Unitary([16, 17, 18], haar_unitary(8, rng))]
+    for idx in (1, 10):
+        sl = full.slice(m, idx)
+        for op in ops:
+            C.functionals.apply(sl, op, n, op.targets)
+    ref = init.astype(np.complex128).reshape(1 << (n - m), 1 << m).copy()
+    for idx in (1, 10):
+        part = ref[idx].copy()
+        for op in ops:
+            O.apply(part, np.array(op.matrix()), [t - (n - m) for t in op.targets], m)
+        ref[idx] = part
+    got = to_np(C, full).reshape(1 << (n - m), 1 << m)
+    assert np.abs(got - ref).max() <= TOL[dtype]
+
+
 def test_invalid_targets_raise(C):
     g = C.operation.gate_operations
     sv = device_state(C, 4)
