@@ -51,10 +51,16 @@ def main():
     apply = O.ref_apply if lib is not None else O.apply
     N = 1 << n
     done, nbytes = 0, 0.0
+    per_class = {}  # gate class -> [gates, seconds]
     t0 = time.perf_counter()
     for op in ops[: a.max_gates]:
         m = np.array(op.matrix())
+        tg = time.perf_counter()
         apply(sv, m, list(op.targets), n)
+        cls = "diagonal_k1" if len(op.targets) == 1 else f"dense_k{len(op.targets)}"
+        c = per_class.setdefault(cls, [0, 0.0])
+        c[0] += 1
+        c[1] += time.perf_counter() - tg
         done += 1
         nbytes += 2.0 * N * 16  # Unitary and RotZ both change every amplitude
         if time.perf_counter() - t0 > a.seconds:
@@ -74,6 +80,7 @@ def main():
         "sample": f"first {done} gates of the {n}-qubit depth-{a.depth} random layered circuit, "
                   f"complex128, cpu::applyGate, {dt:.1f} s",
         "gates": done, "seconds": dt, "cpu_model": model,
+        "per_class_GBps": {k: v[0] * 2.0 * N * 16 / v[1] / 1e9 for k, v in per_class.items()},
         "host_cpus": os.cpu_count(),
     }))
 
