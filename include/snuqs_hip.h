@@ -118,7 +118,9 @@ int sq_apply(void *sv, sq_dtype_t dtype, uint64_t num_elems,
  * matrix is classified and copied to the device once, at creation).
  * gates: num_gates entries; gate g has num_targets[g] targets taken in order
  * from `targets` and a (2^k)^2 complex128 matrix taken in order from
- * `matrices`.  k <= 5 for any gate. */
+ * `matrices`.  k <= 5 for any gate.  A program may be run from several
+ * threads at once except through sq_program_tune and sq_program_run_graph,
+ * which update the program (its launch shapes, its captured graph). */
 int sq_program_create(void **program, sq_dtype_t dtype, uint32_t num_qubits,
                       uint32_t num_gates, const uint32_t *num_targets,
                       const uint32_t *targets, const double *matrices);

@@ -1,4 +1,4 @@
-"""Autotuner for the engine's launch-shape knobs (``sq_tune_set``).
+"""Autotuner for the engine's process-wide launch-shape knobs (``sq_tune_set``).
 
 For each dtype it builds a fused random layered circuit (the benchmark's
 circuit family), times every combination of the knobs with per-kernel HIP
@@ -7,6 +7,9 @@ events and keeps the fastest; ``save`` writes the winners to a JSON file that
 applies.  Run on the target GPU:
 
     python -m snuqs_amd.autotune --qubits 28 --out snuqs_amd/tuning.json
+
+These knobs are the defaults; ``Program.tune(sv)`` (``sq_program_tune``)
+picks a shape per gate of one program on top of them.
 """
 from __future__ import annotations
 
@@ -19,8 +22,9 @@ from . import _C
 
 DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 SPACE = {
-    "mfma_blocks_per_cu": [16, 32, 64, 128, 256, 1024],
+    "mfma_blocks_per_cu": [32, 64, 128, 256, 512, 1024],
     "mfma5_pipeline": [0, 1],
+    "mfma5_order": [0, 1],
 }
 SUFFIX = {"complex64": "c64", "complex128": "c128"}
 

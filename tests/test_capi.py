@@ -32,3 +32,28 @@ def test_version_and_error_string_without_gpu():
     lib.sq_last_error.restype = ctypes.c_char_p
     assert b"gfx950" in lib.sq_version()
     assert isinstance(lib.sq_last_error(), bytes)
+
+
+def test_tuning_knobs_round_trip_without_gpu(tmp_path):
+    """Process-wide launch-shape knobs: validation, and the autotuner's JSON
+    save / load applying them (host-side only)."""
+    import pytest
+
+    from snuqs_amd import _C, autotune
+
+    keys = ["mfma_blocks_per_cu_c64", "mfma5_pipeline_c128", "mfma5_order_c64"]
+    saved = {k: _C.core.tune_get(k) for k in keys}
+    try:
+        table = {"knobs": {"mfma_blocks_per_cu_c64": 64, "mfma5_pipeline_c128": 0,
+                           "mfma5_order_c64": 1}}
+        path = str(tmp_path / "tuning.json")
+        autotune.save(table, path)
+        assert autotune.load(path)["knobs"] == table["knobs"]
+        assert [_C.core.tune_get(k) for k in keys] == [64, 0, 1]
+        with pytest.raises(ValueError):
+            _C.core.tune_set("mfma_blocks_per_cu_c64", 0)
+        with pytest.raises(ValueError):
+            _C.core.tune_set("no_such_knob", 1)
+    finally:
+        for k, v in saved.items():
+            _C.core.tune_set(k, v)
