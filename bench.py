@@ -402,6 +402,8 @@ def main():
             "mfma_frac_fp32": flops / (avg_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TFLOPS if flops else None,
         },
         "kernel_ms_per_step": kernel_ms / args.steps,
+        # the same bytes over the summed kernel times (HIP events), without launch gaps
+        "GBps_kernel_only": step_bytes * args.steps / (kernel_ms * 1e-3) / 1e9,
         "copy_kernel_GBps": copy_gbs,
         "frac_of_copy": value / copy_gbs,
         "kernels": {f"{CLASS_NAMES[c]}_k{k}": {
