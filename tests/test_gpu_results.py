@@ -239,3 +239,30 @@ def test_sampling_fast_mode_mismatches_are_rare(C):
     print("fast-mode mismatches:", mism)
     assert mism <= 20
     assert np.array_equal(C.functionals.sample(sv, u, exact=True), want)
+
+
+def test_edge_cases(C):
+    """1-qubit states, an empty program, empty queries, zero shots and a
+    marginal over every qubit."""
+    g = C.operation.gate_operations
+    for dtype in ("complex128", "complex64"):
+        sv = device_state(C, 1, dtype)
+        C.functionals.apply(sv, g.Hadamard([0]), 1, [0])
+        C.functionals.apply(sv, g.RotZ([0], 0.4), 1, [0])
+        ref = np.array([np.exp(-0.2j), np.exp(0.2j)]) / np.sqrt(2)
+        assert np.abs(to_np(C, sv) - ref).max() < 1e-7
+        prog = C.functionals.Program(1, [], dtype=dtype)
+        assert len(prog) == 0
+        prog.run(sv)
+        prog.run(sv, use_graph=True)
+        assert np.abs(to_np(C, sv) - ref).max() < 1e-7
+        assert C.functionals.amplitudes(sv, []).size == 0
+        assert C.functionals.sample(sv, np.zeros(0)).size == 0
+        p = C.functionals.probabilities(sv)
+        assert np.allclose(p, [0.5, 0.5])
+    rng = np.random.default_rng(2)
+    v, sv = _state(C, 5, rng)
+    m = C.functionals.marginal_probabilities(sv, [4, 3, 2, 1, 0])  # all qubits, reversed
+    p = v.real * v.real + v.imag * v.imag
+    rev = [int(format(i, "05b")[::-1], 2) for i in range(32)]
+    assert np.abs(m - p[rev]).max() < 1e-15
