@@ -6,8 +6,10 @@ accelerator, prefetch, offload, path, block_count)`` signature
 (simulator.py:152-155, 450-461) and its steps: contiguous-qubit remap
 (:278-307), transpile, simulation.evolve, result assembly (:309-381).  The
 Braket SDK and schemas are optional: without them ``run`` takes a JAQCD
-program as a JSON dict (or string) and returns a ``TaskResult`` with the same
-fields as GateModelTaskResult.  Extra keyword arguments: ``precision``
+program as a JSON dict (or string), or an OpenQASM 3 program (an object or dict
+with ``source`` and ``inputs``, or the source text; read by
+``snuqs_amd.openqasm``, run_openqasm at simulator.py:380-437), and returns a
+``TaskResult`` with the same fields as GateModelTaskResult.  Extra keyword arguments: ``precision``
 ("complex64" / "complex128"), ``fuse`` (largest fused block, 0 = off) and
 ``exact_sampling``.
 
@@ -15,8 +17,8 @@ Differences from the reference, by design: the engine runs on the GPU only
 (accelerator None / "cuda" / "hip"; "cpu" / "hybrid" and the offload tiers
 raise), and besides StateVector the result types Amplitude, Probability
 (marginals over up to 20 targets reduced on the GPU), Expectation and Variance
-(Pauli-product observables) are computed on the GPU.  OpenQASM
-source needs the reference's ANTLR interpreter and is not supported.
+(Pauli products with h and i factors; an observable without targets gives
+one value per qubit) are computed on the GPU.
 """
 from __future__ import annotations
 
@@ -28,7 +30,7 @@ from typing import Any, Optional
 
 import numpy as np
 
-from . import _C
+from . import _C, openqasm
 from .gate_operations import BRAKET_GATES, from_braket_instruction
 from .simulation import StateVectorSimulation
 from .subcircuit import Subcircuit
@@ -111,29 +113,32 @@ class BaseSimulator:
         return [[str(b) for b in row] for row in bits.tolist()]
 
     @staticmethod
+    def _is_openqasm(ir) -> bool:
+        if isinstance(ir, bytes):
+            ir = ir.decode()
+        if isinstance(ir, str):
+            return not ir.lstrip().startswith("{")
+        if isinstance(ir, dict):
+            return "source" in ir
+        return _get(ir, "source") is not None
+
+    @staticmethod
     def _parse_program(ir):
         if isinstance(ir, (str, bytes)):
             ir = json.loads(ir)
         if isinstance(ir, dict):
             return ir.get("instructions", []), ir.get("results") or [], \
                 ir.get("basis_rotation_instructions") or []
-        src = _get(ir, "source")
-        if src is not None:
-            raise NotImplementedError("OpenQASM programs need the reference's OpenQASM "
-                                      "interpreter; submit the circuit as JAQCD")
         return (_get(ir, "instructions", []), _get(ir, "results") or [],
                 _get(ir, "basis_rotation_instructions") or [])
 
     def run(self, ir, *args, **kwargs):
+        """Dispatch on the program kind (simulator.py:152-155)."""
+        if self._is_openqasm(ir):
+            return self.run_openqasm(ir, *args, **kwargs)
         return self.run_jaqcd(ir, *args, **kwargs)
 
-    def run_jaqcd(self, ir, *, shots: int = 0, qubit_count_cpu: Optional[int] = None,
-                  qubit_count_cuda: Optional[int] = None,
-                  qubit_count_slice: Optional[int] = None, accelerator: Optional[str] = None,
-                  prefetch: Optional[str] = None, offload: Optional[str] = None,
-                  path=None, block_count: Optional[int] = None,
-                  precision: str = "complex128", fuse: int = 5,
-                  exact_sampling: bool = True):
+    def run_jaqcd(self, ir, *, shots: int = 0, **kwargs):
         instructions, results, basis = self._parse_program(ir)
         ops = [from_braket_instruction(i) for i in instructions]
         if shots > 0 and basis:
@@ -142,6 +147,30 @@ class BaseSimulator:
         for r in results:
             for q in (_get(r, "targets") or []):
                 used.add(int(q))
+        return self._simulate(ir, ops, results, used, shots, None, **kwargs)
+
+    def run_openqasm(self, ir, *, shots: int = 0, inputs: Optional[dict] = None, **kwargs):
+        """OpenQASM 3 program (simulator.py:380-437): gates, result pragmas and
+        measured qubits from ``snuqs_amd.openqasm``; with shots, the
+        observables' basis rotations run before sampling."""
+        if isinstance(ir, bytes):
+            ir = ir.decode()
+        source = ir if isinstance(ir, str) else _get(ir, "source")
+        given = {} if isinstance(ir, str) else dict(_get(ir, "inputs") or {})
+        given.update(inputs or {})
+        circuit = openqasm.parse(source, given)
+        ops = list(circuit.instructions)
+        if shots > 0:
+            ops += openqasm.basis_rotations(circuit.results, sorted(circuit.qubit_set))
+        return self._simulate(ir, ops, circuit.results, set(circuit.qubit_set), shots,
+                              circuit.measured_qubits or None, **kwargs)
+
+    def _simulate(self, ir, ops, results, used, shots, measured_qubits, *,
+                  qubit_count_cpu: Optional[int] = None, qubit_count_cuda: Optional[int] = None,
+                  qubit_count_slice: Optional[int] = None, accelerator: Optional[str] = None,
+                  prefetch: Optional[str] = None, offload: Optional[str] = None,
+                  path=None, block_count: Optional[int] = None, qubit_count: Optional[int] = None,
+                  precision: str = "complex128", fuse: int = 5, exact_sampling: bool = True):
         qmap = self._contiguous_qubit_mapping(used)
         for op in ops:
             op.targets = [qmap[q] for q in op.targets]
@@ -161,7 +190,10 @@ class BaseSimulator:
                 values.append({"type": r, "value": self._calculate(r, sim, qmap, qubit_count)})
         measurements = []
         if shots:
-            measurements = self._formatted_measurements(sim.retrieve_samples(), qubit_count)
+            mapped = [qmap[q] for q in measured_qubits] if measured_qubits else None
+            measurements = self._formatted_measurements(sim.retrieve_samples(), qubit_count,
+                                                        mapped)
+        measured = list(measured_qubits) if measured_qubits else list(range(qubit_count))
         meta = {"id": str(uuid.uuid4()), "shots": shots, "deviceId": self.DEVICE_ID}
         try:  # the Braket schemas, when installed
             from braket.task_result import (AdditionalMetadata, GateModelTaskResult,
@@ -172,9 +204,9 @@ class BaseSimulator:
                 additionalMetadata=AdditionalMetadata(action=ir),
                 resultTypes=[ResultTypeValue.construct(type=v["type"], value=v["value"])
                              for v in values],
-                measurements=measurements, measuredQubits=list(range(qubit_count)))
+                measurements=measurements, measuredQubits=measured)
         except ImportError:
-            return TaskResult(measurements=measurements, measuredQubits=list(range(qubit_count)),
+            return TaskResult(measurements=measurements, measuredQubits=measured,
                               resultTypes=values, taskMetadata=meta)
 
     @staticmethod
@@ -197,16 +229,29 @@ class BaseSimulator:
         if t in ("expectation", "variance"):
             obs = _get(r, "observable")
             letters = [str(o).upper() for o in obs]
+            if any(L not in ("X", "Y", "Z", "I", "H") for L in letters):
+                raise NotImplementedError("only Pauli products with H and I factors are "
+                                          "supported")
             if len(letters) == 1 and not targets:
-                raise ValueError(f"a {t} over all qubits needs targets")
+                # no targets: the observable on each qubit in turn
+                return [BaseSimulator._observable_value(t, sim, letters, [q]) for q in range(n)]
             if len(letters) == 1 and len(targets) > 1:
                 letters = letters * len(targets)
-            if any(L not in ("X", "Y", "Z", "I") for L in letters):
-                raise NotImplementedError("only Pauli-product observables are supported")
-            e = sim.expectation(["".join(letters)], [targets], [1.0])
-            # a Pauli product squares to I: Var = <P^2> - <P>^2 = 1 - <P>^2
-            return e if t == "expectation" else 1.0 - e * e
+            return BaseSimulator._observable_value(t, sim, letters, targets)
         raise NotImplementedError(f"Result type {t} not supported")
+
+    @staticmethod
+    def _observable_value(t, sim: StateVectorSimulation, letters, targets) -> float:
+        # H = (X + Z)/sqrt(2) per factor: a sum of Pauli strings
+        strings, coeffs = [""], [1.0]
+        for L in letters:
+            opts = ("X", "Z") if L == "H" else (L,)
+            w = math.sqrt(0.5) if L == "H" else 1.0
+            strings = [s + o for s in strings for o in opts]
+            coeffs = [c * w for c in coeffs for _ in opts]
+        e = sim.expectation(strings, [targets] * len(strings), coeffs)
+        # every factor squares to I: Var = <O^2> - <O>^2 = 1 - <O>^2
+        return e if t == "expectation" else 1.0 - e * e
 
 
 class StateVectorSimulator(BaseSimulator):
@@ -217,16 +262,34 @@ class StateVectorSimulator(BaseSimulator):
 
     @property
     def properties(self) -> dict:
+        results = [{"name": n} for n in
+                   ("StateVector", "Amplitude", "Probability", "Expectation", "Variance")]
         return {
             "service": {"shotsRange": [0, 1 << 20]},
-            "action": {"braket.ir.jaqcd.program": {
-                "actionType": "braket.ir.jaqcd.program", "version": ["1"],
-                "supportedOperations": sorted(k for k in BRAKET_GATES
-                                              if k not in ("U", "gphase")),
-                "supportedResultTypes": [{"name": n} for n in
-                                         ("StateVector", "Amplitude", "Probability",
-                                          "Expectation", "Variance")],
-            }},
+            "action": {
+                "braket.ir.jaqcd.program": {
+                    "actionType": "braket.ir.jaqcd.program", "version": ["1"],
+                    "supportedOperations": sorted(k for k in BRAKET_GATES
+                                                  if k not in ("U", "gphase")),
+                    "supportedResultTypes": results,
+                },
+                "braket.ir.openqasm.program": {
+                    "actionType": "braket.ir.openqasm.program", "version": ["1"],
+                    "supportedOperations": sorted(
+                        [k for k in BRAKET_GATES if k != "unitary"] +
+                        ["cv", "ecr", "gpi", "gpi2", "ms", "prx"]),
+                    "supportedModifiers": [
+                        {"name": "ctrl"}, {"name": "negctrl"},
+                        {"name": "pow", "exponent_types": ["int", "float"]}, {"name": "inv"}],
+                    "supportedPragmas": ["braket_unitary_matrix", "braket_verbatim",
+                                         "braket_result_type_state_vector",
+                                         "braket_result_type_amplitude",
+                                         "braket_result_type_probability",
+                                         "braket_result_type_expectation",
+                                         "braket_result_type_variance"],
+                    "supportedResultTypes": results,
+                },
+            },
             "paradigm": {"qubitCount": self.max_qubits},
         }
 

@@ -110,3 +110,85 @@ def test_shots_formatting_and_errors():
         sim.run(ghz_program(2), accelerator="cpu")
     with pytest.raises(TypeError):
         sim.run(ghz_program(2), offload="storage")
+
+
+QASM = """OPENQASM 3.0;
+input float theta;
+bit[3] b;
+qubit[4] q;
+gate entangle(t) a, c { h a; cnot a, c; rz(t) c; }
+entangle(theta) q[0], q[3];
+ctrl @ ry(0.4) q[3], q[1];
+negctrl @ pow(1/2) @ x q[1], q[0];
+inv @ entangle(0.3) q[1], q[0];
+#pragma braket unitary([[0.6, 0.8im], [0.8im, 0.6]]) q[3]
+gphase(0.25);
+b[0] = measure q[3];
+b[1] = measure q[0];
+b[2] = measure q[1];
+"""
+
+
+@pytest.mark.parametrize("precision", ["complex128", "complex64"])
+def test_run_openqasm_against_oracle(precision):
+    """OpenQASM 3 through StateVectorSimulator.run: the declared-but-unused
+    qubit q[2] is dropped by the contiguous remap (simulator.py:278-307), the
+    result pragmas are computed on the GPU and checked against the CPU oracle
+    on the frontend's own gate list."""
+    from snuqs_amd.openqasm import parse
+    from snuqs_amd.simulator import StateVectorSimulator
+
+    results = """#pragma braket result state_vector
+#pragma braket result expectation z(q[0]) @ h(q[3])
+#pragma braket result variance x(q[1])
+#pragma braket result expectation z all
+#pragma braket result probability q[3], q[0]
+#pragma braket result amplitude "010", "111"
+"""
+    src = QASM.replace("b[0] = measure q[3];\nb[1] = measure q[0];\nb[2] = measure q[1];\n",
+                       results)
+    res = StateVectorSimulator().run({"source": src, "inputs": {"theta": 0.9}},
+                                     precision=precision)
+    c = parse(src, {"theta": 0.9})
+    qmap = {0: 0, 1: 1, 3: 2}
+    for op in c.instructions:
+        op.targets = [qmap[q] for q in op.targets]
+    ref = run_oracle(3, c.instructions)
+    tol = 1e-12 if precision == "complex128" else 1e-5
+    vals = [r["value"] for r in res.resultTypes]
+    assert np.abs(vals[0].astype(np.complex128) - ref).max() < tol
+    s2 = np.sqrt(0.5)
+    want = s2 * (O.expect_pauli(ref, "ZX", [0, 2], 3) + O.expect_pauli(ref, "ZZ", [0, 2], 3))
+    assert abs(vals[1] - want.real) < tol
+    assert abs(vals[2] - (1 - O.expect_pauli(ref, "X", [1], 3).real ** 2)) < 4 * tol
+    assert np.abs(np.array(vals[3]) -
+                  [O.expect_pauli(ref, "Z", [q], 3).real for q in range(3)]).max() < tol
+    pm = (np.abs(ref) ** 2).reshape(2, 2, 2)
+    assert np.abs(vals[4] - pm.sum(axis=1).T.reshape(-1)).max() < tol
+    assert abs(vals[5]["010"] - ref[2]) < tol and abs(vals[5]["111"] - ref[7]) < tol
+
+
+def test_run_openqasm_shots_and_measured_qubits():
+    """Measured qubits order the bit characters; a register-wide observable
+    adds its basis rotation before sampling (circuit.py:95-143)."""
+    from snuqs_amd.openqasm import parse
+    from snuqs_amd.simulator import StateVectorSimulator
+
+    sim = StateVectorSimulator()
+    np.random.seed(5)
+    res = sim.run({"source": QASM, "inputs": {"theta": 0.9}}, shots=2000)
+    assert res.measuredQubits == [3, 0, 1]
+    c = parse(QASM, {"theta": 0.9})
+    qmap = {0: 0, 1: 1, 3: 2}
+    for op in c.instructions:
+        op.targets = [qmap[q] for q in op.targets]
+    p = np.abs(run_oracle(3, c.instructions)) ** 2
+    np.random.seed(5)
+    idx = np.random.choice(8, p=p, size=2000)
+    want = [[str((i >> (2 - q)) & 1) for q in (2, 0, 1)] for i in idx]
+    assert res.measurements == want
+    # GHZ measured in the X basis on every qubit: parity of the outcomes is even
+    ghz = ("OPENQASM 3.0;\nqubit[3] q;\nh q[0];\ncnot q[0], q[1];\ncnot q[1], q[2];\n"
+           "#pragma braket result expectation x all\n")
+    res = sim.run(ghz, shots=500)
+    assert all(sum(int(ch) for ch in m) % 2 == 0 for m in res.measurements)
