@@ -105,9 +105,13 @@ __device__ __forceinline__ uint64_t slot_beta(const QInstDev &q, int l, int part
 //   L1: j = (t & 15) | r << 4 | (t >> 4) << 8      (bits 4-7 in r)
 //   L0: j = r | t << 4                             (bits 0-3 in r)
 // RX on the r bits is applied in registers; layouts change through LDS with
-// slot swz(j), which keeps every one of the three access patterns free of
-// bank conflicts for 8 B and 16 B elements.
-__device__ __forceinline__ int qswz(int j) { return j ^ ((j >> 5) & 15) ^ (((j >> 8) & 1) << 4); }
+// slot swz(j) = j + j / 16 (one pad slot per 16), which makes every slot a
+// per-thread base plus a multiple of r -- 17 r (L0, L1: the pads shift the
+// 16-element rows) or 272 r (L2) -- so the addresses fold into instruction
+// offsets, and keeps the three patterns free of bank conflicts for 8 B and
+// 16 B elements (L2 with 8 B elements: one 2-way pair per 32 lanes).
+constexpr int kQSlots = kQChunk + kQChunk / 16;
+__device__ __forceinline__ int qswz(int j) { return j + (j >> 4); }
 __device__ __forceinline__ int qlay(int layout, int t, int r) {
   return layout == 2 ? t + 256 * r
          : layout == 1 ? (t & 15) | (r << 4) | ((t >> 4) << 8)
@@ -141,9 +145,18 @@ __device__ __forceinline__ void qrx16(T (&v)[16], R c0, R s0) {
     }
 }
 
+// Im(conj(l) a) of one amplitude pair in the state's precision: for complex64
+// one FMA after one product (|error| <= 2 ulp of |l||a|, the state's own
+// rounding scale), so only the per-pair result is widened to FP64 and the
+// amplitudes stay in 32-bit registers.
+template <typename T>
+__device__ __forceinline__ auto qimcl(const T &l, const T &a) {
+  return fma(l.x, a.y, -(l.y * a.x));
+}
+
 // sum over the 4 register bits b and pairs (r, r^b) of
 // Im(conj(l_r) a_{r^b} + conj(l_{r^b}) a_r)  (= Im <l| X_b |a> over the group),
-// products and sums in FP64 as two dependent FMA chains (few live registers)
+// accumulated in FP64
 template <typename T>
 __device__ __forceinline__ double qxpart16(const T (&a)[16], const T (&l)[16]) {
   double s0 = 0.0, s1 = 0.0;
@@ -153,10 +166,8 @@ __device__ __forceinline__ double qxpart16(const T (&a)[16], const T (&l)[16]) {
     for (int r = 0; r < 16; ++r) {
       if (r & (1 << b)) continue;
       const int u = r | (1 << b);
-      s0 = fma((double)l[r].x, (double)a[u].y, s0);
-      s0 = fma(-(double)l[r].y, (double)a[u].x, s0);
-      s1 = fma((double)l[u].x, (double)a[r].y, s1);
-      s1 = fma(-(double)l[u].y, (double)a[r].x, s1);
+      s0 += (double)qimcl(l[r], a[u]);
+      s1 += (double)qimcl(l[u], a[r]);
     }
   return s0 + s1;
 }
@@ -214,7 +225,7 @@ __global__ void __launch_bounds__(kBlock)
                         const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
                         int count, int layer) {
   using T = typename CV<R>::T;
-  __shared__ T X[kQChunk];
+  __shared__ T X[kQSlots];
   __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
   __shared__ int rint[16];
   const int ei = find_entry(tab, count, blockIdx.x);
@@ -441,8 +452,7 @@ __global__ void __launch_bounds__(kBlock)
       for (int r = 0; r < D; ++r) {
         if (r & (1 << b)) continue;
         const int u = r | (1 << b);
-        acc += (double)l[r].x * a[u].y - (double)l[r].y * a[u].x + (double)l[u].x * a[r].y -
-               (double)l[u].y * a[r].x;
+        acc += (double)qimcl(l[r], a[u]) + (double)qimcl(l[u], a[r]);
       }
     // RX(-2b) = [[c, i s], [i s, c]] on every bit of the group
 #pragma unroll
@@ -472,13 +482,13 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 template <typename R>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
     k_qaoa_low_rev_full(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
                         const uint8_t *__restrict__ edges, const double *__restrict__ gam,
                         const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
                         int count, int layer, double *__restrict__ partial) {
   using T = typename CV<R>::T;
-  __shared__ T X[kQChunk];
+  __shared__ T X[kQSlots];
   __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
   __shared__ int rint[16];
   const int ei = find_entry(tab, count, blockIdx.x);
