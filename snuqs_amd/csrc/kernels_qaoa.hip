@@ -34,8 +34,8 @@ namespace sq {
 constexpr int kQL = 12;            // log2 chunk size
 constexpr int kQChunk = 1 << kQL;  // amplitudes per chunk
 constexpr int kQMaxEdges = 128;
-constexpr int kQGF = 4;            // high bits per forward pass
-constexpr int kQGR = 3;            // high bits per reverse pass (psi and lambda in registers)
+constexpr int kQGF = 5;            // high bits per forward pass
+constexpr int kQGR = 4;            // high bits per reverse pass (psi and lambda in registers)
 
 struct QInstDev {
   uint64_t psi, lam;  // amplitude offsets into the batch state buffer
@@ -839,7 +839,8 @@ static int enqueue_t(QaoaBatch *B, hipStream_t st) {
                              B->p, tab, L.count, L.layer);
         break;
       case 1: {
-        auto k = L.nb == 4 ? k_qaoa_high_fwd<R, 4>
+        auto k = L.nb == 5 ? k_qaoa_high_fwd<R, 5>
+                 : L.nb == 4 ? k_qaoa_high_fwd<R, 4>
                  : L.nb == 3 ? k_qaoa_high_fwd<R, 3>
                  : L.nb == 2 ? k_qaoa_high_fwd<R, 2> : k_qaoa_high_fwd<R, 1>;
         hipLaunchKernelGGL(k, g, bl, 0, st, s, B->inst, bet, B->p, tab, L.count, L.layer, L.bit);
@@ -850,7 +851,8 @@ static int enqueue_t(QaoaBatch *B, hipStream_t st) {
                            B->partial);
         break;
       case 3: {
-        auto k = L.nb == 3 ? k_qaoa_high_rev<R, 3>
+        auto k = L.nb == 4 ? k_qaoa_high_rev<R, 4>
+                 : L.nb == 3 ? k_qaoa_high_rev<R, 3>
                  : L.nb == 2 ? k_qaoa_high_rev<R, 2> : k_qaoa_high_rev<R, 1>;
         hipLaunchKernelGGL(k, g, bl, 0, st, s, B->inst, bet, B->p, tab, L.count, L.layer, L.bit,
                            B->partial);

@@ -79,12 +79,12 @@ def test_qaoa_batch_energies_and_gradients(C, dtype, etol, gtol):
 def test_qaoa_gradients_match_general_adjoint(C):
     """The batch kernels against the general engine's adjoint on the same
     circuit (ZZ(2 gamma): dE/dgamma = 2 * sum of the ZZ-angle derivatives).
-    Sizes 13-20 give every high-bit group size of the forward (1-4 bits) and
-    reverse (1-3 bits) passes, and instances that stop inside a group."""
+    Sizes 13-20 give every high-bit group size of the forward (1-5 bits) and
+    reverse (1-4 bits) passes, and instances that stop inside a group."""
     from snuqs_amd.circuits import qaoa_maxcut
     from snuqs_amd.qaoa import qaoa_batch
 
-    sizes = [14, 10, 17, 19, 13, 15, 20]
+    sizes = [14, 10, 17, 19, 13, 15, 20, 16]
     p = 2
     graphs, gam, bet = _batch(sizes, p, 5)
     out = qaoa_batch(sizes, graphs, p, dtype="complex128").run(gam, bet, use_graph=False)
