@@ -18,7 +18,11 @@ serialised to, producing the engine's gate objects directly:
 * ``gphase(x);`` and controlled phases (``ctrl @ gphase(x) a``);
 * ``#pragma braket unitary(<matrix>) qubits``; ``#pragma braket result``
   state_vector / probability / amplitude / expectation / variance of Pauli
-  products with ``h`` and ``i`` factors (``all`` or per-factor qubits);
+  products with ``h`` and ``i`` factors (``all`` or per-factor qubits), and
+  ``adjoint_gradient expectation(<sum of c * products>) <inputs | all>``:
+  inputs are carried as dual numbers through the angle expressions, so each
+  gate records d angle / d input and the engine's adjoint sweep gives the
+  gradient by the chain rule;
 * ``measure`` in its assignment, bare and ``->`` forms; ``barrier`` and
   ``box { }`` (verbatim blocks) are transparent.
 
@@ -100,6 +104,96 @@ _MATRIX_GATES = {  # name -> (qubits, accepted parameter counts, matrix function
     "ms": (2, (2, 3), _ms), "ecr": (2, (0,), lambda: _ECR),
 }
 _NAMED = {k: v for k, v in BRAKET_GATES.items() if k not in ("unitary", "gphase")}
+# one-parameter gates exp(-i t A / 2)-style, for which G(t)^k = G(k t)
+_EXP_FAMILY = {"rx", "ry", "rz", "phaseshift", "cphaseshift", "cphaseshift00", "cphaseshift01",
+               "cphaseshift10", "xx", "yy", "zz", "xy"}
+
+
+def _fraction(p) -> Fraction:
+    if isinstance(p, Dual):
+        p = p.v
+    return Fraction(p) if isinstance(p, (int, Fraction)) else Fraction(float(p))
+
+
+class Dual:
+    """A real value with its partial derivatives with respect to the program's
+    inputs (forward-mode differentiation of the angle expressions)."""
+    __slots__ = ("v", "d")
+
+    def __init__(self, v, d=None):
+        self.v, self.d = float(v), dict(d or {})
+
+    @staticmethod
+    def _lift(x):
+        return x if isinstance(x, Dual) else Dual(x)
+
+    def _comb(self, o, dv_self, dv_o, v):
+        o = Dual._lift(o)
+        d = {k: dv_self * x for k, x in self.d.items()}
+        for k, x in o.d.items():
+            d[k] = d.get(k, 0.0) + dv_o * x
+        return Dual(v, d)
+
+    def __add__(self, o):
+        return self._comb(o, 1.0, 1.0, self.v + float(Dual._lift(o).v))
+
+    __radd__ = __add__
+
+    def __sub__(self, o):
+        return self._comb(o, 1.0, -1.0, self.v - Dual._lift(o).v)
+
+    def __rsub__(self, o):
+        return Dual._lift(o) - self
+
+    def __mul__(self, o):
+        o = Dual._lift(o)
+        return self._comb(o, o.v, self.v, self.v * o.v)
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, o):
+        o = Dual._lift(o)
+        return self._comb(o, 1.0 / o.v, -self.v / (o.v * o.v), self.v / o.v)
+
+    def __rtruediv__(self, o):
+        return Dual._lift(o) / self
+
+    def __pow__(self, o):
+        o = Dual._lift(o)
+        v = self.v ** o.v
+        dlog = math.log(self.v) if self.v > 0 else 0.0
+        return self._comb(o, o.v * self.v ** (o.v - 1) if self.v else 0.0, v * dlog, v)
+
+    def __rpow__(self, o):
+        return Dual._lift(o) ** self
+
+    def __neg__(self):
+        return Dual(-self.v, {k: -x for k, x in self.d.items()})
+
+    def __mod__(self, o):
+        return Dual(self.v % float(Dual._lift(o).v), self.d)
+
+    def __float__(self):
+        return self.v
+
+    def __int__(self):
+        return int(self.v)
+
+
+_DERIV = {"sin": math.cos, "cos": lambda x: -math.sin(x), "tan": lambda x: 1 / math.cos(x) ** 2,
+          "arcsin": lambda x: 1 / math.sqrt(1 - x * x), "arccos": lambda x: -1 / math.sqrt(1 - x * x),
+          "arctan": lambda x: 1 / (1 + x * x), "exp": math.exp, "log": lambda x: 1 / x,
+          "sqrt": lambda x: 0.5 / math.sqrt(x)}
+
+
+def _apply_function(name, args):
+    f = _FUNCTIONS[name]
+    if any(isinstance(a, Dual) for a in args):
+        if name not in _DERIV or len(args) != 1:
+            raise NotImplementedError(f"{name} of a program input cannot be differentiated")
+        x = args[0]
+        return Dual(f(x.v), {k: _DERIV[name](x.v) * dx for k, dx in x.d.items()})
+    return f(*[float(a) for a in args])
 
 
 def unitary_power(m: np.ndarray, p: float) -> np.ndarray:
@@ -124,9 +218,15 @@ class Circuit:
     results: list = field(default_factory=list)
     measured_qubits: list = field(default_factory=list)
     qubit_set: set = field(default_factory=set)
+    # per instruction: [(angle index, {input: d angle / d input})], or a set of
+    # input names for a gate whose dependence on them is not differentiable
+    # here (fractional powers, matrix gates)
+    input_deps: list = field(default_factory=list)
+    inputs: list = field(default_factory=list)
 
-    def add_instruction(self, op) -> None:
+    def add_instruction(self, op, deps=None) -> None:
         self.instructions.append(op)
+        self.input_deps.append(deps if deps is not None else [])
         self.qubit_set |= set(op.targets)
 
 
@@ -280,6 +380,10 @@ def _eval(e, env):
             return a - b
         if op == "*":
             return a * b
+        if isinstance(a, Fraction) and isinstance(b, Dual):
+            a = float(a)
+        if isinstance(b, Fraction) and isinstance(a, Dual):
+            b = float(b)
         if op == "/":
             if isinstance(a, int) and isinstance(b, int) and not isinstance(a, bool):
                 return Fraction(a, b) if a % b else a // b
@@ -288,16 +392,17 @@ def _eval(e, env):
             return a % b
         return a ** b
     if k == "call":
-        f = _FUNCTIONS.get(e[1])
-        if f is None:
+        if e[1] not in _FUNCTIONS:
             raise QasmError(f"line {e[3]}: unknown function {e[1]!r}")
-        return f(*[float(_eval(a, env)) for a in e[2]])
+        return _apply_function(e[1], [_eval(a, env) for a in e[2]])
     if k == "list":
         return [_eval(x, env) for x in e[1]]
     raise QasmError(f"bad expression {e!r}")
 
 
 def _num(v) -> float:
+    if isinstance(v, Dual):
+        return v.v
     if isinstance(v, complex):
         if v.imag:
             raise QasmError(f"a real number is needed, got {v}")
@@ -600,7 +705,8 @@ class _Interpreter:
         elif k == "input":
             if s[1] not in self.inputs:
                 raise QasmError(f"line {s[2]}: no value was given for input {s[1]!r}")
-            self.env[s[1]] = self.inputs[s[1]]
+            self.env[s[1]] = Dual(self.inputs[s[1]], {s[1]: 1.0})
+            self.circuit.inputs.append(s[1])
         elif k == "const":
             self.env[s[1]] = _eval(s[2], self.env)
         elif k == "gatedef":
@@ -648,27 +754,30 @@ class _Interpreter:
             elif mname == "inv":
                 power = -power
             else:
-                p = _eval(arg, ev)
-                power *= Fraction(p) if isinstance(p, (int, Fraction)) else Fraction(float(p))
+                power *= _fraction(_eval(arg, ev))
         if len(set(qubits)) != len(qubits):
             raise ValueError(f"line {line}: gate {name} is applied to repeated qubits {qubits}")
         nc = len(ctrl)
         if nc > len(qubits):
             raise ValueError(f"line {line}: {nc} control(s) need more than {len(qubits)} qubit(s)")
         cq, tq = qubits[:nc], qubits[nc:]
-        self.emit(name, [_num(p) for p in params], cq, ctrl, tq, power, line)
+        self.emit(name, list(params), cq, ctrl, tq, power, line)
 
-    def emit(self, name, params, cq, ctrl, tq, power: Fraction, line) -> None:
+    def emit(self, name, raw, cq, ctrl, tq, power: Fraction, line) -> None:
+        """raw: parameter values, Dual where they depend on inputs."""
         if power == 0:
             return
         integer = power.denominator == 1
+        params = [_num(p) for p in raw]
+        deps = [p.d if isinstance(p, Dual) else {} for p in raw]
+        used = {k for d in deps for k in d}
         if name in self.gates:
             fparams, fq, body = self.gates[name]
             if len(params) != len(fparams) or len(tq) != len(fq):
                 raise ValueError(f"line {line}: gate {name} takes {len(fparams)} parameter(s) "
                                  f"and {len(fq)} qubit(s)")
             if integer:
-                seq = self._expand(name, params, tq)
+                seq = self._expand(name, raw, tq)
                 if power < 0:
                     seq = [(n, p, q, c, -pw) for (n, p, q, c, pw) in reversed(seq)]
                 for _ in range(abs(int(power))):
@@ -676,7 +785,7 @@ class _Interpreter:
                         self.emit(n, p, cq + q[:len(c)], ctrl + c, q[len(c):], pw, line)
             else:
                 m = self._body_matrix(name, params, tq)
-                self._unitary(cq + tq, unitary_power(m, float(power)), ctrl)
+                self._unitary(cq + tq, unitary_power(m, float(power)), ctrl, used)
             return
         if name == "gphase":  # on qubits without controls it is still a global phase
             theta = params[0] if len(params) == 1 else None
@@ -688,12 +797,13 @@ class _Interpreter:
                 phase = float(np.angle(unitary_power(np.array([[np.exp(1j * theta)]]),
                                                      float(power))[0, 0]))
             if ctrl:
-                self._add(g.GPhase(cq, phase, ctrl_modifiers=ctrl))
+                d = [(0, {k: x * int(power) for k, x in deps[0].items()})] if integer else used
+                self._add(g.GPhase(cq, phase, ctrl_modifiers=ctrl), d if deps[0] else None)
             else:
-                self._add(g.GPhase([], phase))
+                self._add(g.GPhase([], phase))  # global: no effect on expectations
             return
         if name == "cv":
-            self.emit("v", params, cq + tq[:1], ctrl + [1], tq[1:], power, line)
+            self.emit("v", raw, cq + tq[:1], ctrl + [1], tq[1:], power, line)
             return
         if name in _MATRIX_GATES:
             nq, npar, fn = _MATRIX_GATES[name]
@@ -701,17 +811,28 @@ class _Interpreter:
                 raise ValueError(f"line {line}: gate {name} takes {nq} qubit(s) and "
                                  f"{' or '.join(map(str, npar))} parameter(s)")
             m = fn(*params)
-            self._unitary(cq + tq, m if power == 1 else unitary_power(m, float(power)), ctrl)
+            self._unitary(cq + tq, m if power == 1 else unitary_power(m, float(power)), ctrl,
+                          used)
             return
         cls = _NAMED.get(name)
         if cls is None:
             raise QasmError(f"line {line}: gate {name!r} is not defined")
         try:
+            if integer and power != 1 and name in _EXP_FAMILY:
+                # G(t)^k = G(k t): keeps the gate differentiable
+                k = int(power)
+                params = [k * x for x in params]
+                deps = [{n: k * x for n, x in d.items()} for d in deps]
+                power = Fraction(1)
             if integer:
-                self._add(cls(cq + tq, *params, ctrl_modifiers=ctrl, power=int(power)))
+                op = cls(cq + tq, *params, ctrl_modifiers=ctrl, power=int(power))
+                if used and power != 1:
+                    self._add(op, used)
+                else:
+                    self._add(op, [(a, d) for a, d in enumerate(deps) if d])
             else:
                 base = np.array(cls(list(range(len(tq))), *params).matrix())
-                self._unitary(cq + tq, unitary_power(base, float(power)), ctrl)
+                self._unitary(cq + tq, unitary_power(base, float(power)), ctrl, used)
         except (TypeError, ValueError) as e:
             raise ValueError(f"line {line}: {name}: {e}") from None
 
@@ -745,12 +866,12 @@ class _Interpreter:
             m = om[0, 0] * m if not t else _apply_to_block(m, local, om, t)
         return m
 
-    def _unitary(self, qubits, m, ctrl) -> None:
+    def _unitary(self, qubits, m, ctrl, used=None) -> None:
         self._add(g.Unitary(list(qubits), np.asarray(m, dtype=np.complex128),
-                            ctrl_modifiers=list(ctrl)))
+                            ctrl_modifiers=list(ctrl)), set(used) if used else None)
 
-    def _add(self, op) -> None:
-        self.circuit.add_instruction(op)
+    def _add(self, op, deps=None) -> None:
+        self.circuit.add_instruction(op, deps)
 
     # -- pragmas --------------------------------------------------------
     def pragma(self, text: str, line: int) -> None:
@@ -801,6 +922,8 @@ class _Interpreter:
                 elif tok[1] != ",":
                     raise QasmError(f"line {line}: amplitude takes quoted bitstrings")
             return {"type": "amplitude", "states": states}
+        if kind == "adjoint_gradient":
+            return self._adjoint_result(ts, line)
         if kind in ("expectation", "variance", "sample"):
             if kind == "sample":
                 raise NotImplementedError("the sample result type needs shots; use measurements")
@@ -828,6 +951,59 @@ class _Interpreter:
         raise NotImplementedError(f"line {line}: result type {kind!r} is not supported")
 
 
+def _adjoint_result(self, ts: _Tokens, line: int) -> dict:
+    """adjoint_gradient expectation(<sum of coefficient * Pauli products>)
+    <input names | all>"""
+    ts.expect("expectation")
+    ts.expect("(")
+    terms, touched = [], set()
+    while True:
+        sign = 1.0
+        while ts.at("+") or ts.at("-"):
+            if ts.next()[1] == "-":
+                sign = -sign
+        coef = 1.0
+        if ts.peek()[0] == "num":
+            coef = float(ts.next()[1])
+            ts.expect("*")
+        letters, qubits = [], []
+        while True:
+            obs = ts.ident().lower()
+            if obs not in ("x", "y", "z", "h", "i"):
+                raise NotImplementedError(f"line {line}: observable {obs!r} is not supported "
+                                          "(Pauli products with h and i factors are)")
+            ts.expect("(")
+            q = [x for qa in _parse_qargs(ts, end=(")",)) for x in self._resolve(qa, None)]
+            ts.expect(")")
+            letters += [obs] * len(q)
+            qubits += q
+            if not ts.accept("@"):
+                break
+        terms.append((sign * coef, letters, qubits))
+        touched |= set(qubits)
+        if ts.at(")"):
+            break
+        if not (ts.at("+") or ts.at("-")):
+            raise QasmError(f"line {line}: expected '+', '-' or ')' in the observable")
+    ts.expect(")")
+    if ts.accept("all"):
+        names = None
+    else:
+        names = [ts.ident()]
+        while ts.accept(","):
+            names.append(ts.ident())
+        unknown = [n for n in names if n not in self.circuit.inputs]
+        if unknown:
+            raise QasmError(f"line {line}: {unknown} are not inputs of the program")
+    if not ts.done():
+        raise QasmError(f"line {line}: unexpected {ts.peek()[1]!r} after the parameters")
+    self.circuit.qubit_set |= touched
+    return {"type": "adjoint_gradient", "observable": terms, "parameters": names}
+
+
+_Interpreter._adjoint_result = _adjoint_result
+
+
 class _Recorder(_Interpreter):
     """Collects a gate body's calls instead of emitting gate objects."""
 
@@ -844,9 +1020,8 @@ class _Recorder(_Interpreter):
             elif mname == "inv":
                 power = -power
             else:
-                p = _eval(arg, ev)
-                power *= Fraction(p) if isinstance(p, (int, Fraction)) else Fraction(float(p))
-        self.out.append((name, [_num(p) for p in params], list(qubits), ctrl, power))
+                power *= _fraction(_eval(arg, ev))
+        self.out.append((name, list(params), list(qubits), ctrl, power))
 
 
 def parse(source: str, inputs: Optional[dict] = None) -> Circuit:

@@ -18,7 +18,9 @@ Differences from the reference, by design: the engine runs on the GPU only
 raise), and besides StateVector the result types Amplitude, Probability
 (marginals over up to 20 targets reduced on the GPU), Expectation and Variance
 (Pauli products with h and i factors; an observable without targets gives
-one value per qubit) are computed on the GPU.
+one value per qubit) are computed on the GPU, and OpenQASM programs may ask
+for Braket's AdjointGradient (expectation of a Pauli sum and its gradient
+with respect to the program's inputs, from the engine's adjoint sweep).
 """
 from __future__ import annotations
 
@@ -162,18 +164,21 @@ class BaseSimulator:
         ops = list(circuit.instructions)
         if shots > 0:
             ops += openqasm.basis_rotations(circuit.results, sorted(circuit.qubit_set))
+        ctx = {"deps": list(circuit.input_deps), "inputs": list(circuit.inputs)}
         return self._simulate(ir, ops, circuit.results, set(circuit.qubit_set), shots,
-                              circuit.measured_qubits or None, **kwargs)
+                              circuit.measured_qubits or None, adjoint_ctx=ctx, **kwargs)
 
     def _simulate(self, ir, ops, results, used, shots, measured_qubits, *,
                   qubit_count_cpu: Optional[int] = None, qubit_count_cuda: Optional[int] = None,
                   qubit_count_slice: Optional[int] = None, accelerator: Optional[str] = None,
                   prefetch: Optional[str] = None, offload: Optional[str] = None,
                   path=None, block_count: Optional[int] = None, qubit_count: Optional[int] = None,
-                  precision: str = "complex128", fuse: int = 5, exact_sampling: bool = True):
+                  precision: str = "complex128", fuse: int = 5, exact_sampling: bool = True,
+                  adjoint_ctx: Optional[dict] = None):
         qmap = self._contiguous_qubit_mapping(used)
         for op in ops:
             op.targets = [qmap[q] for q in op.targets]
+        in_order = list(ops)
         qubit_count = max(len(qmap), 1)
         acc = self._validate_simulation_type(accelerator)
         self._validate_prefetch_type(prefetch)
@@ -185,9 +190,16 @@ class BaseSimulator:
                                     exact_sampling=exact_sampling)
         sim.evolve(sub)
         values = []
+        if shots and any(_get(r, "type") == "adjoint_gradient" for r in results):
+            raise ValueError("adjoint_gradient needs shots=0")
         if not shots:
             for r in results:
-                values.append({"type": r, "value": self._calculate(r, sim, qmap, qubit_count)})
+                if _get(r, "type") == "adjoint_gradient":
+                    v = self._adjoint_gradient(r, sim, in_order, adjoint_ctx, qmap, qubit_count,
+                                               precision)
+                else:
+                    v = self._calculate(r, sim, qmap, qubit_count)
+                values.append({"type": r, "value": v})
         measurements = []
         if shots:
             mapped = [qmap[q] for q in measured_qubits] if measured_qubits else None
@@ -241,6 +253,68 @@ class BaseSimulator:
         raise NotImplementedError(f"Result type {t} not supported")
 
     @staticmethod
+    def _adjoint_gradient(r, sim: StateVectorSimulation, ops, ctx, qmap, n, precision) -> dict:
+        """Braket's AdjointGradient result: the expectation of a sum of Pauli
+        products and its gradient with respect to program inputs.  Every
+        (gate, angle) that depends on a requested input is one parameter of
+        the engine's adjoint sweep (Program.adjoint); the recorded
+        d angle / d input combine those into d<H>/d input (chain rule)."""
+        if ctx is None:
+            raise NotImplementedError("adjoint_gradient needs an OpenQASM program with inputs")
+        names = r["parameters"] if r["parameters"] is not None else list(ctx["inputs"])
+        want = set(names)
+        keep, pmap, slots = [], [], []
+        for op, d in zip(ops, ctx["deps"]):
+            if isinstance(d, set):
+                if d & want:
+                    raise NotImplementedError(
+                        f"{op} depends on {sorted(d & want)} through a fractional power or a "
+                        "matrix gate; its gradient is not supported")
+                d = []
+            if not list(op.targets):
+                continue  # a global phase changes no expectation
+            entry = []
+            for a, coefs in d:
+                if want.intersection(coefs):
+                    entry.append((a, len(slots)))
+                    slots.append(coefs)
+            keep.append(op)
+            pmap.append(entry)
+        paulis, qubits, coeffs, const = [], [], [], 0.0
+        for coef, letters, qs in r["observable"]:
+            strings = [("", [], float(coef))]
+            for L, q in zip(letters, qs):
+                L = L.upper()
+                if L == "I":
+                    continue
+                opts, w = (("X", "Z"), math.sqrt(0.5)) if L == "H" else ((L,), 1.0)
+                strings = [(st + o, t + [qmap[q]], c * w) for st, t, c in strings for o in opts]
+            for st, t, c in strings:
+                if st:
+                    paulis.append(st)
+                    qubits.append(t)
+                    coeffs.append(c)
+                else:
+                    const += c
+        grads = [0.0] * len(slots)
+        if not paulis:
+            e = 0.0
+        elif not slots:
+            e = sim.expectation(paulis, qubits, coeffs)
+        else:
+            mk = lambda: _C.result_types.StateVector(_C.DeviceType.CUDA, n,  # noqa: E731
+                                                     dtype=precision)
+            prog = _C.functionals.Program(n, keep, dtype=precision)
+            e, grads = prog.adjoint(mk(), mk(), mk(), [], pmap, len(slots), paulis, qubits,
+                                    coeffs)
+        gradient = {nm: 0.0 for nm in names}
+        for k, coefs in enumerate(slots):
+            for nm, c in coefs.items():
+                if nm in gradient:
+                    gradient[nm] += c * float(grads[k])
+        return {"expectation": float(e) + const, "gradient": gradient}
+
+    @staticmethod
     def _observable_value(t, sim: StateVectorSimulation, letters, targets) -> float:
         # H = (X + Z)/sqrt(2) per factor: a sum of Pauli strings
         strings, coeffs = [""], [1.0]
@@ -282,6 +356,7 @@ class StateVectorSimulator(BaseSimulator):
                         {"name": "ctrl"}, {"name": "negctrl"},
                         {"name": "pow", "exponent_types": ["int", "float"]}, {"name": "inv"}],
                     "supportedPragmas": ["braket_unitary_matrix", "braket_verbatim",
+                                         "braket_result_type_adjoint_gradient",
                                          "braket_result_type_state_vector",
                                          "braket_result_type_amplitude",
                                          "braket_result_type_probability",

@@ -192,3 +192,67 @@ def test_run_openqasm_shots_and_measured_qubits():
            "#pragma braket result expectation x all\n")
     res = sim.run(ghz, shots=500)
     assert all(sum(int(ch) for ch in m) % 2 == 0 for m in res.measurements)
+
+
+ADJ = """OPENQASM 3.0;
+input float theta;
+input float phi;
+qubit[4] q;
+gate layer(a) x, y { rx(2*a) x; ctrl @ rz(a) x, y; cnot y, x; }
+h q;
+ry(theta) q[0];
+inv @ layer(phi) q[1], q[2];
+pow(2) @ xx(theta * phi) q[0], q[3];
+ctrl @ gphase(sin(phi)) q[2];
+negctrl @ ry(theta / 3) q[3], q[1];
+U(theta, 0.2, phi) q[2];
+layer(theta - phi) q[3], q[0];
+#pragma braket result adjoint_gradient expectation(-0.6 * y(q[0]) @ i(q[1]) + 0.75 * x(q[2]) @ z(q[1]) - 1.5 * h(q[3]) @ z(q[0])) theta, phi
+"""
+
+
+def _oracle_energy(src, inputs):
+    from snuqs_amd.openqasm import parse
+
+    c = parse(src, inputs)
+    psi = run_oracle(4, [op for op in c.instructions if list(op.targets)])
+    e = 0.0
+    for coef, letters, qs in c.results[0]["observable"]:
+        terms = [("", [], coef)]
+        for L, q in zip(letters, qs):
+            if L == "i":
+                continue
+            opts, w = (("X", "Z"), np.sqrt(0.5)) if L == "h" else ((L.upper(),), 1.0)
+            terms = [(s + o, t + [q], cc * w) for s, t, cc in terms for o in opts]
+        for s, t, cc in terms:
+            e += cc * O.expect_pauli(psi, s, t, 4).real
+    return e
+
+
+@pytest.mark.parametrize("precision", ["complex128", "complex64"])
+def test_openqasm_adjoint_gradient_matches_finite_differences(precision):
+    """Braket's AdjointGradient result through StateVectorSimulator.run:
+    expectation and d/d(theta, phi) against the CPU oracle's energy and its
+    central differences (the chain rule through rx(2a), xx(theta*phi)^2,
+    inverted user gates, a controlled phase, a negated control and U)."""
+    from snuqs_amd.simulator import StateVectorSimulator
+
+    x0 = {"theta": 0.37, "phi": -1.1}
+    res = StateVectorSimulator().run({"source": ADJ, "inputs": x0}, precision=precision)
+    v = res.resultTypes[0]["value"]
+    tol = 1e-9 if precision == "complex128" else 1e-5
+    assert abs(v["expectation"] - _oracle_energy(ADJ, x0)) < tol
+    h = 1e-5
+    for name in ("theta", "phi"):
+        up = _oracle_energy(ADJ, {**x0, name: x0[name] + h})
+        dn = _oracle_energy(ADJ, {**x0, name: x0[name] - h})
+        fd = (up - dn) / (2 * h)
+        assert abs(v["gradient"][name] - fd) < max(tol, 1e-8), (name, v, fd)
+    # 'all' names every input; a gradient with shots is refused
+    src_all = ADJ.replace(") theta, phi", ") all")
+    v2 = StateVectorSimulator().run({"source": src_all, "inputs": x0},
+                                    precision=precision).resultTypes[0]["value"]
+    assert sorted(v2["gradient"]) == ["phi", "theta"]
+    assert abs(v2["gradient"]["phi"] - v["gradient"]["phi"]) < 1e-12
+    with pytest.raises(ValueError):
+        StateVectorSimulator().run({"source": ADJ, "inputs": x0}, shots=10)

@@ -256,3 +256,67 @@ def test_unitary_power_closed_forms():
 def test_errors(src, exc):
     with pytest.raises(exc):
         parse(f"OPENQASM 3.0;\nqubit[3] q;\n{src}\n")
+
+
+def test_dual_numbers_differentiate_angle_expressions():
+    """d angle / d input recorded per gate matches central differences of the
+    same expression (forward-mode rules for + - * / ** and the functions)."""
+    exprs = ["2*theta", "sin(phi)", "theta/phi", "theta**2", "exp(-theta)*cos(phi)",
+             "sqrt(phi) - arctan(theta)", "2**theta", "-(theta - 3*phi)/4", "log(phi)*π",
+             "arcsin(theta) + arccos(theta/2) + tan(phi)"]
+    x0 = {"theta": 0.3, "phi": 0.7}
+    h = 1e-6
+
+    def angle(expr, vals):
+        c = parse(f"OPENQASM 3.0;\ninput float theta;\ninput float phi;\nqubit q;\n"
+                  f"rx({expr}) q;\n", vals)
+        return c.instructions[0].angles[0], dict(c.input_deps[0][0][1]) if c.input_deps[0] else {}
+
+    for expr in exprs:
+        _, d = angle(expr, x0)
+        for name in ("theta", "phi"):
+            up, _ = angle(expr, {**x0, name: x0[name] + h})
+            dn, _ = angle(expr, {**x0, name: x0[name] - h})
+            fd = (up - dn) / (2 * h)
+            assert abs(d.get(name, 0.0) - fd) < 1e-7, (expr, name, d, fd)
+
+
+def test_input_dependencies_through_modifiers_and_user_gates():
+    c = parse("""OPENQASM 3.0;
+input float theta;
+input float phi;
+qubit[3] q;
+gate layer(a) x, y { rx(2*a) x; ctrl @ rz(a) x, y; }
+inv @ rz(theta + phi) q[2];
+pow(2) @ xx(theta) q[1], q[2];
+ctrl @ gphase(phi) q[2];
+inv @ layer(phi) q[0], q[1];
+pow(1/2) @ rx(phi) q[1];
+h q[0];
+#pragma braket result adjoint_gradient expectation(-6 * y(q[0]) @ i(q[1]) + 0.75 * y(q[2]) @ z(q[1]) - h(q[0])) theta, phi
+""", {"theta": 0.3, "phi": 0.7})
+    deps = c.input_deps
+    assert deps[0] == [(0, {"theta": -1.0, "phi": -1.0})]      # inv @ rz = rz(-t)
+    assert deps[1] == [(0, {"theta": 2.0})]                     # pow(2) @ xx = xx(2 t)
+    assert deps[2] == [(0, {"phi": 1.0})]                       # controlled phase
+    # inv @ layer: reversed body, each statement inverted
+    assert [op.name for op in c.instructions[3:5]] == ["RotZ", "RotX"]
+    assert deps[3] == [(0, {"phi": -1.0})] and deps[4] == [(0, {"phi": -2.0})]
+    assert deps[5] == {"phi"}                                   # fractional power: not differentiable
+    assert deps[6] == []
+    assert c.results[0] == {"type": "adjoint_gradient", "parameters": ["theta", "phi"],
+                            "observable": [(-6.0, ["y", "i"], [0, 1]),
+                                           (0.75, ["y", "z"], [2, 1]), (-1.0, ["h"], [0])]}
+    with pytest.raises(QasmError):
+        parse("OPENQASM 3.0;\ninput float t;\nqubit q;\nrx(t) q;\n"
+              "#pragma braket result adjoint_gradient expectation(z(q)) s\n", {"t": 0.1})
+
+
+@pytest.mark.parametrize("a,b", [
+    ("pow(3) @ rx(0.4) q[0];", "rx(1.2000000000000002) q[0];"),
+    ("pow(-2) @ cphaseshift(0.3) q[0], q[1];", "cphaseshift(-0.6) q[0], q[1];"),
+    ("pow(2) @ ctrl @ yy(0.25) q[2], q[0], q[1];", "ctrl @ yy(0.5) q[2], q[0], q[1];"),
+    ("pow(2) @ pswap(0.3) q[0], q[1];", "pswap(0.3) q[0], q[1]; pswap(0.3) q[0], q[1];"),
+])
+def test_integer_powers_of_rotations_scale_the_angle(a, b):
+    _same(a, b)
