@@ -52,13 +52,42 @@ struct QEntry {
   uint64_t first_block;
 };
 
-__device__ __forceinline__ int find_entry(const QEntry *tab, int count, uint64_t b) {
+__device__ __forceinline__ int find_entry_global(const QEntry *tab, int count, uint64_t b) {
   int lo = 0, hi = count - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (tab[mid].first_block <= b) lo = mid; else hi = mid - 1;
   }
   return lo;
+}
+
+// The launch's entry (instance) whose block range holds workgroup b, and that
+// instance's index.  With at most kBlock entries the table's first blocks are
+// staged in LDS by one load per thread and searched there: one global round
+// trip instead of a chain of dependent loads before the chunk's own loads can
+// start.  Every thread of the workgroup must call it (it synchronises).
+struct QFound {
+  int entry, inst;
+  uint64_t first;
+};
+__device__ __forceinline__ QFound find_entry(const QEntry *tab, int count, uint64_t b) {
+  __shared__ uint64_t sfb[kBlock];
+  __shared__ int sinst[kBlock];
+  if (count > kBlock) {
+    const int e = find_entry_global(tab, count, b);
+    return {e, tab[e].inst, tab[e].first_block};
+  }
+  if ((int)threadIdx.x < count) {
+    sfb[threadIdx.x] = tab[threadIdx.x].first_block;
+    sinst[threadIdx.x] = tab[threadIdx.x].inst;
+  }
+  __syncthreads();
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sfb[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return {lo, sinst[lo], sfb[lo]};
 }
 
 __device__ __forceinline__ int cut_of(uint64_t i, const uint8_t *e, int ne) {
@@ -228,13 +257,26 @@ __global__ void __launch_bounds__(kBlock)
   __shared__ T X[kQSlots];
   __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
   __shared__ int rint[16];
-  const int ei = find_entry(tab, count, blockIdx.x);
-  const QInstDev q = inst[tab[ei].inst];
-  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const int ii = fe.inst;
+  const QInstDev q = inst[ii];
+  const uint64_t chunk = blockIdx.x - fe.first;
   const uint64_t base = chunk << kQL;
   const uint8_t *e = edges + 2 * q.edge_off;
-  const double g = gam[tab[ei].inst * p + layer], b = bet[tab[ei].inst * p + layer];
+  const double g = gam[ii * p + layer], b = bet[ii * p + layer];
   const int t = threadIdx.x;
+  // the chunk's loads go out first; the phase table and cut setup overlap them
+  T v[16];
+  const R amp0 = (R)(1.0 / sqrt((double)(1ull << q.n)));
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    if (layer == 0) {
+      v[r].x = amp0;
+      v[r].y = 0;
+    } else {
+      v[r] = ld_nt(st + q.psi + base + t + 256 * r);
+    }
+  }
   for (int c = t; c <= q.nedges; c += kBlock) {
     double sn, cs;
     sincos(-g * (double)(q.nedges - 2 * c), &sn, &cs);
@@ -243,19 +285,9 @@ __global__ void __launch_bounds__(kBlock)
   }
   const QCut cc = qcut_setup(base, t, e, q.nedges, rint);
   __syncthreads();
-  T v[16];
-  const R amp0 = (R)(1.0 / sqrt((double)(1ull << q.n)));
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    R x, y;
-    if (layer == 0) {
-      x = amp0;
-      y = 0;
-    } else {
-      const T w = ld_nt(st + q.psi + base + t + 256 * r);
-      x = w.x;
-      y = w.y;
-    }
+    R x = v[r].x, y = v[r].y;
     const int c = qcut_at(cc, rint, r);
     cmul(x, y, ph_r[c], ph_i[c]);
     v[r].x = x;
@@ -289,13 +321,14 @@ __global__ void __launch_bounds__(kBlock)
   using T = typename CV<R>::T;
   __shared__ R xr[kQChunk / 2], xi[kQChunk / 2];  // chunks below 2^12 (n < 12)
   __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
-  const int ei = find_entry(tab, count, blockIdx.x);
-  const QInstDev q = inst[tab[ei].inst];
-  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const int ii = fe.inst;
+  const QInstDev q = inst[ii];
+  const uint64_t chunk = blockIdx.x - fe.first;
   const int m = 1 << q.lc;
   const uint64_t base = chunk << q.lc;
   const uint8_t *e = edges + 2 * q.edge_off;
-  const double g = gam[tab[ei].inst * p + layer], b = bet[tab[ei].inst * p + layer];
+  const double g = gam[ii * p + layer], b = bet[ii * p + layer];
   // phases exp(-i g s) for s = |E| - 2c, c = 0..|E|, in double
   for (int c = threadIdx.x; c <= q.nedges; c += kBlock) {
     double sn, cs;
@@ -360,11 +393,12 @@ __global__ void __launch_bounds__(kBlock)
                     int count, int layer, int h0) {
   using T = typename CV<R>::T;
   constexpr int D = 1 << G;
-  const int ei = find_entry(tab, count, blockIdx.x);
-  const QInstDev q = inst[tab[ei].inst];
-  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const int ii = fe.inst;
+  const QInstDev q = inst[ii];
+  const uint64_t chunk = blockIdx.x - fe.first;
   double sn, cs;
-  sincos(bet[tab[ei].inst * p + layer], &sn, &cs);
+  sincos(bet[ii * p + layer], &sn, &cs);
   const R c0 = (R)cs, s0 = (R)sn;  // RX(2b) = [[c, -i s], [-i s, c]]
   const uint64_t ngrp = 1ull << (q.lc - G);  // groups per chunk-sized share
   for (uint64_t t = threadIdx.x; t < ngrp; t += kBlock) {
@@ -396,9 +430,10 @@ __global__ void __launch_bounds__(kBlock)
                   double *__restrict__ partial) {
   using T = typename CV<R>::T;
   __shared__ int rint[16];
-  const int ei = find_entry(tab, count, blockIdx.x);
-  const QInstDev q = inst[tab[ei].inst];
-  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const int ii = fe.inst;
+  const QInstDev q = inst[ii];
+  const uint64_t chunk = blockIdx.x - fe.first;
   const int m = 1 << q.lc;
   const uint64_t base = chunk << q.lc;
   const uint8_t *e = edges + 2 * q.edge_off;
@@ -407,14 +442,30 @@ __global__ void __launch_bounds__(kBlock)
   if (full) cc = qcut_setup(base, threadIdx.x, e, q.nedges, rint);
   __syncthreads();
   double acc = 0.0;
-  for (int j = threadIdx.x; j < m; j += kBlock) {
-    const T v = ld_nt(st + q.psi + base + j);
-    const int c = full ? qcut_at(cc, rint, j >> 8) : cut_of(base + j, e, q.nedges);
-    acc += (double)c * ((double)v.x * v.x + (double)v.y * v.y);
-    T o;
-    o.x = (R)c * v.x;
-    o.y = (R)c * v.y;
-    st_nt(st + q.lam + base + j, o);
+  if (full) {
+    // all 16 loads of the thread in flight at once (m = 16 x kBlock)
+    T v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = ld_nt(st + q.psi + base + threadIdx.x + kBlock * r);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int c = qcut_at(cc, rint, r);
+      acc += (double)c * ((double)v[r].x * v[r].x + (double)v[r].y * v[r].y);
+      T o;
+      o.x = (R)c * v[r].x;
+      o.y = (R)c * v[r].y;
+      st_nt(st + q.lam + base + threadIdx.x + kBlock * r, o);
+    }
+  } else {
+    for (int j = threadIdx.x; j < m; j += kBlock) {
+      const T v = ld_nt(st + q.psi + base + j);
+      const int c = cut_of(base + j, e, q.nedges);
+      acc += (double)c * ((double)v.x * v.x + (double)v.y * v.y);
+      T o;
+      o.x = (R)c * v.x;
+      o.y = (R)c * v.y;
+      st_nt(st + q.lam + base + j, o);
+    }
   }
   acc = qblock_sum(acc);
   if (threadIdx.x == 0) partial[slot_energy(q) + chunk] = acc;
@@ -429,11 +480,12 @@ __global__ void __launch_bounds__(kBlock)
                     int count, int layer, int h0, double *__restrict__ partial) {
   using T = typename CV<R>::T;
   constexpr int D = 1 << G;
-  const int ei = find_entry(tab, count, blockIdx.x);
-  const QInstDev q = inst[tab[ei].inst];
-  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const int ii = fe.inst;
+  const QInstDev q = inst[ii];
+  const uint64_t chunk = blockIdx.x - fe.first;
   double sn, cs;
-  sincos(bet[tab[ei].inst * p + layer], &sn, &cs);
+  sincos(bet[ii * p + layer], &sn, &cs);
   const R c0 = (R)cs, s0 = (R)sn;
   const uint64_t ngrp = 1ull << (q.lc - G);
   double acc = 0.0;
@@ -491,13 +543,21 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
   __shared__ T X[kQSlots];
   __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
   __shared__ int rint[16];
-  const int ei = find_entry(tab, count, blockIdx.x);
-  const QInstDev q = inst[tab[ei].inst];
-  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const int ii = fe.inst;
+  const QInstDev q = inst[ii];
+  const uint64_t chunk = blockIdx.x - fe.first;
   const uint64_t base = chunk << kQL;
   const uint8_t *e = edges + 2 * q.edge_off;
-  const double g = gam[tab[ei].inst * p + layer], b = bet[tab[ei].inst * p + layer];
+  const double g = gam[ii * p + layer], b = bet[ii * p + layer];
   const int t = threadIdx.x;
+  // the chunk's loads go out first; the phase table and cut setup overlap them
+  T a[16], l[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    a[r] = ld_nt(st + q.psi + base + t + 256 * r);
+    l[r] = ld_nt(st + q.lam + base + t + 256 * r);
+  }
   for (int c = t; c <= q.nedges; c += kBlock) {
     double sn, cs;
     sincos(g * (double)(q.nedges - 2 * c), &sn, &cs);  // undo: exp(+i g s)
@@ -505,12 +565,6 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
     ph_i[c] = (R)sn;
   }
   const QCut cc = qcut_setup(base, t, e, q.nedges, rint);
-  T a[16], l[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    a[r] = ld_nt(st + q.psi + base + t + 256 * r);
-    l[r] = ld_nt(st + q.lam + base + t + 256 * r);
-  }
   double sn, cs;
   sincos(b, &sn, &cs);
   const R c0 = (R)cs, s0 = -(R)sn;  // RX(-2b)
@@ -571,13 +625,14 @@ __global__ void __launch_bounds__(kBlock)
   using T = typename CV<R>::T;
   __shared__ R xr[kQChunk / 2], xi[kQChunk / 2], yr[kQChunk / 2], yi[kQChunk / 2];  // n < 12
   __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
-  const int ei = find_entry(tab, count, blockIdx.x);
-  const QInstDev q = inst[tab[ei].inst];
-  const uint64_t chunk = blockIdx.x - tab[ei].first_block;
+  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const int ii = fe.inst;
+  const QInstDev q = inst[ii];
+  const uint64_t chunk = blockIdx.x - fe.first;
   const int m = 1 << q.lc;
   const uint64_t base = chunk << q.lc;
   const uint8_t *e = edges + 2 * q.edge_off;
-  const double g = gam[tab[ei].inst * p + layer], b = bet[tab[ei].inst * p + layer];
+  const double g = gam[ii * p + layer], b = bet[ii * p + layer];
   for (int c = threadIdx.x; c <= q.nedges; c += kBlock) {
     double sn, cs;
     sincos(g * (double)(q.nedges - 2 * c), &sn, &cs);  // undo: exp(+i g s)
