@@ -17,8 +17,9 @@ serialised to, producing the engine's gate objects directly:
 * ``gate name(params) a, b { ... }`` definitions, expanded at the call site;
 * ``gphase(x);`` and controlled phases (``ctrl @ gphase(x) a``);
 * ``#pragma braket unitary(<matrix>) qubits``; ``#pragma braket result``
-  state_vector / probability / amplitude / expectation / variance of Pauli
-  products with ``h`` and ``i`` factors (``all`` or per-factor qubits), and
+  state_vector / probability / amplitude / expectation / variance of tensor
+  products of ``x y z h i`` and ``hermitian(<matrix>)`` factors (``all`` or
+  per-factor qubits), and
   ``adjoint_gradient expectation(<sum of c * products>) <inputs | all>``:
   inputs are carried as dual numbers through the angle expressions, so each
   gate records d angle / d input and the engine's adjoint sweep gives the
@@ -930,9 +931,23 @@ class _Interpreter:
             letters, targets = [], []
             while True:
                 obs = ts.ident().lower()
+                if obs == "hermitian":
+                    ts.expect("(")
+                    m = hermitian_matrix(_eval(_parse_expr(ts), self.env))
+                    ts.expect(")")
+                    q = [x for qa in _parse_qargs(ts, end=("@",)) for x in self._resolve(qa, None)]
+                    if (1 << len(q)) != m.shape[0]:
+                        raise ValueError(f"line {line}: a {m.shape[0]}x{m.shape[0]} Hermitian "
+                                         f"needs {m.shape[0].bit_length() - 1} qubit(s)")
+                    letters.append(m)
+                    targets += q
+                    if not ts.accept("@"):
+                        break
+                    continue
                 if obs not in ("x", "y", "z", "h", "i"):
                     raise NotImplementedError(f"line {line}: observable {obs!r} is not supported "
-                                              "(Pauli products with h and i factors are)")
+                                              "(Pauli products with h and i factors and Hermitian "
+                                              "matrices are)")
                 if ts.accept("all"):
                     if letters:
                         raise QasmError(f"line {line}: 'all' cannot be part of a product")
@@ -1038,25 +1053,73 @@ def parse(source: str, inputs: Optional[dict] = None) -> Circuit:
     return interp.circuit
 
 
+_PAULI_FACTORS = {
+    "x": (lambda q: [g.Hadamard(q)], (1.0, -1.0)),
+    "y": (lambda q: [g.PauliZ(q), g.S(q), g.Hadamard(q)], (1.0, -1.0)),
+    "z": (lambda q: [], (1.0, -1.0)),
+    "h": (lambda q: [g.RotY(q, -math.pi / 4)], (1.0, -1.0)),
+    "i": (lambda q: [], (1.0, 1.0)),
+}
+
+
+def hermitian_matrix(m) -> np.ndarray:
+    """A Hermitian observable's matrix from a complex array or the JAQCD
+    [[[re, im], ...], ...] form; checked square, power-of-two, Hermitian."""
+    a = np.asarray(m)
+    if a.ndim == 3 and a.shape[-1] == 2 and not np.iscomplexobj(a):
+        a = a[..., 0] + 1j * a[..., 1]
+    a = np.asarray(a, dtype=np.complex128)
+    d = a.shape[0] if a.ndim == 2 else 0
+    if a.ndim != 2 or a.shape != (d, d) or d < 2 or d & (d - 1):
+        raise ValueError(f"a Hermitian observable needs a square 2^k matrix, got {a.shape}")
+    if not np.allclose(a, a.conj().T, atol=1e-8):
+        raise ValueError("the observable matrix is not Hermitian")
+    return a
+
+
+def observable_factor(o):
+    """(qubits, rotation builder, eigenvalues) of one observable factor: the
+    gates that map its eigenbasis to the computational basis and the
+    eigenvalue of each outcome (observables.py:136-305: Pauli X/Y/Z, H, I and
+    Hermitian's eigendecomposition)."""
+    if isinstance(o, str):
+        f = _PAULI_FACTORS.get(o.lower())
+        if f is None:
+            raise NotImplementedError(f"observable {o!r} is not supported")
+        make, eig = f
+        return 1, (lambda qs, make=make: make([qs[0]])), np.array(eig)
+    m = hermitian_matrix(o)
+    lam, v = np.linalg.eigh(m)
+    k = m.shape[0].bit_length() - 1
+    d = v.conj().T
+    return k, (lambda qs, d=d: [g.Unitary(list(qs), d)]), lam
+
+
 def basis_rotations(results: list, qubits) -> list:
     """Gates that rotate each observable's eigenbasis to Z before sampling
-    (openqasm/circuit.py:95-143): x -> H; y -> Z, S, H; h -> RotY(-pi/4)."""
-    per = {}
+    (openqasm/circuit.py:95-143): x -> H; y -> Z, S, H; h -> RotY(-pi/4);
+    a Hermitian -> the adjoint of its eigenvector matrix.  A qubit measured
+    in two different bases is an error."""
+    per, out = {}, []
     for r in results:
         if r.get("type") not in ("expectation", "variance"):
             continue
-        letters, targets = r["observable"], r.get("targets")
+        obs, targets = list(r["observable"]), r.get("targets")
         if not targets:
-            letters, targets = letters * len(qubits), list(qubits)
-        for L, q in zip(letters, targets):
-            if per.setdefault(q, L) != L:
-                raise ValueError("Conflicting result types applied to a single qubit")
-    out = []
-    for q, L in per.items():
-        if L == "x":
-            out.append(g.Hadamard([q]))
-        elif L == "y":
-            out += [g.PauliZ([q]), g.S([q]), g.Hadamard([q])]
-        elif L == "h":
-            out.append(g.RotY([q], -math.pi / 4))
+            obs, targets = obs * len(qubits), list(qubits)
+        k = 0
+        for o in obs:
+            nq, make, _ = observable_factor(o)
+            qs = tuple(targets[k:k + nq])
+            k += nq
+            key = o.lower() if isinstance(o, str) else hermitian_matrix(o).tobytes()
+            for q in qs:
+                prev = per.get(q)
+                if prev is not None and prev != (key, qs):
+                    raise ValueError("Conflicting result types applied to a single qubit")
+            if all(per.get(q) == (key, qs) for q in qs):
+                continue
+            for q in qs:
+                per[q] = (key, qs)
+            out += make(list(qs))
     return out

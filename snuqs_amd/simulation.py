@@ -161,6 +161,21 @@ class StateVectorSimulation(Simulation):
     def amplitudes(self, indices) -> np.ndarray:
         return _C.functionals.amplitudes(self._state_vector, [int(i) for i in indices])
 
+    def diagonalized_moments(self, rotations, targets, eigenvalues) -> tuple:
+        """<O> and <O^2> of an observable O = D^dagger diag(eigenvalues) D:
+        the diagonalizing gates D act on a copy of the state, whose marginal
+        distribution over `targets` (targets[0] the most significant outcome
+        bit, at most 20) weights the eigenvalues (observables.py:245-305:
+        eigendecomposition and diagonalizing gate of a Hermitian)."""
+        work = _C.result_types.StateVector(_C.DeviceType.CUDA, self._qubit_count,
+                                           dtype=self._precision)
+        work.copy(self._state_vector)
+        for op in rotations:
+            _C.functionals.apply(work, op, self._qubit_count, list(op.targets))
+        p = np.asarray(_C.functionals.marginal_probabilities(work, list(targets)), np.float64)
+        lam = np.asarray(eigenvalues, dtype=np.float64)
+        return float(p @ lam), float(p @ (lam * lam))
+
     def expectation(self, paulis, qubits, coeffs) -> float:
         return _C.functionals.expectation(self._state_vector, list(paulis),
                                           [list(q) for q in qubits], list(coeffs))[0]

@@ -17,8 +17,9 @@ Differences from the reference, by design: the engine runs on the GPU only
 (accelerator None / "cuda" / "hip"; "cpu" / "hybrid" and the offload tiers
 raise), and besides StateVector the result types Amplitude, Probability
 (marginals over up to 20 targets reduced on the GPU), Expectation and Variance
-(Pauli products with h and i factors; an observable without targets gives
-one value per qubit) are computed on the GPU, and OpenQASM programs may ask
+(Pauli products with h and i factors, and Hermitian matrices through their
+diagonalizing gates and eigenvalues; an observable without targets gives one
+value per qubit) are computed on the GPU, and OpenQASM programs may ask
 for Braket's AdjointGradient (expectation of a Pauli sum and its gradient
 with respect to the program's inputs, from the engine's adjoint sweep).
 """
@@ -240,10 +241,12 @@ class BaseSimulator:
             return sim.marginal_probabilities(targets)
         if t in ("expectation", "variance"):
             obs = _get(r, "observable")
+            if any(not isinstance(o, str) for o in obs):
+                return BaseSimulator._hermitian_value(t, sim, obs, targets, n)
             letters = [str(o).upper() for o in obs]
             if any(L not in ("X", "Y", "Z", "I", "H") for L in letters):
-                raise NotImplementedError("only Pauli products with H and I factors are "
-                                          "supported")
+                raise NotImplementedError("only Pauli products with H and I factors and "
+                                          "Hermitian matrices are supported")
             if len(letters) == 1 and not targets:
                 # no targets: the observable on each qubit in turn
                 return [BaseSimulator._observable_value(t, sim, letters, [q]) for q in range(n)]
@@ -313,6 +316,29 @@ class BaseSimulator:
                 if nm in gradient:
                     gradient[nm] += c * float(grads[k])
         return {"expectation": float(e) + const, "gradient": gradient}
+
+    @staticmethod
+    def _hermitian_value(t, sim: StateVectorSimulation, obs, targets, n):
+        """Expectation or variance of a tensor product with Hermitian-matrix
+        factors (JAQCD: a matrix as [[[re, im], ...], ...]; OpenQASM:
+        hermitian(...)), each factor taking log2(dim) consecutive targets."""
+        factors = [openqasm.observable_factor(o) for o in obs]
+        if not targets:
+            if len(factors) != 1 or factors[0][0] != 1:
+                raise ValueError(f"a {t} without targets needs a one-qubit observable")
+            return [BaseSimulator._hermitian_value(t, sim, obs, [q], n) for q in range(n)]
+        if sum(f[0] for f in factors) != len(targets):
+            raise ValueError("observable factors do not match the number of targets")
+        if len(set(targets)) != len(targets) or len(targets) > 20:
+            raise ValueError("observable targets must be distinct and at most 20")
+        rotations, lam, k = [], np.ones(1), 0
+        for nq, make_rotation, eig in factors:
+            qs = list(targets[k:k + nq])
+            k += nq
+            rotations += make_rotation(qs)
+            lam = np.kron(lam, eig)
+        e, e2 = sim.diagonalized_moments(rotations, targets, lam)
+        return e if t == "expectation" else e2 - e * e
 
     @staticmethod
     def _observable_value(t, sim: StateVectorSimulation, letters, targets) -> float:

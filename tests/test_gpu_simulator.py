@@ -256,3 +256,62 @@ def test_openqasm_adjoint_gradient_matches_finite_differences(precision):
     assert abs(v2["gradient"]["phi"] - v["gradient"]["phi"]) < 1e-12
     with pytest.raises(ValueError):
         StateVectorSimulator().run({"source": ADJ, "inputs": x0}, shots=10)
+
+
+@pytest.mark.parametrize("precision", ["complex128", "complex64"])
+def test_hermitian_observables(precision):
+    """Expectation and Variance of Hermitian-matrix observables (JAQCD matrix
+    form and OpenQASM hermitian(...)), alone, on two qubits and in a tensor
+    product with Pauli factors, against <psi|O|psi> and <O^2> - <O>^2 from
+    the oracle's state; with shots, the diagonalizing gate runs before
+    sampling and the sample mean of the eigenvalues tracks <O>."""
+    from snuqs_amd.simulator import StateVectorSimulator
+
+    rng = np.random.default_rng(2)
+    h2 = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+    h2 = (h2 + h2.conj().T) / 2
+    h4 = rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4))
+    h4 = (h4 + h4.conj().T) / 2
+    jm = lambda m: [[[z.real, z.imag] for z in row] for row in m]  # noqa: E731
+    ins = [{"type": "h", "target": 0}, {"type": "ry", "target": 1, "angle": 0.7},
+           {"type": "cnot", "control": 0, "target": 2}, {"type": "rx", "target": 2, "angle": -0.4},
+           {"type": "xy", "targets": [1, 2], "angle": 0.9}]
+    results = [{"type": "expectation", "observable": [jm(h2)], "targets": [1]},
+               {"type": "variance", "observable": [jm(h4)], "targets": [2, 0]},
+               {"type": "expectation", "observable": ["x", jm(h2)], "targets": [0, 2]},
+               {"type": "expectation", "observable": [jm(h2)]}]
+    res = StateVectorSimulator().run({"instructions": ins, "results": results},
+                                     precision=precision)
+    from snuqs_amd.gate_operations import from_braket_instruction
+    psi = run_oracle(3, [from_braket_instruction(i) for i in ins])
+    X = np.array([[0, 1], [1, 0]])
+
+    def emb(m, qs):
+        from test_openqasm import _embed
+        return _embed(m, qs, 3)
+
+    tol = 1e-10 if precision == "complex128" else 2e-5
+    vals = [r["value"] for r in res.resultTypes]
+    ev = lambda op: np.vdot(psi, op @ psi).real  # noqa: E731
+    assert abs(vals[0] - ev(emb(h2, [1]))) < tol
+    o = emb(h4, [2, 0])
+    assert abs(vals[1] - (ev(o @ o) - ev(o) ** 2)) < 4 * tol
+    assert abs(vals[2] - ev(emb(X, [0]) @ emb(h2, [2]))) < tol
+    assert np.abs(np.array(vals[3]) - [ev(emb(h2, [q])) for q in range(3)]).max() < tol
+    # OpenQASM form of the tensor product
+    qasm = ("OPENQASM 3.0;\nqubit[3] q;\nh q[0];\nry(0.7) q[1];\ncnot q[0], q[2];\n"
+            "rx(-0.4) q[2];\nxy(0.9) q[1], q[2];\n"
+            "#pragma braket result expectation x(q[0]) @ hermitian(" +
+            str([[complex(z) for z in row] for row in h2]).replace("j", "im").replace("(", "")
+            .replace(")", "") + ") q[2]\n")
+    v = StateVectorSimulator().run(qasm, precision=precision).resultTypes[0]["value"]
+    assert abs(v - vals[2]) < tol
+    # shots (OpenQASM derives the rotations from the result pragmas): the
+    # eigenvalue sample mean after the diagonalizing rotation tracks <O>
+    lam = np.linalg.eigvalsh(h2)
+    head, pragma = qasm.rstrip("\n").rsplit("\n", 1)
+    qasm1 = head + "\n" + pragma.replace("x(q[0]) @ ", "").replace(") q[2]", ") q[1]") + "\n"
+    shot_res = StateVectorSimulator().run(qasm1, shots=20000, precision=precision)
+    bits = np.array([[int(ch) for ch in m] for m in shot_res.measurements])
+    mean = lam[bits[:, 1]].mean()
+    assert abs(mean - vals[0]) < 5 * np.ptp(lam) / np.sqrt(20000)

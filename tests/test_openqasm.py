@@ -248,8 +248,9 @@ def test_unitary_power_closed_forms():
     ("#pragma braket unitary([[1, 1], [0, 1]]) q[0]", ValueError),
     ("#pragma braket unitary([[0, 1], [1, 0]]) q[0], q[1]", ValueError),
     ("#pragma braket result density_matrix", NotImplementedError),
-    ("#pragma braket result expectation hermitian([[1, 0], [0, -1]]) q[0]",
-     NotImplementedError),
+    ("#pragma braket result expectation hermitian([[1, 1], [0, -1]]) q[0]", ValueError),
+    ("#pragma braket result expectation hermitian([[1, 0], [0, -1]]) q[0], q[1]", ValueError),
+    ("#pragma braket result expectation foo(q[0])", NotImplementedError),
     ("h q[0]", QasmError),
     ("ctrl(3) @ x q[0], q[1];", ValueError),
 ])
@@ -320,3 +321,68 @@ h q[0];
 ])
 def test_integer_powers_of_rotations_scale_the_angle(a, b):
     _same(a, b)
+
+
+def test_hermitian_observables_by_diagonalization():
+    """<psi|O|psi> of a tensor product with Hermitian factors equals the
+    eigenvalue-weighted distribution after the factors' diagonalizing gates
+    (what the GPU path computes), for random Hermitians on 1 and 2 qubits
+    next to Pauli and H factors; and the OpenQASM form parses to the same."""
+    from snuqs_amd.openqasm import observable_factor
+
+    rng = np.random.default_rng(11)
+    n = 4
+    psi = _random_state(n, seed=3)
+
+    def rand_herm(d):
+        a = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+        return (a + a.conj().T) / 2
+
+    def full_op(factors, targets):
+        # Kronecker product in qubit order 0..n-1 (qubit 0 = MSB)
+        mats = {"x": np.array([[0, 1], [1, 0]]), "y": np.array([[0, -1j], [1j, 0]]),
+                "z": np.diag([1, -1]), "i": np.eye(2),
+                "h": np.array([[1, 1], [1, -1]]) / np.sqrt(2)}
+        ops, k = [], 0
+        for f in factors:
+            m = mats[f] if isinstance(f, str) else f
+            nq = m.shape[0].bit_length() - 1
+            ops.append((m, targets[k:k + nq]))
+            k += nq
+        big = np.eye(1 << n, dtype=complex)
+        for m, qs in ops:
+            big = _embed(m, qs, n) @ big
+        return big
+
+    for factors, targets in [([rand_herm(2)], [2]), ([rand_herm(4)], [3, 1]),
+                             ([rand_herm(2), "x", rand_herm(4)], [1, 0, 3, 2]),
+                             (["h", rand_herm(2), "y"], [3, 0, 2])]:
+        want = np.vdot(psi, full_op(factors, targets) @ psi).real
+        rot, lam, k = [], np.ones(1), 0
+        for f in factors:
+            nq, make, eig = observable_factor(f)
+            rot += make(list(targets[k:k + nq]))
+            k += nq
+            lam = np.kron(lam, eig)
+        out = run_oracle(n, rot, init=psi.copy())
+        p = (np.abs(out) ** 2).reshape([2] * n).transpose(
+            targets + [q for q in range(n) if q not in targets]).reshape(1 << len(targets), -1)
+        assert abs(p.sum(axis=1) @ lam - want) < 1e-12
+    c = parse("OPENQASM 3.0;\nqubit[3] q;\n#pragma braket result variance "
+              "hermitian([[2, 1im], [-1im, 0]]) q[1] @ x(q[2])\n")
+    r = c.results[0]
+    assert r["targets"] == [1, 2] and r["observable"][1] == "x"
+    assert np.allclose(r["observable"][0], [[2, 1j], [-1j, 0]])
+    rot = basis_rotations(c.results, [0, 1, 2])
+    assert [op.name for op in rot] == ["Unitary", "Hadamard"]
+
+
+def _embed(m, qs, n):
+    """m on qubits qs (qs[0] the most significant) as a 2^n matrix."""
+    k = len(qs)
+    others = [q for q in range(n) if q not in qs]
+    full = np.kron(m, np.eye(1 << (n - k)))
+    order = list(qs) + others  # axis j of `full` is qubit order[j]
+    perm = [order.index(q) for q in range(n)]
+    t = full.reshape([2] * (2 * n)).transpose(perm + [n + p for p in perm])
+    return t.reshape(1 << n, 1 << n)
