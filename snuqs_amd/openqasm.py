@@ -926,8 +926,6 @@ class _Interpreter:
         if kind == "adjoint_gradient":
             return self._adjoint_result(ts, line)
         if kind in ("expectation", "variance", "sample"):
-            if kind == "sample":
-                raise NotImplementedError("the sample result type needs shots; use measurements")
             letters, targets = [], []
             while True:
                 obs = ts.ident().lower()
@@ -1102,7 +1100,7 @@ def basis_rotations(results: list, qubits) -> list:
     in two different bases is an error."""
     per, out = {}, []
     for r in results:
-        if r.get("type") not in ("expectation", "variance"):
+        if r.get("type") not in ("expectation", "variance", "sample"):
             continue
         obs, targets = list(r["observable"]), r.get("targets")
         if not targets:

@@ -191,9 +191,16 @@ class BaseSimulator:
                                     exact_sampling=exact_sampling)
         sim.evolve(sub)
         values = []
-        if shots and any(_get(r, "type") == "adjoint_gradient" for r in results):
+        kinds = [str(getattr(_get(r, "type"), "value", _get(r, "type"))).lower() for r in results]
+        if shots and "adjoint_gradient" in kinds:
             raise ValueError("adjoint_gradient needs shots=0")
-        if not shots:
+        if not shots and "sample" in kinds:
+            raise ValueError("the Sample result type needs shots > 0")
+        if shots:
+            # as simulator.py:380-437: with shots the result types go back without
+            # values (the client derives them from the measurements)
+            values = [{"type": r, "value": None} for r in results]
+        else:
             for r in results:
                 if _get(r, "type") == "adjoint_gradient":
                     v = self._adjoint_gradient(r, sim, in_order, adjoint_ctx, qmap, qubit_count,

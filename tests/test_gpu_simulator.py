@@ -192,6 +192,13 @@ def test_run_openqasm_shots_and_measured_qubits():
            "#pragma braket result expectation x all\n")
     res = sim.run(ghz, shots=500)
     assert all(sum(int(ch) for ch in m) % 2 == 0 for m in res.measurements)
+    # Sample of X x X x Y: the rotation runs before sampling; the result type
+    # returns without a value (derived from the measurements by the client)
+    ghz_s = ghz.replace("expectation x all", "sample x(q[0]) @ x(q[1]) @ y(q[2])")
+    res = sim.run(ghz_s, shots=400)
+    assert res.resultTypes[0]["value"] is None and len(res.measurements) == 400
+    with pytest.raises(ValueError):
+        sim.run(ghz_s, shots=0)
 
 
 ADJ = """OPENQASM 3.0;
