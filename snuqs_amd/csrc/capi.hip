@@ -159,6 +159,7 @@ static int *tune_slot(const std::string &k) {
       {"mfma5_pipeline_c128", [] { return &tuning().mfma5_pipeline[1]; }},
       {"mfma5_order_c64", [] { return &tuning().mfma5_order[0]; }},
       {"mfma5_order_c128", [] { return &tuning().mfma5_order[1]; }},
+      {"mfma5_wide_c64", [] { return &tuning().mfma5_wide; }},
   };
   for (auto &e : table)
     if (k == e.name) return e.slot();
@@ -171,7 +172,8 @@ int sq_tune_set(const char *key, int value) {
   if (!slot) return fail(SQ_ERR_INVALID, "unknown tuning key '" + k + "'");
   if (k.rfind("mfma_blocks_per_cu", 0) == 0 && (value < 1 || value > 4096))
     return fail(SQ_ERR_INVALID, "mfma_blocks_per_cu must be in [1, 4096]");
-  const bool flag = k.rfind("mfma5_pipeline", 0) == 0 || k.rfind("mfma5_order", 0) == 0;
+  const bool flag = k.rfind("mfma5_pipeline", 0) == 0 || k.rfind("mfma5_order", 0) == 0 ||
+                    k.rfind("mfma5_wide", 0) == 0;
   *slot = flag ? (value ? 1 : 0) : value;
   return SQ_OK;
 }
@@ -531,8 +533,11 @@ int sq_program_tune(void *program, void *sv, uint64_t n, int reps, void *stream,
   // after launch, mispredicts it); every gate then keeps its fastest shape.
   std::vector<GatePlan::Shape> cands;
   for (int order = 0; order < 2; ++order) {
-    for (int bpc : {64, 256, 1024}) cands.push_back({bpc, order, 1});
-    for (int bpc : {256, 1024}) cands.push_back({bpc, order, 0});
+    for (int bpc : {64, 256, 1024}) cands.push_back({bpc, order, 1, 0});
+    for (int bpc : {256, 1024}) cands.push_back({bpc, order, 0, 0});
+    if (p->dt == SQ_COMPLEX64)  // 64-group tiles: longer runs, half the waves per CU
+      for (int pipe = 0; pipe < 2; ++pipe)
+        for (int bpc : {256, 1024}) cands.push_back({bpc, order, pipe, 1});
   }
   std::vector<hipEvent_t> ev(ng + 1);
   for (auto &e : ev) SQ_HIP(hipEventCreate(&e));

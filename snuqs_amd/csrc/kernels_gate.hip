@@ -579,6 +579,118 @@ __global__ void __launch_bounds__(kBlock)
 #undef SQ_KW
 }
 
+// complex64, wide tile: 64 groups (the lowest 6 free positions) x the 32
+// amplitudes of the block per wave, so every global access is 32 lanes x 16 B
+// = 512 B runs (half as many runs per tile as the 32-group tile).  The tile
+// sits in LDS as two 32-group halves (slot bit 10 = free position 5), each
+// multiplied exactly as in k_mfma5_c64 (same operands, same MFMA order, so
+// the results are bitwise those of the narrow tile).  80 KB of LDS per
+// workgroup: two workgroups per CU.
+struct StageArgsW {
+  uint64_t a[11];  // address stride (amplitudes) of tile bit i
+  uint32_t w[11];  // LDS slot weight of tile bit i
+};
+
+template <bool PIPE, bool BLK>
+__global__ void __launch_bounds__(kBlock)
+    k_mfma5w_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgsW sa,
+                 const float4 *__restrict__ G) {
+  constexpr int D = 32, NS = 16, NK = 16;
+  __shared__ float4 As[D * D];          // [o][r], 16 KB
+  __shared__ float2 Xs[4][2 * D * 32];  // one tile per wave, 64 KB
+  for (int e = threadIdx.x; e < D * D; e += kBlock) As[e] = G[e];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 31, half = lane >> 5;
+  float2 *X = Xs[wave];
+  TileRange tr = tile_range<BLK>(ntiles, wave);
+  uint64_t tile = tr.first;
+  if (tile >= tr.end) return;
+  // j = 2 lane + 128 k + e: lane bits are tile bits 1-6, k bits tile bits 7-10
+  uint64_t la = 0;
+  uint32_t lw = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+    if ((lane >> i) & 1) {
+      la += sa.a[i + 1];
+      lw ^= sa.w[i + 1];
+    }
+#define SQ_KA(k)                                                                          \
+  ((((k) & 1) ? sa.a[7] : 0) + (((k) & 2) ? sa.a[8] : 0) + (((k) & 4) ? sa.a[9] : 0) + \
+   (((k) & 8) ? sa.a[10] : 0))
+#define SQ_KW(k)                                                                              \
+  ((((k) & 1) ? sa.w[7] : 0u) ^ (((k) & 2) ? sa.w[8] : 0u) ^ (((k) & 4) ? sa.w[9] : 0u) ^ \
+   (((k) & 8) ? sa.w[10] : 0u))
+  const uint32_t w0 = sa.w[0];
+  float4 cur[NK], nxt[NK];
+  uint64_t base = deposit(tile * 64, im) + la;
+#pragma unroll
+  for (int k = 0; k < NK; ++k)
+    cur[k] = ld_nt(reinterpret_cast<const float4 *>(sv + base + SQ_KA(k)));
+  while (true) {
+    const uint64_t ntile = tile + tr.step;
+    const bool more = ntile < tr.end;
+    uint64_t nbase = 0;
+    if (PIPE && more) {
+      nbase = deposit(ntile * 64, im) + la;
+#pragma unroll
+      for (int k = 0; k < NK; ++k)
+        nxt[k] = ld_nt(reinterpret_cast<const float4 *>(sv + nbase + SQ_KA(k)));
+    }
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+      X[lw ^ SQ_KW(k)] = make_float2(cur[k].x, cur[k].y);
+      X[lw ^ SQ_KW(k) ^ w0] = make_float2(cur[k].z, cur[k].w);
+    }
+    wave_lds_sync();
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      float2 *Xh = X + 1024 * h;
+      float2 b[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) b[s] = Xh[64 * s + 2 * (col ^ s) + half];
+      floatx16 t1, t2, t3;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) t1[v] = t2[v] = t3[v] = 0.f;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const float4 a = As[(2 * s + half) * D + col];
+        t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b[s].x, t1, 0, 0, 0);
+        t2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b[s].y, t2, 0, 0, 0);
+        t3 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b[s].x + b[s].y, t3, 0, 0, 0);
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int r = 8 * (v / 4) + 4 * half + (v % 4), rs = r >> 1;
+        Xh[64 * rs + 2 * (col ^ (rs & 15)) + (r & 1)] =
+            make_float2(t1[v] - t2[v], t3[v] - t1[v] - t2[v]);
+      }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+      const float2 y0 = X[lw ^ SQ_KW(k)], y1 = X[lw ^ SQ_KW(k) ^ w0];
+      st_nt(reinterpret_cast<float4 *>(sv + base + SQ_KA(k)), make_float4(y0.x, y0.y, y1.x, y1.y));
+    }
+    wave_lds_sync();
+    if (!more) break;
+    tile = ntile;
+    if (PIPE) {
+      base = nbase;
+#pragma unroll
+      for (int k = 0; k < NK; ++k) cur[k] = nxt[k];
+    } else {
+      base = deposit(tile * 64, im) + la;
+#pragma unroll
+      for (int k = 0; k < NK; ++k)
+        cur[k] = ld_nt(reinterpret_cast<const float4 *>(sv + base + SQ_KA(k)));
+    }
+  }
+#undef SQ_KA
+#undef SQ_KW
+}
+
 template <bool PIPE, bool BLK>
 __global__ void __launch_bounds__(kBlock)
     k_mfma5_c128(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
@@ -940,6 +1052,31 @@ static bool make_stage(const GatePlan &p, bool c64, StageArgs *sa) {
   return i == nfb + p.kk && (!c64 || sa->a[0] == 1);
 }
 
+// Geometry of the wide complex64 tile: the 5 targets and the lowest 6 free
+// positions; free position 5 selects the LDS half (slot weight 1024).
+static bool make_stage_wide(const GatePlan &p, StageArgsW *sa) {
+  const uint64_t tmask = target_mask(p);
+  int i = 0, nf = 0;
+  for (int b = 0; b < 64 && i < 6 + p.kk; ++b) {
+    const uint64_t bit = 1ull << b;
+    if (p.fixed_mask & bit) continue;
+    uint32_t slot;
+    if (tmask & bit) {
+      int q = 0;
+      while (p.tbit[q] != b) ++q;
+      slot = q == 0 ? 1u : 1u << (q + 5);
+    } else {
+      if (nf == 6) continue;
+      slot = nf < 5 ? 2u << nf : 1024u;
+      ++nf;
+    }
+    sa->a[i] = bit;
+    sa->w[i] = swz_c64(slot);
+    ++i;
+  }
+  return p.kk == 5 && i == 11 && sa->a[0] == 1;
+}
+
 // LDS-tiled path of a <= 3-target DENSE / PERM plan: taken when a target or
 // control sits below bit 4 (complex64) / 3 (complex128), where lane-strided
 // accesses would touch runs shorter than 128 B.  Tile = the targets plus the
@@ -1141,9 +1278,18 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
           if (blocks > cap) blocks = cap;
           using Q = typename Q4<R>::T;
           if constexpr (c64) {
+            const bool wide = (p.shape.wide >= 0 ? p.shape.wide : tu.mfma5_wide) != 0;
+            StageArgsW sw{};
             if (p.kk == 4) {
               hipLaunchKernelGGL(k_mfma4_c64, dim3(blocks), dim3(kBlock), 0, st,
                                  (float2 *)sv, ntiles, im, off, (const Q *)gdev);
+            } else if (wide && ntiles % 2 == 0 && make_stage_wide(p, &sw)) {
+              const uint64_t wt = ntiles / 2;  // 64-group tiles
+              const uint64_t wb = (wt + 3) / 4 < cap ? (wt + 3) / 4 : cap;
+              auto k = pipe ? (blk ? k_mfma5w_c64<true, true> : k_mfma5w_c64<true, false>)
+                            : (blk ? k_mfma5w_c64<false, true> : k_mfma5w_c64<false, false>);
+              hipLaunchKernelGGL(k, dim3(wb), dim3(kBlock), 0, st, (float2 *)sv, wt, im, sw,
+                                 (const Q *)gdev);
             } else {
               auto k = pipe ? (blk ? k_mfma5_c64<true, true> : k_mfma5_c64<true, false>)
                             : (blk ? k_mfma5_c64<false, true> : k_mfma5_c64<false, false>);

@@ -72,6 +72,7 @@ struct GatePlan {
     int blocks_per_cu = -1;  // grid cap / 256 CUs
     int order = -1;          // 0: tiles dealt cyclically, 1: contiguous per block
     int pipe = -1;           // prefetch the next tile
+    int wide = -1;           // complex64: 64-group tiles (512 B runs)
   } shape;
 };
 

@@ -79,38 +79,44 @@ def test_unitary_k_targets(C, dtype, k, n):
 
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
-@pytest.mark.parametrize("grid,pipe,order", [(256, 1, 0), (1, 1, 0), (1, 0, 0), (1, 1, 1),
-                                             (3, 0, 1), (1024, 1, 1)])
-def test_unitary_5_target_tiles(C, dtype, grid, pipe, order):
+@pytest.mark.parametrize("grid,pipe,order,wide", [
+    (256, 1, 0, 0), (1, 1, 0, 0), (1, 0, 0, 0), (1, 1, 1, 0), (3, 0, 1, 0), (1024, 1, 1, 0),
+    (256, 1, 0, 1), (1, 1, 0, 1), (1, 0, 1, 1), (1024, 0, 1, 1)])
+def test_unitary_5_target_tiles(C, dtype, grid, pipe, order, wide):
     """5-target MFMA kernels (LDS-staged tiles) on target sets that put the
     tile's bits everywhere: the lowest five address bits, the highest, spread
     and given out of order, plus 4-target and controlled 4-target blocks; a
     1-block-per-CU grid cap makes every wave loop over several tiles with and
-    without the prefetch."""
+    without the prefetch.  wide = 64-group complex64 tiles (22 qubits, so the
+    1-block cap still loops)."""
     from snuqs_amd.circuits import haar_unitary
 
+    if wide and dtype == "complex128":
+        pytest.skip("64-group tiles are complex64 only")
     g = C.operation.gate_operations
-    n = 21
+    n = 22 if wide else 21
     rng = np.random.default_rng(55)
     init = random_state(n, rng, np.complex64 if dtype == "complex64" else np.complex128)
     ref0 = init.astype(np.complex128)
     suffix = "c64" if dtype == "complex64" else "c128"
     keys = (f"mfma_blocks_per_cu_{suffix}", f"mfma5_pipeline_{suffix}",
-            f"mfma5_order_{suffix}")
+            f"mfma5_order_{suffix}", "mfma5_wide_c64")
     saved = [C.core.tune_get(k) for k in keys]
     try:
         C.core.tune_set(keys[0], grid)
         C.core.tune_set(keys[1], pipe)
         C.core.tune_set(keys[2], order)
+        C.core.tune_set(keys[3], wide)
+        top = n - 1
         ops = [g.Unitary(t, haar_unitary(32, rng))
-               for t in ([16, 17, 18, 19, 20], [0, 1, 2, 3, 4], [20, 0, 13, 6, 17],
-                         [18, 11, 5, 19, 2], [20, 18, 16, 14, 12])]
+               for t in ([16, 17, 18, 19, top], [0, 1, 2, 3, 4], [top, 0, 13, 6, 17],
+                         [18, 11, 5, 19, 2], [top, 18, 16, 14, 12])]
         # 4-target blocks (widened to 5 targets with an identity factor on the
         # lowest free bit) and controlled 4-target blocks: control on the
         # lowest bit (not widened) and on a middle bit (widened)
         ops += [g.Unitary(t, haar_unitary(16, rng))
-                for t in ([17, 18, 19, 20], [0, 7, 19, 3], [1, 2, 3, 20])]
-        for t in ([20, 0, 5, 11, 16], [9, 20, 19, 2, 14]):
+                for t in ([17, 18, 19, top], [0, 7, 19, 3], [1, 2, 3, top])]
+        for t in ([top, 0, 5, 11, 16], [9, top, 19, 2, 14]):
             m = np.eye(32, dtype=complex)
             m[16:, 16:] = haar_unitary(16, rng)  # targets[0] is the control
             ops.append(g.Unitary(t, m))
@@ -123,6 +129,28 @@ def test_unitary_5_target_tiles(C, dtype, grid, pipe, order):
     finally:
         for k, v in zip(keys, saved):
             C.core.tune_set(k, v)
+
+
+def test_wide_tiles_are_bitwise_narrow_tiles(C):
+    """The 64-group complex64 tile multiplies the same operands in the same
+    MFMA order as the 32-group tile: outputs are bitwise equal."""
+    from snuqs_amd.circuits import haar_unitary
+
+    g = C.operation.gate_operations
+    n = 22
+    rng = np.random.default_rng(8)
+    init = random_state(n, rng, np.complex64)
+    saved = C.core.tune_get("mfma5_wide_c64")
+    try:
+        for t in ([21, 3, 9, 15, 0], [5, 6, 7, 8, 9], [20, 17, 12, 1, 10]):
+            op = g.Unitary(t, haar_unitary(32, rng))
+            out = []
+            for wide in (0, 1):
+                C.core.tune_set("mfma5_wide_c64", wide)
+                out.append(run_engine(C, n, [op], "complex64", init))
+            assert np.array_equal(out[0], out[1]), t
+    finally:
+        C.core.tune_set("mfma5_wide_c64", saved)
 
 
 def test_ghz20_closed_form_bitwise(C):
