@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <array>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -548,6 +549,25 @@ int sq_program_tune(void *program, void *sv, uint64_t n, int reps, void *stream,
     for (int r = 0; r < reps && rc == SQ_OK; ++r)
       rc = run_timed_into(p, sv, n, st, ev, ms[c].data());
   }
+  // Second round: each gate's three fastest candidates are timed again,
+  // reps + 1 times each (a run sets every gate to its k-th candidate, so one
+  // run re-times different shapes of different gates); the pick uses these
+  // times only, which damps the selection noise of the screening round.
+  constexpr int kTop = 3;
+  std::vector<std::array<size_t, kTop>> top(ng);
+  for (size_t g : todo) {
+    std::vector<size_t> order(cands.size());
+    for (size_t c = 0; c < cands.size(); ++c) order[c] = c;
+    std::partial_sort(order.begin(), order.begin() + kTop, order.end(),
+                      [&](size_t a, size_t b) { return ms[a][g] < ms[b][g]; });
+    for (int k = 0; k < kTop; ++k) top[g][k] = order[k];
+  }
+  std::vector<std::vector<double>> ms2(kTop, std::vector<double>(ng, 0.0));
+  for (int k = 0; k < kTop && rc == SQ_OK; ++k) {
+    for (size_t g : todo) p->fwd[g].shape = cands[top[g][k]];
+    for (int r = 0; r <= reps && rc == SQ_OK; ++r)
+      rc = run_timed_into(p, sv, n, st, ev, ms2[k].data());
+  }
   for (auto &e : ev) (void)hipEventDestroy(e);
   if (rc != SQ_OK) {
     for (size_t g : todo) p->fwd[g].shape = GatePlan::Shape{};
@@ -555,11 +575,12 @@ int sq_program_tune(void *program, void *sv, uint64_t n, int reps, void *stream,
   }
   std::lock_guard<std::mutex> lk(g_shape_mutex);
   for (size_t g : todo) {
-    size_t best = 0;
-    for (size_t c = 1; c < cands.size(); ++c)
-      if (ms[c][g] < ms[best][g]) best = c;
-    p->fwd[g].shape = p->inv[g].shape = cands[best];
-    g_shape_cache[keys[g]] = cands[best];
+    int best = 0;
+    for (int k = 1; k < kTop; ++k)
+      if (ms2[k][g] < ms2[best][g]) best = k;
+    const GatePlan::Shape &sh = cands[top[g][best]];
+    p->fwd[g].shape = p->inv[g].shape = sh;
+    g_shape_cache[keys[g]] = sh;
     ++tuned;
   }
   if (num_tuned) *num_tuned = tuned;
