@@ -46,7 +46,10 @@ KERNEL_NAMES = {1: "k_phase", 2: "k_diag", 3: "k_perm", 4: "k_dense", 5: "k_mfma
 
 
 def kernel_name(cls, kk, dtype):
-    return KERNEL_NAMES.get(cls, "?").format(k=kk, d="c64" if dtype == "complex64" else "c128")
+    name = KERNEL_NAMES.get(cls, "?").format(k=kk, d="c64" if dtype == "complex64" else "c128")
+    if name == "k_mfma5_c64":  # the tuner picks the 32- or 64-group tile per gate
+        name += "/k_mfma5w_c64"
+    return name
 
 
 def pmc_traffic(name):
@@ -395,7 +398,8 @@ def main():
             "bound": "hbm", "kernel": kernel_name(dcls, dkk, dtype),
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": pmc_traffic(kernel_name(dcls, dkk, dtype)) if n == 30 else None,
+            "traffic": (pmc_traffic(kernel_name(dcls, dkk, dtype).split("/")[0])
+                        if n == 30 else None),
             "algorithmic_bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_ms,
             "launches": d["launches"],
             "mfma_tflops": flops / (avg_ms * 1e-3) / 1e12 if flops else None,
