@@ -189,12 +189,22 @@ _DERIV = {"sin": math.cos, "cos": lambda x: -math.sin(x), "tan": lambda x: 1 / m
 
 def _apply_function(name, args):
     f = _FUNCTIONS[name]
-    if any(isinstance(a, Dual) for a in args):
-        if name not in _DERIV or len(args) != 1:
-            raise NotImplementedError(f"{name} of a program input cannot be differentiated")
-        x = args[0]
-        return Dual(f(x.v), {k: _DERIV[name](x.v) * dx for k, dx in x.d.items()})
-    return f(*[float(a) for a in args])
+    if not any(isinstance(a, Dual) for a in args):
+        return f(*[float(a) for a in args])
+    if name == "pow" and len(args) == 2:
+        return Dual._lift(args[0]) ** args[1]
+    if name == "mod" and len(args) == 2:
+        return Dual._lift(args[0]) % args[1]
+    if len(args) != 1:
+        raise ValueError(f"{name} takes one argument")
+    x = Dual._lift(args[0])
+    if name == "abs":
+        d = math.copysign(1.0, x.v) if x.v else 0.0
+    elif name in ("ceiling", "floor"):
+        d = 0.0  # piecewise constant
+    else:
+        d = _DERIV[name](x.v)
+    return Dual(f(x.v), {k: d * dx for k, dx in x.d.items()})
 
 
 def unitary_power(m: np.ndarray, p: float) -> np.ndarray:

@@ -264,7 +264,8 @@ def test_dual_numbers_differentiate_angle_expressions():
     same expression (forward-mode rules for + - * / ** and the functions)."""
     exprs = ["2*theta", "sin(phi)", "theta/phi", "theta**2", "exp(-theta)*cos(phi)",
              "sqrt(phi) - arctan(theta)", "2**theta", "-(theta - 3*phi)/4", "log(phi)*π",
-             "arcsin(theta) + arccos(theta/2) + tan(phi)"]
+             "arcsin(theta) + arccos(theta/2) + tan(phi)", "abs(theta - phi) + floor(phi)",
+             "pow(phi, 3) + mod(theta, 0.2) + ceiling(theta)"]
     x0 = {"theta": 0.3, "phi": 0.7}
     h = 1e-6
 
