@@ -969,6 +969,83 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   for (uint32_t c = 0; c < num_checkpoints; ++c)
     bound[c] = (uint32_t)(((uint64_t)(c + 1) * ng) / (num_checkpoints + 1));
 
+  // Runs of consecutive diagonal gates (1-2 targets, at most one derivative
+  // each, diagonal derivative) are undone in one pass in the reverse sweep
+  // (launch_adjoint_diag_run).  A gate where a checkpoint is restored stays
+  // out of every run.  run_top[g] = index of the run whose highest gate is g.
+  std::vector<DiagRunGate> rgates;
+  std::vector<int> rslots;
+  struct Run {
+    uint32_t lo, first, m, first_slot, nslots;
+  };
+  std::vector<Run> runs;
+  std::vector<int> run_top(ng, -1);
+  std::vector<char> is_bound(ng, 0);
+  for (uint32_t c = 0; c < num_checkpoints; ++c)
+    if (bound[c] < ng) is_bound[bound[c]] = 1;
+  if ((n % 1024) == 0) {
+    auto is_diag = [](const double *m, int k) {
+      const int d = 1 << k;
+      for (int r = 0; r < d; ++r)
+        for (int c = 0; c < d; ++c)
+          if (r != c && (m[2 * (r * d + c)] != 0.0 || m[2 * (r * d + c) + 1] != 0.0)) return false;
+      return true;
+    };
+    auto diag_ok = [&](uint32_t g) {
+      const int k = p->nt[g];
+      if (is_bound[g] || k < 1 || k > 2 || by_gate[g].size() > 1) return false;
+      if (!is_diag(p->mats.data() + p->moffs[g], k)) return false;
+      return by_gate[g].empty() || is_diag(dmatrices + doffs[by_gate[g][0]], k);
+    };
+    for (int g = (int)ng - 1; g >= 0;) {
+      int lo = g, nsl = 0;
+      while (lo >= 0 && diag_ok((uint32_t)lo) && g - lo < kMaxDiagRun &&
+             nsl + (int)by_gate[lo].size() <= kMaxDiagSlots) {
+        nsl += (int)by_gate[lo].size();
+        --lo;
+      }
+      ++lo;  // lowest gate of the run
+      if (g - lo + 1 >= 2) {
+        Run r{(uint32_t)lo, (uint32_t)rgates.size(), (uint32_t)(g - lo + 1),
+              (uint32_t)rslots.size(), 0};
+        for (int q = g; q >= lo; --q) {
+          const int k = p->nt[q];
+          const int d = 1 << k;
+          const double *G = p->mats.data() + p->moffs[q];
+          DiagRunGate e{};
+          e.b0 = p->nq - 1 - (int)p->targets[p->toff[q]];
+          e.b1 = k == 2 ? p->nq - 1 - (int)p->targets[p->toff[q] + 1] : -1;
+          e.slot = -1;
+          for (int l = 0; l < d; ++l) {  // G^dagger diagonal = conj(G[l][l])
+            e.ginv[l][0] = G[2 * (l * d + l)];
+            e.ginv[l][1] = -G[2 * (l * d + l) + 1];
+          }
+          if (!by_gate[q].empty()) {
+            const uint32_t di = by_gate[q][0];
+            const double *D = dmatrices + doffs[di];
+            e.slot = (int)r.nslots++;
+            rslots.push_back((int)deriv_param[di]);
+            for (int l = 0; l < d; ++l) {  // (dG G^dagger)_ll = dG_ll conj(G_ll)
+              const double dr = D[2 * (l * d + l)], dim = D[2 * (l * d + l) + 1];
+              const double gr = e.ginv[l][0], gim = e.ginv[l][1];
+              e.dgg[l][0] = dr * gr - dim * gim;
+              e.dgg[l][1] = dr * gim + dim * gr;
+            }
+          }
+          rgates.push_back(e);
+        }
+        run_top[g] = (int)runs.size();
+        runs.push_back(r);
+        g = lo - 1;
+      } else {
+        g = g - 1;
+      }
+    }
+  }
+  const size_t rg_b = (rgates.size() * sizeof(DiagRunGate) + 255) / 256 * 256;
+  const size_t rs_b = (rslots.size() * sizeof(int) + 255) / 256 * 256;
+  const size_t rp_b = runs.empty() ? 0 : (size_t)kMaxDiagSlots * 4096 * sizeof(double);
+
   int dev;
   SQ_TRY(current_device(&dev));
   std::lock_guard<std::mutex> lk(g_ws_mutex[dev]);
@@ -976,7 +1053,8 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   const size_t edev = expect_ws_dev(nterms);
   const size_t apb = nterms * pauli_apply_term_bytes() + 256;
   Workspace *w;
-  SQ_TRY(ws_reserve(dev, edev + grad_b + apb + 4096, nterms * 64 + grad_b + apb + 4096, &w));
+  SQ_TRY(ws_reserve(dev, edev + grad_b + apb + rg_b + rs_b + rp_b + 4096,
+                    nterms * 64 + grad_b + apb + rg_b + rs_b + 4096, &w));
   hipStream_t st = (hipStream_t)stream;
   char *dbase = (char *)w->dev;
   double *partial = (double *)dbase;                      // reduction partials
@@ -984,6 +1062,15 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   void *aterms = al((char *)gslots + grad_b);             // Pauli apply terms
   double *two = gslots + 2 * num_params;                  // {2, 0}
   double *eslot = two + 2;                                // energy (re, im)
+  DiagRunGate *rgates_dev = (DiagRunGate *)al((char *)aterms + apb);
+  int *rslots_dev = (int *)al((char *)rgates_dev + rg_b);
+  double *rpartial = (double *)al((char *)rslots_dev + rs_b);
+  if (!runs.empty()) {  // staged through the pinned host buffer, after the gradient slots
+    char *hs = (char *)al((char *)w->host + apb + grad_b);
+    std::memcpy(hs, rgates.data(), rgates.size() * sizeof(DiagRunGate));
+    std::memcpy(hs + rg_b, rslots.data(), rslots.size() * sizeof(int));
+    SQ_HIP(hipMemcpyAsync(rgates_dev, hs, rg_b + rs_b, hipMemcpyHostToDevice, st));
+  }
 
   // forward pass with checkpoints
   SQ_TRY(launch_init(sv, dt, n, true, st));
@@ -1011,6 +1098,14 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   // reverse sweep
   int cidx = (int)num_checkpoints - 1;
   for (int g = (int)ng - 1; g >= 0; --g) {
+    if (run_top[g] >= 0) {
+      const Run &r = runs[run_top[g]];
+      SQ_TRY(launch_adjoint_diag_run(sv, work0, dt, n, rgates_dev + r.first, (int)r.m,
+                                     (int)r.nslots, rslots_dev + r.first_slot, rpartial, gslots,
+                                     st));
+      g = (int)r.lo;  // the loop steps below the run
+      continue;
+    }
     bool restored = false;
     if (cidx >= 0 && bound[cidx] == (uint32_t)g) {
       SQ_HIP(hipMemcpyAsync(sv, checkpoints[cidx], n * itemsize(dt), hipMemcpyDeviceToDevice, st));

@@ -241,6 +241,104 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
+// ---- reverse step over a run of diagonal gates ---------------------------------
+// Each thread takes U amplitudes i = base + t + kBlock u of a chunk; per slot
+// gate the chunk's sum of Re(conj(lam_i) psi_i dgg(i)) is reduced over the
+// wave and added to an LDS row; then every amplitude of psi and lambda is
+// multiplied by the run's G^dagger phase product, formed in FP64 (one
+// rounding per amplitude for complex64).
+template <typename T, typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_adjoint_diag_run(T *__restrict__ psi, T *__restrict__ lam, uint64_t n,
+                       const DiagRunGate *__restrict__ gates, int m, int nslots,
+                       double *__restrict__ partial) {
+  constexpr int U = 4, W = kBlock / 64;
+  __shared__ DiagRunGate sg[kMaxDiagRun];
+  __shared__ double srow[kMaxDiagSlots][W];
+  {
+    const double *src = reinterpret_cast<const double *>(gates);
+    double *dst = reinterpret_cast<double *>(sg);
+    const int words = m * (int)(sizeof(DiagRunGate) / sizeof(double));
+    for (int e = threadIdx.x; e < words; e += kBlock) dst[e] = src[e];
+    for (int e = threadIdx.x; e < kMaxDiagSlots * W; e += kBlock) (&srow[0][0])[e] = 0.0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t chunk = (uint64_t)kBlock * U;
+  for (uint64_t base = (uint64_t)blockIdx.x * chunk; base < n; base += (uint64_t)gridDim.x * chunk) {
+    T a[U], l[U];
+    double wr[U], wi[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = base + threadIdx.x + (uint64_t)kBlock * u;
+      a[u] = ld_nt(psi + i);
+      l[u] = ld_nt(lam + i);
+      // w = conj(lam) psi
+      wr[u] = (double)l[u].x * a[u].x + (double)l[u].y * a[u].y;
+      wi[u] = (double)l[u].x * a[u].y - (double)l[u].y * a[u].x;
+    }
+    double pr[U], pi[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      pr[u] = 1.0;
+      pi[u] = 0.0;
+    }
+    for (int k = 0; k < m; ++k) {
+      const DiagRunGate &gk = sg[k];
+      double acc = 0.0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint64_t i = base + threadIdx.x + (uint64_t)kBlock * u;
+        const int loc = gk.b1 < 0 ? (int)((i >> gk.b0) & 1)
+                                  : (int)((((i >> gk.b0) & 1) << 1) | ((i >> gk.b1) & 1));
+        const double gr = gk.ginv[loc][0], gi = gk.ginv[loc][1];
+        const double tr = pr[u] * gr - pi[u] * gi;
+        pi[u] = pr[u] * gi + pi[u] * gr;
+        pr[u] = tr;
+        if (gk.slot >= 0) acc += wr[u] * gk.dgg[loc][0] - wi[u] * gk.dgg[loc][1];
+      }
+      if (gk.slot >= 0) {  // uniform over the block
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if (lane == 0) srow[gk.slot][wave] += acc;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = base + threadIdx.x + (uint64_t)kBlock * u;
+      T o, q;
+      o.x = (R)((double)a[u].x * pr[u] - (double)a[u].y * pi[u]);
+      o.y = (R)((double)a[u].x * pi[u] + (double)a[u].y * pr[u]);
+      q.x = (R)((double)l[u].x * pr[u] - (double)l[u].y * pi[u]);
+      q.y = (R)((double)l[u].x * pi[u] + (double)l[u].y * pr[u]);
+      st_nt(psi + i, o);
+      st_nt(lam + i, q);
+    }
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < nslots; s += kBlock) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) v += srow[s][w];
+    partial[(uint64_t)s * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+// out[2 slot_param[v]] += alpha * sum_b partial[v][b] for v = 0..nslots-1, in
+// order by one block: slots of gates that share a parameter add to the same
+// output, so they must not be finalized concurrently.
+__global__ void __launch_bounds__(kBlock)
+    k_finalize_slots(const double *__restrict__ partial, int nblocks, int nslots,
+                     const int *__restrict__ slot_param, double alpha, double *out) {
+  for (int v = 0; v < nslots; ++v) {
+    double acc = 0.0, dummy = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) acc += partial[(uint64_t)v * nblocks + b];
+    block_sum2(acc, dummy);
+    if (threadIdx.x == 0) out[2 * slot_param[v]] += alpha * acc;
+    __syncthreads();  // block_sum2's LDS is reused by the next slot
+  }
+}
+
 // ---- marginal distribution over k targets ------------------------------------
 // out[r] = sum of |sv[i]|^2 over the i whose target bits spell r (target 0
 // the most significant bit of r).  Block b sums outcome r = b / nb over the
@@ -413,6 +511,29 @@ int launch_adjoint_step1(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int bi
   }
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, st, partial, nb, alpha_dev, 1.0,
                      out_dev);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
+                            const DiagRunGate *gates_dev, int m, int nslots,
+                            const int *slot_param_dev, double *partial, double *out_dev,
+                            hipStream_t st) {
+  if (m < 1 || m > kMaxDiagRun || nslots < 0 || nslots > kMaxDiagSlots)
+    return fail(SQ_ERR_INVALID, "internal: diagonal run out of range");
+  const uint64_t chunk = (uint64_t)kBlock * 4;
+  if (n % chunk) return fail(SQ_ERR_INVALID, "internal: diagonal run needs n % 1024 == 0");
+  uint64_t nb = n / chunk;
+  if (nb > 4096) nb = 4096;
+  if (dt == SQ_COMPLEX64)
+    hipLaunchKernelGGL((k_adjoint_diag_run<float2, float>), dim3((unsigned)nb), dim3(kBlock), 0,
+                       st, (float2 *)psi, (float2 *)lam, n, gates_dev, m, nslots, partial);
+  else
+    hipLaunchKernelGGL((k_adjoint_diag_run<double2, double>), dim3((unsigned)nb), dim3(kBlock), 0,
+                       st, (double2 *)psi, (double2 *)lam, n, gates_dev, m, nslots, partial);
+  if (nslots)
+    hipLaunchKernelGGL(k_finalize_slots, dim3(1), dim3(kBlock), 0, st, partial, (int)nb, nslots,
+                       slot_param_dev, 2.0, out_dev);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
 }

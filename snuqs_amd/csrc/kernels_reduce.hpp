@@ -30,6 +30,26 @@ int launch_inner(const void *a, const void *b, sq_dtype_t dt, uint64_t n,
 int launch_adjoint_step1(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int bit,
                          const double *ginv, const double *dgm, int undo_psi, double *partial,
                          double *out_dev, const double *alpha_dev, hipStream_t st);
+// Reverse step over a run of m diagonal gates (each on 1 or 2 targets): psi
+// and lambda both multiplied by the product of the gates' G^dagger diagonals,
+// and for every gate with a slot, out_dev[2 slot_param[slot]] += 2 Re
+// sum_i conj(lam_i) psi_i dgg(i) at the run's end point (dgg = diagonal of
+// dG G^dagger; diagonal gates commute, so this is each gate's sequential
+// derivative term).  gates_dev: m DiagRunGate records; slot_param_dev: the
+// parameter of each of the nslots slots.
+constexpr int kMaxDiagRun = 64;
+constexpr int kMaxDiagSlots = 32;
+struct DiagRunGate {
+  int b0, b1;   // address bits of local index bit 1 / bit 0 (b1 < 0: one target, local = bit b0)
+  int slot;     // partial row of this gate's derivative, or -1
+  int pad;
+  double ginv[4][2];  // diagonal of G^dagger, by local index
+  double dgg[4][2];   // diagonal of dG G^dagger, by local index (slot >= 0)
+};
+int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
+                            const DiagRunGate *gates_dev, int m, int nslots,
+                            const int *slot_param_dev, double *partial, double *out_dev,
+                            hipStream_t st);
 int launch_pauli_terms(const void *sv, sq_dtype_t dt, uint64_t n, int nterms,
                        const void *terms_dev, const double *alpha_dev,
                        double *partial, double *out_dev, hipStream_t st);
