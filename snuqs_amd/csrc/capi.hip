@@ -1042,9 +1042,52 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       }
     }
   }
+  // forward runs: the same gates, multiplied by their own diagonals, so the
+  // forward sweep also takes one pass per run; fwd_start[g] = run starting at g
+  std::vector<int> fwd_start(ng, -1);
+  std::vector<Run> fruns;
+  if ((n % 1024) == 0) {
+    auto is_diag_g = [&](uint32_t g) {
+      const int k = p->nt[g];
+      if (is_bound[g] || k < 1 || k > 2) return false;
+      const int d = 1 << k;
+      const double *m = p->mats.data() + p->moffs[g];
+      for (int r = 0; r < d; ++r)
+        for (int c = 0; c < d; ++c)
+          if (r != c && (m[2 * (r * d + c)] != 0.0 || m[2 * (r * d + c) + 1] != 0.0)) return false;
+      return true;
+    };
+    for (uint32_t g = 0; g < ng;) {
+      uint32_t hi = g;
+      while (hi < ng && is_diag_g(hi) && hi - g < (uint32_t)kMaxDiagRun) ++hi;
+      if (hi - g >= 2) {
+        Run r{g, (uint32_t)rgates.size(), hi - g, 0, 0};
+        for (uint32_t q = g; q < hi; ++q) {
+          const int k = p->nt[q];
+          const int d = 1 << k;
+          const double *G = p->mats.data() + p->moffs[q];
+          DiagRunGate e{};
+          e.b0 = p->nq - 1 - (int)p->targets[p->toff[q]];
+          e.b1 = k == 2 ? p->nq - 1 - (int)p->targets[p->toff[q] + 1] : -1;
+          e.slot = -1;
+          for (int l = 0; l < d; ++l) {  // the gate's own diagonal
+            e.ginv[l][0] = G[2 * (l * d + l)];
+            e.ginv[l][1] = G[2 * (l * d + l) + 1];
+          }
+          rgates.push_back(e);
+        }
+        fwd_start[g] = (int)fruns.size();
+        fruns.push_back(r);
+        g = hi;
+      } else {
+        ++g;
+      }
+    }
+  }
+  const bool any_runs = !runs.empty() || !fruns.empty();
   const size_t rg_b = (rgates.size() * sizeof(DiagRunGate) + 255) / 256 * 256;
   const size_t rs_b = (rslots.size() * sizeof(int) + 255) / 256 * 256;
-  const size_t rp_b = runs.empty() ? 0 : (size_t)kMaxDiagSlots * 4096 * sizeof(double);
+  const size_t rp_b = any_runs ? (size_t)kMaxDiagSlots * 4096 * sizeof(double) : 0;
 
   int dev;
   SQ_TRY(current_device(&dev));
@@ -1065,7 +1108,7 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   DiagRunGate *rgates_dev = (DiagRunGate *)al((char *)aterms + apb);
   int *rslots_dev = (int *)al((char *)rgates_dev + rg_b);
   double *rpartial = (double *)al((char *)rslots_dev + rs_b);
-  if (!runs.empty()) {  // staged through the pinned host buffer, after the gradient slots
+  if (any_runs) {  // staged through the pinned host buffer, after the gradient slots
     char *hs = (char *)al((char *)w->host + apb + grad_b);
     std::memcpy(hs, rgates.data(), rgates.size() * sizeof(DiagRunGate));
     std::memcpy(hs + rg_b, rslots.data(), rslots.size() * sizeof(int));
@@ -1079,6 +1122,13 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     while (cnext < num_checkpoints && bound[cnext] == g) {
       SQ_HIP(hipMemcpyAsync(checkpoints[cnext], sv, n * itemsize(dt), hipMemcpyDeviceToDevice, st));
       ++cnext;
+    }
+    if (fwd_start[g] >= 0) {
+      const Run &r = fruns[fwd_start[g]];
+      SQ_TRY(launch_adjoint_diag_run(sv, nullptr, dt, n, rgates_dev + r.first, (int)r.m, 0,
+                                     nullptr, rpartial, nullptr, st));
+      g = r.lo + r.m - 1;  // the loop steps past the run
+      continue;
     }
     SQ_TRY(launch_plan(p->fwd[g], sv, dt, n, st, p->fwd_dev[g]));
   }

@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(kBlock)
 // wave and added to an LDS row; then every amplitude of psi and lambda is
 // multiplied by the run's G^dagger phase product, formed in FP64 (one
 // rounding per amplitude for complex64).
-template <typename T, typename R>
+template <typename T, typename R, bool LAM>
 __global__ void __launch_bounds__(kBlock)
     k_adjoint_diag_run(T *__restrict__ psi, T *__restrict__ lam, uint64_t n,
                        const DiagRunGate *__restrict__ gates, int m, int nslots,
@@ -272,10 +272,14 @@ __global__ void __launch_bounds__(kBlock)
     for (int u = 0; u < U; ++u) {
       const uint64_t i = base + threadIdx.x + (uint64_t)kBlock * u;
       a[u] = ld_nt(psi + i);
-      l[u] = ld_nt(lam + i);
-      // w = conj(lam) psi
-      wr[u] = (double)l[u].x * a[u].x + (double)l[u].y * a[u].y;
-      wi[u] = (double)l[u].x * a[u].y - (double)l[u].y * a[u].x;
+      if (LAM) {
+        l[u] = ld_nt(lam + i);
+        // w = conj(lam) psi
+        wr[u] = (double)l[u].x * a[u].x + (double)l[u].y * a[u].y;
+        wi[u] = (double)l[u].x * a[u].y - (double)l[u].y * a[u].x;
+      } else {
+        wr[u] = wi[u] = 0.0;
+      }
     }
     double pr[U], pi[U];
 #pragma unroll
@@ -295,9 +299,9 @@ __global__ void __launch_bounds__(kBlock)
         const double tr = pr[u] * gr - pi[u] * gi;
         pi[u] = pr[u] * gi + pi[u] * gr;
         pr[u] = tr;
-        if (gk.slot >= 0) acc += wr[u] * gk.dgg[loc][0] - wi[u] * gk.dgg[loc][1];
+        if (LAM && gk.slot >= 0) acc += wr[u] * gk.dgg[loc][0] - wi[u] * gk.dgg[loc][1];
       }
-      if (gk.slot >= 0) {  // uniform over the block
+      if (LAM && gk.slot >= 0) {  // uniform over the block
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
         if (lane == 0) srow[gk.slot][wave] += acc;
@@ -306,15 +310,19 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t i = base + threadIdx.x + (uint64_t)kBlock * u;
-      T o, q;
+      T o;
       o.x = (R)((double)a[u].x * pr[u] - (double)a[u].y * pi[u]);
       o.y = (R)((double)a[u].x * pi[u] + (double)a[u].y * pr[u]);
-      q.x = (R)((double)l[u].x * pr[u] - (double)l[u].y * pi[u]);
-      q.y = (R)((double)l[u].x * pi[u] + (double)l[u].y * pr[u]);
       st_nt(psi + i, o);
-      st_nt(lam + i, q);
+      if (LAM) {
+        T q;
+        q.x = (R)((double)l[u].x * pr[u] - (double)l[u].y * pi[u]);
+        q.y = (R)((double)l[u].x * pi[u] + (double)l[u].y * pr[u]);
+        st_nt(lam + i, q);
+      }
     }
   }
+  if (!LAM) return;
   __syncthreads();
   for (int s = threadIdx.x; s < nslots; s += kBlock) {
     double v = 0.0;
@@ -525,12 +533,17 @@ int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
   if (n % chunk) return fail(SQ_ERR_INVALID, "internal: diagonal run needs n % 1024 == 0");
   uint64_t nb = n / chunk;
   if (nb > 4096) nb = 4096;
-  if (dt == SQ_COMPLEX64)
-    hipLaunchKernelGGL((k_adjoint_diag_run<float2, float>), dim3((unsigned)nb), dim3(kBlock), 0,
-                       st, (float2 *)psi, (float2 *)lam, n, gates_dev, m, nslots, partial);
-  else
-    hipLaunchKernelGGL((k_adjoint_diag_run<double2, double>), dim3((unsigned)nb), dim3(kBlock), 0,
-                       st, (double2 *)psi, (double2 *)lam, n, gates_dev, m, nslots, partial);
+  if (!lam && nslots) return fail(SQ_ERR_INVALID, "internal: derivative slots need lambda");
+  if (dt == SQ_COMPLEX64) {
+    auto k = lam ? k_adjoint_diag_run<float2, float, true> : k_adjoint_diag_run<float2, float, false>;
+    hipLaunchKernelGGL(k, dim3((unsigned)nb), dim3(kBlock), 0, st, (float2 *)psi, (float2 *)lam,
+                       n, gates_dev, m, nslots, partial);
+  } else {
+    auto k = lam ? k_adjoint_diag_run<double2, double, true>
+                 : k_adjoint_diag_run<double2, double, false>;
+    hipLaunchKernelGGL(k, dim3((unsigned)nb), dim3(kBlock), 0, st, (double2 *)psi, (double2 *)lam,
+                       n, gates_dev, m, nslots, partial);
+  }
   if (nslots)
     hipLaunchKernelGGL(k_finalize_slots, dim3(1), dim3(kBlock), 0, st, partial, (int)nb, nslots,
                        slot_param_dev, 2.0, out_dev);

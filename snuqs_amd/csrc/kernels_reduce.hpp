@@ -36,7 +36,9 @@ int launch_adjoint_step1(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int bi
 // sum_i conj(lam_i) psi_i dgg(i) at the run's end point (dgg = diagonal of
 // dG G^dagger; diagonal gates commute, so this is each gate's sequential
 // derivative term).  gates_dev: m DiagRunGate records; slot_param_dev: the
-// parameter of each of the nslots slots.
+// parameter of each of the nslots slots.  lam == nullptr (and nslots == 0):
+// psi alone is multiplied by the records' `ginv` diagonals (used with the
+// gates' own diagonals for a forward run).
 constexpr int kMaxDiagRun = 64;
 constexpr int kMaxDiagSlots = 32;
 struct DiagRunGate {
