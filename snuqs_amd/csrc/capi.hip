@@ -1042,6 +1042,75 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       }
     }
   }
+  // Groups of 2-4 consecutive one-target gates with one derivative each, on
+  // distinct amplitude bits above bit 3 and outside the runs, take one
+  // reverse pass (launch_adjoint_step_group); grp_top[g] = group whose highest
+  // gate is g.
+  struct Group {
+    uint32_t lo, k, first_slot;
+    int bits[4];
+    double ginv[32], dgg[32];
+  };
+  std::vector<Group> groups;
+  std::vector<int> grp_top(ng, -1);
+  {
+    std::vector<char> in_run(ng, 0);
+    for (const Run &r : runs)
+      for (uint32_t q = r.lo; q < r.lo + r.m; ++q) in_run[q] = 1;
+    auto grp_ok = [&](int g) {
+      return !in_run[g] && !is_bound[g] && p->nt[g] == 1 && by_gate[g].size() == 1 &&
+             p->nq - 1 - (int)p->targets[p->toff[g]] >= 4;
+    };
+    for (int g = (int)ng - 1; g >= 0;) {
+      int lo = g;
+      uint64_t used = 0;
+      while (lo >= 0 && g - lo < 4 && grp_ok(lo)) {
+        const int b = p->nq - 1 - (int)p->targets[p->toff[lo]];
+        if ((used >> b) & 1) break;
+        used |= 1ull << b;
+        --lo;
+      }
+      ++lo;
+      if (g - lo + 1 < 2) {
+        --g;
+        continue;
+      }
+      Group gr{};
+      gr.lo = (uint32_t)lo;
+      gr.k = (uint32_t)(g - lo + 1);
+      gr.first_slot = (uint32_t)rslots.size();
+      for (int j = 0; j < (int)gr.k; ++j) {
+        const int q = g - j;
+        const uint32_t di = by_gate[q][0];
+        gr.bits[j] = p->nq - 1 - (int)p->targets[p->toff[q]];
+        rslots.push_back((int)deriv_param[di]);
+        const double *G = p->mats.data() + p->moffs[q];
+        const double *D = dmatrices + doffs[di];
+        double *gi = gr.ginv + 8 * j, *dg = gr.dgg + 8 * j;
+        for (int r = 0; r < 2; ++r)
+          for (int c = 0; c < 2; ++c) {  // G^dagger[r][c] = conj(G[c][r])
+            gi[2 * (2 * r + c)] = G[2 * (2 * c + r)];
+            gi[2 * (2 * r + c) + 1] = -G[2 * (2 * c + r) + 1];
+          }
+        for (int r = 0; r < 2; ++r)
+          for (int c = 0; c < 2; ++c) {  // (dG G^dagger)[r][c]
+            double re = 0, im = 0;
+            for (int t = 0; t < 2; ++t) {
+              const double ar = D[2 * (2 * r + t)], ai = D[2 * (2 * r + t) + 1];
+              const double br = gi[2 * (2 * t + c)], bi = gi[2 * (2 * t + c) + 1];
+              re += ar * br - ai * bi;
+              im += ar * bi + ai * br;
+            }
+            dg[2 * (2 * r + c)] = re;
+            dg[2 * (2 * r + c) + 1] = im;
+          }
+      }
+      grp_top[g] = (int)groups.size();
+      groups.push_back(gr);
+      g = lo - 1;
+    }
+  }
+
   // forward runs: the same gates, multiplied by their own diagonals, so the
   // forward sweep also takes one pass per run; fwd_start[g] = run starting at g
   std::vector<int> fwd_start(ng, -1);
@@ -1084,7 +1153,7 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       }
     }
   }
-  const bool any_runs = !runs.empty() || !fruns.empty();
+  const bool any_runs = !runs.empty() || !fruns.empty() || !groups.empty();
   const size_t rg_b = (rgates.size() * sizeof(DiagRunGate) + 255) / 256 * 256;
   const size_t rs_b = (rslots.size() * sizeof(int) + 255) / 256 * 256;
   const size_t rp_b = any_runs ? (size_t)kMaxDiagSlots * 4096 * sizeof(double) : 0;
@@ -1154,6 +1223,13 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
                                      (int)r.nslots, rslots_dev + r.first_slot, rpartial, gslots,
                                      st));
       g = (int)r.lo;  // the loop steps below the run
+      continue;
+    }
+    if (grp_top[g] >= 0) {
+      const Group &gr = groups[grp_top[g]];
+      SQ_TRY(launch_adjoint_step_group(sv, work0, dt, n, (int)gr.k, gr.bits, gr.ginv, gr.dgg,
+                                       rslots_dev + gr.first_slot, rpartial, gslots, st));
+      g = (int)gr.lo;
       continue;
     }
     bool restored = false;

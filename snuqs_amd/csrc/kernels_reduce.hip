@@ -347,6 +347,79 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// ---- reverse step over a group of one-target gates on distinct bits ------------
+// A lane holds the 2^K amplitudes of psi and lambda of one group (the K target
+// bits vary): for each gate j, the pairs along local bit j give
+// Re conj(lam) (A_j psi) (A_j = dG_j G_j^dagger), then every G_j^dagger is
+// applied to both.  Gate j is local bit K-1-j (gate 0 the most significant).
+template <typename R, int K>
+struct GroupArgs {
+  Mat2<R> gi[K], dg[K];
+  uint64_t off[1 << K];  // address offset of local index r
+};
+
+template <typename T, typename R, int K>
+__global__ void __launch_bounds__(kBlock)
+    k_adjoint_step_group(T *__restrict__ psi, T *__restrict__ lam, uint64_t ngroups, IdxMap im,
+                         GroupArgs<R, K> a, double *__restrict__ partial) {
+  constexpr int D = 1 << K;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  for (uint64_t f = (uint64_t)blockIdx.x * kBlock + threadIdx.x; f < ngroups;
+       f += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t base = deposit(f, im);
+    T x[D], l[D];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      x[r] = ld_nt(psi + base + a.off[r]);
+      l[r] = ld_nt(lam + base + a.off[r]);
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int b = 1 << (K - 1 - j);
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        if (r & b) continue;
+        T d0, d1;
+        mat2_apply(a.dg[j], x[r], x[r | b], d0, d1);
+        acc[j] += (double)l[r].x * d0.x + (double)l[r].y * d0.y + (double)l[r | b].x * d1.x +
+                  (double)l[r | b].y * d1.y;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int b = 1 << (K - 1 - j);
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        if (r & b) continue;
+        T y0, y1;
+        mat2_apply(a.gi[j], x[r], x[r | b], y0, y1);
+        x[r] = y0;
+        x[r | b] = y1;
+        mat2_apply(a.gi[j], l[r], l[r | b], y0, y1);
+        l[r] = y0;
+        l[r | b] = y1;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      st_nt(psi + base + a.off[r], x[r]);
+      st_nt(lam + base + a.off[r], l[r]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < K; j += 2) {
+    double u = acc[j], v = j + 1 < K ? acc[j + 1] : 0.0;
+    block_sum2(u, v);
+    if (threadIdx.x == 0) {
+      partial[(uint64_t)j * gridDim.x + blockIdx.x] = u;
+      if (j + 1 < K) partial[(uint64_t)(j + 1) * gridDim.x + blockIdx.x] = v;
+    }
+    __syncthreads();  // block_sum2's LDS is reused
+  }
+}
+
 // ---- marginal distribution over k targets ------------------------------------
 // out[r] = sum of |sv[i]|^2 over the i whose target bits spell r (target 0
 // the most significant bit of r).  Block b sums outcome r = b / nb over the
@@ -547,6 +620,56 @@ int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
   if (nslots)
     hipLaunchKernelGGL(k_finalize_slots, dim3(1), dim3(kBlock), 0, st, partial, (int)nb, nslots,
                        slot_param_dev, 2.0, out_dev);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+template <typename T, typename R, int K>
+static void launch_group_t(void *psi, void *lam, uint64_t n, const int *bits, const double *ginv,
+                           const double *dgg, double *partial, int nb, hipStream_t st) {
+  GroupArgs<R, K> a{};
+  uint64_t ins = 0;
+  for (int j = 0; j < K; ++j) {
+    ins |= 1ull << bits[j];
+    for (int e = 0; e < 4; ++e) {
+      a.gi[j].r[e] = (R)ginv[8 * j + 2 * e];
+      a.gi[j].i[e] = (R)ginv[8 * j + 2 * e + 1];
+      a.dg[j].r[e] = (R)dgg[8 * j + 2 * e];
+      a.dg[j].i[e] = (R)dgg[8 * j + 2 * e + 1];
+    }
+  }
+  for (int r = 0; r < (1 << K); ++r) {
+    uint64_t o = 0;
+    for (int j = 0; j < K; ++j)
+      if ((r >> (K - 1 - j)) & 1) o += 1ull << bits[j];
+    a.off[r] = o;
+  }
+  hipLaunchKernelGGL((k_adjoint_step_group<T, R, K>), dim3(nb), dim3(kBlock), 0, st, (T *)psi,
+                     (T *)lam, n >> K, make_idxmap(ins, 0), a, partial);
+}
+
+int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int k,
+                              const int *bits, const double *ginv, const double *dgg,
+                              const int *slot_param_dev, double *partial, double *out_dev,
+                              hipStream_t st) {
+  if (k < 2 || k > 4) return fail(SQ_ERR_INVALID, "internal: gate group size out of range");
+  const int nb = red_blocks(n >> k);
+  const bool c64 = dt == SQ_COMPLEX64;
+#define SQ_GROUP(K)                                                                       \
+  case K:                                                                                 \
+    if (c64)                                                                              \
+      launch_group_t<float2, float, K>(psi, lam, n, bits, ginv, dgg, partial, nb, st);    \
+    else                                                                                  \
+      launch_group_t<double2, double, K>(psi, lam, n, bits, ginv, dgg, partial, nb, st);  \
+    break;
+  switch (k) {
+    SQ_GROUP(2)
+    SQ_GROUP(3)
+    SQ_GROUP(4)
+  }
+#undef SQ_GROUP
+  hipLaunchKernelGGL(k_finalize_slots, dim3(1), dim3(kBlock), 0, st, partial, nb, k,
+                     slot_param_dev, 2.0, out_dev);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
 }

@@ -52,6 +52,15 @@ int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
                             const DiagRunGate *gates_dev, int m, int nslots,
                             const int *slot_param_dev, double *partial, double *out_dev,
                             hipStream_t st);
+// Reverse step over K = 2..4 one-target gates on distinct amplitude bits
+// (bits[j] = bit of gate j, none of the lowest 4): psi and lambda undone by
+// every gate's G^dagger (ginv, row-major complex128 2x2 each), and for gate j
+// out_dev[2 slot_param[j]] += 2 Re <lam| dG_j G_j^dagger |psi> at the group's
+// end point (the gates commute).  One pass; partial needs K x kRedBlocks.
+int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int k,
+                              const int *bits, const double *ginv, const double *dgg,
+                              const int *slot_param_dev, double *partial, double *out_dev,
+                              hipStream_t st);
 int launch_pauli_terms(const void *sv, sq_dtype_t dt, uint64_t n, int nterms,
                        const void *terms_dev, const double *alpha_dev,
                        double *partial, double *out_dev, hipStream_t st);
