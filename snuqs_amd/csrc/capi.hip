@@ -1153,6 +1153,45 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       }
     }
   }
+  // forward groups: 2-4 consecutive one-target gates on distinct bits above
+  // bit 3 (outside forward runs, not at a checkpoint) applied in one pass
+  std::vector<Group> fgroups;
+  std::vector<int> fgrp_start(ng, -1);
+  {
+    std::vector<char> in_frun(ng, 0);
+    for (const Run &r : fruns)
+      for (uint32_t q = r.lo; q < r.lo + r.m; ++q) in_frun[q] = 1;
+    auto fok = [&](uint32_t g) {
+      return !in_frun[g] && !is_bound[g] && p->nt[g] == 1 &&
+             p->nq - 1 - (int)p->targets[p->toff[g]] >= 4;
+    };
+    for (uint32_t g = 0; g < ng;) {
+      uint32_t hi = g;
+      uint64_t used = 0;
+      while (hi < ng && hi - g < 4 && fok(hi)) {
+        const int b = p->nq - 1 - (int)p->targets[p->toff[hi]];
+        if ((used >> b) & 1) break;
+        used |= 1ull << b;
+        ++hi;
+      }
+      if (hi - g < 2) {
+        ++g;
+        continue;
+      }
+      Group gr{};
+      gr.lo = g;
+      gr.k = hi - g;
+      for (uint32_t j = 0; j < gr.k; ++j) {
+        const uint32_t q = g + j;
+        gr.bits[j] = p->nq - 1 - (int)p->targets[p->toff[q]];
+        const double *G = p->mats.data() + p->moffs[q];
+        for (int e = 0; e < 8; ++e) gr.ginv[8 * j + e] = G[e];  // the gate itself
+      }
+      fgrp_start[g] = (int)fgroups.size();
+      fgroups.push_back(gr);
+      g = hi;
+    }
+  }
   const bool any_runs = !runs.empty() || !fruns.empty() || !groups.empty();
   const size_t rg_b = (rgates.size() * sizeof(DiagRunGate) + 255) / 256 * 256;
   const size_t rs_b = (rslots.size() * sizeof(int) + 255) / 256 * 256;
@@ -1197,6 +1236,13 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       SQ_TRY(launch_adjoint_diag_run(sv, nullptr, dt, n, rgates_dev + r.first, (int)r.m, 0,
                                      nullptr, rpartial, nullptr, st));
       g = r.lo + r.m - 1;  // the loop steps past the run
+      continue;
+    }
+    if (fgrp_start[g] >= 0) {
+      const Group &gr = fgroups[fgrp_start[g]];
+      SQ_TRY(launch_adjoint_step_group(sv, nullptr, dt, n, (int)gr.k, gr.bits, gr.ginv, gr.dgg,
+                                       nullptr, rpartial, nullptr, st));
+      g = gr.lo + gr.k - 1;
       continue;
     }
     SQ_TRY(launch_plan(p->fwd[g], sv, dt, n, st, p->fwd_dev[g]));

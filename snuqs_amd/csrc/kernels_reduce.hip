@@ -358,7 +358,7 @@ struct GroupArgs {
   uint64_t off[1 << K];  // address offset of local index r
 };
 
-template <typename T, typename R, int K>
+template <typename T, typename R, int K, bool LAM>
 __global__ void __launch_bounds__(kBlock)
     k_adjoint_step_group(T *__restrict__ psi, T *__restrict__ lam, uint64_t ngroups, IdxMap im,
                          GroupArgs<R, K> a, double *__restrict__ partial) {
@@ -373,7 +373,24 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int r = 0; r < D; ++r) {
       x[r] = ld_nt(psi + base + a.off[r]);
-      l[r] = ld_nt(lam + base + a.off[r]);
+      if (LAM) l[r] = ld_nt(lam + base + a.off[r]);
+    }
+    if (!LAM) {  // forward: psi <- (every gi) psi
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const int b = 1 << (K - 1 - j);
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+          if (r & b) continue;
+          T y0, y1;
+          mat2_apply(a.gi[j], x[r], x[r | b], y0, y1);
+          x[r] = y0;
+          x[r | b] = y1;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < D; ++r) st_nt(psi + base + a.off[r], x[r]);
+      continue;
     }
 #pragma unroll
     for (int j = 0; j < K; ++j) {
@@ -408,6 +425,7 @@ __global__ void __launch_bounds__(kBlock)
       st_nt(lam + base + a.off[r], l[r]);
     }
   }
+  if (!LAM) return;
 #pragma unroll
   for (int j = 0; j < K; j += 2) {
     double u = acc[j], v = j + 1 < K ? acc[j + 1] : 0.0;
@@ -644,8 +662,12 @@ static void launch_group_t(void *psi, void *lam, uint64_t n, const int *bits, co
       if ((r >> (K - 1 - j)) & 1) o += 1ull << bits[j];
     a.off[r] = o;
   }
-  hipLaunchKernelGGL((k_adjoint_step_group<T, R, K>), dim3(nb), dim3(kBlock), 0, st, (T *)psi,
-                     (T *)lam, n >> K, make_idxmap(ins, 0), a, partial);
+  if (lam)
+    hipLaunchKernelGGL((k_adjoint_step_group<T, R, K, true>), dim3(nb), dim3(kBlock), 0, st,
+                       (T *)psi, (T *)lam, n >> K, make_idxmap(ins, 0), a, partial);
+  else
+    hipLaunchKernelGGL((k_adjoint_step_group<T, R, K, false>), dim3(nb), dim3(kBlock), 0, st,
+                       (T *)psi, (T *)nullptr, n >> K, make_idxmap(ins, 0), a, partial);
 }
 
 int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int k,
@@ -653,7 +675,10 @@ int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, i
                               const int *slot_param_dev, double *partial, double *out_dev,
                               hipStream_t st) {
   if (k < 2 || k > 4) return fail(SQ_ERR_INVALID, "internal: gate group size out of range");
-  const int nb = red_blocks(n >> k);
+  // a forward group (lam == nullptr) only applies the ginv matrices to psi:
+  // callers pass the gates themselves there; more blocks, no reduction
+  const uint64_t fb = (n >> k) / kBlock + 1;
+  const int nb = lam ? red_blocks(n >> k) : (int)(fb < (1u << 16) ? fb : (1u << 16));
   const bool c64 = dt == SQ_COMPLEX64;
 #define SQ_GROUP(K)                                                                       \
   case K:                                                                                 \
@@ -668,8 +693,9 @@ int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, i
     SQ_GROUP(4)
   }
 #undef SQ_GROUP
-  hipLaunchKernelGGL(k_finalize_slots, dim3(1), dim3(kBlock), 0, st, partial, nb, k,
-                     slot_param_dev, 2.0, out_dev);
+  if (lam)
+    hipLaunchKernelGGL(k_finalize_slots, dim3(1), dim3(kBlock), 0, st, partial, nb, k,
+                       slot_param_dev, 2.0, out_dev);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
 }

@@ -57,6 +57,8 @@ int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
 // every gate's G^dagger (ginv, row-major complex128 2x2 each), and for gate j
 // out_dev[2 slot_param[j]] += 2 Re <lam| dG_j G_j^dagger |psi> at the group's
 // end point (the gates commute).  One pass; partial needs K x kRedBlocks.
+// lam == nullptr: forward group, psi <- (every `ginv` matrix) psi, where the
+// caller passes the gates themselves; no reduction.
 int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int k,
                               const int *bits, const double *ginv, const double *dgg,
                               const int *slot_param_dev, double *partial, double *out_dev,
