@@ -287,9 +287,35 @@ __global__ void __launch_bounds__(kBlock)
       pr[u] = 1.0;
       pi[u] = 0.0;
     }
+    // gates whose target bits are all >= log2(chunk) see one local index in
+    // the whole chunk: their phase goes into one chunk factor and their term
+    // uses the thread's sum of w
+    double cr = 1.0, ci = 0.0, swr = 0.0, swi = 0.0;
+    if (LAM) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        swr += wr[u];
+        swi += wi[u];
+      }
+    }
     for (int k = 0; k < m; ++k) {
       const DiagRunGate &gk = sg[k];
       double acc = 0.0;
+      if (gk.pad) {  // chunk-uniform gate (uniform over the block)
+        const int loc = gk.b1 < 0 ? (int)((base >> gk.b0) & 1)
+                                  : (int)((((base >> gk.b0) & 1) << 1) | ((base >> gk.b1) & 1));
+        const double gr = gk.ginv[loc][0], gi = gk.ginv[loc][1];
+        const double tr = cr * gr - ci * gi;
+        ci = cr * gi + ci * gr;
+        cr = tr;
+        if (LAM && gk.slot >= 0) {
+          acc = swr * gk.dgg[loc][0] - swi * gk.dgg[loc][1];
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+          if (lane == 0) srow[gk.slot][wave] += acc;
+        }
+        continue;
+      }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const uint64_t i = base + threadIdx.x + (uint64_t)kBlock * u;
@@ -310,6 +336,9 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t i = base + threadIdx.x + (uint64_t)kBlock * u;
+      const double t = pr[u] * cr - pi[u] * ci;
+      pi[u] = pr[u] * ci + pi[u] * cr;
+      pr[u] = t;
       T o;
       o.x = (R)((double)a[u].x * pr[u] - (double)a[u].y * pi[u]);
       o.y = (R)((double)a[u].x * pi[u] + (double)a[u].y * pr[u]);
