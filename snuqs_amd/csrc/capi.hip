@@ -1195,6 +1195,12 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     }
   }
   const bool any_runs = !runs.empty() || !fruns.empty() || !groups.empty();
+  auto distinct_params = [&](uint32_t first, uint32_t count) {
+    for (uint32_t a = first; a < first + count; ++a)
+      for (uint32_t b = a + 1; b < first + count; ++b)
+        if (rslots[a] == rslots[b]) return false;
+    return true;
+  };
   const size_t rg_b = (rgates.size() * sizeof(DiagRunGate) + 255) / 256 * 256;
   const size_t rs_b = (rslots.size() * sizeof(int) + 255) / 256 * 256;
   const size_t rp_b = any_runs ? (size_t)kMaxDiagSlots * 4096 * sizeof(double) : 0;
@@ -1236,14 +1242,14 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     if (fwd_start[g] >= 0) {
       const Run &r = fruns[fwd_start[g]];
       SQ_TRY(launch_adjoint_diag_run(sv, nullptr, dt, n, rgates_dev + r.first, (int)r.m, 0,
-                                     nullptr, rpartial, nullptr, st));
+                                     nullptr, false, rpartial, nullptr, st));
       g = r.lo + r.m - 1;  // the loop steps past the run
       continue;
     }
     if (fgrp_start[g] >= 0) {
       const Group &gr = fgroups[fgrp_start[g]];
       SQ_TRY(launch_adjoint_step_group(sv, nullptr, dt, n, (int)gr.k, gr.bits, gr.ginv, gr.dgg,
-                                       nullptr, rpartial, nullptr, st));
+                                       nullptr, false, rpartial, nullptr, st));
       g = gr.lo + gr.k - 1;
       continue;
     }
@@ -1268,7 +1274,8 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     if (run_top[g] >= 0) {
       const Run &r = runs[run_top[g]];
       SQ_TRY(launch_adjoint_diag_run(sv, work0, dt, n, rgates_dev + r.first, (int)r.m,
-                                     (int)r.nslots, rslots_dev + r.first_slot, rpartial, gslots,
+                                     (int)r.nslots, rslots_dev + r.first_slot,
+                                     distinct_params(r.first_slot, r.nslots), rpartial, gslots,
                                      st));
       g = (int)r.lo;  // the loop steps below the run
       continue;
@@ -1276,7 +1283,9 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     if (grp_top[g] >= 0) {
       const Group &gr = groups[grp_top[g]];
       SQ_TRY(launch_adjoint_step_group(sv, work0, dt, n, (int)gr.k, gr.bits, gr.ginv, gr.dgg,
-                                       rslots_dev + gr.first_slot, rpartial, gslots, st));
+                                       rslots_dev + gr.first_slot,
+                                       distinct_params(gr.first_slot, gr.k), rpartial, gslots,
+                                       st));
       g = (int)gr.lo;
       continue;
     }

@@ -367,6 +367,14 @@ __global__ void __launch_bounds__(kBlock)
 __global__ void __launch_bounds__(kBlock)
     k_finalize_slots(const double *__restrict__ partial, int nblocks, int nslots,
                      const int *__restrict__ slot_param, double alpha, double *out) {
+  if (gridDim.x > 1) {  // slots of distinct parameters: one block each
+    const int v = blockIdx.x;
+    double acc = 0.0, dummy = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) acc += partial[(uint64_t)v * nblocks + b];
+    block_sum2(acc, dummy);
+    if (threadIdx.x == 0) out[2 * slot_param[v]] += alpha * acc;
+    return;
+  }
   for (int v = 0; v < nslots; ++v) {
     double acc = 0.0, dummy = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += kBlock) acc += partial[(uint64_t)v * nblocks + b];
@@ -645,8 +653,8 @@ int launch_adjoint_step1(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int bi
 
 int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
                             const DiagRunGate *gates_dev, int m, int nslots,
-                            const int *slot_param_dev, double *partial, double *out_dev,
-                            hipStream_t st) {
+                            const int *slot_param_dev, bool distinct, double *partial,
+                            double *out_dev, hipStream_t st) {
   if (m < 1 || m > kMaxDiagRun || nslots < 0 || nslots > kMaxDiagSlots)
     return fail(SQ_ERR_INVALID, "internal: diagonal run out of range");
   const uint64_t chunk = (uint64_t)kBlock * 4;
@@ -665,8 +673,8 @@ int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
                        n, gates_dev, m, nslots, partial);
   }
   if (nslots)
-    hipLaunchKernelGGL(k_finalize_slots, dim3(1), dim3(kBlock), 0, st, partial, (int)nb, nslots,
-                       slot_param_dev, 2.0, out_dev);
+    hipLaunchKernelGGL(k_finalize_slots, dim3(distinct ? nslots : 1), dim3(kBlock), 0, st,
+                       partial, (int)nb, nslots, slot_param_dev, 2.0, out_dev);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
 }
@@ -701,8 +709,8 @@ static void launch_group_t(void *psi, void *lam, uint64_t n, const int *bits, co
 
 int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int k,
                               const int *bits, const double *ginv, const double *dgg,
-                              const int *slot_param_dev, double *partial, double *out_dev,
-                              hipStream_t st) {
+                              const int *slot_param_dev, bool distinct, double *partial,
+                              double *out_dev, hipStream_t st) {
   if (k < 2 || k > 4) return fail(SQ_ERR_INVALID, "internal: gate group size out of range");
   // a forward group (lam == nullptr) only applies the ginv matrices to psi:
   // callers pass the gates themselves there; more blocks, no reduction
@@ -723,8 +731,8 @@ int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, i
   }
 #undef SQ_GROUP
   if (lam)
-    hipLaunchKernelGGL(k_finalize_slots, dim3(1), dim3(kBlock), 0, st, partial, nb, k,
-                       slot_param_dev, 2.0, out_dev);
+    hipLaunchKernelGGL(k_finalize_slots, dim3(distinct ? k : 1), dim3(kBlock), 0, st, partial,
+                       nb, k, slot_param_dev, 2.0, out_dev);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
 }

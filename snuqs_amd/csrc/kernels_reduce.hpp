@@ -48,10 +48,12 @@ struct DiagRunGate {
   double ginv[4][2];  // diagonal of G^dagger, by local index
   double dgg[4][2];   // diagonal of dG G^dagger, by local index (slot >= 0)
 };
+// distinct: the slots' parameters differ (their sums may be finalised in
+// parallel); otherwise one block adds them in order.
 int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
                             const DiagRunGate *gates_dev, int m, int nslots,
-                            const int *slot_param_dev, double *partial, double *out_dev,
-                            hipStream_t st);
+                            const int *slot_param_dev, bool distinct, double *partial,
+                            double *out_dev, hipStream_t st);
 // Reverse step over K = 2..4 one-target gates on distinct amplitude bits
 // (bits[j] = bit of gate j, none of the lowest 4): psi and lambda undone by
 // every gate's G^dagger (ginv, row-major complex128 2x2 each), and for gate j
@@ -61,8 +63,8 @@ int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
 // caller passes the gates themselves; no reduction.
 int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int k,
                               const int *bits, const double *ginv, const double *dgg,
-                              const int *slot_param_dev, double *partial, double *out_dev,
-                              hipStream_t st);
+                              const int *slot_param_dev, bool distinct, double *partial,
+                              double *out_dev, hipStream_t st);
 int launch_pauli_terms(const void *sv, sq_dtype_t dt, uint64_t n, int nterms,
                        const void *terms_dev, const double *alpha_dev,
                        double *partial, double *out_dev, hipStream_t st);
