@@ -123,32 +123,42 @@ def run_adjoint_bench(C, n=24, layers=8, nterms=64, weight=4, shift_params=12):
         dt = (time.perf_counter() - t0) / reps
         res[dtype] = {"energy": e, "grads": gr, "seconds": dt}
         log(f"adjoint {dtype}: {dt * 1e3:.1f} ms per {thetas.size}-component gradient")
-    # parameter shift on the same GPU (complex64, same per-gate kernels):
-    # programs are built before the clock starts; each shifted circuit is run
-    # from |0..0> and its energy reduced on the device.
+    # parameter shift on the same GPU (complex64): each shifted circuit runs
+    # from |0..0> through the engine's fastest forward path (the circuit fused
+    # into <= 5-qubit blocks; the per-gate program is timed too) and its
+    # energy is reduced on the device; programs are built before the clock.
+    from snuqs_amd.transpile import fuse_operations
+
     sub = np.random.default_rng(1).choice(thetas.size, shift_params, replace=False)
-    progs = []
-    for j in sub:
-        for sgn in (+1, -1):
-            th = thetas.copy()
-            th[j] += sgn * np.pi / 2
-            progs.append(C.functionals.Program(n, hea_ansatz(n, layers, th)[0], dtype="complex64"))
     sv = C.result_types.StateVector(C.DeviceType.CUDA, n, dtype="complex64")
-    C.functionals.initialize_basis_z(sv)
-    progs[0].run(sv)
-    C.functionals.expectation(sv, *obs)
-    C.core.cuda.device_synchronize()
-    t0 = time.perf_counter()
-    es = []
-    for p in progs:
+    rates, es_by = {}, {}
+    for mode in ("fused", "per_gate"):
+        progs = []
+        for j in sub:
+            for sgn in (+1, -1):
+                th = thetas.copy()
+                th[j] += sgn * np.pi / 2
+                ops_j = hea_ansatz(n, layers, th)[0]
+                if mode == "fused":
+                    ops_j = fuse_operations(ops_j, 5)
+                progs.append(C.functionals.Program(n, ops_j, dtype="complex64"))
         C.functionals.initialize_basis_z(sv)
-        p.run(sv)
-        es.append(C.functionals.expectation(sv, *obs)[0])
-    dt_ps = time.perf_counter() - t0
+        progs[0].run(sv)
+        C.functionals.expectation(sv, *obs)
+        C.core.cuda.device_synchronize()
+        t0 = time.perf_counter()
+        es = []
+        for p in progs:
+            C.functionals.initialize_basis_z(sv)
+            p.run(sv)
+            es.append(C.functionals.expectation(sv, *obs)[0])
+        rates[mode] = len(sub) / (time.perf_counter() - t0)
+        es_by[mode] = es
+    es = es_by["fused"]
     ps = 0.5 * (np.array(es[0::2]) - np.array(es[1::2]))
     g64, g128 = res["complex64"]["grads"], res["complex128"]["grads"]
     adj_rate = thetas.size / res["complex64"]["seconds"]
-    ps_rate = len(sub) / dt_ps
+    ps_rate = rates["fused"]
     return {
         "config": f"{n}-qubit hardware-efficient ansatz, {layers} layers of RotY+RotZ on every "
                   f"qubit + CZ ring, {thetas.size} parameters; observable: {nterms} random "
@@ -158,8 +168,10 @@ def run_adjoint_bench(C, n=24, layers=8, nterms=64, weight=4, shift_params=12):
         "adjoint_ms_per_gradient_c128": res["complex128"]["seconds"] * 1e3,
         "param_shift_grads_per_s": ps_rate,
         "param_shift_sample": f"{len(sub)} of {thetas.size} components ({2 * len(sub)} shifted "
-                              f"circuits), complex64",
+                              f"circuits), complex64, circuits fused into <= 5-qubit blocks",
         "speedup": adj_rate / ps_rate,
+        "param_shift_per_gate_grads_per_s": rates["per_gate"],
+        "speedup_vs_per_gate_param_shift": adj_rate / rates["per_gate"],
         "max_abs_grad_err_c64_vs_c128": float(np.abs(g64 - g128).max()),
         "max_abs_param_shift_vs_adjoint_c128": float(np.abs(ps - g128[sub]).max()),
         "energy_c128": res["complex128"]["energy"],
