@@ -1016,7 +1016,7 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
           e.b0 = p->nq - 1 - (int)p->targets[p->toff[q]];
           e.b1 = k == 2 ? p->nq - 1 - (int)p->targets[p->toff[q] + 1] : -1;
           e.slot = -1;
-          e.pad = (e.b0 >= 10 && (e.b1 < 0 || e.b1 >= 10)) ? 1 : 0;
+          e.uniform = (e.b0 >= 10 && (e.b1 < 0 || e.b1 >= 10)) ? 1 : 0;
           for (int l = 0; l < d; ++l) {  // G^dagger diagonal = conj(G[l][l])
             e.ginv[l][0] = G[2 * (l * d + l)];
             e.ginv[l][1] = -G[2 * (l * d + l) + 1];
@@ -1140,7 +1140,7 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
           e.b0 = p->nq - 1 - (int)p->targets[p->toff[q]];
           e.b1 = k == 2 ? p->nq - 1 - (int)p->targets[p->toff[q] + 1] : -1;
           e.slot = -1;
-          e.pad = (e.b0 >= 10 && (e.b1 < 0 || e.b1 >= 10)) ? 1 : 0;
+          e.uniform = (e.b0 >= 10 && (e.b1 < 0 || e.b1 >= 10)) ? 1 : 0;
           for (int l = 0; l < d; ++l) {  // the gate's own diagonal
             e.ginv[l][0] = G[2 * (l * d + l)];
             e.ginv[l][1] = G[2 * (l * d + l) + 1];

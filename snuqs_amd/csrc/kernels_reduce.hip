@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int k = 0; k < m; ++k) {
       const DiagRunGate &gk = sg[k];
       double acc = 0.0;
-      if (gk.pad) {  // chunk-uniform gate (uniform over the block)
+      if (gk.uniform) {  // chunk-uniform gate (the branch is uniform over the block)
         const int loc = gk.b1 < 0 ? (int)((base >> gk.b0) & 1)
                                   : (int)((((base >> gk.b0) & 1) << 1) | ((base >> gk.b1) & 1));
         const double gr = gk.ginv[loc][0], gi = gk.ginv[loc][1];

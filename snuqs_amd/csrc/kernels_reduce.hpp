@@ -44,7 +44,7 @@ constexpr int kMaxDiagSlots = 32;
 struct DiagRunGate {
   int b0, b1;   // address bits of local index bit 1 / bit 0 (b1 < 0: one target, local = bit b0)
   int slot;     // partial row of this gate's derivative, or -1
-  int pad;      // 1: both target bits >= 10 (constant over a 1024-amplitude chunk)
+  int uniform;  // 1: both target bits >= 10 (constant over a 1024-amplitude chunk)
   double ginv[4][2];  // diagonal of G^dagger, by local index
   double dgg[4][2];   // diagonal of dG G^dagger, by local index (slot >= 0)
 };

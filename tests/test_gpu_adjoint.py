@@ -130,3 +130,65 @@ def test_hea_adjoint_16q_vs_gpu_parameter_shift(C, dtype, tol):
         tm[j] -= np.pi / 2
         ps[j] = 0.5 * (energy(tp) - energy(tm))
     assert np.abs(grads - ps).max() < tol
+
+
+def _planner_circuit(n, thetas):
+    """Gates that exercise the adjoint's pass planning: a diagonal run of 80
+    gates (RotZ on every qubit, ZZ / CPhaseShift / PhaseShift pairs, CZ) with
+    shared parameters and more than 32 derivatives, layers of one-target
+    rotations on distinct qubits (low address bits included), and dense
+    two-qubit gates between them."""
+    from snuqs_amd import _C as C
+
+    g = C.operation.gate_operations
+    ops, pmap = [], []
+
+    def add(op, params=()):
+        ops.append(op)
+        pmap.append([(a, p) for a, p in params])
+
+    for q in range(n):
+        add(g.Hadamard([q]))
+    for rep in range(2):
+        for q in range(n):                                   # RotY layer, own params
+            add(g.RotY([q], thetas[q]), [(0, q)])
+        for q in range(n):                                   # diagonal run starts
+            add(g.RotZ([q], thetas[n + q % 3]), [(0, n + q % 3)])  # shared params
+        for q in range(n - 1):
+            add(g.ZZ([q, q + 1], thetas[n + 3 + q % 5]), [(0, n + 3 + q % 5)])
+            add(g.CZ([q, (q + 3) % n]))
+            add(g.CPhaseShift([(q + 2) % n, q], thetas[n + 8]), [(0, n + 8)])
+            add(g.PhaseShift([q], thetas[n + 9 + q % 2]), [(0, n + 9 + q % 2)])
+        for q in range(n):                                   # the run goes past 64 gates
+            add(g.S([q]))
+            add(g.T([(q + 1) % n]))
+        for q in range(0, n, 2):                             # shared RotX params
+            add(g.RotX([q], thetas[n + 11]), [(0, n + 11)])
+        add(g.XX([0, n - 1], thetas[n + 12]), [(0, n + 12)])
+    return ops, pmap
+
+
+@pytest.mark.parametrize("n", [9, 12])
+@pytest.mark.parametrize("checkpoints", [0, 2, 5])
+def test_adjoint_pass_planning_vs_finite_differences(C, n, checkpoints):
+    """Diagonal runs (split at 64 gates and 32 derivatives, shared parameters
+    finalised in order), one-target groups, checkpoint gates that cut runs and
+    groups, and a 9-qubit state too small for runs: complex128 gradients
+    against central differences of the CPU oracle."""
+    from snuqs_amd.circuits import random_pauli_sum
+
+    rng = np.random.default_rng(n + checkpoints)
+    nparams = n + 13
+    thetas = rng.uniform(0, 2 * np.pi, nparams)
+    ops, pmap = _planner_circuit(n, thetas)
+    obs = random_pauli_sum(n, 6, 3, np.random.default_rng(4))
+    e, grads = _adjoint(C, n, ops, pmap, nparams, obs, "complex128", checkpoints)
+    assert abs(e - oracle_energy(n, ops, *obs)) < 1e-10
+    h = 1e-6
+    for p in range(nparams):
+        up, dn = thetas.copy(), thetas.copy()
+        up[p] += h
+        dn[p] -= h
+        fd = (oracle_energy(n, _planner_circuit(n, up)[0], *obs) -
+              oracle_energy(n, _planner_circuit(n, dn)[0], *obs)) / (2 * h)
+        assert abs(grads[p] - fd) < 1e-7, (p, grads[p], fd)
