@@ -250,7 +250,9 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs,
 
 /* ---- batched QAOA MaxCut (new; no reference counterpart).
  * num_instances circuits of uneven qubit counts num_qubits[j] (2..30) on
- * graphs with num_edges[j] edges (qubit pairs, back to back in `edges`):
+ * simple graphs with num_edges[j] edges (qubit pairs, back to back in
+ * `edges`; a pair given twice, in either order, is rejected with
+ * SQ_ERR_INVALID):
  * H on every qubit, then p layers of ZZ(2 gamma_l) on every edge and
  * RX(2 beta_l) on every qubit.  A run computes for every instance the energy
  * <C> = sum_edges (1 - <Z_a Z_b>) / 2 and its adjoint gradient.  Every launch

@@ -90,6 +90,17 @@ __device__ __forceinline__ QFound find_entry(const QEntry *tab, int count, uint6
   return {lo, sinst[lo], sfb[lo]};
 }
 
+// find_entry with its results moved to scalar registers: every lane holds the
+// same values, and marking them uniform lets the compiler load the instance's
+// descriptor and tables with scalar loads and keep chunk-wide terms in SGPRs.
+__device__ __forceinline__ QFound find_entry_u(const QEntry *tab, int count, uint64_t b) {
+  const QFound f = find_entry(tab, count, b);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)f.first);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(f.first >> 32));
+  return {__builtin_amdgcn_readfirstlane(f.entry), __builtin_amdgcn_readfirstlane(f.inst),
+          (uint64_t)hi << 32 | lo};
+}
+
 __device__ __forceinline__ int cut_of(uint64_t i, const uint8_t *e, int ne) {
   int c = 0;
   for (int k = 0; k < ne; ++k) c += (int)(((i >> e[2 * k]) ^ (i >> e[2 * k + 1])) & 1);
@@ -201,71 +212,102 @@ __device__ __forceinline__ double qxpart16(const T (&a)[16], const T (&l)[16]) {
   return s0 + s1;
 }
 
-// Cut counts of a full chunk at `base` for thread t's amplitudes j = t + 256 r:
-// cut = K + sum_k u_k r_k + rint[r], where the edges touching no bit 8-11
-// give K, an edge from bit 8+k to a known bit (of t or of base) gives u_k,
-// and rint[16] (shared) holds the edges inside bits 8-11.
-struct QCut {
-  int K, u0, u1, u2, u3;
+// Cut counts of full chunks (n >= 12).  With i = base + j, j = t + 256 r < 2^12,
+// split the edges by where their bits lie: both below 12 ("lo"), both at 12 or
+// above ("hi"), or one of each ("cross", low bit a, high bit b):
+//   cut(i) = lo(j) + hi(base) + sum_cross (j_a xor base_b)
+//          = lo(j) + C + sum_{a < 12} j_a w_a,
+//   C   = hi(base) + sum_a popc(base & M_a),   w_a = popc(M_a) - 2 popc(base & M_a),
+// with M_a the mask of bit a's neighbours at bits >= 12.  lo(j) is a 4096-byte
+// table per instance, built at batch creation and stored [t][r] so a thread's 16
+// values are one 16-byte load; C and w_a depend on the chunk only and are
+// computed once per wave in scalar registers.
+struct QCutTab {
+  uint32_t m[kQL];  // low bit a: its neighbours at bits >= 12
+  uint32_t nh[32];  // high bit b: its neighbours at bits above b
 };
 
-__device__ __forceinline__ int qbit(uint64_t x, int b) { return (int)((x >> b) & 1); }
+struct QCutF {
+  int K;                  // C + sum_{a < 8} t_a w_a
+  int w8, w9, w10, w11;   // w_a of the register bits (uniform)
+  uint4 lo;               // lo(t + 256 r) in byte r & 3 of word r >> 2
+};
 
-__device__ __forceinline__ QCut qcut_setup(uint64_t base, int t, const uint8_t *e, int ne,
-                                           int *rint) {
-  QCut c{0, 0, 0, 0, 0};
-  for (int k = 0; k < ne; ++k) {
-    const int a = e[2 * k], b = e[2 * k + 1];
-    const bool ra = a >= 8 && a < 12, rb = b >= 8 && b < 12;
-    if (ra && rb) continue;
-    const int va = a < 8 ? qbit((uint64_t)t, a) : qbit(base, a);
-    const int vb = b < 8 ? qbit((uint64_t)t, b) : qbit(base, b);
-    if (!ra && !rb) {
-      c.K += va ^ vb;
-    } else {
-      const int kk = (ra ? a : b) - 8, v = ra ? vb : va, d = 1 - 2 * v;
-      c.K += v;
-      c.u0 += kk == 0 ? d : 0;
-      c.u1 += kk == 1 ? d : 0;
-      c.u2 += kk == 2 ? d : 0;
-      c.u3 += kk == 3 ? d : 0;
-    }
+// `base` and `ii` must be wave-uniform.
+__device__ __forceinline__ QCutF qcut_full(const QCutTab *__restrict__ cuts,
+                                           const uint4 *__restrict__ lotab, int ii, int n,
+                                           uint32_t base, int t) {
+  QCutF f;
+  f.lo = lotab[(size_t)ii * 256 + t];
+  const QCutTab &c = cuts[ii];
+  int C = 0, w[kQL];
+#pragma unroll
+  for (int a = 0; a < kQL; ++a) {
+    const int pm = __popc(base & c.m[a]);
+    C += pm;
+    w[a] = __popc(c.m[a]) - 2 * pm;
   }
-  if (t < 16) {
-    int ci = 0;
-    for (int k = 0; k < ne; ++k) {
-      const int a = e[2 * k], b = e[2 * k + 1];
-      if (a >= 8 && a < 12 && b >= 8 && b < 12) ci += qbit((uint64_t)t, a - 8) ^ qbit((uint64_t)t, b - 8);
-    }
-    rint[t] = ci;
-  }
-  return c;
+  for (int b = kQL; b < n; ++b) C += __popc(c.nh[b] & (((base >> b) & 1) ? ~base : base));
+  int K = C;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) K += ((t >> a) & 1) * w[a];
+  f.K = K;
+  f.w8 = w[8];
+  f.w9 = w[9];
+  f.w10 = w[10];
+  f.w11 = w[11];
+  return f;
 }
 
-__device__ __forceinline__ int qcut_at(const QCut &c, const int *rint, int r) {
-  return c.K + (r & 1) * c.u0 + ((r >> 1) & 1) * c.u1 + ((r >> 2) & 1) * c.u2 +
-         ((r >> 3) & 1) * c.u3 + rint[r];
+__device__ __forceinline__ int qcut_r(const QCutF &f, int r) {
+  const uint32_t word = r < 4 ? f.lo.x : r < 8 ? f.lo.y : r < 12 ? f.lo.z : f.lo.w;
+  return f.K + (int)((word >> (8 * (r & 3))) & 0xffu) + (r & 1) * f.w8 + ((r >> 1) & 1) * f.w9 +
+         ((r >> 2) & 1) * f.w10 + ((r >> 3) & 1) * f.w11;
+}
+
+// Per (instance, layer): the cost-layer phases exp(-i g s_c), s_c = |E| - 2c for
+// c = 0..|E|, then (cos b, sin b) of the mixer, evaluated in FP64 and rounded to
+// the state's precision once per run by k_qaoa_phases.
+constexpr int kQPh = kQMaxEdges + 2;
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_qaoa_phases(typename CV<R>::T *__restrict__ ph, const QInstDev *__restrict__ inst,
+                  const double *__restrict__ gam, const double *__restrict__ bet, int p) {
+  using T = typename CV<R>::T;
+  const int ii = blockIdx.x / p, layer = blockIdx.x % p;
+  const int ne = inst[ii].nedges;
+  const double g = gam[ii * p + layer];
+  T *o = ph + (size_t)blockIdx.x * kQPh;
+  for (int c = threadIdx.x; c <= ne; c += kBlock) {
+    double sn, cs;
+    sincos(-g * (double)(ne - 2 * c), &sn, &cs);
+    o[c] = T{(R)cs, (R)sn};
+  }
+  if (threadIdx.x == 0) {
+    double sn, cs;
+    sincos(bet[ii * p + layer], &sn, &cs);
+    o[kQPh - 1] = T{(R)cs, (R)sn};
+  }
 }
 
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
     k_qaoa_low_fwd_full(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                        const uint8_t *__restrict__ edges, const double *__restrict__ gam,
-                        const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
-                        int count, int layer) {
+                        const QCutTab *__restrict__ cuts, const uint4 *__restrict__ lotab,
+                        const typename CV<R>::T *__restrict__ ph, int p,
+                        const QEntry *__restrict__ tab, int count, int layer) {
   using T = typename CV<R>::T;
   __shared__ T X[kQSlots];
   __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
-  __shared__ int rint[16];
-  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const QFound fe = find_entry_u(tab, count, blockIdx.x);
   const int ii = fe.inst;
   const QInstDev q = inst[ii];
-  const uint64_t chunk = blockIdx.x - fe.first;
-  const uint64_t base = chunk << kQL;
-  const uint8_t *e = edges + 2 * q.edge_off;
-  const double g = gam[ii * p + layer], b = bet[ii * p + layer];
+  const uint32_t chunk = (uint32_t)(blockIdx.x - fe.first);
+  const uint64_t base = (uint64_t)chunk << kQL;
+  const T *phl = ph + (size_t)(ii * p + layer) * kQPh;
   const int t = threadIdx.x;
-  // the chunk's loads go out first; the phase table and cut setup overlap them
+  // the chunk's loads go out first; the phase table and cut terms overlap them
   T v[16];
   const R amp0 = (R)(1.0 / sqrt((double)(1ull << q.n)));
 #pragma unroll
@@ -277,25 +319,23 @@ __global__ void __launch_bounds__(kBlock)
       v[r] = ld_nt(st + q.psi + base + t + 256 * r);
     }
   }
-  for (int c = t; c <= q.nedges; c += kBlock) {
-    double sn, cs;
-    sincos(-g * (double)(q.nedges - 2 * c), &sn, &cs);
-    ph_r[c] = (R)cs;
-    ph_i[c] = (R)sn;
+  if (t <= q.nedges) {
+    const T z = phl[t];
+    ph_r[t] = z.x;
+    ph_i[t] = z.y;
   }
-  const QCut cc = qcut_setup(base, t, e, q.nedges, rint);
+  const QCutF cc = qcut_full(cuts, lotab, ii, q.n, (uint32_t)base, t);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     R x = v[r].x, y = v[r].y;
-    const int c = qcut_at(cc, rint, r);
+    const int c = qcut_r(cc, r);
     cmul(x, y, ph_r[c], ph_i[c]);
     v[r].x = x;
     v[r].y = y;
   }
-  double sn, cs;
-  sincos(b, &sn, &cs);
-  const R c0 = (R)cs, s0 = (R)sn;  // RX(2b) = [[c, -i s], [-i s, c]]
+  const T cb = phl[kQPh - 1];
+  const R c0 = cb.x, s0 = cb.y;    // RX(2b) = [[c, -i s], [-i s, c]]
   qrx16(v, c0, s0);                // bits 8-11
   qtranspose(v, X, t, 2, 1);
   qrx16(v, c0, s0);                // bits 4-7
@@ -315,26 +355,23 @@ __global__ void __launch_bounds__(kBlock)
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
     k_qaoa_low_fwd(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                   const uint8_t *__restrict__ edges, const double *__restrict__ gam,
-                   const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
-                   int count, int layer) {
+                   const uint8_t *__restrict__ edges, const typename CV<R>::T *__restrict__ ph,
+                   int p, const QEntry *__restrict__ tab, int count, int layer) {
   using T = typename CV<R>::T;
   __shared__ R xr[kQChunk / 2], xi[kQChunk / 2];  // chunks below 2^12 (n < 12)
   __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
-  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const QFound fe = find_entry_u(tab, count, blockIdx.x);
   const int ii = fe.inst;
   const QInstDev q = inst[ii];
   const uint64_t chunk = blockIdx.x - fe.first;
   const int m = 1 << q.lc;
   const uint64_t base = chunk << q.lc;
   const uint8_t *e = edges + 2 * q.edge_off;
-  const double g = gam[ii * p + layer], b = bet[ii * p + layer];
-  // phases exp(-i g s) for s = |E| - 2c, c = 0..|E|, in double
-  for (int c = threadIdx.x; c <= q.nedges; c += kBlock) {
-    double sn, cs;
-    sincos(-g * (double)(q.nedges - 2 * c), &sn, &cs);
-    ph_r[c] = (R)cs;
-    ph_i[c] = (R)sn;
+  const T *phl = ph + (size_t)(ii * p + layer) * kQPh;
+  if ((int)threadIdx.x <= q.nedges) {
+    const T z = phl[threadIdx.x];
+    ph_r[threadIdx.x] = z.x;
+    ph_i[threadIdx.x] = z.y;
   }
   __syncthreads();
   const R amp0 = (R)(1.0 / sqrt((double)(1ull << q.n)));
@@ -354,9 +391,8 @@ __global__ void __launch_bounds__(kBlock)
     xi[j] = y;
   }
   __syncthreads();
-  double sn, cs;
-  sincos(b, &sn, &cs);
-  const R c0 = (R)cs, s0 = (R)sn;  // RX(2b) = [[c, -i s], [-i s, c]]
+  const T cb = phl[kQPh - 1];
+  const R c0 = cb.x, s0 = cb.y;  // RX(2b) = [[c, -i s], [-i s, c]]
   for (int bit = 0; bit < q.lc; ++bit) {
     for (int f = threadIdx.x; f < m / 2; f += kBlock) {
       const int lo = f & ((1 << bit) - 1);
@@ -389,17 +425,16 @@ __device__ __forceinline__ uint64_t qgroup_base(uint64_t f, int h0) {
 template <typename R, int G>
 __global__ void __launch_bounds__(kBlock)
     k_qaoa_high_fwd(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                    const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
-                    int count, int layer, int h0) {
+                    const typename CV<R>::T *__restrict__ ph, int p,
+                    const QEntry *__restrict__ tab, int count, int layer, int h0) {
   using T = typename CV<R>::T;
   constexpr int D = 1 << G;
-  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const QFound fe = find_entry_u(tab, count, blockIdx.x);
   const int ii = fe.inst;
   const QInstDev q = inst[ii];
   const uint64_t chunk = blockIdx.x - fe.first;
-  double sn, cs;
-  sincos(bet[ii * p + layer], &sn, &cs);
-  const R c0 = (R)cs, s0 = (R)sn;  // RX(2b) = [[c, -i s], [-i s, c]]
+  const T cb = ph[(size_t)(ii * p + layer) * kQPh + kQPh - 1];
+  const R c0 = cb.x, s0 = cb.y;  // RX(2b) = [[c, -i s], [-i s, c]]
   const uint64_t ngrp = 1ull << (q.lc - G);  // groups per chunk-sized share
   for (uint64_t t = threadIdx.x; t < ngrp; t += kBlock) {
     const uint64_t i0 = qgroup_base<G>(chunk * ngrp + t, h0);
@@ -426,35 +461,33 @@ __global__ void __launch_bounds__(kBlock)
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
     k_qaoa_energy(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                  const uint8_t *__restrict__ edges, const QEntry *__restrict__ tab, int count,
+                  const uint8_t *__restrict__ edges, const QCutTab *__restrict__ cuts,
+                  const uint4 *__restrict__ lotab, const QEntry *__restrict__ tab, int count,
                   double *__restrict__ partial) {
   using T = typename CV<R>::T;
-  __shared__ int rint[16];
-  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const QFound fe = find_entry_u(tab, count, blockIdx.x);
   const int ii = fe.inst;
   const QInstDev q = inst[ii];
   const uint64_t chunk = blockIdx.x - fe.first;
   const int m = 1 << q.lc;
   const uint64_t base = chunk << q.lc;
   const uint8_t *e = edges + 2 * q.edge_off;
-  const bool full = q.lc == kQL;  // uniform per workgroup
-  QCut cc{0, 0, 0, 0, 0};
-  if (full) cc = qcut_setup(base, threadIdx.x, e, q.nedges, rint);
-  __syncthreads();
   double acc = 0.0;
-  if (full) {
+  if (q.lc == kQL) {  // uniform per workgroup
     // all 16 loads of the thread in flight at once (m = 16 x kBlock)
+    const int t = threadIdx.x;
     T v[16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = ld_nt(st + q.psi + base + threadIdx.x + kBlock * r);
+    for (int r = 0; r < 16; ++r) v[r] = ld_nt(st + q.psi + base + t + kBlock * r);
+    const QCutF cc = qcut_full(cuts, lotab, ii, q.n, (uint32_t)base, t);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int c = qcut_at(cc, rint, r);
+      const int c = qcut_r(cc, r);
       acc += (double)c * ((double)v[r].x * v[r].x + (double)v[r].y * v[r].y);
       T o;
       o.x = (R)c * v[r].x;
       o.y = (R)c * v[r].y;
-      st_nt(st + q.lam + base + threadIdx.x + kBlock * r, o);
+      st_nt(st + q.lam + base + t + kBlock * r, o);
     }
   } else {
     for (int j = threadIdx.x; j < m; j += kBlock) {
@@ -476,17 +509,17 @@ __global__ void __launch_bounds__(kBlock)
 template <typename R, int G>
 __global__ void __launch_bounds__(kBlock)
     k_qaoa_high_rev(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                    const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
-                    int count, int layer, int h0, double *__restrict__ partial) {
+                    const typename CV<R>::T *__restrict__ ph, int p,
+                    const QEntry *__restrict__ tab, int count, int layer, int h0,
+                    double *__restrict__ partial) {
   using T = typename CV<R>::T;
   constexpr int D = 1 << G;
-  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const QFound fe = find_entry_u(tab, count, blockIdx.x);
   const int ii = fe.inst;
   const QInstDev q = inst[ii];
   const uint64_t chunk = blockIdx.x - fe.first;
-  double sn, cs;
-  sincos(bet[ii * p + layer], &sn, &cs);
-  const R c0 = (R)cs, s0 = (R)sn;
+  const T cb = ph[(size_t)(ii * p + layer) * kQPh + kQPh - 1];
+  const R c0 = cb.x, s0 = cb.y;
   const uint64_t ngrp = 1ull << (q.lc - G);
   double acc = 0.0;
   for (uint64_t t = threadIdx.x; t < ngrp; t += kBlock) {
@@ -536,38 +569,35 @@ __global__ void __launch_bounds__(kBlock)
 template <typename R>
 __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
     k_qaoa_low_rev_full(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                        const uint8_t *__restrict__ edges, const double *__restrict__ gam,
-                        const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
-                        int count, int layer, double *__restrict__ partial) {
+                        const QCutTab *__restrict__ cuts, const uint4 *__restrict__ lotab,
+                        const typename CV<R>::T *__restrict__ ph, int p,
+                        const QEntry *__restrict__ tab, int count, int layer,
+                        double *__restrict__ partial) {
   using T = typename CV<R>::T;
   __shared__ T X[kQSlots];
   __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
-  __shared__ int rint[16];
-  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const QFound fe = find_entry_u(tab, count, blockIdx.x);
   const int ii = fe.inst;
   const QInstDev q = inst[ii];
-  const uint64_t chunk = blockIdx.x - fe.first;
-  const uint64_t base = chunk << kQL;
-  const uint8_t *e = edges + 2 * q.edge_off;
-  const double g = gam[ii * p + layer], b = bet[ii * p + layer];
+  const uint32_t chunk = (uint32_t)(blockIdx.x - fe.first);
+  const uint64_t base = (uint64_t)chunk << kQL;
+  const T *phl = ph + (size_t)(ii * p + layer) * kQPh;
   const int t = threadIdx.x;
-  // the chunk's loads go out first; the phase table and cut setup overlap them
+  // the chunk's loads go out first; the phase table and cut terms overlap them
   T a[16], l[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     a[r] = ld_nt(st + q.psi + base + t + 256 * r);
     l[r] = ld_nt(st + q.lam + base + t + 256 * r);
   }
-  for (int c = t; c <= q.nedges; c += kBlock) {
-    double sn, cs;
-    sincos(g * (double)(q.nedges - 2 * c), &sn, &cs);  // undo: exp(+i g s)
-    ph_r[c] = (R)cs;
-    ph_i[c] = (R)sn;
+  if (t <= q.nedges) {  // undo: exp(+i g s), the conjugate phases
+    const T z = phl[t];
+    ph_r[t] = z.x;
+    ph_i[t] = -z.y;
   }
-  const QCut cc = qcut_setup(base, t, e, q.nedges, rint);
-  double sn, cs;
-  sincos(b, &sn, &cs);
-  const R c0 = (R)cs, s0 = -(R)sn;  // RX(-2b)
+  const QCutF cc = qcut_full(cuts, lotab, ii, q.n, (uint32_t)base, t);
+  const T cb = phl[kQPh - 1];
+  const R c0 = cb.x, s0 = -cb.y;  // RX(-2b)
   // beta partial (any point inside the mixer layer) and undo, 4 bits at a time
   double accb = qxpart16(a, l);  // bits 8-11
   qrx16(a, c0, s0);
@@ -593,7 +623,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
   double accg = 0.0;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    const int c = qcut_at(cc, rint, r);
+    const int c = qcut_r(cc, r);
     const double sv = (double)(q.nedges - 2 * c);
     R x = a[r].x, y = a[r].y, u = l[r].x, w = l[r].y;
     accg += sv * ((double)u * y - (double)w * x);
@@ -619,25 +649,24 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
     k_qaoa_low_rev(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                   const uint8_t *__restrict__ edges, const double *__restrict__ gam,
-                   const double *__restrict__ bet, int p, const QEntry *__restrict__ tab,
-                   int count, int layer, double *__restrict__ partial) {
+                   const uint8_t *__restrict__ edges, const typename CV<R>::T *__restrict__ ph,
+                   int p, const QEntry *__restrict__ tab, int count, int layer,
+                   double *__restrict__ partial) {
   using T = typename CV<R>::T;
   __shared__ R xr[kQChunk / 2], xi[kQChunk / 2], yr[kQChunk / 2], yi[kQChunk / 2];  // n < 12
   __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
-  const QFound fe = find_entry(tab, count, blockIdx.x);
+  const QFound fe = find_entry_u(tab, count, blockIdx.x);
   const int ii = fe.inst;
   const QInstDev q = inst[ii];
   const uint64_t chunk = blockIdx.x - fe.first;
   const int m = 1 << q.lc;
   const uint64_t base = chunk << q.lc;
   const uint8_t *e = edges + 2 * q.edge_off;
-  const double g = gam[ii * p + layer], b = bet[ii * p + layer];
-  for (int c = threadIdx.x; c <= q.nedges; c += kBlock) {
-    double sn, cs;
-    sincos(g * (double)(q.nedges - 2 * c), &sn, &cs);  // undo: exp(+i g s)
-    ph_r[c] = (R)cs;
-    ph_i[c] = (R)sn;
+  const T *phl = ph + (size_t)(ii * p + layer) * kQPh;
+  if ((int)threadIdx.x <= q.nedges) {  // undo: exp(+i g s), the conjugate phases
+    const T z = phl[threadIdx.x];
+    ph_r[threadIdx.x] = z.x;
+    ph_i[threadIdx.x] = -z.y;
   }
   for (int j = threadIdx.x; j < m; j += kBlock) {
     const T v = ld_nt(st + q.psi + base + j), w = ld_nt(st + q.lam + base + j);
@@ -647,9 +676,8 @@ __global__ void __launch_bounds__(kBlock)
     yi[j] = w.y;
   }
   __syncthreads();
-  double sn, cs;
-  sincos(b, &sn, &cs);
-  const R c0 = (R)cs, s0 = (R)sn;
+  const T cb = phl[kQPh - 1];
+  const R c0 = cb.x, s0 = cb.y;
   double accb = 0.0;
   for (int bit = 0; bit < q.lc; ++bit) {
     for (int f = threadIdx.x; f < m / 2; f += kBlock) {
@@ -733,6 +761,9 @@ struct QaoaBatch {
   void *states = nullptr;
   QInstDev *inst = nullptr;
   uint8_t *edges = nullptr;
+  QCutTab *cuts = nullptr;   // per instance
+  uint8_t *lotab = nullptr;  // per instance, 4096 bytes
+  void *phase = nullptr;     // per (instance, layer), kQPh amplitudes
   double *params = nullptr;  // gam [ninst*p], bet [ninst*p]
   double *partial = nullptr;
   double *out = nullptr;
@@ -753,8 +784,9 @@ struct QaoaBatch {
 static void free_batch(QaoaBatch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  for (void *ptr : {(void *)b->states, (void *)b->inst, (void *)b->edges, (void *)b->params,
-                    (void *)b->partial, (void *)b->out, (void *)b->tables})
+  for (void *ptr : {(void *)b->states, (void *)b->inst, (void *)b->edges, (void *)b->cuts,
+                    (void *)b->lotab, (void *)b->phase, (void *)b->params, (void *)b->partial,
+                    (void *)b->out, (void *)b->tables})
     if (ptr) (void)hipFree(ptr);
 }
 
@@ -770,6 +802,8 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
   uint64_t amp = 0, slot = 0;
   size_t eoff = 0;
   std::vector<uint8_t> hedges;
+  std::vector<QCutTab> hcuts(ninst);
+  std::vector<uint8_t> hlo((size_t)ninst * kQChunk, 0);
   for (uint32_t j = 0; j < ninst; ++j) {
     const int n = (int)nq[j];
     if (n < 2 || n > 30) return fail(SQ_ERR_INVALID, "qaoa: qubit counts must be in [2, 30]");
@@ -781,12 +815,36 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
     q.ngroups = (n - q.lc + kQGR - 1) / kQGR;
     q.nedges = (int)nedges[j];
     q.edge_off = (int)(hedges.size() / 2);
+    std::vector<uint32_t> seen(n, 0);  // seen[a]: bit b set if edge {a, b} was given
     for (uint32_t k = 0; k < nedges[j]; ++k) {
       const uint32_t a = edges[2 * (eoff + k)], b = edges[2 * (eoff + k) + 1];
       if (a >= (uint32_t)n || b >= (uint32_t)n || a == b)
         return fail(SQ_ERR_INVALID, "qaoa: bad edge");
+      // simple graphs only: the cut tables hold neighbour sets as bit masks
+      if ((seen[a] >> b) & 1) return fail(SQ_ERR_INVALID, "qaoa: repeated edge");
+      seen[a] |= 1u << b;
+      seen[b] |= 1u << a;
       hedges.push_back((uint8_t)(n - 1 - a));  // qubit -> bit position
       hedges.push_back((uint8_t)(n - 1 - b));
+    }
+    if (q.lc == kQL) {  // cut tables of full chunks (see qcut_full)
+      QCutTab &ct = hcuts[j];
+      std::vector<std::pair<int, int>> lo_edges;
+      for (uint32_t k = 0; k < nedges[j]; ++k) {
+        const int u = n - 1 - (int)edges[2 * (eoff + k)], v = n - 1 - (int)edges[2 * (eoff + k) + 1];
+        const int x = std::min(u, v), y = std::max(u, v);
+        if (y < kQL) lo_edges.emplace_back(x, y);
+        else if (x < kQL) ct.m[x] |= 1u << y;
+        else ct.nh[x] |= 1u << y;
+      }
+      uint8_t *lo = hlo.data() + (size_t)j * kQChunk;
+      for (int t = 0; t < 256; ++t)
+        for (int r = 0; r < 16; ++r) {
+          const int jj = t + 256 * r;
+          int c = 0;
+          for (auto &e : lo_edges) c += ((jj >> e.first) ^ (jj >> e.second)) & 1;
+          lo[t * 16 + r] = (uint8_t)c;
+        }
     }
     eoff += nedges[j];
     q.psi = amp;
@@ -811,6 +869,9 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
   if (!rc) rc = dmalloc(&B->states, amp * item);
   if (!rc) rc = dmalloc((void **)&B->inst, ninst * sizeof(QInstDev));
   if (!rc) rc = dmalloc((void **)&B->edges, hedges.size());
+  if (!rc) rc = dmalloc((void **)&B->cuts, ninst * sizeof(QCutTab));
+  if (!rc) rc = dmalloc((void **)&B->lotab, hlo.size());
+  if (!rc) rc = dmalloc(&B->phase, (size_t)ninst * p * kQPh * item);
   if (!rc) rc = dmalloc((void **)&B->params, 2 * ninst * p * sizeof(double));
   if (!rc) rc = dmalloc((void **)&B->partial, std::max<uint64_t>(slot, 1) * sizeof(double));
   if (!rc) rc = dmalloc((void **)&B->out, ninst * (1 + 2 * p) * sizeof(double));
@@ -858,6 +919,9 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
   if (hipMemcpy(B->inst, B->host_inst.data(), ninst * sizeof(QInstDev), hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemcpy(B->edges, hedges.data(), hedges.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(B->cuts, hcuts.data(), ninst * sizeof(QCutTab), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(B->lotab, hlo.data(), hlo.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(B->tables, tabs.data(), tabs.size() * sizeof(QEntry), hipMemcpyHostToDevice) !=
           hipSuccess) {
     free_batch(B.get());
@@ -880,46 +944,51 @@ template <typename R>
 static int enqueue_t(QaoaBatch *B, hipStream_t st) {
   using T = typename CV<R>::T;
   T *s = (T *)B->states;
-  const double *gam = B->params, *bet = B->params + (size_t)B->ninst * B->p;
+  const T *ph = (const T *)B->phase;
+  const uint4 *lo = (const uint4 *)B->lotab;
+  const int p = B->p;
+  const double *gam = B->params, *bet = B->params + (size_t)B->ninst * p;
+  hipLaunchKernelGGL(k_qaoa_phases<R>, dim3(B->ninst * p), dim3(kBlock), 0, st, (T *)B->phase,
+                     B->inst, gam, bet, p);
   for (auto &L : B->launches) {
     const QEntry *tab = B->tables + L.tab_off;
     const dim3 g((unsigned)L.blocks), bl(kBlock);
     switch (L.kind) {
       case 0:
         if (L.nb)
-          hipLaunchKernelGGL(k_qaoa_low_fwd_full<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet,
-                             B->p, tab, L.count, L.layer);
+          hipLaunchKernelGGL(k_qaoa_low_fwd_full<R>, g, bl, 0, st, s, B->inst, B->cuts, lo, ph, p,
+                             tab, L.count, L.layer);
         else
-          hipLaunchKernelGGL(k_qaoa_low_fwd<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet,
-                             B->p, tab, L.count, L.layer);
+          hipLaunchKernelGGL(k_qaoa_low_fwd<R>, g, bl, 0, st, s, B->inst, B->edges, ph, p, tab,
+                             L.count, L.layer);
         break;
       case 1: {
         auto k = L.nb == 5 ? k_qaoa_high_fwd<R, 5>
                  : L.nb == 4 ? k_qaoa_high_fwd<R, 4>
                  : L.nb == 3 ? k_qaoa_high_fwd<R, 3>
                  : L.nb == 2 ? k_qaoa_high_fwd<R, 2> : k_qaoa_high_fwd<R, 1>;
-        hipLaunchKernelGGL(k, g, bl, 0, st, s, B->inst, bet, B->p, tab, L.count, L.layer, L.bit);
+        hipLaunchKernelGGL(k, g, bl, 0, st, s, B->inst, ph, p, tab, L.count, L.layer, L.bit);
         break;
       }
       case 2:
-        hipLaunchKernelGGL(k_qaoa_energy<R>, g, bl, 0, st, s, B->inst, B->edges, tab, L.count,
-                           B->partial);
+        hipLaunchKernelGGL(k_qaoa_energy<R>, g, bl, 0, st, s, B->inst, B->edges, B->cuts, lo, tab,
+                           L.count, B->partial);
         break;
       case 3: {
         auto k = L.nb == 4 ? k_qaoa_high_rev<R, 4>
                  : L.nb == 3 ? k_qaoa_high_rev<R, 3>
                  : L.nb == 2 ? k_qaoa_high_rev<R, 2> : k_qaoa_high_rev<R, 1>;
-        hipLaunchKernelGGL(k, g, bl, 0, st, s, B->inst, bet, B->p, tab, L.count, L.layer, L.bit,
+        hipLaunchKernelGGL(k, g, bl, 0, st, s, B->inst, ph, p, tab, L.count, L.layer, L.bit,
                            B->partial);
         break;
       }
       case 4:
         if (L.nb)
-          hipLaunchKernelGGL(k_qaoa_low_rev_full<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet,
-                             B->p, tab, L.count, L.layer, B->partial);
+          hipLaunchKernelGGL(k_qaoa_low_rev_full<R>, g, bl, 0, st, s, B->inst, B->cuts, lo, ph, p,
+                             tab, L.count, L.layer, B->partial);
         else
-          hipLaunchKernelGGL(k_qaoa_low_rev<R>, g, bl, 0, st, s, B->inst, B->edges, gam, bet,
-                             B->p, tab, L.count, L.layer, B->partial);
+          hipLaunchKernelGGL(k_qaoa_low_rev<R>, g, bl, 0, st, s, B->inst, B->edges, ph, p, tab,
+                             L.count, L.layer, B->partial);
         break;
     }
   }
@@ -979,7 +1048,7 @@ int qaoa_destroy(void *handle) {
 
 int qaoa_stats(void *handle, uint64_t *launches, double *state_bytes, double *traffic_bytes) {
   QaoaBatch *B = (QaoaBatch *)handle;
-  if (launches) *launches = B->launches.size() + 1;
+  if (launches) *launches = B->launches.size() + 2;  // + phase tables, finaliser
   if (state_bytes) *state_bytes = B->state_bytes;
   if (traffic_bytes) *traffic_bytes = B->traffic_bytes;
   return SQ_OK;

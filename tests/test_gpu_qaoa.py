@@ -80,11 +80,12 @@ def test_qaoa_gradients_match_general_adjoint(C):
     """The batch kernels against the general engine's adjoint on the same
     circuit (ZZ(2 gamma): dE/dgamma = 2 * sum of the ZZ-angle derivatives).
     Sizes 13-20 give every high-bit group size of the forward (1-5 bits) and
-    reverse (1-4 bits) passes, and instances that stop inside a group."""
+    reverse (1-4 bits) passes, and instances that stop inside a group; 24
+    puts cut-table neighbours up to bit 23."""
     from snuqs_amd.circuits import qaoa_maxcut
     from snuqs_amd.qaoa import qaoa_batch
 
-    sizes = [14, 10, 17, 19, 13, 15, 20, 16]
+    sizes = [14, 10, 17, 19, 13, 15, 20, 16, 24]
     p = 2
     graphs, gam, bet = _batch(sizes, p, 5)
     out = qaoa_batch(sizes, graphs, p, dtype="complex128").run(gam, bet, use_graph=False)
@@ -118,3 +119,13 @@ def test_qaoa_graph_replays_match_eager(C):
         assert np.array_equal(eager, replay), r
     st = batch.stats()
     assert st["launches"] > 1 and st["state_bytes"] > 0
+
+
+def test_qaoa_rejects_repeated_edges(C):
+    """MaxCut graphs are simple: an edge given twice (either order) is an
+    invalid argument, reported before anything is allocated."""
+    from snuqs_amd.qaoa import qaoa_batch
+
+    with pytest.raises(Exception, match="repeated edge"):
+        qaoa_batch([14, 6], [[(0, 1), (2, 3)], [(0, 1), (1, 0)]], 1)
+    qaoa_batch([14, 6], [[(0, 1), (2, 3)], [(0, 1), (1, 2)]], 1)
