@@ -11,9 +11,9 @@
 //
 // Work unit: a chunk of 2^L amplitudes (L = min(12, n)); one workgroup per
 // (instance, chunk).  Qubits on bits below L ("low") are applied inside the
-// chunk in LDS; the higher bits go in groups of up to kQGF (forward) / kQGR
-// (reverse, which holds psi and lambda) bits per pass over
-// the state (each lane holds the 2^g amplitudes one group of bits spans).
+// chunk in LDS together with the cost layer; the higher bits go in groups of
+// up to kQGM bits per "mid" pass over the state, whose workgroups hold 16
+// consecutive amplitudes for each of the 2^8 values of the group's bits.
 // Grouping is exact for the gradients too: every X_q commutes with the
 // whole mixer layer, so <lambda| X_q |psi> has the same value anywhere inside
 // it and a group's partial can be taken before the group is undone.  Each
@@ -34,14 +34,13 @@ namespace sq {
 constexpr int kQL = 12;            // log2 chunk size
 constexpr int kQChunk = 1 << kQL;  // amplitudes per chunk
 constexpr int kQMaxEdges = 128;
-constexpr int kQGF = 5;            // high bits per forward pass
-constexpr int kQGR = 4;            // high bits per reverse pass (psi and lambda in registers)
+constexpr int kQGM = 8;            // high bits per mid pass
 
 struct QInstDev {
   uint64_t psi, lam;  // amplitude offsets into the batch state buffer
   uint64_t slot;      // first partial slot
   int n, nedges, edge_off, lc;  // lc = min(L, n)
-  int nchunks, ngroups;         // ngroups = ceil((n - lc) / kQGR) reverse high passes
+  int nchunks, ngroups;         // ngroups = ceil((n - lc) / kQGM) mid passes per layer
   int pad1, pad2;
 };
 
@@ -129,7 +128,7 @@ __device__ double qblock_sum(double v) {
 }
 
 // Slot layout of instance j (nch = nchunks): energy [nch], then per layer l:
-// gamma [nch], beta [(1 + ngroups) * nch] (low pass first, then high groups).
+// gamma [nch], beta [(1 + ngroups) * nch] (low pass first, then mid groups).
 __device__ __forceinline__ uint64_t slot_energy(const QInstDev &q) { return q.slot; }
 __device__ __forceinline__ uint64_t slot_gamma(const QInstDev &q, int l) {
   return q.slot + (uint64_t)q.nchunks * (1 + (uint64_t)l * (2 + q.ngroups));
@@ -413,50 +412,6 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// ---- high pass, forward: RX(2b) on bits h0 .. h0+G-1 -----------------------
-// Lane t takes one group: the 2^G amplitudes f with bits h0.. set in every
-// combination (consecutive lanes: consecutive low bits, coalesced).
-template <int G>
-__device__ __forceinline__ uint64_t qgroup_base(uint64_t f, int h0) {
-  const uint64_t lo = f & ((1ull << h0) - 1);
-  return ((f ^ lo) << G) | lo;
-}
-
-template <typename R, int G>
-__global__ void __launch_bounds__(kBlock)
-    k_qaoa_high_fwd(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                    const typename CV<R>::T *__restrict__ ph, int p,
-                    const QEntry *__restrict__ tab, int count, int layer, int h0) {
-  using T = typename CV<R>::T;
-  constexpr int D = 1 << G;
-  const QFound fe = find_entry_u(tab, count, blockIdx.x);
-  const int ii = fe.inst;
-  const QInstDev q = inst[ii];
-  const uint64_t chunk = blockIdx.x - fe.first;
-  const T cb = ph[(size_t)(ii * p + layer) * kQPh + kQPh - 1];
-  const R c0 = cb.x, s0 = cb.y;  // RX(2b) = [[c, -i s], [-i s, c]]
-  const uint64_t ngrp = 1ull << (q.lc - G);  // groups per chunk-sized share
-  for (uint64_t t = threadIdx.x; t < ngrp; t += kBlock) {
-    const uint64_t i0 = qgroup_base<G>(chunk * ngrp + t, h0);
-    T a[D];
-#pragma unroll
-    for (int r = 0; r < D; ++r) a[r] = ld_nt(st + q.psi + i0 + ((uint64_t)r << h0));
-#pragma unroll
-    for (int b = 0; b < G; ++b)
-#pragma unroll
-      for (int r = 0; r < D; ++r) {
-        if (r & (1 << b)) continue;
-        const T x = a[r], y = a[r | (1 << b)];
-        a[r].x = c0 * x.x + s0 * y.y;
-        a[r].y = c0 * x.y - s0 * y.x;
-        a[r | (1 << b)].x = c0 * y.x + s0 * x.y;
-        a[r | (1 << b)].y = c0 * y.y - s0 * x.x;
-      }
-#pragma unroll
-    for (int r = 0; r < D; ++r) st_nt(st + q.psi + i0 + ((uint64_t)r << h0), a[r]);
-  }
-}
-
 // ---- energy: lambda = C psi, partial sum of cut(i) |psi_i|^2 ----------------
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
@@ -502,68 +457,6 @@ __global__ void __launch_bounds__(kBlock)
   }
   acc = qblock_sum(acc);
   if (threadIdx.x == 0) partial[slot_energy(q) + chunk] = acc;
-}
-
-// ---- high pass, reverse: beta partial of bits h0.., undo their RX on psi and
-// lambda --------------------------------------------------------------------
-template <typename R, int G>
-__global__ void __launch_bounds__(kBlock)
-    k_qaoa_high_rev(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                    const typename CV<R>::T *__restrict__ ph, int p,
-                    const QEntry *__restrict__ tab, int count, int layer, int h0,
-                    double *__restrict__ partial) {
-  using T = typename CV<R>::T;
-  constexpr int D = 1 << G;
-  const QFound fe = find_entry_u(tab, count, blockIdx.x);
-  const int ii = fe.inst;
-  const QInstDev q = inst[ii];
-  const uint64_t chunk = blockIdx.x - fe.first;
-  const T cb = ph[(size_t)(ii * p + layer) * kQPh + kQPh - 1];
-  const R c0 = cb.x, s0 = cb.y;
-  const uint64_t ngrp = 1ull << (q.lc - G);
-  double acc = 0.0;
-  for (uint64_t t = threadIdx.x; t < ngrp; t += kBlock) {
-    const uint64_t i0 = qgroup_base<G>(chunk * ngrp + t, h0);
-    T a[D], l[D];
-#pragma unroll
-    for (int r = 0; r < D; ++r) {
-      a[r] = ld_nt(st + q.psi + i0 + ((uint64_t)r << h0));
-      l[r] = ld_nt(st + q.lam + i0 + ((uint64_t)r << h0));
-    }
-    // sum over the group's bits b and pairs (r, r^b): Im(conj(l_r) a_{r^b} + conj(l_{r^b}) a_r)
-#pragma unroll
-    for (int b = 0; b < G; ++b)
-#pragma unroll
-      for (int r = 0; r < D; ++r) {
-        if (r & (1 << b)) continue;
-        const int u = r | (1 << b);
-        acc += (double)qimcl(l[r], a[u]) + (double)qimcl(l[u], a[r]);
-      }
-    // RX(-2b) = [[c, i s], [i s, c]] on every bit of the group
-#pragma unroll
-    for (int b = 0; b < G; ++b)
-#pragma unroll
-      for (int r = 0; r < D; ++r) {
-        if (r & (1 << b)) continue;
-        const int u = r | (1 << b);
-        const T x = a[r], y = a[u], v = l[r], w = l[u];
-        a[r].x = c0 * x.x - s0 * y.y;
-        a[r].y = c0 * x.y + s0 * y.x;
-        a[u].x = c0 * y.x - s0 * x.y;
-        a[u].y = c0 * y.y + s0 * x.x;
-        l[r].x = c0 * v.x - s0 * w.y;
-        l[r].y = c0 * v.y + s0 * w.x;
-        l[u].x = c0 * w.x - s0 * v.y;
-        l[u].y = c0 * w.y + s0 * v.x;
-      }
-#pragma unroll
-    for (int r = 0; r < D; ++r) {
-      st_nt(st + q.psi + i0 + ((uint64_t)r << h0), a[r]);
-      st_nt(st + q.lam + i0 + ((uint64_t)r << h0), l[r]);
-    }
-  }
-  acc = qblock_sum(acc);
-  if (threadIdx.x == 0) partial[slot_beta(q, layer, 1 + (h0 - q.lc) / kQGR) + chunk] = acc;
 }
 
 template <typename R>
@@ -641,6 +534,158 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
   if (t == 0) partial[slot_beta(q, layer, 0) + chunk] = accb;
   accg = qblock_sum(accg);
   if (t == 0) partial[slot_gamma(q, layer) + chunk] = accg;
+}
+
+// ---- mid pass: RX(2b) (forward) or RX(-2b) and the beta partial (reverse) on
+// W = min(8, n - h0) high bits h0 .. h0+W-1 -----------------------------------
+// A workgroup holds 2^12 amplitudes whose local index j maps to the global
+// index bit by bit: local bits 0-3 -> global bits 0-3 (16 consecutive
+// amplitudes, 128 B of complex64), local bits 4..3+W -> h0..h0+W-1 (the bits
+// rotated), local bits 4+W..11 -> global bits 4..11-W (carried unchanged);
+// the chunk number fills the other n - 12 bits.  Loads use layout L2 (local
+// bits 8-11 in the register index), then one LDS transpose to L1 (bits 4-7),
+// and the stores go out in L1.  Eight rotated bits per pass, against the 4-5
+// a lane can hold in registers without LDS, halves the passes over high bits.
+__device__ __forceinline__ int qmid_pos(int k, int W, int h0) {
+  return k < 4 ? k : k < 4 + W ? h0 + (k - 4) : k - W;
+}
+
+struct QMid {
+  uint32_t base;     // global offset of the chunk (uniform)
+  uint32_t t2, t1;   // thread part of the L2 / L1 offsets
+  int W;
+};
+
+__device__ __forceinline__ QMid qmid_setup(uint32_t chunk, int n, int h0, int t) {
+  QMid m;
+  m.W = min(8, n - h0);
+  const int A = h0 - 12 + m.W;  // chunk bits below h0, at 12-W ..
+  m.base = ((chunk & ((1u << A) - 1)) << (12 - m.W)) | ((chunk >> A) << (h0 + m.W));
+  m.t2 = (uint32_t)(t & 15);
+  m.t1 = (uint32_t)(t & 15);
+#pragma unroll
+  for (int k = 4; k < 8; ++k) m.t2 |= (uint32_t)((t >> k) & 1) << qmid_pos(k, m.W, h0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) m.t1 |= (uint32_t)((t >> (4 + k)) & 1) << qmid_pos(8 + k, m.W, h0);
+  return m;
+}
+
+// register part of the offset: L2 holds local bits 8-11 in r, L1 bits 4-7
+__device__ __forceinline__ uint32_t qmid_r(int r, int lo, int W, int h0) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) o |= (uint32_t)((r >> k) & 1) << qmid_pos(lo + k, W, h0);
+  return o;
+}
+
+// RX on the register-index bits set in `mask` (uniform), as qrx16
+template <typename T, typename R>
+__device__ __forceinline__ void qrx16m(T (&v)[16], R c0, R s0, int mask) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    if (!((mask >> b) & 1)) continue;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      if (r & (1 << b)) continue;
+      const T x = v[r], y = v[r | (1 << b)];
+      v[r].x = c0 * x.x + s0 * y.y;
+      v[r].y = c0 * x.y - s0 * y.x;
+      v[r | (1 << b)].x = c0 * y.x + s0 * x.y;
+      v[r | (1 << b)].y = c0 * y.y - s0 * x.x;
+    }
+  }
+}
+
+// qxpart16 over the register-index bits set in `mask` (uniform)
+template <typename T>
+__device__ __forceinline__ double qxpart16m(const T (&a)[16], const T (&l)[16], int mask) {
+  double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    if (!((mask >> b) & 1)) continue;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      if (r & (1 << b)) continue;
+      const int u = r | (1 << b);
+      s0 += (double)qimcl(l[r], a[u]);
+      s1 += (double)qimcl(l[u], a[r]);
+    }
+  }
+  return s0 + s1;
+}
+
+// register-bit masks of the rotated local bits 4..3+W: L2 phase (local 8-11),
+// L1 phase (local 4-7)
+__device__ __forceinline__ int qmid_mask2(int W) { return (((1 << W) - 1) << 4) >> 8; }
+__device__ __forceinline__ int qmid_mask1(int W) { return ((1 << W) - 1) & 15; }
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_qaoa_mid_fwd(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
+                   const typename CV<R>::T *__restrict__ ph, int p,
+                   const QEntry *__restrict__ tab, int count, int layer, int h0) {
+  using T = typename CV<R>::T;
+  __shared__ T X[kQSlots];
+  const QFound fe = find_entry_u(tab, count, blockIdx.x);
+  const int ii = fe.inst;
+  const QInstDev q = inst[ii];
+  const int t = threadIdx.x;
+  const QMid m = qmid_setup((uint32_t)(blockIdx.x - fe.first), q.n, h0, t);
+  T *s = st + q.psi + m.base;
+  T v[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = ld_nt(s + (m.t2 | qmid_r(r, 8, m.W, h0)));
+  const T cb = ph[(size_t)(ii * p + layer) * kQPh + kQPh - 1];
+  const R c0 = cb.x, s0 = cb.y;  // RX(2b) = [[c, -i s], [-i s, c]]
+  qrx16m(v, c0, s0, qmid_mask2(m.W));
+  qtranspose(v, X, t, 2, 1);
+  qrx16m(v, c0, s0, qmid_mask1(m.W));
+#pragma unroll
+  for (int r = 0; r < 16; ++r) st_nt(s + (m.t1 | qmid_r(r, 4, m.W, h0)), v[r]);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
+    k_qaoa_mid_rev(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
+                   const typename CV<R>::T *__restrict__ ph, int p,
+                   const QEntry *__restrict__ tab, int count, int layer, int h0,
+                   double *__restrict__ partial) {
+  using T = typename CV<R>::T;
+  __shared__ T X[kQSlots];
+  const QFound fe = find_entry_u(tab, count, blockIdx.x);
+  const int ii = fe.inst;
+  const QInstDev q = inst[ii];
+  const int t = threadIdx.x;
+  const uint32_t chunk = (uint32_t)(blockIdx.x - fe.first);
+  const QMid m = qmid_setup(chunk, q.n, h0, t);
+  T *sa = st + q.psi + m.base, *sl = st + q.lam + m.base;
+  T a[16], l[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const uint32_t o = m.t2 | qmid_r(r, 8, m.W, h0);
+    a[r] = ld_nt(sa + o);
+    l[r] = ld_nt(sl + o);
+  }
+  const T cb = ph[(size_t)(ii * p + layer) * kQPh + kQPh - 1];
+  const R c0 = cb.x, s0 = -cb.y;  // RX(-2b)
+  const int m2 = qmid_mask2(m.W), m1 = qmid_mask1(m.W);
+  double acc = qxpart16m(a, l, m2);
+  qrx16m(a, c0, s0, m2);
+  qrx16m(l, c0, s0, m2);
+  qtranspose(a, X, t, 2, 1);
+  __syncthreads();
+  qtranspose(l, X, t, 2, 1);
+  acc += qxpart16m(a, l, m1);
+  qrx16m(a, c0, s0, m1);
+  qrx16m(l, c0, s0, m1);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const uint32_t o = m.t1 | qmid_r(r, 4, m.W, h0);
+    st_nt(sa + o, a[r]);
+    st_nt(sl + o, l[r]);
+  }
+  acc = qblock_sum(acc);
+  if (t == 0) partial[slot_beta(q, layer, 1 + (h0 - kQL) / kQGM) + chunk] = acc;
 }
 
 // ---- low pass, reverse, chunks below 2^12 (n < 12): beta partials and undo RX
@@ -769,8 +814,8 @@ struct QaoaBatch {
   double *out = nullptr;
   QEntry *tables = nullptr;
   struct Launch {
-    int kind, layer, bit;  // kind: 0 low_fwd, 1 high_fwd, 2 energy, 3 high_rev, 4 low_rev
-    int nb;                // high passes: bits bit .. bit + nb - 1
+    int kind, layer, bit;  // kind: 0 low_fwd, 2 energy, 4 low_rev, 5 mid_fwd, 6 mid_rev
+    int nb;                // low passes: 1 for the full-chunk kernel
     size_t tab_off;
     int count;
     uint64_t blocks;
@@ -812,7 +857,7 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
     q.n = n;
     q.lc = std::min(n, kQL);
     q.nchunks = 1 << (n - q.lc);
-    q.ngroups = (n - q.lc + kQGR - 1) / kQGR;
+    q.ngroups = (n - q.lc + kQGM - 1) / kQGM;
     q.nedges = (int)nedges[j];
     q.edge_off = (int)(hedges.size() / 2);
     std::vector<uint32_t> seen(n, 0);  // seen[a]: bit b set if edge {a, b} was given
@@ -889,25 +934,18 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
     if (L.count) B->launches.push_back(L);
   };
   auto all = [](const QInstDev &) { return true; };
-  // high group of up to gs bits starting at h0 with nb bits: the instances
-  // whose group there has exactly nb bits (a smaller last group for
-  // n - h0 < gs)
-  auto has_group = [](int h0, int nb, int gs) {
-    return [h0, nb, gs](const QInstDev &q) { return q.n > h0 && std::min(gs, q.n - h0) == nb; };
-  };
   auto full = [](const QInstDev &q) { return q.lc == kQL; };
   auto small = [](const QInstDev &q) { return q.lc < kQL; };
+  auto above = [](int h0) { return [h0](const QInstDev &q) { return q.n > h0; }; };
   for (int l = 0; l < (int)p; ++l) {
     add(0, l, 0, 1, full);  // nb = 1: full-chunk kernel
     add(0, l, 0, 0, small);
-    for (int h0 = kQL; h0 < B->maxn; h0 += kQGF)
-      for (int nb = kQGF; nb >= 1; --nb) add(1, l, h0, nb, has_group(h0, nb, kQGF));
+    for (int h0 = kQL; h0 < B->maxn; h0 += kQGM) add(5, l, h0, 0, above(h0));
   }
   add(2, 0, 0, 0, all);
-  const int last = kQL + ((B->maxn - kQL - 1) / kQGR) * kQGR;
   for (int l = (int)p - 1; l >= 0; --l) {
-    for (int h0 = last; h0 >= kQL; h0 -= kQGR)
-      for (int nb = kQGR; nb >= 1; --nb) add(3, l, h0, nb, has_group(h0, nb, kQGR));
+    for (int h0 = kQL + ((B->maxn - kQL - 1) / kQGM) * kQGM; h0 >= kQL; h0 -= kQGM)
+      add(6, l, h0, 0, above(h0));
     add(4, l, 0, 1, full);
     add(4, l, 0, 0, small);
   }
@@ -929,7 +967,7 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
   }
   // HBM traffic of one run (reads + writes of the states)
   for (auto &L : B->launches) {
-    const double per = (L.kind == 0 || L.kind == 1) ? 2.0 : (L.kind == 2 ? 3.0 : 4.0);
+    const double per = (L.kind == 0 || L.kind == 5) ? 2.0 : (L.kind == 2 ? 3.0 : 4.0);
     for (int k = 0; k < L.count; ++k) {
       const QInstDev &q = B->host_inst[tabs[L.tab_off + k].inst];
       const double amps = std::ldexp(1.0, q.n);
@@ -962,26 +1000,18 @@ static int enqueue_t(QaoaBatch *B, hipStream_t st) {
           hipLaunchKernelGGL(k_qaoa_low_fwd<R>, g, bl, 0, st, s, B->inst, B->edges, ph, p, tab,
                              L.count, L.layer);
         break;
-      case 1: {
-        auto k = L.nb == 5 ? k_qaoa_high_fwd<R, 5>
-                 : L.nb == 4 ? k_qaoa_high_fwd<R, 4>
-                 : L.nb == 3 ? k_qaoa_high_fwd<R, 3>
-                 : L.nb == 2 ? k_qaoa_high_fwd<R, 2> : k_qaoa_high_fwd<R, 1>;
-        hipLaunchKernelGGL(k, g, bl, 0, st, s, B->inst, ph, p, tab, L.count, L.layer, L.bit);
-        break;
-      }
       case 2:
         hipLaunchKernelGGL(k_qaoa_energy<R>, g, bl, 0, st, s, B->inst, B->edges, B->cuts, lo, tab,
                            L.count, B->partial);
         break;
-      case 3: {
-        auto k = L.nb == 4 ? k_qaoa_high_rev<R, 4>
-                 : L.nb == 3 ? k_qaoa_high_rev<R, 3>
-                 : L.nb == 2 ? k_qaoa_high_rev<R, 2> : k_qaoa_high_rev<R, 1>;
-        hipLaunchKernelGGL(k, g, bl, 0, st, s, B->inst, ph, p, tab, L.count, L.layer, L.bit,
-                           B->partial);
+      case 5:
+        hipLaunchKernelGGL(k_qaoa_mid_fwd<R>, g, bl, 0, st, s, B->inst, ph, p, tab, L.count,
+                           L.layer, L.bit);
         break;
-      }
+      case 6:
+        hipLaunchKernelGGL(k_qaoa_mid_rev<R>, g, bl, 0, st, s, B->inst, ph, p, tab, L.count,
+                           L.layer, L.bit, B->partial);
+        break;
       case 4:
         if (L.nb)
           hipLaunchKernelGGL(k_qaoa_low_rev_full<R>, g, bl, 0, st, s, B->inst, B->cuts, lo, ph, p,

@@ -79,9 +79,9 @@ def test_qaoa_batch_energies_and_gradients(C, dtype, etol, gtol):
 def test_qaoa_gradients_match_general_adjoint(C):
     """The batch kernels against the general engine's adjoint on the same
     circuit (ZZ(2 gamma): dE/dgamma = 2 * sum of the ZZ-angle derivatives).
-    Sizes 13-20 give every high-bit group size of the forward (1-5 bits) and
-    reverse (1-4 bits) passes, and instances that stop inside a group; 24
-    puts cut-table neighbours up to bit 23."""
+    Sizes 13-20 give every width (1-8 bits) of the first pass over the bits
+    above 12, 24 a second pass (bits 20-23) and cut-table neighbours up to
+    bit 23."""
     from snuqs_amd.circuits import qaoa_maxcut
     from snuqs_amd.qaoa import qaoa_batch
 
