@@ -78,10 +78,13 @@ def test_unitary_k_targets(C, dtype, k, n):
         assert err <= TOL[dtype], (t, err)
 
 
-@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
-@pytest.mark.parametrize("grid,pipe,order,wide", [
-    (256, 1, 0, 0), (1, 1, 0, 0), (1, 0, 0, 0), (1, 1, 1, 0), (3, 0, 1, 0), (1024, 1, 1, 0),
-    (256, 1, 0, 1), (1, 1, 0, 1), (1, 0, 1, 1), (1024, 0, 1, 1)])
+_NARROW = [(256, 1, 0, 0), (1, 1, 0, 0), (1, 0, 0, 0), (1, 1, 1, 0), (3, 0, 1, 0), (1024, 1, 1, 0)]
+_WIDE = [(256, 1, 0, 1), (1, 1, 0, 1), (1, 0, 1, 1), (1024, 0, 1, 1)]  # complex64 only
+
+
+@pytest.mark.parametrize("dtype,grid,pipe,order,wide",
+                         [(d,) + c for d in ("complex128", "complex64") for c in _NARROW]
+                         + [("complex64",) + c for c in _WIDE])
 def test_unitary_5_target_tiles(C, dtype, grid, pipe, order, wide):
     """5-target MFMA kernels (LDS-staged tiles) on target sets that put the
     tile's bits everywhere: the lowest five address bits, the highest, spread
@@ -91,8 +94,6 @@ def test_unitary_5_target_tiles(C, dtype, grid, pipe, order, wide):
     1-block cap still loops)."""
     from snuqs_amd.circuits import haar_unitary
 
-    if wide and dtype == "complex128":
-        pytest.skip("64-group tiles are complex64 only")
     g = C.operation.gate_operations
     n = 22 if wide else 21
     rng = np.random.default_rng(55)
