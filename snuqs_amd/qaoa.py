@@ -6,7 +6,7 @@ with rejection; 3-regular graphs need an even qubit count, so sizes are the
 even values of 12..26) and parameters gamma ~ U[0, pi], beta ~ U[0, pi/2].
 ``QaoaBatch`` (the engine, ``_C.functionals.QaoaBatch``) returns per instance
 E = sum_edges (1 - <Z_a Z_b>) / 2 and dE/dgamma, dE/dbeta by the adjoint
-method.
+method.  Graphs must be simple (an edge given twice raises).
 """
 from __future__ import annotations
 
