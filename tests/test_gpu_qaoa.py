@@ -80,12 +80,12 @@ def test_qaoa_gradients_match_general_adjoint(C):
     """The batch kernels against the general engine's adjoint on the same
     circuit (ZZ(2 gamma): dE/dgamma = 2 * sum of the ZZ-angle derivatives).
     Sizes 13-20 give every width (1-8 bits) of the first pass over the bits
-    above 12, 24 a second pass (bits 20-23) and cut-table neighbours up to
-    bit 23."""
+    above 12, 24 and 26 a second pass (bits 20-23, 20-25) and cut-table
+    neighbours up to bit 25 (26 is the largest size of the benchmark batch)."""
     from snuqs_amd.circuits import qaoa_maxcut
     from snuqs_amd.qaoa import qaoa_batch
 
-    sizes = [14, 10, 17, 19, 13, 15, 20, 16, 24]
+    sizes = [14, 10, 17, 19, 13, 15, 20, 16, 24, 26]
     p = 2
     graphs, gam, bet = _batch(sizes, p, 5)
     out = qaoa_batch(sizes, graphs, p, dtype="complex128").run(gam, bet, use_graph=False)
@@ -100,6 +100,29 @@ def test_qaoa_gradients_match_general_adjoint(C):
         e += 0.5 * len(graphs[j])
         assert abs(out[j, 0] - e) < 1e-10
         assert np.abs(out[j, 1:] - 2.0 * g).max() < 1e-9
+
+
+def test_qaoa_complex64_batch_matches_complex128_at_benchmark_sizes(C):
+    """complex64 states with FP64 reductions against the complex128 batch on
+    3-regular graphs of the benchmark's sizes (p = 4): energies within 1e-5
+    relative; gradients, sums of products of lambda = C psi (|C| up to |E|)
+    and S psi (|S| up to |E|), within 1e-5 of their term scale |E|^2."""
+    from snuqs_amd.circuits import random_3_regular
+    from snuqs_amd.qaoa import qaoa_batch
+
+    rng = np.random.default_rng(21)
+    sizes = [26, 22, 12, 24, 18]
+    p = 4
+    graphs = [random_3_regular(n, rng) for n in sizes]
+    gam = rng.uniform(0, np.pi, (len(sizes), p))
+    bet = rng.uniform(0, np.pi / 2, (len(sizes), p))
+    ref = qaoa_batch(sizes, graphs, p, dtype="complex128").run(gam, bet, use_graph=False)
+    got = qaoa_batch(sizes, graphs, p, dtype="complex64").run(gam, bet, use_graph=True)
+    assert np.all(np.abs(got[:, 0] - ref[:, 0]) <= 1e-5 * np.abs(ref[:, 0])), (got[:, 0], ref[:, 0])
+    scale = np.array([len(g) ** 2 for g in graphs], dtype=float)[:, None]
+    gerr = np.abs(got[:, 1:] - ref[:, 1:]) / scale
+    print("complex64 energy rel err", np.abs(got[:, 0] / ref[:, 0] - 1).max(), "grad err / |E|^2", gerr.max())
+    assert gerr.max() < 1e-5, gerr.max()
 
 
 def test_qaoa_graph_replays_match_eager(C):
