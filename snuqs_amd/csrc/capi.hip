@@ -1044,9 +1044,10 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     }
   }
   // Groups of 2-4 consecutive one-target gates with one derivative each, on
-  // distinct amplitude bits above bit 3 and outside the runs, take one
-  // reverse pass (launch_adjoint_step_group); grp_top[g] = group whose highest
-  // gate is g.
+  // distinct amplitude bits, outside the runs, take one reverse pass
+  // (launch_adjoint_step_group): bits all above 3 (a lane holds each group of
+  // amplitudes) or all below 4 (pairs across the lanes of a wave; states of
+  // at least 64 amplitudes).  grp_top[g] = group whose highest gate is g.
   struct Group {
     uint32_t lo, k, first_slot;
     int bits[4];
@@ -1058,16 +1059,18 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     std::vector<char> in_run(ng, 0);
     for (const Run &r : runs)
       for (uint32_t q = r.lo; q < r.lo + r.m; ++q) in_run[q] = 1;
+    auto gbit = [&](int g) { return p->nq - 1 - (int)p->targets[p->toff[g]]; };
     auto grp_ok = [&](int g) {
       return !in_run[g] && !is_bound[g] && p->nt[g] == 1 && by_gate[g].size() == 1 &&
-             p->nq - 1 - (int)p->targets[p->toff[g]] >= 4;
+             (gbit(g) >= 4 || p->nq >= 6);
     };
     for (int g = (int)ng - 1; g >= 0;) {
       int lo = g;
       uint64_t used = 0;
       while (lo >= 0 && g - lo < 4 && grp_ok(lo)) {
-        const int b = p->nq - 1 - (int)p->targets[p->toff[lo]];
+        const int b = gbit(lo);
         if ((used >> b) & 1) break;
+        if (lo < g && (b >= 4) != (gbit(g) >= 4)) break;  // one kind per group
         used |= 1ull << b;
         --lo;
       }
@@ -1155,24 +1158,26 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       }
     }
   }
-  // forward groups: 2-4 consecutive one-target gates on distinct bits above
-  // bit 3 (outside forward runs, not at a checkpoint) applied in one pass
+  // forward groups: 2-4 consecutive one-target gates on distinct bits, all
+  // above 3 or all below 4 as in the reverse groups (outside forward runs, not
+  // at a checkpoint), applied in one pass
   std::vector<Group> fgroups;
   std::vector<int> fgrp_start(ng, -1);
   {
     std::vector<char> in_frun(ng, 0);
     for (const Run &r : fruns)
       for (uint32_t q = r.lo; q < r.lo + r.m; ++q) in_frun[q] = 1;
+    auto fbit = [&](uint32_t g) { return p->nq - 1 - (int)p->targets[p->toff[g]]; };
     auto fok = [&](uint32_t g) {
-      return !in_frun[g] && !is_bound[g] && p->nt[g] == 1 &&
-             p->nq - 1 - (int)p->targets[p->toff[g]] >= 4;
+      return !in_frun[g] && !is_bound[g] && p->nt[g] == 1 && (fbit(g) >= 4 || p->nq >= 6);
     };
     for (uint32_t g = 0; g < ng;) {
       uint32_t hi = g;
       uint64_t used = 0;
       while (hi < ng && hi - g < 4 && fok(hi)) {
-        const int b = p->nq - 1 - (int)p->targets[p->toff[hi]];
+        const int b = fbit(hi);
         if ((used >> b) & 1) break;
+        if (hi > g && (b >= 4) != (fbit(g) >= 4)) break;  // one kind per group
         used |= 1ull << b;
         ++hi;
       }

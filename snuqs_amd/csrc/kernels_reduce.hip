@@ -6,6 +6,8 @@
 // expectation, norm or inner-product code.  Every reduction here accumulates
 // in FP64 (also for complex64 states) with a fixed grid and a fixed
 // combination order, so results are bitwise reproducible run to run.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "kernels_reduce.hpp"
 
@@ -475,6 +477,96 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// The same step for one-target gates on distinct bits below 6, i.e. inside
+// the 64 consecutive amplitudes a wave holds: lane t owns amplitude i
+// (consecutive lanes, consecutive amplitudes: every access coalesced), a gate
+// on bit b pairs lanes t and t ^ 2^b, and the partner's value comes from a
+// cross-lane shuffle.  Lane-per-group addressing would put a gate's pair in
+// one lane but make every load instruction stride over the lines, which
+// non-temporal accesses then re-fetch.  n is a multiple of 64, so the
+// "i < n" tests are uniform per wave and every lane of a shuffle is active.
+template <typename R, int K>
+struct LaneArgs {
+  Mat2<R> gi[K], dg[K];
+  int bit[K];
+};
+
+template <typename T, typename R>
+__device__ __forceinline__ T lane_xor(const T &v, int mask) {
+  T o;
+  o.x = __shfl_xor(v.x, mask, 64);
+  o.y = __shfl_xor(v.y, mask, 64);
+  return o;
+}
+
+// y = m[s][s] v + m[s][1-s] p, the row of m that belongs to this lane's bit s
+template <typename T, typename R>
+__device__ __forceinline__ T lane_row(const Mat2<R> &m, int s, const T &v, const T &p) {
+  const R ar = s ? m.r[3] : m.r[0], ai = s ? m.i[3] : m.i[0];
+  const R br = s ? m.r[2] : m.r[1], bi = s ? m.i[2] : m.i[1];
+  T y;
+  y.x = ar * v.x - ai * v.y + br * p.x - bi * p.y;
+  y.y = ar * v.y + ai * v.x + br * p.y + bi * p.x;
+  return y;
+}
+
+template <typename T, typename R, int K, bool LAM>
+__global__ void __launch_bounds__(kBlock)
+    k_adjoint_step_lanes(T *__restrict__ psi, T *__restrict__ lam, uint64_t n,
+                         LaneArgs<R, K> a, double *__restrict__ partial) {
+  constexpr int U = 4;  // amplitudes per lane in flight
+  const int lane = threadIdx.x & 63;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x; i0 < n;
+       i0 += (uint64_t)gridDim.x * (kBlock * U)) {
+    T x[U], l[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = i0 + (uint64_t)u * kBlock;
+      if (i < n) {
+        x[u] = ld_nt(psi + i);
+        if (LAM) l[u] = ld_nt(lam + i);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = i0 + (uint64_t)u * kBlock;
+      if (i >= n) continue;
+      if (LAM) {
+        // every derivative term at the group's starting point (the gates
+        // act on distinct qubits and commute), then undo the gates
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+          const int s = (lane >> a.bit[j]) & 1;
+          const T d = lane_row<T, R>(a.dg[j], s, x[u], lane_xor<T, R>(x[u], 1 << a.bit[j]));
+          acc[j] += (double)l[u].x * d.x + (double)l[u].y * d.y;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const int s = (lane >> a.bit[j]) & 1, mask = 1 << a.bit[j];
+        x[u] = lane_row<T, R>(a.gi[j], s, x[u], lane_xor<T, R>(x[u], mask));
+        if (LAM) l[u] = lane_row<T, R>(a.gi[j], s, l[u], lane_xor<T, R>(l[u], mask));
+      }
+      st_nt(psi + i, x[u]);
+      if (LAM) st_nt(lam + i, l[u]);
+    }
+  }
+  if (!LAM) return;
+#pragma unroll
+  for (int j = 0; j < K; j += 2) {
+    double u = acc[j], v = j + 1 < K ? acc[j + 1] : 0.0;
+    block_sum2(u, v);
+    if (threadIdx.x == 0) {
+      partial[(uint64_t)j * gridDim.x + blockIdx.x] = u;
+      if (j + 1 < K) partial[(uint64_t)(j + 1) * gridDim.x + blockIdx.x] = v;
+    }
+    __syncthreads();  // block_sum2's LDS is reused
+  }
+}
+
 // ---- marginal distribution over k targets ------------------------------------
 // out[r] = sum of |sv[i]|^2 over the i whose target bits spell r (target 0
 // the most significant bit of r).  Block b sums outcome r = b / nb over the
@@ -680,6 +772,27 @@ int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
 }
 
 template <typename T, typename R, int K>
+static void launch_lanes_t(void *psi, void *lam, uint64_t n, const int *bits, const double *ginv,
+                           const double *dgg, double *partial, int nb, hipStream_t st) {
+  LaneArgs<R, K> a{};
+  for (int j = 0; j < K; ++j) {
+    a.bit[j] = bits[j];
+    for (int e = 0; e < 4; ++e) {
+      a.gi[j].r[e] = (R)ginv[8 * j + 2 * e];
+      a.gi[j].i[e] = (R)ginv[8 * j + 2 * e + 1];
+      a.dg[j].r[e] = (R)dgg[8 * j + 2 * e];
+      a.dg[j].i[e] = (R)dgg[8 * j + 2 * e + 1];
+    }
+  }
+  if (lam)
+    hipLaunchKernelGGL((k_adjoint_step_lanes<T, R, K, true>), dim3(nb), dim3(kBlock), 0, st,
+                       (T *)psi, (T *)lam, n, a, partial);
+  else
+    hipLaunchKernelGGL((k_adjoint_step_lanes<T, R, K, false>), dim3(nb), dim3(kBlock), 0, st,
+                       (T *)psi, (T *)nullptr, n, a, partial);
+}
+
+template <typename T, typename R, int K>
 static void launch_group_t(void *psi, void *lam, uint64_t n, const int *bits, const double *ginv,
                            const double *dgg, double *partial, int nb, hipStream_t st) {
   GroupArgs<R, K> a{};
@@ -712,17 +825,33 @@ int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, i
                               const int *slot_param_dev, bool distinct, double *partial,
                               double *out_dev, hipStream_t st) {
   if (k < 2 || k > 4) return fail(SQ_ERR_INVALID, "internal: gate group size out of range");
+  // bits all below 6: the lane kernel (cross-lane pairs); bits all above 3:
+  // the register kernel (a lane holds each group of 2^k amplitudes)
+  int lo = 64, hi = -1;
+  for (int j = 0; j < k; ++j) {
+    lo = std::min(lo, bits[j]);
+    hi = std::max(hi, bits[j]);
+  }
+  const bool lanes = hi < 6;
+  if (lanes && (n < 64 || n % 64 != 0))
+    return fail(SQ_ERR_INVALID, "internal: lane group on a state below 64 amplitudes");
+  if (!lanes && lo < 4) return fail(SQ_ERR_INVALID, "internal: gate group mixes low and high bits");
   // a forward group (lam == nullptr) only applies the ginv matrices to psi:
   // callers pass the gates themselves there; more blocks, no reduction
-  const uint64_t fb = (n >> k) / kBlock + 1;
-  const int nb = lam ? red_blocks(n >> k) : (int)(fb < (1u << 16) ? fb : (1u << 16));
+  const uint64_t work = lanes ? n / 4 : n >> k;  // lanes: 4 amplitudes per lane
+  const uint64_t fb = work / kBlock + 1;
+  const int nb = lam ? red_blocks(work) : (int)(fb < (1u << 16) ? fb : (1u << 16));
   const bool c64 = dt == SQ_COMPLEX64;
-#define SQ_GROUP(K)                                                                       \
-  case K:                                                                                 \
-    if (c64)                                                                              \
-      launch_group_t<float2, float, K>(psi, lam, n, bits, ginv, dgg, partial, nb, st);    \
-    else                                                                                  \
-      launch_group_t<double2, double, K>(psi, lam, n, bits, ginv, dgg, partial, nb, st);  \
+#define SQ_GROUP(K)                                                                         \
+  case K:                                                                                   \
+    if (lanes && c64)                                                                       \
+      launch_lanes_t<float2, float, K>(psi, lam, n, bits, ginv, dgg, partial, nb, st);      \
+    else if (lanes)                                                                         \
+      launch_lanes_t<double2, double, K>(psi, lam, n, bits, ginv, dgg, partial, nb, st);    \
+    else if (c64)                                                                           \
+      launch_group_t<float2, float, K>(psi, lam, n, bits, ginv, dgg, partial, nb, st);      \
+    else                                                                                    \
+      launch_group_t<double2, double, K>(psi, lam, n, bits, ginv, dgg, partial, nb, st);    \
     break;
   switch (k) {
     SQ_GROUP(2)

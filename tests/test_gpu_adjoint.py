@@ -168,13 +168,15 @@ def _planner_circuit(n, thetas):
     return ops, pmap
 
 
-@pytest.mark.parametrize("n", [9, 12])
+@pytest.mark.parametrize("n", [6, 9, 12])
 @pytest.mark.parametrize("checkpoints", [0, 2, 5])
 def test_adjoint_pass_planning_vs_finite_differences(C, n, checkpoints):
     """Diagonal runs (split at 64 gates and 32 derivatives, shared parameters
-    finalised in order), one-target groups, checkpoint gates that cut runs and
-    groups, and a 9-qubit state too small for runs: complex128 gradients
-    against central differences of the CPU oracle."""
+    finalised in order), one-target groups on high bits (a lane per group) and
+    on bits 0-3 (pairs across lanes; at 6 qubits the state is one wave),
+    checkpoint gates that cut runs and groups, and 6- and 9-qubit states too
+    small for runs: complex128 gradients against central differences of the
+    CPU oracle."""
     from snuqs_amd.circuits import random_pauli_sum
 
     rng = np.random.default_rng(n + checkpoints)
