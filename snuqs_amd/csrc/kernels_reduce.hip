@@ -244,17 +244,38 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---- reverse step over a run of diagonal gates ---------------------------------
-// Each thread takes U amplitudes i = base + t + kBlock u of a chunk; per slot
-// gate the chunk's sum of Re(conj(lam_i) psi_i dgg(i)) is reduced over the
-// wave and added to an LDS row; then every amplitude of psi and lambda is
-// multiplied by the run's G^dagger phase product, formed in FP64 (one
-// rounding per amplitude for complex64).
+// Each thread takes U amplitudes i = base + t + kBlock u of 1024-amplitude
+// chunks, so bits 0-9 of i are fixed per (thread, u) and bits 10+ per chunk.
+// Every amplitude of psi and lambda is multiplied by the run's G^dagger phase
+// product, formed in FP64 (one rounding per amplitude for complex64):
+//   - gates on bits below 10 ("local"): their product is formed once per
+//     thread and u, before the chunk loop;
+//   - gates on bits 10 and above ("uniform"): one factor per chunk;
+//   - gates with one bit on each side ("mixed"): per amplitude.
+// Derivative terms Re sum_i conj(lam_i) psi_i dgg(i) use w = conj(lam) psi,
+// which the run does not change (psi and lambda get the same unit-modulus
+// phase), so for a local gate the sum is sum_u Re dgg(t, u) W_u with W_u the
+// thread's sum of w over its chunks (one wave sum per gate at the end); a
+// uniform gate takes dgg(chunk) times the wave's sum of w in that chunk (one
+// wave sum per chunk for all of them); a mixed gate is summed per chunk.
+__device__ __forceinline__ int diag_loc(const DiagRunGate &g, uint64_t i) {
+  return g.b1 < 0 ? (int)((i >> g.b0) & 1)
+                  : (int)((((i >> g.b0) & 1) << 1) | ((i >> g.b1) & 1));
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
 template <typename T, typename R, bool LAM>
 __global__ void __launch_bounds__(kBlock)
     k_adjoint_diag_run(T *__restrict__ psi, T *__restrict__ lam, uint64_t n,
                        const DiagRunGate *__restrict__ gates, int m, int nslots,
                        double *__restrict__ partial) {
   constexpr int U = 4, W = kBlock / 64;
+  constexpr int kLocBits = 10;  // log2(kBlock * U)
   __shared__ DiagRunGate sg[kMaxDiagRun];
   __shared__ double srow[kMaxDiagSlots][W];
   {
@@ -266,6 +287,31 @@ __global__ void __launch_bounds__(kBlock)
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  auto is_local = [](const DiagRunGate &g) {
+    return g.b0 < kLocBits && (g.b1 < 0 || g.b1 < kLocBits);
+  };
+  // phase product of the local gates at local index t + kBlock u
+  double lr[U], li[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    lr[u] = 1.0;
+    li[u] = 0.0;
+  }
+  for (int k = 0; k < m; ++k) {
+    const DiagRunGate &gk = sg[k];
+    if (!is_local(gk)) continue;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int loc = diag_loc(gk, (uint64_t)threadIdx.x + (uint64_t)kBlock * u);
+      const double gr = gk.ginv[loc][0], gi = gk.ginv[loc][1];
+      const double tr = lr[u] * gr - li[u] * gi;
+      li[u] = lr[u] * gi + li[u] * gr;
+      lr[u] = tr;
+    }
+  }
+  double war[U], wai[U];  // the thread's sums of w over its chunks, per u
+#pragma unroll
+  for (int u = 0; u < U; ++u) war[u] = wai[u] = 0.0;
   const uint64_t chunk = (uint64_t)kBlock * U;
   for (uint64_t base = (uint64_t)blockIdx.x * chunk; base < n; base += (uint64_t)gridDim.x * chunk) {
     T a[U], l[U];
@@ -279,6 +325,8 @@ __global__ void __launch_bounds__(kBlock)
         // w = conj(lam) psi
         wr[u] = (double)l[u].x * a[u].x + (double)l[u].y * a[u].y;
         wi[u] = (double)l[u].x * a[u].y - (double)l[u].y * a[u].x;
+        war[u] += wr[u];
+        wai[u] += wi[u];
       } else {
         wr[u] = wi[u] = 0.0;
       }
@@ -286,43 +334,41 @@ __global__ void __launch_bounds__(kBlock)
     double pr[U], pi[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      pr[u] = 1.0;
-      pi[u] = 0.0;
+      pr[u] = lr[u];
+      pi[u] = li[u];
     }
-    // gates whose target bits are all >= log2(chunk) see one local index in
-    // the whole chunk: their phase goes into one chunk factor and their term
-    // uses the thread's sum of w
-    double cr = 1.0, ci = 0.0, swr = 0.0, swi = 0.0;
-    if (LAM) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        swr += wr[u];
-        swi += wi[u];
-      }
-    }
+    double cr = 1.0, ci = 0.0;
+    double swr = 0.0, swi = 0.0;  // the wave's sum of w in this chunk, formed on first use
+    bool have_sw = false;
     for (int k = 0; k < m; ++k) {
       const DiagRunGate &gk = sg[k];
-      double acc = 0.0;
-      if (gk.uniform) {  // chunk-uniform gate (the branch is uniform over the block)
-        const int loc = gk.b1 < 0 ? (int)((base >> gk.b0) & 1)
-                                  : (int)((((base >> gk.b0) & 1) << 1) | ((base >> gk.b1) & 1));
+      if (gk.uniform) {  // the branch is uniform over the block
+        const int loc = diag_loc(gk, base);
         const double gr = gk.ginv[loc][0], gi = gk.ginv[loc][1];
         const double tr = cr * gr - ci * gi;
         ci = cr * gi + ci * gr;
         cr = tr;
         if (LAM && gk.slot >= 0) {
-          acc = swr * gk.dgg[loc][0] - swi * gk.dgg[loc][1];
+          if (!have_sw) {
+            double tr2 = 0.0, ti2 = 0.0;
 #pragma unroll
-          for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-          if (lane == 0) srow[gk.slot][wave] += acc;
+            for (int u = 0; u < U; ++u) {
+              tr2 += wr[u];
+              ti2 += wi[u];
+            }
+            swr = wave_sum_d(tr2);
+            swi = wave_sum_d(ti2);
+            have_sw = true;
+          }
+          if (lane == 0) srow[gk.slot][wave] += swr * gk.dgg[loc][0] - swi * gk.dgg[loc][1];
         }
         continue;
       }
+      if (is_local(gk)) continue;
+      double acc = 0.0;  // mixed gate
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const uint64_t i = base + threadIdx.x + (uint64_t)kBlock * u;
-        const int loc = gk.b1 < 0 ? (int)((i >> gk.b0) & 1)
-                                  : (int)((((i >> gk.b0) & 1) << 1) | ((i >> gk.b1) & 1));
+        const int loc = diag_loc(gk, base + threadIdx.x + (uint64_t)kBlock * u);
         const double gr = gk.ginv[loc][0], gi = gk.ginv[loc][1];
         const double tr = pr[u] * gr - pi[u] * gi;
         pi[u] = pr[u] * gi + pi[u] * gr;
@@ -330,8 +376,7 @@ __global__ void __launch_bounds__(kBlock)
         if (LAM && gk.slot >= 0) acc += wr[u] * gk.dgg[loc][0] - wi[u] * gk.dgg[loc][1];
       }
       if (LAM && gk.slot >= 0) {  // uniform over the block
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        acc = wave_sum_d(acc);
         if (lane == 0) srow[gk.slot][wave] += acc;
       }
     }
@@ -354,6 +399,19 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   if (!LAM) return;
+  // local gates' derivative terms from the per-u sums of w
+  for (int k = 0; k < m; ++k) {
+    const DiagRunGate &gk = sg[k];
+    if (gk.slot < 0 || !is_local(gk)) continue;
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int loc = diag_loc(gk, (uint64_t)threadIdx.x + (uint64_t)kBlock * u);
+      acc += war[u] * gk.dgg[loc][0] - wai[u] * gk.dgg[loc][1];
+    }
+    acc = wave_sum_d(acc);
+    if (lane == 0) srow[gk.slot][wave] += acc;
+  }
   __syncthreads();
   for (int s = threadIdx.x; s < nslots; s += kBlock) {
     double v = 0.0;

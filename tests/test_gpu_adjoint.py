@@ -102,14 +102,17 @@ def test_adjoint_all_parametric_gate_kinds(C):
     assert np.abs(grads - fd).max() < 1e-7
 
 
+@pytest.mark.parametrize("n", [16, 23])
 @pytest.mark.parametrize("dtype,tol", [("complex128", 1e-9), ("complex64", 2e-4)])
-def test_hea_adjoint_16q_vs_gpu_parameter_shift(C, dtype, tol):
-    """16 qubits (targets on every bit position, including the ones above the
-    lowest eight): adjoint gradients against parameter shift evaluated with the
-    engine itself in complex128 (exact for the exp(-i theta/2 P) rotations)."""
+def test_hea_adjoint_vs_gpu_parameter_shift(C, dtype, tol, n):
+    """16 and 23 qubits (targets on every bit position, including the ones
+    above the lowest eight; at 23 qubits each workgroup of a diagonal run
+    covers two 1024-amplitude chunks): adjoint gradients against parameter
+    shift evaluated with the engine itself in complex128 (exact for the
+    exp(-i theta/2 P) rotations)."""
     from snuqs_amd.circuits import hea_ansatz, random_pauli_sum
 
-    n, layers = 16, 2
+    layers = 2
     rng = np.random.default_rng(16)
     thetas = rng.uniform(0, 2 * np.pi, 2 * n * layers)
     obs = random_pauli_sum(n, 10, 4, rng)
