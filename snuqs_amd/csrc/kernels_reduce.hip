@@ -809,8 +809,10 @@ int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
     return fail(SQ_ERR_INVALID, "internal: diagonal run out of range");
   const uint64_t chunk = (uint64_t)kBlock * 4;
   if (n % chunk) return fail(SQ_ERR_INVALID, "internal: diagonal run needs n % 1024 == 0");
+  // at most 1536 workgroups (2 rounds of 3 per CU): each then covers several
+  // chunks, over which its local phase products and sums of w are shared
   uint64_t nb = n / chunk;
-  if (nb > 4096) nb = 4096;
+  if (nb > 1536) nb = 1536;
   if (!lam && nslots) return fail(SQ_ERR_INVALID, "internal: derivative slots need lambda");
   if (dt == SQ_COMPLEX64) {
     auto k = lam ? k_adjoint_diag_run<float2, float, true> : k_adjoint_diag_run<float2, float, false>;
