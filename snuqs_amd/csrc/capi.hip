@@ -1050,8 +1050,8 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   // at least 64 amplitudes).  grp_top[g] = group whose highest gate is g.
   struct Group {
     uint32_t lo, k, first_slot;
-    int bits[4];
-    double ginv[32], dgg[32];
+    int bits[5];
+    double ginv[40], dgg[40];
   };
   std::vector<Group> groups;
   std::vector<int> grp_top(ng, -1);
@@ -1158,9 +1158,10 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       }
     }
   }
-  // forward groups: 2-4 consecutive one-target gates on distinct bits, all
+  // forward groups: 2-5 consecutive one-target gates on distinct bits, all
   // above 3 or all below 4 as in the reverse groups (outside forward runs, not
-  // at a checkpoint), applied in one pass
+  // at a checkpoint), applied in one pass (5: psi alone leaves the registers
+  // for 32 amplitudes per lane; the reverse holds psi and lambda and stops at 4)
   std::vector<Group> fgroups;
   std::vector<int> fgrp_start(ng, -1);
   {
@@ -1174,7 +1175,7 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     for (uint32_t g = 0; g < ng;) {
       uint32_t hi = g;
       uint64_t used = 0;
-      while (hi < ng && hi - g < 4 && fok(hi)) {
+      while (hi < ng && hi - g < 5 && fok(hi)) {
         const int b = fbit(hi);
         if ((used >> b) & 1) break;
         if (hi > g && (b >= 4) != (fbit(g) >= 4)) break;  // one kind per group

@@ -844,12 +844,15 @@ static void launch_lanes_t(void *psi, void *lam, uint64_t n, const int *bits, co
       a.dg[j].i[e] = (R)dgg[8 * j + 2 * e + 1];
     }
   }
-  if (lam)
-    hipLaunchKernelGGL((k_adjoint_step_lanes<T, R, K, true>), dim3(nb), dim3(kBlock), 0, st,
-                       (T *)psi, (T *)lam, n, a, partial);
-  else
-    hipLaunchKernelGGL((k_adjoint_step_lanes<T, R, K, false>), dim3(nb), dim3(kBlock), 0, st,
-                       (T *)psi, (T *)nullptr, n, a, partial);
+  if constexpr (K <= 4) {
+    if (lam) {
+      hipLaunchKernelGGL((k_adjoint_step_lanes<T, R, K, true>), dim3(nb), dim3(kBlock), 0, st,
+                         (T *)psi, (T *)lam, n, a, partial);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_adjoint_step_lanes<T, R, K, false>), dim3(nb), dim3(kBlock), 0, st,
+                     (T *)psi, (T *)nullptr, n, a, partial);
 }
 
 template <typename T, typename R, int K>
@@ -872,19 +875,23 @@ static void launch_group_t(void *psi, void *lam, uint64_t n, const int *bits, co
       if ((r >> (K - 1 - j)) & 1) o += 1ull << bits[j];
     a.off[r] = o;
   }
-  if (lam)
-    hipLaunchKernelGGL((k_adjoint_step_group<T, R, K, true>), dim3(nb), dim3(kBlock), 0, st,
-                       (T *)psi, (T *)lam, n >> K, make_idxmap(ins, 0), a, partial);
-  else
-    hipLaunchKernelGGL((k_adjoint_step_group<T, R, K, false>), dim3(nb), dim3(kBlock), 0, st,
-                       (T *)psi, (T *)nullptr, n >> K, make_idxmap(ins, 0), a, partial);
+  if constexpr (K <= 4) {  // reverse groups hold psi and lambda: at most 4 gates
+    if (lam) {
+      hipLaunchKernelGGL((k_adjoint_step_group<T, R, K, true>), dim3(nb), dim3(kBlock), 0, st,
+                         (T *)psi, (T *)lam, n >> K, make_idxmap(ins, 0), a, partial);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_adjoint_step_group<T, R, K, false>), dim3(nb), dim3(kBlock), 0, st,
+                     (T *)psi, (T *)nullptr, n >> K, make_idxmap(ins, 0), a, partial);
 }
 
 int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int k,
                               const int *bits, const double *ginv, const double *dgg,
                               const int *slot_param_dev, bool distinct, double *partial,
                               double *out_dev, hipStream_t st) {
-  if (k < 2 || k > 4) return fail(SQ_ERR_INVALID, "internal: gate group size out of range");
+  if (k < 2 || k > (lam ? 4 : 5))
+    return fail(SQ_ERR_INVALID, "internal: gate group size out of range");
   // bits all below 6: the lane kernel (cross-lane pairs); bits all above 3:
   // the register kernel (a lane holds each group of 2^k amplitudes)
   int lo = 64, hi = -1;
@@ -917,6 +924,7 @@ int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, i
     SQ_GROUP(2)
     SQ_GROUP(3)
     SQ_GROUP(4)
+    SQ_GROUP(5)
   }
 #undef SQ_GROUP
   if (lam)
