@@ -250,14 +250,16 @@ __global__ void __launch_bounds__(kBlock)
 // product, formed in FP64 (one rounding per amplitude for complex64):
 //   - gates on bits below 10 ("local"): their product is formed once per
 //     thread and u, before the chunk loop;
-//   - gates on bits 10 and above ("uniform"): one factor per chunk;
+//   - gates on bits 10 and above ("uniform"): one factor per chunk, formed
+//     across a wave's lanes (one gate per lane) and a product over the wave;
 //   - gates with one bit on each side ("mixed"): per amplitude.
 // Derivative terms Re sum_i conj(lam_i) psi_i dgg(i) use w = conj(lam) psi,
 // which the run does not change (psi and lambda get the same unit-modulus
 // phase), so for a local gate the sum is sum_u Re dgg(t, u) W_u with W_u the
 // thread's sum of w over its chunks (one wave sum per gate at the end); a
 // uniform gate takes dgg(chunk) times the wave's sum of w in that chunk (one
-// wave sum per chunk for all of them); a mixed gate is summed per chunk.
+// wave sum per chunk for all of them, each lane adding its own gate's term:
+// slots are distinct per gate); a mixed gate is summed per chunk.
 __device__ __forceinline__ int diag_loc(const DiagRunGate &g, uint64_t i) {
   return g.b1 < 0 ? (int)((i >> g.b0) & 1)
                   : (int)((((i >> g.b0) & 1) << 1) | ((i >> g.b1) & 1));
@@ -290,6 +292,24 @@ __global__ void __launch_bounds__(kBlock)
   auto is_local = [](const DiagRunGate &g) {
     return g.b0 < kLocBits && (g.b1 < 0 || g.b1 < kLocBits);
   };
+  // the uniform gates (at most 64: one per lane) and the mixed ones
+  __shared__ unsigned char uni[kMaxDiagRun], mix[kMaxDiagRun];
+  __shared__ int nuni, nmix, uslots;
+  if (threadIdx.x == 0) {
+    int a = 0, b = 0, any = 0;
+    for (int k = 0; k < m; ++k) {
+      if (sg[k].uniform) {
+        uni[a++] = (unsigned char)k;
+        any |= sg[k].slot >= 0;
+      } else if (!is_local(sg[k])) {
+        mix[b++] = (unsigned char)k;
+      }
+    }
+    nuni = a;
+    nmix = b;
+    uslots = any;
+  }
+  __syncthreads();
   // phase product of the local gates at local index t + kBlock u
   double lr[U], li[U];
 #pragma unroll
@@ -337,35 +357,39 @@ __global__ void __launch_bounds__(kBlock)
       pr[u] = lr[u];
       pi[u] = li[u];
     }
+    // chunk factor: lane j takes uniform gate j, then a product over the
+    // wave (lane 0's value for every lane: one rounding order)
     double cr = 1.0, ci = 0.0;
-    double swr = 0.0, swi = 0.0;  // the wave's sum of w in this chunk, formed on first use
-    bool have_sw = false;
-    for (int k = 0; k < m; ++k) {
-      const DiagRunGate &gk = sg[k];
-      if (gk.uniform) {  // the branch is uniform over the block
-        const int loc = diag_loc(gk, base);
-        const double gr = gk.ginv[loc][0], gi = gk.ginv[loc][1];
-        const double tr = cr * gr - ci * gi;
-        ci = cr * gi + ci * gr;
-        cr = tr;
-        if (LAM && gk.slot >= 0) {
-          if (!have_sw) {
-            double tr2 = 0.0, ti2 = 0.0;
+    int myk = -1, myloc = 0;
+    if (lane < nuni) {
+      myk = uni[lane];
+      myloc = diag_loc(sg[myk], base);
+      cr = sg[myk].ginv[myloc][0];
+      ci = sg[myk].ginv[myloc][1];
+    }
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-              tr2 += wr[u];
-              ti2 += wi[u];
-            }
-            swr = wave_sum_d(tr2);
-            swi = wave_sum_d(ti2);
-            have_sw = true;
-          }
-          if (lane == 0) srow[gk.slot][wave] += swr * gk.dgg[loc][0] - swi * gk.dgg[loc][1];
-        }
-        continue;
+    for (int off = 1; off < 64; off <<= 1) {
+      const double orr = __shfl_xor(cr, off), oi = __shfl_xor(ci, off);
+      const double tr = cr * orr - ci * oi;
+      ci = cr * oi + ci * orr;
+      cr = tr;
+    }
+    cr = __shfl(cr, 0);
+    ci = __shfl(ci, 0);
+    if (LAM && uslots) {  // uniform slot gates: dgg(chunk) times the wave's sum of w
+      double tr2 = 0.0, ti2 = 0.0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        tr2 += wr[u];
+        ti2 += wi[u];
       }
-      if (is_local(gk)) continue;
-      double acc = 0.0;  // mixed gate
+      const double swr = wave_sum_d(tr2), swi = wave_sum_d(ti2);
+      if (myk >= 0 && sg[myk].slot >= 0)  // slots are distinct per gate
+        srow[sg[myk].slot][wave] += swr * sg[myk].dgg[myloc][0] - swi * sg[myk].dgg[myloc][1];
+    }
+    for (int j = 0; j < nmix; ++j) {
+      const DiagRunGate &gk = sg[mix[j]];
+      double acc = 0.0;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int loc = diag_loc(gk, base + threadIdx.x + (uint64_t)kBlock * u);
