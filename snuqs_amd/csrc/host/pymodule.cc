@@ -392,7 +392,24 @@ class Program {
                     size_t num_params, const std::vector<std::string> &paulis,
                     const std::vector<std::vector<size_t>> &qubits,
                     const std::vector<double> &coeffs, std::shared_ptr<Stream> stream) {
-    sv.require_device("adjoint");
+    // The C ABI takes one size for every buffer, so each one is checked here
+    // against the program before anything is launched.
+    auto require_match = [&](const StateVector &s, const std::string &what) {
+      s.require_device(what.c_str());
+      if (s.dtype() != dt_)
+        throw py::value_error("adjoint: " + what + " has dtype " + dtype_name(s.dtype()) +
+                              ", the program " + dtype_name(dt_));
+      if (s.num_qubits() != nq_)
+        throw py::value_error("adjoint: " + what + " has " + std::to_string(s.num_qubits()) +
+                              " qubits, the program " + std::to_string(nq_));
+    };
+    require_match(sv, "state_vector");
+    require_match(w0, "work0");
+    require_match(w1, "work1");
+    for (size_t c = 0; c < checkpoints.size(); ++c) {
+      if (!checkpoints[c]) throw py::value_error("adjoint: checkpoint " + std::to_string(c) + " is None");
+      require_match(*checkpoints[c], "checkpoint " + std::to_string(c));
+    }
     if (param_map.size() != ops_.size()) throw py::value_error("param_map needs one entry per gate");
     PauliSum ps = make_pauli_sum(nq_, paulis, qubits, coeffs);
     std::vector<uint32_t> dg, dp;

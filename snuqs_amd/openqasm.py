@@ -162,8 +162,16 @@ class Dual:
     def __pow__(self, o):
         o = Dual._lift(o)
         v = self.v ** o.v
+        if self.v != 0.0:
+            dbase = o.v * self.v ** (o.v - 1)
+        elif o.v > 1.0 or o.v == 0.0:
+            dbase = 0.0  # d/dx x^e at x = 0: 0 for e > 1; x^0 is constant
+        elif o.v == 1.0:
+            dbase = 1.0
+        else:
+            dbase = math.inf  # e < 1: the slope at 0 is unbounded
         dlog = math.log(self.v) if self.v > 0 else 0.0
-        return self._comb(o, o.v * self.v ** (o.v - 1) if self.v else 0.0, v * dlog, v)
+        return self._comb(o, dbase, v * dlog, v)
 
     def __rpow__(self, o):
         return Dual._lift(o) ** self
@@ -172,7 +180,13 @@ class Dual:
         return Dual(-self.v, {k: -x for k, x in self.d.items()})
 
     def __mod__(self, o):
-        return Dual(self.v % float(Dual._lift(o).v), self.d)
+        # a % b = a - b floor(a/b) (Python's floored remainder), so
+        # d(a % b) = da - floor(a/b) db away from the jumps
+        o = Dual._lift(o)
+        return self._comb(o, 1.0, -math.floor(self.v / o.v), self.v % o.v)
+
+    def __rmod__(self, o):
+        return Dual._lift(o) % self
 
     def __float__(self):
         return self.v

@@ -150,7 +150,9 @@ class BaseSimulator:
         for r in results:
             for q in (_get(r, "targets") or []):
                 used.add(int(q))
-        return self._simulate(ir, ops, results, used, shots, None, **kwargs)
+        # with shots the reference returns no result types for JAQCD
+        # (simulator.py:497-510)
+        return self._simulate(ir, ops, results, used, shots, None, shots_results=[], **kwargs)
 
     def run_openqasm(self, ir, *, shots: int = 0, inputs: Optional[dict] = None, **kwargs):
         """OpenQASM 3 program (simulator.py:380-437): gates, result pragmas and
@@ -166,8 +168,11 @@ class BaseSimulator:
         if shots > 0:
             ops += openqasm.basis_rotations(circuit.results, sorted(circuit.qubit_set))
         ctx = {"deps": list(circuit.input_deps), "inputs": list(circuit.inputs)}
+        # with shots the reference returns the program's result list itself
+        # (simulator.py:437-447)
         return self._simulate(ir, ops, circuit.results, set(circuit.qubit_set), shots,
-                              circuit.measured_qubits or None, adjoint_ctx=ctx, **kwargs)
+                              circuit.measured_qubits or None, adjoint_ctx=ctx,
+                              shots_results=list(circuit.results), **kwargs)
 
     def _simulate(self, ir, ops, results, used, shots, measured_qubits, *,
                   qubit_count_cpu: Optional[int] = None, qubit_count_cuda: Optional[int] = None,
@@ -175,7 +180,7 @@ class BaseSimulator:
                   prefetch: Optional[str] = None, offload: Optional[str] = None,
                   path=None, block_count: Optional[int] = None, qubit_count: Optional[int] = None,
                   precision: str = "complex128", fuse: int = 5, exact_sampling: bool = True,
-                  adjoint_ctx: Optional[dict] = None):
+                  adjoint_ctx: Optional[dict] = None, shots_results: Optional[list] = None):
         qmap = self._contiguous_qubit_mapping(used)
         for op in ops:
             op.targets = [qmap[q] for q in op.targets]
@@ -196,11 +201,7 @@ class BaseSimulator:
             raise ValueError("adjoint_gradient needs shots=0")
         if not shots and "sample" in kinds:
             raise ValueError("the Sample result type needs shots > 0")
-        if shots:
-            # as simulator.py:380-437: with shots the result types go back without
-            # values (the client derives them from the measurements)
-            values = [{"type": r, "value": None} for r in results]
-        else:
+        if not shots:
             for r in results:
                 if _get(r, "type") == "adjoint_gradient":
                     v = self._adjoint_gradient(r, sim, in_order, adjoint_ctx, qmap, qubit_count,
@@ -222,12 +223,14 @@ class BaseSimulator:
             return GateModelTaskResult.construct(
                 taskMetadata=TaskMetadata(**meta),
                 additionalMetadata=AdditionalMetadata(action=ir),
-                resultTypes=[ResultTypeValue.construct(type=v["type"], value=v["value"])
-                             for v in values],
+                resultTypes=(list(shots_results or []) if shots else
+                             [ResultTypeValue.construct(type=v["type"], value=v["value"])
+                              for v in values]),
                 measurements=measurements, measuredQubits=measured)
         except ImportError:
             return TaskResult(measurements=measurements, measuredQubits=measured,
-                              resultTypes=values, taskMetadata=meta)
+                              resultTypes=list(shots_results or []) if shots else values,
+                              taskMetadata=meta)
 
     @staticmethod
     def _calculate(r, sim: StateVectorSimulation, qmap, n) -> Any:

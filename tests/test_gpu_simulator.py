@@ -105,6 +105,7 @@ def test_shots_formatting_and_errors():
     sim = StateVectorSimulator()
     res = sim.run(ghz_program(4), shots=100)
     assert len(res.measurements) == 100
+    assert res.resultTypes == []  # JAQCD with shots: no result types (simulator.py:497-510)
     assert all("".join(m) in ("0000", "1111") for m in res.measurements)
     with pytest.raises(TypeError):
         sim.run(ghz_program(2), accelerator="cpu")
@@ -192,11 +193,12 @@ def test_run_openqasm_shots_and_measured_qubits():
            "#pragma braket result expectation x all\n")
     res = sim.run(ghz, shots=500)
     assert all(sum(int(ch) for ch in m) % 2 == 0 for m in res.measurements)
-    # Sample of X x X x Y: the rotation runs before sampling; the result type
-    # returns without a value (derived from the measurements by the client)
+    # Sample of X x X x Y: the rotation runs before sampling; with shots the
+    # program's result list comes back unchanged (simulator.py:437-447), the
+    # values being derived from the measurements by the client
     ghz_s = ghz.replace("expectation x all", "sample x(q[0]) @ x(q[1]) @ y(q[2])")
     res = sim.run(ghz_s, shots=400)
-    assert res.resultTypes[0]["value"] is None and len(res.measurements) == 400
+    assert res.resultTypes == parse(ghz_s).results and len(res.measurements) == 400
     with pytest.raises(ValueError):
         sim.run(ghz_s, shots=0)
 

@@ -265,7 +265,9 @@ def test_dual_numbers_differentiate_angle_expressions():
     exprs = ["2*theta", "sin(phi)", "theta/phi", "theta**2", "exp(-theta)*cos(phi)",
              "sqrt(phi) - arctan(theta)", "2**theta", "-(theta - 3*phi)/4", "log(phi)*π",
              "arcsin(theta) + arccos(theta/2) + tan(phi)", "abs(theta - phi) + floor(phi)",
-             "pow(phi, 3) + mod(theta, 0.2) + ceiling(theta)"]
+             "pow(phi, 3) + mod(theta, 0.2) + ceiling(theta)",
+             # the divisor's term of a % b: -floor(a/b) db
+             "mod(2.0, theta)", "2.0 % theta", "phi % theta", "mod(theta + 1, phi)"]
     x0 = {"theta": 0.3, "phi": 0.7}
     h = 1e-6
 
@@ -281,6 +283,20 @@ def test_dual_numbers_differentiate_angle_expressions():
             dn, _ = angle(expr, {**x0, name: x0[name] - h})
             fd = (up - dn) / (2 * h)
             assert abs(d.get(name, 0.0) - fd) < 1e-7, (expr, name, d, fd)
+
+
+def test_dual_power_at_base_zero():
+    """x**e at x = 0: slope 1 for e = 1, 0 for e > 1 or e = 0, unbounded below 1."""
+    from snuqs_amd.openqasm import Dual
+
+    x = Dual(0.0, {"t": 1.0})
+    assert (x ** 1).d["t"] == 1.0
+    assert (x ** 2).d["t"] == 0.0
+    assert (x ** 0).d["t"] == 0.0
+    assert (x ** 0.5).d["t"] == float("inf")
+    assert (x ** 1).v == 0.0 and (x ** 0).v == 1.0
+    # the exponent's derivative at base 0 (e > 0) is 0
+    assert (0.0 ** Dual(2.0, {"e": 1.0})).d["e"] == 0.0
 
 
 def test_input_dependencies_through_modifiers_and_user_gates():

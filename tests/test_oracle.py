@@ -206,3 +206,96 @@ def test_probabilities_match_numpy_form():
     sv = rand_state(10, rng)
     p, want = O.probabilities(sv), np.abs(sv) ** 2
     assert np.max(np.abs(p - want) / want) < 1e-15
+
+
+def test_inner_product_conjugates_its_first_argument():
+    """sqo_inner is <a|b> = sum conj(a_i) b_i, the convention of np.vdot.
+    Checked on two different states with a large imaginary part, so that a
+    conjugate on the wrong side (the planted <b|a>) or no conjugate at all
+    fails the same comparison."""
+    rng = np.random.default_rng(11)
+    n = 9
+    a = rand_state(n, rng)
+    b = a * np.exp(0.7j) + 0.5 * rand_state(n, rng)  # <a|b> ~ exp(0.7i)
+    b /= np.linalg.norm(b)
+    want = np.vdot(a, b)
+    assert abs(want.imag) > 0.3
+    assert abs(O.inner(a, b) - want) < 1e-14
+    # planted mistakes must fail the same bar
+    assert abs(O.inner(b, a) - want) > 0.5           # conjugate-swap
+    assert abs(complex(np.sum(a * b)) - want) > 0.5  # no conjugate
+    # linearity in the second argument, antilinearity in the first
+    c = 0.3 - 1.7j
+    assert abs(O.inner(a, c * b) - c * want) < 1e-13
+    assert abs(O.inner(c * a, b) - np.conj(c) * want) < 1e-13
+
+
+class _Op:
+    """A gate stand-in carrying only what the reorder reads."""
+
+    def __init__(self, name, targets):
+        self.name, self.targets = name, list(targets)
+
+    def sliceable(self):
+        return False
+
+
+def _order(ops):
+    return [op.name for op in O.pseudo_sort_operations_descending(ops)]
+
+
+# Expected orders worked by hand from transpile.py:33-54 (pseudo_gt) and
+# :74-89 (the insertion sort).  The sort passes (op, i) and (op, j) with
+# j < i, so "pos1 < pos2" is always False there: gates sharing a qubit, or
+# two zero-target gates, stop the insertion.
+_PSEUDO_SORT_CASES = [
+    # disjoint one-target gates: descending by target
+    ("disjoint", [("a", [0]), ("b", [2]), ("c", [1])], ["b", "c", "a"]),
+    # disjoint, min differs from max: B (min 1) passes A (min 0); by max
+    # (2 vs 3) it would not
+    ("min-not-max", [("A", [0, 3]), ("B", [1, 2])], ["B", "A"]),
+    # insertion stops at the first gate that is not smaller
+    ("stops", [("A", [3]), ("B", [1]), ("C", [2])], ["A", "C", "B"]),
+    # zero targets on one side: the global phase moves to the front, and a
+    # later gate does not pass it
+    ("gphase", [("X0", [0]), ("G", []), ("H1", [1])], ["G", "H1", "X0"]),
+    # two zero-target gates keep their order
+    ("two-gphase", [("G1", []), ("X0", [0]), ("G2", [])], ["G1", "G2", "X0"]),
+    # a shared qubit blocks the move: C stays after B
+    ("shared", [("A", [0]), ("B", [2]), ("C", [2])], ["B", "C", "A"]),
+    # shared qubit between two-target gates, with a free gate passing both
+    ("shared2", [("X0", [0]), ("CX01", [0, 1]), ("H2", [2])], ["H2", "X0", "CX01"]),
+]
+
+
+@pytest.mark.parametrize("name,ops,want", _PSEUDO_SORT_CASES, ids=[c[0] for c in _PSEUDO_SORT_CASES])
+def test_pseudo_sort_hand_worked_orders(name, ops, want):
+    assert _order([_Op(nm, t) for nm, t in ops]) == want
+
+
+def _planted_gt(kind):
+    def gt(op1, pos1, op2, pos2, sliceable=lambda op: False):
+        t1, t2 = op1.targets, op2.targets
+        if len(t1) == 0 and len(t2) == 0:
+            return pos1 < pos2
+        if len(t1) == 0:
+            return kind != "zero-flip"
+        if len(t2) == 0:
+            return kind == "zero-flip"
+        if set(t1) & set(t2):
+            return True if kind == "shared-moves" else pos1 < pos2
+        if kind == "max":
+            return max(t1) > max(t2)
+        if kind == "lt":
+            return min(t1) < min(t2)
+        return min(t1) > min(t2)
+    return gt
+
+
+@pytest.mark.parametrize("kind", ["max", "lt", "zero-flip", "shared-moves"])
+def test_pseudo_sort_planted_mistakes_fail(kind, monkeypatch):
+    """Each planted slip in pseudo_gt changes at least one hand-worked order."""
+    monkeypatch.setattr(O, "_pseudo_gt", _planted_gt(kind))
+    failed = [name for name, ops, want in _PSEUDO_SORT_CASES
+              if _order([_Op(nm, t) for nm, t in ops]) != want]
+    assert failed, kind

@@ -44,3 +44,13 @@ def test_fusion_of_layered_circuit_uses_five_qubit_blocks():
     fused = fuse_operations(ops, 5)
     assert max(len(op.targets) for op in fused) == 5
     assert np.abs(run_oracle(n, fused) - run_oracle(n, ops)).max() < 1e-12
+
+
+def test_pseudo_sort_hand_worked_orders_product():
+    """The product's reorder on the hand-worked cases that pin the oracle."""
+    from test_oracle import _PSEUDO_SORT_CASES, _Op
+    from snuqs_amd.transpile import pseudo_sort_operations_descending
+
+    for name, ops, want in _PSEUDO_SORT_CASES:
+        got = pseudo_sort_operations_descending([_Op(nm, t) for nm, t in ops])
+        assert [op.name for op in got] == want, name
