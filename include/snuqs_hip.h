@@ -127,6 +127,16 @@ int sq_program_create(void **program, sq_dtype_t dtype, uint32_t num_qubits,
                       uint32_t num_gates, const uint32_t *num_targets,
                       const uint32_t *targets, const double *matrices);
 int sq_program_destroy(void *program);
+/* New matrices for the same gates (no reference counterpart: the reference
+ * builds new GateOperation objects, and their device copies, for every
+ * parameter set, operation.cc:96-127).  `matrices` has the layout of
+ * sq_program_create; every gate keeps its targets.  Allocates nothing: a
+ * gate of 4-5 targets owns fixed device slots from creation.  The device
+ * copy is ordered after the work already queued on `stream` and the call
+ * returns when it is done; work on other streams that runs this program must
+ * have finished.  A captured hipGraph is dropped (recaptured on the next
+ * sq_program_run_graph); tuned launch shapes are kept. */
+int sq_program_set_matrices(void *program, const double *matrices, void *stream);
 /* Number of kernels a run launches and its algorithmic HBM bytes
  * (2 x touched amplitudes x itemsize, summed over the gates). */
 int sq_program_stats(void *program, uint64_t *num_launches,

@@ -139,13 +139,15 @@ void sqo_expect_pauli_c128(const double complex *sv, size_t n, uint64_t xmask,
                            uint64_t zmask, int ny, double *out) {
   static const double complex ipow[4] = {1.0, I, -1.0, -I};
   double complex ph = ipow[ny & 3];
-  double complex acc = 0;
+  double re = 0.0, im = 0.0; /* FP64 partial sums per thread */
+#pragma omp parallel for reduction(+ : re, im)
   for (size_t i = 0; i < n; ++i) {
     double complex t = conj(sv[i ^ xmask]) * sv[i];
     if (popcount64(i & zmask) & 1) t = -t;
-    acc += t;
+    re += creal(t);
+    im += cimag(t);
   }
-  acc *= ph;
+  double complex acc = (re + I * im) * ph;
   out[0] = creal(acc);
   out[1] = cimag(acc);
 }

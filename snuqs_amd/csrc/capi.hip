@@ -292,7 +292,9 @@ struct Program {
   std::vector<double> mats;            // the gates' complex128 matrices (as given)
   std::vector<size_t> moffs;           // offset of a gate's matrix in mats
   std::vector<const void *> fwd_dev, inv_dev;  // real-block matrices (kk >= 4)
+  std::vector<size_t> slot;            // byte offset of a gate's block slots, or -1
   void *dev_mats = nullptr;
+  size_t dev_cap = 0;                  // bytes of dev_mats
   uint64_t launches = 0;
   double bytes = 0.0;
   // hipGraph of the forward launch sequence on one state (sq_program_run_graph)
@@ -309,42 +311,107 @@ static void drop_graph(Program *p) {
   p->graph_sv = nullptr;
 }
 
-// Upload the real-block matrices of the dense 4-5 target plans in `plans`.
-static int upload_blocks(const std::vector<const GatePlan *> &plans, sq_dtype_t dt,
-                         void **dev_base, std::vector<const void *> *ptrs) {
-  const size_t esz = dt == SQ_COMPLEX64 ? 4 : 8;
-  std::vector<size_t> off(plans.size(), (size_t)-1);
+// Real-block matrices of dense 4-5 target plans (layout of fused_block_3m).
+// `off[i]` is plan i's byte offset in the image, or -1 for a plan that keeps
+// its matrix in kernel arguments; the image is `total` bytes.
+static size_t block_bytes(int kk, sq_dtype_t dt) {
+  const size_t D = (size_t)2 << kk;
+  return ((D * D * (dt == SQ_COMPLEX64 ? 4 : 8) + 255) / 256) * 256;
+}
+static bool has_block(const GatePlan &p) { return p.kind == Kind::DENSE && p.kk >= 4; }
+
+// Offsets packed by the plans' own sizes (derivative matrices of one call).
+static size_t block_layout(const std::vector<const GatePlan *> &plans, sq_dtype_t dt,
+                           std::vector<size_t> *off) {
+  off->assign(plans.size(), (size_t)-1);
   size_t total = 0;
-  for (size_t i = 0; i < plans.size(); ++i) {
-    const GatePlan &p = *plans[i];
-    if (p.kind == Kind::DENSE && p.kk >= 4) {
-      const size_t D = (size_t)2 << p.kk;
-      off[i] = total;
-      total += ((D * D * esz + 255) / 256) * 256;
+  for (size_t i = 0; i < plans.size(); ++i)
+    if (has_block(*plans[i])) {
+      (*off)[i] = total;
+      total += block_bytes(plans[i]->kk, dt);
     }
-  }
-  ptrs->assign(plans.size(), nullptr);
-  *dev_base = nullptr;
-  if (total == 0) return SQ_OK;
-  std::vector<char> host(total);
+  return total;
+}
+
+// Host image of the blocks at the given offsets.
+static void block_image(const std::vector<const GatePlan *> &plans, sq_dtype_t dt,
+                        const std::vector<size_t> &off, char *img) {
   for (size_t i = 0; i < plans.size(); ++i) {
-    if (off[i] == (size_t)-1) continue;
+    if (off[i] == (size_t)-1 || !has_block(*plans[i])) continue;
     std::vector<double> g = fused_block_3m(*plans[i]);
     if (dt == SQ_COMPLEX64) {
-      float *h = (float *)(host.data() + off[i]);
+      float *h = (float *)(img + off[i]);
       for (size_t e = 0; e < g.size(); ++e) h[e] = (float)g[e];
     } else {
-      std::memcpy(host.data() + off[i], g.data(), g.size() * 8);
+      std::memcpy(img + off[i], g.data(), g.size() * 8);
     }
   }
-  if (hipMalloc(dev_base, total) != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(SQ_ERR_OOM, "program matrices: device allocation failed");
+}
+
+// Plans of a program's gates from `matrices` (the create-time layout: gate g's
+// row-major complex128 matrix at 2 * sum_{h<g} 4^nt[h] doubles), its dagger
+// for the reverse sweep, and the totals.  Launch shapes tuned for a gate are
+// kept while its plan stays a 5-target block.
+static int plan_program(Program *prog, const double *matrices) {
+  const size_t ng = prog->nt.size();
+  std::vector<GatePlan> fwd(ng), inv(ng);
+  uint64_t launches = 0;
+  double bytes = 0.0;
+  size_t moff = 0;
+  prog->moffs.resize(ng);
+  for (size_t g = 0; g < ng; ++g) {
+    const int k = prog->nt[g];
+    prog->moffs[g] = moff;
+    const uint32_t *t = prog->targets.data() + prog->toff[g];
+    const double *m = matrices + moff;
+    SQ_TRY(make_plan(m, t, k, prog->nq, prog->nq, &fwd[g]));
+    std::vector<double> md = dagger(m, k);
+    SQ_TRY(make_plan(md.data(), t, k, prog->nq, prog->nq, &inv[g]));
+    if (g < prog->fwd.size() && has_block(prog->fwd[g]) && has_block(fwd[g]) &&
+        fwd[g].kk == prog->fwd[g].kk) {
+      fwd[g].shape = prog->fwd[g].shape;
+      inv[g].shape = prog->inv[g].shape;
+    }
+    moff += 2ull << (2 * k);
+    if (fwd[g].kind != Kind::NOOP) {
+      launches++;
+      bytes += 2.0 * fwd[g].touched * std::ldexp(1.0, prog->nq) * itemsize(prog->dt);
+    }
   }
-  SQ_HIP(hipMemcpy(*dev_base, host.data(), total, hipMemcpyHostToDevice));
-  for (size_t i = 0; i < plans.size(); ++i)
-    if (off[i] != (size_t)-1) (*ptrs)[i] = (const char *)*dev_base + off[i];
+  prog->fwd.swap(fwd);
+  prog->inv.swap(inv);
+  prog->mats.assign(matrices, matrices + moff);
+  prog->launches = launches;
+  prog->bytes = bytes;
   return SQ_OK;
+}
+
+// Host image of a program's blocks in its fixed slots: every gate of 4 or more
+// targets owns a 5-target slot for its plan and one for its dagger, so new
+// matrices for the same gates never change the layout (sq_program_set_matrices
+// allocates nothing).  Sets the per-plan device pointers.
+static std::vector<char> program_image(Program *p) {
+  const size_t ng = p->fwd.size();
+  std::vector<char> img(p->dev_cap);
+  std::vector<const GatePlan *> plans;
+  std::vector<size_t> off;
+  for (size_t g = 0; g < ng; ++g) {
+    plans.push_back(&p->fwd[g]);
+    off.push_back(p->slot[g]);
+  }
+  for (size_t g = 0; g < ng; ++g) {
+    plans.push_back(&p->inv[g]);
+    off.push_back(p->slot[g] == (size_t)-1 ? (size_t)-1 : p->slot[g] + block_bytes(kMaxK, p->dt));
+  }
+  block_image(plans, p->dt, off, img.data());
+  p->fwd_dev.assign(ng, nullptr);
+  p->inv_dev.assign(ng, nullptr);
+  for (size_t i = 0; i < plans.size(); ++i) {
+    if (off[i] == (size_t)-1 || !has_block(*plans[i])) continue;
+    const void *d = (const char *)p->dev_mats + off[i];
+    if (i < ng) p->fwd_dev[i] = d; else p->inv_dev[i - ng] = d;
+  }
+  return img;
 }
 
 }  // namespace sq
@@ -362,37 +429,52 @@ int sq_program_create(void **program, sq_dtype_t dt, uint32_t nq, uint32_t ng,
   prog->dt = dt;
   prog->nq = (int)nq;
   SQ_TRY(current_device(&prog->device));
-  size_t toff = 0, moff = 0;
-  prog->fwd.resize(ng);
-  prog->inv.resize(ng);
+  size_t toff = 0;
   for (uint32_t g = 0; g < ng; ++g) {
     const int k = (int)num_targets[g];
     if (k > kMaxK) return fail(SQ_ERR_UNSUPPORTED, "gate with more than 5 targets");
-    const uint32_t *t = targets + toff;
-    const double *m = matrices + moff;
-    SQ_TRY(make_plan(m, t, k, (int)nq, (int)nq, &prog->fwd[g]));
-    std::vector<double> md = dagger(m, k);
-    SQ_TRY(make_plan(md.data(), t, k, (int)nq, (int)nq, &prog->inv[g]));
     prog->nt.push_back(k);
     prog->toff.push_back(toff);
-    prog->targets.insert(prog->targets.end(), t, t + k);
-    prog->moffs.push_back(prog->mats.size());
-    prog->mats.insert(prog->mats.end(), m, m + (2ull << (2 * k)));
+    prog->targets.insert(prog->targets.end(), targets + toff, targets + toff + k);
     toff += k;
-    moff += 2ull << (2 * k);
-    if (prog->fwd[g].kind != Kind::NOOP) {
-      prog->launches++;
-      prog->bytes += 2.0 * prog->fwd[g].touched * std::ldexp(1.0, (int)nq) * itemsize(dt);
+  }
+  SQ_TRY(plan_program(prog.get(), matrices));
+  for (uint32_t g = 0; g < ng; ++g) {
+    prog->slot.push_back(prog->nt[g] >= 4 ? prog->dev_cap : (size_t)-1);
+    if (prog->nt[g] >= 4) prog->dev_cap += 2 * block_bytes(kMaxK, dt);
+  }
+  if (prog->dev_cap) {
+    if (hipMalloc(&prog->dev_mats, prog->dev_cap) != hipSuccess) {
+      (void)hipGetLastError();
+      prog->dev_mats = nullptr;
+      return fail(SQ_ERR_OOM, "program matrices: device allocation failed");
     }
   }
-  std::vector<const GatePlan *> all;
-  for (auto &p : prog->fwd) all.push_back(&p);
-  for (auto &p : prog->inv) all.push_back(&p);
-  std::vector<const void *> ptrs;
-  SQ_TRY(upload_blocks(all, dt, &prog->dev_mats, &ptrs));
-  prog->fwd_dev.assign(ptrs.begin(), ptrs.begin() + ng);
-  prog->inv_dev.assign(ptrs.begin() + ng, ptrs.end());
+  std::vector<char> img = program_image(prog.get());
+  if (prog->dev_cap) {
+    if (hipMemcpy(prog->dev_mats, img.data(), prog->dev_cap, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(prog->dev_mats);
+      return fail(SQ_ERR_HIP, "program matrices: upload failed");
+    }
+  }
   *program = prog.release();
+  return SQ_OK;
+}
+
+int sq_program_set_matrices(void *program, const double *matrices, void *stream) {
+  Program *p = (Program *)program;
+  if (!p) return fail(SQ_ERR_INVALID, "null program");
+  if (!matrices && !p->nt.empty()) return fail(SQ_ERR_INVALID, "null matrices");
+  hipStream_t st = (hipStream_t)stream;
+  // the blocks may still be read by work queued on `stream`; the new image is
+  // copied behind it, and the call returns once the copy is done
+  SQ_TRY(plan_program(p, matrices));
+  drop_graph(p);  // kernel arguments (1-3 target matrices) changed
+  std::vector<char> img = program_image(p);
+  if (p->dev_cap) {
+    SQ_HIP(hipMemcpyAsync(p->dev_mats, img.data(), p->dev_cap, hipMemcpyHostToDevice, st));
+    SQ_HIP(hipStreamSynchronize(st));
+  }
   return SQ_OK;
 }
 
@@ -952,17 +1034,12 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     moff += 2ull << (2 * k);
     by_gate[g].push_back(d);
   }
+  // derivative plans of 4-5 target gates keep their blocks in the workspace
   std::vector<const GatePlan *> dptr;
   for (auto &dp : dplans) dptr.push_back(&dp);
-  void *dmat_dev = nullptr;
-  std::vector<const void *> dmat_ptr;
-  SQ_TRY(upload_blocks(dptr, dt, &dmat_dev, &dmat_ptr));
-  struct Free {
-    void *q;
-    ~Free() {
-      if (q) (void)hipFree(q);
-    }
-  } free_guard{dmat_dev};
+  std::vector<size_t> dblk_off;
+  const size_t dblk_b = block_layout(dptr, dt, &dblk_off);
+  std::vector<const void *> dmat_ptr(num_derivs, nullptr);
 
   // checkpoint c holds the state before gate bound[c]
   std::vector<uint32_t> bound(num_checkpoints);
@@ -1218,8 +1295,8 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   const size_t edev = expect_ws_dev(nterms);
   const size_t apb = nterms * pauli_apply_term_bytes() + 256;
   Workspace *w;
-  SQ_TRY(ws_reserve(dev, edev + grad_b + apb + rg_b + rs_b + rp_b + 4096,
-                    nterms * 64 + grad_b + apb + rg_b + rs_b + 4096, &w));
+  SQ_TRY(ws_reserve(dev, edev + grad_b + apb + rg_b + rs_b + rp_b + dblk_b + 4096,
+                    nterms * 64 + grad_b + apb + rg_b + rs_b + dblk_b + 4096, &w));
   hipStream_t st = (hipStream_t)stream;
   char *dbase = (char *)w->dev;
   double *partial = (double *)dbase;                      // reduction partials
@@ -1230,11 +1307,20 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   DiagRunGate *rgates_dev = (DiagRunGate *)al((char *)aterms + apb);
   int *rslots_dev = (int *)al((char *)rgates_dev + rg_b);
   double *rpartial = (double *)al((char *)rslots_dev + rs_b);
-  if (any_runs) {  // staged through the pinned host buffer, after the gradient slots
-    char *hs = (char *)al((char *)w->host + apb + grad_b);
+  char *dblk_dev = al((char *)rpartial + rp_b);           // derivative blocks
+  // staged through the pinned host buffer, after the gradient slots
+  char *hs = (char *)al((char *)w->host + apb + grad_b);
+  if (any_runs) {
     std::memcpy(hs, rgates.data(), rgates.size() * sizeof(DiagRunGate));
     std::memcpy(hs + rg_b, rslots.data(), rslots.size() * sizeof(int));
     SQ_HIP(hipMemcpyAsync(rgates_dev, hs, rg_b + rs_b, hipMemcpyHostToDevice, st));
+  }
+  if (dblk_b) {
+    char *hb = al(hs + rg_b + rs_b);
+    block_image(dptr, dt, dblk_off, hb);
+    SQ_HIP(hipMemcpyAsync(dblk_dev, hb, dblk_b, hipMemcpyHostToDevice, st));
+    for (uint32_t d = 0; d < num_derivs; ++d)
+      if (dblk_off[d] != (size_t)-1) dmat_ptr[d] = dblk_dev + dblk_off[d];
   }
 
   // forward pass with checkpoints

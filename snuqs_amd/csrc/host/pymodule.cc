@@ -338,6 +338,29 @@ class Program {
   ~Program() {
     if (prog_) sq_program_destroy(prog_);
   }
+  // New gate objects with the same targets (new parameters); no device
+  // allocation.
+  void set_gates(const std::vector<std::shared_ptr<GateOperation>> &ops,
+                 std::shared_ptr<Stream> stream) {
+    if (ops.size() != ops_.size())
+      throw py::value_error("set_gates: " + std::to_string(ops.size()) + " gates given, the program has " +
+                            std::to_string(ops_.size()));
+    std::vector<double> mats;
+    for (size_t g = 0; g < ops.size(); ++g) {
+      if (ops[g]->get_targets() != ops_[g]->get_targets() ||
+          ops[g]->num_matrix_qubits() != ops_[g]->num_matrix_qubits())
+        throw py::value_error("set_gates: gate " + std::to_string(g) + " (" +
+                              ops[g]->formatted_string() + ") has other targets than " +
+                              ops_[g]->formatted_string());
+      const double *m = mat_ptr(*ops[g]);
+      mats.insert(mats.end(), m, m + 2 * ops[g]->matrix().size());
+    }
+    {
+      py::gil_scoped_release nogil;
+      check(sq_program_set_matrices(prog_, mats.data(), sraw(stream)));
+    }
+    ops_ = ops;
+  }
   void run(StateVector &sv, bool use_graph, std::shared_ptr<Stream> stream) {
     sv.require_device("Program.run");
     if (sv.dtype() != dt_) throw py::value_error("state dtype differs from the program's");
@@ -818,6 +841,8 @@ PYBIND11_MODULE(_C, m) {
       .def("gate_info", &Program::gate_info)
       .def("__len__", &Program::num_gates)
       .def("stats", &Program::stats)
+      .def("set_gates", &Program::set_gates, py::arg("ops"), py::kw_only(),
+           py::arg("stream") = nullptr)
       .def("adjoint", &Program::adjoint, py::arg("state_vector"), py::arg("work0"),
            py::arg("work1"), py::arg("checkpoints"), py::arg("param_map"),
            py::arg("num_params"), py::arg("paulis"), py::arg("qubits"), py::arg("coeffs"),
