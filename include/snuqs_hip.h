@@ -217,6 +217,22 @@ int sq_expect_pauli_sum(const void *sv, sq_dtype_t dtype, uint64_t num_elems,
                         const uint64_t *zmask, const uint32_t *ny,
                         const double *coeffs, double *out,
                         double *term_values, void *stream);
+/* sq_expect_pauli_sum without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate memory (code under hipGraph
+ * capture, tight serving loops, hosts that forbid hipMalloc).  Same arguments,
+ * kernels and result; the partial sums and term tables live in a fixed
+ * per-device array of about 1 MiB, held until the process ends, and the host
+ * staging lives on the stack.  At most 256 terms per call (more returns
+ * SQ_ERR_INVALID).  Every call on one GPU shares that GPU's array, so calls
+ * that overlapped on one GPU would race: the library takes a per-device lock
+ * around each call and synchronises `stream` before returning, so concurrent
+ * callers wait for one another instead. */
+int sq_expect_pauli_sum_noalloc(const void *sv, sq_dtype_t dtype,
+                                uint64_t num_elems, uint32_t num_terms,
+                                const uint64_t *xmask, const uint64_t *zmask,
+                                const uint32_t *ny, const double *coeffs,
+                                double *out, double *term_values,
+                                void *stream);
 /* out = sum_t coeffs[t] P_t |in>  (device -> device, out != in). */
 int sq_apply_pauli_sum(const void *in, void *out, sq_dtype_t dtype,
                        uint64_t num_elems, uint32_t num_terms,

@@ -944,6 +944,97 @@ int sq_expect_pauli_sum(const void *sv, sq_dtype_t dt, uint64_t n, uint32_t nter
   return SQ_OK;
 }
 
+}  // extern "C"
+
+// ---- the same expectation without allocation --------------------------------
+// For callers that cannot allocate (code under hipGraph capture, serving
+// loops, hosts that forbid hipMalloc): sq_expect_pauli_sum's kernels with
+// their partial sums, term table, term factors and term values in a fixed
+// device array instead of the grow-only workspace.  Each GPU holds its own
+// copy of the array until the process ends.  Every slot is written before it
+// is read (partials by their block, values by the finaliser with beta = 0),
+// so nothing needs clearing between calls.
+namespace sq {
+constexpr uint32_t kNoAllocMaxTerms = 256;
+constexpr size_t kNoAllocTermBytes = 64;  // room per term; >= sizeof(PauliTerm)
+constexpr size_t kNoAllocScratchBytes =
+    (size_t)(kRedBlocks * 64 + 64) * sizeof(double) +
+    kNoAllocMaxTerms * (kNoAllocTermBytes + 2 * sizeof(double)) + 4 * 256;
+}  // namespace sq
+
+static __device__ __align__(256) char scratch[sq::kNoAllocScratchBytes];
+
+namespace sq {
+static std::mutex g_scratch_mutex[64];
+static char *g_scratch_dev[64];  // scratch's address on each device, once known
+
+// Caller holds g_scratch_mutex[dev] and has dev current.
+static int scratch_base(int dev, char **base) {
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(SQ_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_scratch_dev[dev] = (char *)p;
+  }
+  *base = g_scratch_dev[dev];
+  return SQ_OK;
+}
+}  // namespace sq
+
+extern "C" {
+
+int sq_expect_pauli_sum_noalloc(const void *sv, sq_dtype_t dt, uint64_t n, uint32_t nterms,
+                                const uint64_t *x, const uint64_t *z, const uint32_t *ny,
+                                const double *coeffs, double *out, double *term_values,
+                                void *stream) {
+  SQ_TRY(check_state(sv, dt, n));
+  if (!out) return fail(SQ_ERR_INVALID, "null output");
+  if (nterms > kNoAllocMaxTerms)
+    return fail(SQ_ERR_INVALID, "expect_pauli_sum_noalloc: at most " +
+                                    std::to_string(kNoAllocMaxTerms) + " terms per call");
+  if (nterms && (!x || !z || !ny)) return fail(SQ_ERR_INVALID, "null Pauli masks");
+  SQ_TRY(check_terms(nterms, x, z, ny, n));
+  if (nterms == 0) {
+    *out = 0.0;
+    return SQ_OK;
+  }
+  const size_t tb = pauli_term_bytes();
+  if (tb > kNoAllocTermBytes) return fail(SQ_ERR_UNSUPPORTED, "internal: Pauli term too large");
+  int dev;
+  SQ_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mutex[dev]);
+  char *base;
+  SQ_TRY(scratch_base(dev, &base));
+  double *partial = (double *)base;
+  char *terms = al(base + expect_partial_bytes());
+  double *alpha = (double *)al(terms + nterms * tb);
+  double *vals = (double *)al((char *)(alpha + nterms));
+  // host staging on the stack (no heap, no pinned allocation); the call
+  // synchronises before these go out of scope
+  alignas(16) unsigned char hterms[kNoAllocMaxTerms * kNoAllocTermBytes];
+  double ones[kNoAllocMaxTerms], halpha[kNoAllocMaxTerms], hvals[kNoAllocMaxTerms];
+  for (uint32_t t = 0; t < nterms; ++t) ones[t] = 1.0;
+  fill_pauli_terms((int)nterms, x, z, ny, ones, hterms, halpha);
+  hipStream_t st = (hipStream_t)stream;
+  SQ_HIP(hipMemcpyAsync(terms, hterms, nterms * tb, hipMemcpyHostToDevice, st));
+  SQ_HIP(hipMemcpyAsync(alpha, halpha, nterms * sizeof(double), hipMemcpyHostToDevice, st));
+  for (uint32_t t0 = 0; t0 < nterms; t0 += 64) {
+    const uint32_t c = std::min<uint32_t>(64, nterms - t0);
+    SQ_TRY(launch_pauli_terms(sv, dt, n, (int)c, terms + t0 * tb, alpha + t0, partial,
+                              vals + t0, st));
+  }
+  SQ_HIP(hipMemcpyAsync(hvals, vals, nterms * sizeof(double), hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  double acc = 0.0;
+  for (uint32_t t = 0; t < nterms; ++t) acc += (coeffs ? coeffs[t] : 1.0) * hvals[t];
+  *out = acc;
+  if (term_values) std::memcpy(term_values, hvals, nterms * sizeof(double));
+  return SQ_OK;
+}
+
 int sq_apply_pauli_sum(const void *in, void *out, sq_dtype_t dt, uint64_t n, uint32_t nterms,
                        const uint64_t *x, const uint64_t *z, const uint32_t *ny,
                        const double *coeffs, void *stream) {
