@@ -178,6 +178,28 @@ def run_adjoint_bench(C, n=24, layers=8, nterms=64, weight=4, shift_params=12):
     }
 
 
+def state_error(a64, a128, chunk=1 << 26):
+    """Error of a complex64 state against the complex128 state of the same
+    circuit, in FP64: max-abs, max-abs relative to the largest amplitude,
+    norm-wise relative ||a64 - a128|| / ||a128|| and 1 - fidelity
+    |<a64|a128>|^2 / (<a64|a64><a128|a128>)."""
+    mx = amax = d2 = n128 = n64 = 0.0
+    ov = 0j
+    for s in range(0, a128.size, chunk):
+        x = a64[s:s + chunk].astype(np.complex128)
+        y = a128[s:s + chunk]
+        d = x - y
+        mx = max(mx, float(np.abs(d).max()))
+        amax = max(amax, float(np.abs(y).max()))
+        d2 += float(np.vdot(d, d).real)
+        n128 += float(np.vdot(y, y).real)
+        n64 += float(np.vdot(x, x).real)
+        ov += complex(np.vdot(x, y))
+    return {"max_abs": mx, "max_abs_over_max_amplitude": mx / amax,
+            "normwise_relative": (d2 / n128) ** 0.5,
+            "one_minus_fidelity": 1.0 - abs(ov) ** 2 / (n64 * n128)}
+
+
 def run_other_configs(C, depth):
     """The remaining configurations, one timed run each after a warm-up:
     config 2 in complex128 (30 qubits, 16 GiB), config 1 (GHZ-20 complex128,
@@ -210,7 +232,14 @@ def run_other_configs(C, depth):
     prog, sv, dt, nbytes = timed_program(30, ops, "complex128")
     out["config2_c128_30q"] = {"ms": dt * 1e3, "GBps": nbytes / dt / 1e9, "kernels": len(ops)}
     log(f"config 2 c128: {dt * 1e3:.0f} ms, {nbytes / dt / 1e9:.0f} GB/s")
-    del prog, sv
+    del prog
+    # the complex64 state of the same circuit against this complex128 one
+    sv64 = C.result_types.StateVector(C.DeviceType.CUDA, 30, dtype="complex64")
+    C.functionals.initialize_basis_z(sv64)
+    C.functionals.Program(30, ops, dtype="complex64").run(sv64)
+    out["config2_c64_vs_c128_30q"] = state_error(np.asarray(sv64.cpu()), np.asarray(sv.cpu()))
+    log("config 2 c64 vs c128:", out["config2_c64_vs_c128_30q"])
+    del sv, sv64
     # config 1: GHZ-20 complex128
     prog, sv, dt, nbytes = timed_program(20, ghz(20), "complex128", tune=False)
     a = C.functionals.amplitudes(sv, [0, (1 << 20) - 1])
@@ -239,6 +268,58 @@ def run_other_configs(C, depth):
         log(f"config 5 {dtype} {n} q: {dt * 1e3:.0f} ms, {nbytes / dt / 1e9:.0f} GB/s")
         del prog, sv
     return out
+
+
+def run_batch_bench(C, n=20, layers=8, nsets=64, nterms=64, weight=4, reps=3):
+    """Parameter batch: 64 parameter sets of a 20-qubit, 8-layer
+    hardware-efficient ansatz (320 parameters), complex64, energies of a
+    64-string weight-4 Pauli sum.  One batch = |0..0> in all 64 states, the
+    fused circuit as one launch per block over all states (hipGraph) and one
+    batched reduction; against the same 64 circuits run one after another
+    (one fused program and one expectation call per set)."""
+    from snuqs_amd.batch import ParameterBatch, fuse_operations_batch
+    from snuqs_amd.circuits import hea_ansatz, random_pauli_sum
+
+    rng = np.random.default_rng(2020)
+    thetas = rng.uniform(0, 2 * np.pi, (nsets, 2 * n * layers))
+    obs = random_pauli_sum(n, nterms, weight, rng)
+    build = lambda t: hea_ansatz(n, layers, t)[0]  # noqa: E731
+    pb = ParameterBatch(n, build, thetas, dtype="complex64")
+    e_b = pb.energies(*obs, use_graph=True)  # warm-up and capture
+    C.core.cuda.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e_b = pb.energies(*obs, use_graph=True)
+    dt_b = (time.perf_counter() - t0) / reps
+    # the same circuits one after another (programs built before the clock)
+    fused = fuse_operations_batch([build(t) for t in thetas], 5)
+    progs = [C.functionals.Program(n, f, dtype="complex64") for f in fused]
+    sv = C.result_types.StateVector(C.DeviceType.CUDA, n, dtype="complex64")
+    C.functionals.initialize_basis_z(sv)
+    progs[0].run(sv, use_graph=True)
+    C.core.cuda.device_synchronize()
+    t0 = time.perf_counter()
+    e_s = []
+    for p in progs:
+        C.functionals.initialize_basis_z(sv)
+        p.run(sv)
+        e_s.append(C.functionals.expectation(sv, *obs)[0])
+    dt_s = time.perf_counter() - t0
+    st = pb.program.stats()
+    log(f"parameter batch: {dt_b * 1e3:.1f} ms per {nsets} sets ({dt_s * 1e3:.1f} ms one by one)")
+    return {
+        "config": f"{nsets} parameter sets of a {n}-qubit hardware-efficient ansatz, {layers} layers "
+                  f"({2 * n * layers} parameters), complex64, fused into <= 5-qubit blocks; energies "
+                  f"of {nterms} random weight-{weight} Pauli strings",
+        "ms_per_batch": dt_b * 1e3,
+        "sets_per_s": nsets / dt_b,
+        "ms_sequential": dt_s * 1e3,
+        "sets_per_s_sequential": nsets / dt_s,
+        "speedup": dt_s / dt_b,
+        "launches_per_batch": st["launches"],
+        "gate_GBps": st["algorithmic_bytes"] / dt_b / 1e9,
+        "max_abs_energy_diff_batch_vs_sequential": float(np.abs(e_b - np.array(e_s)).max()),
+    }
 
 
 def run_qaoa_bench(C, reps=2):
@@ -298,6 +379,7 @@ def main():
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-qaoa", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-configs", action="store_true",
                     help="skip the config 1, config 2 complex128 and config 5 legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -385,6 +467,7 @@ def main():
     adj = None if args.skip_adjoint else run_adjoint_bench(C)
     others = None if args.skip_configs else run_other_configs(C, args.depth)
     qaoa = None if args.skip_qaoa else run_qaoa_bench(C)
+    batch = None if args.skip_batch else run_batch_bench(C)
     cpu = None if args.skip_cpu else cpu_baseline(n, args.depth, args.cpu_seconds)
 
     out = {
@@ -433,6 +516,7 @@ def main():
         "unfused": unfused,
         "adjoint": adj,
         "qaoa_batch": qaoa,
+        "parameter_batch": batch,
         "other_configs": others,
         "cpu_baseline": cpu,
     }

@@ -171,6 +171,50 @@ int sq_program_tune(void *program, void *sv, uint64_t num_elems, int reps,
 int sq_program_gate_info(void *program, uint32_t gate, int *kernel_class,
                          int *num_targets, double *algorithmic_bytes);
 
+/* ---- parameter batches (new; the reference has no batching: its
+ * optimize_operations stub, transpile.py:233-234, is where it would go).
+ * One gate structure, num_sets parameter sets: the states are num_sets
+ * consecutive blocks of num_elems = 2^num_qubits amplitudes in one device
+ * buffer (state y at states + y * num_elems), and every gate of the batch
+ * runs as ONE launch over all of them, each state with its own matrix.
+ * `matrices` holds set 0's gate matrices in the layout of sq_program_create,
+ * then set 1's, and so on; the sets must use the same targets per gate.  A
+ * gate whose sets' matrices fall into different classes (e.g. an angle of 0
+ * in one set) runs as a dense gate in every set.  1 <= num_sets <= 65535. */
+int sq_batch_create(void **batch, sq_dtype_t dtype, uint32_t num_qubits,
+                    uint32_t num_sets, uint32_t num_gates,
+                    const uint32_t *num_targets, const uint32_t *targets,
+                    const double *matrices);
+int sq_batch_destroy(void *batch);
+/* New matrices for the same gates (layout of sq_batch_create).  Ordered
+ * after the work already queued on `stream`; returns when the device copy
+ * is done.  Reallocates the batch's argument tables only when a gate's plan
+ * grows.  A captured hipGraph is dropped. */
+int sq_batch_set_matrices(void *batch, const double *matrices, void *stream);
+/* Apply every gate to every state (asynchronous on `stream`). */
+int sq_batch_run(void *batch, void *states, uint64_t num_elems, void *stream);
+/* sq_batch_run through a hipGraph captured on the first call for `states`
+ * and replayed on later calls with the same states. */
+int sq_batch_run_graph(void *batch, void *states, uint64_t num_elems,
+                       void *stream);
+/* Launches per run and algorithmic HBM bytes per run (all states). */
+int sq_batch_stats(void *batch, uint64_t *num_launches,
+                   double *algorithmic_bytes);
+/* |0...0> in each of num_sets consecutive states (one launch). */
+int sq_initialize_basis_z_batch(void *states, sq_dtype_t dtype,
+                                uint64_t num_elems, uint32_t num_sets,
+                                void *stream);
+/* out[y] = sum_t coeffs[t] <psi_y|P_t|psi_y> for the num_sets consecutive
+ * states (terms as in sq_expect_pauli_sum), FP64; term_values (host,
+ * num_sets x num_terms, may be NULL) gets the per-term values.  Launches
+ * cover every state at once.  Returns when the values are on the host. */
+int sq_expect_pauli_sum_batch(const void *states, sq_dtype_t dtype,
+                              uint64_t num_elems, uint32_t num_sets,
+                              uint32_t num_terms, const uint64_t *xmask,
+                              const uint64_t *zmask, const uint32_t *ny,
+                              const double *coeffs, double *out,
+                              double *term_values, void *stream);
+
 /* ---- results: simulation.py:196-220 (state_vector, probabilities,
  *      retrieve_samples) and the expectation/inner-product reductions that
  *      the reference lacks (result_types.py:149-151 registers StateVector

@@ -1532,3 +1532,339 @@ int sq_qaoa_stats(void *batch, uint64_t *launches, double *state_bytes, double *
 int sq_qaoa_destroy(void *batch) { return qaoa_destroy(batch); }
 
 }  // extern "C"
+
+// ---- parameter batches ---------------------------------------------------------
+// One circuit structure, many parameter sets: nsets states of 2^nq amplitudes
+// in one device allocation (state y at y * 2^nq), each evolved by the same
+// gates with its own matrices, one launch per gate for the whole batch.
+namespace sq {
+
+struct Batch {
+  sq_dtype_t dt;
+  int nq;
+  uint32_t nsets;
+  int device;
+  std::vector<int> nt;
+  std::vector<size_t> toff;
+  std::vector<uint32_t> targets;
+  std::vector<size_t> moffs;    // offset of a gate's matrix within one set's matrices
+  size_t set_doubles = 0;       // doubles of one set's matrices
+  struct Gate {
+    std::vector<GatePlan> plans;  // one per set (empty: the gate does nothing)
+    size_t tab_off = 0;           // per-set kernel arguments
+    size_t blk_off = (size_t)-1;  // per-set block matrices (4-5 targets)
+    uint64_t gstride = 0;         // elements between two sets' blocks
+  };
+  std::vector<Gate> gates;
+  void *dev = nullptr;
+  size_t dev_cap = 0;
+  uint64_t launches = 0;
+  double bytes = 0.0;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  void *graph_sv = nullptr;
+};
+
+static void drop_batch_graph(Batch *b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  b->graph_sv = nullptr;
+}
+
+// Plans of one gate share their structure when these agree; the kernel
+// arguments then differ in matrix entries only.
+static bool same_structure(const GatePlan &a, const GatePlan &b) {
+  if (a.kind != b.kind || a.kk != b.kk || a.fixed_mask != b.fixed_mask ||
+      a.fixed_val != b.fixed_val)
+    return false;
+  for (int j = 0; j < a.kk; ++j)
+    if (a.tbit[j] != b.tbit[j]) return false;
+  if (a.kind == Kind::PERM) {
+    if (a.moved != b.moved) return false;
+    for (int r = 0; r < (1 << a.kk); ++r)
+      if (a.perm[r] != b.perm[r]) return false;
+  }
+  return true;
+}
+
+// Plans of every gate and set, the host image of the argument tables and
+// block matrices, and its size.  A gate whose sets' matrices fall into
+// different classes (e.g. RotY(0) = identity next to RotY(0.3)) is planned as
+// a dense gate on all its targets in every set.
+static int plan_batch(Batch *b, const double *matrices, std::vector<char> *img) {
+  const uint64_t n = 1ull << b->nq;
+  const size_t ng = b->nt.size();
+  std::vector<Batch::Gate> gates(ng);
+  std::vector<char> tab;
+  std::vector<std::pair<size_t, const GatePlan *>> blocks;  // (offset, plan)
+  size_t blk_total = 0;
+  uint64_t launches = 0;
+  double bytes = 0.0;
+  for (size_t g = 0; g < ng; ++g) {
+    const int k = b->nt[g];
+    const uint32_t *t = b->targets.data() + b->toff[g];
+    std::vector<GatePlan> &P = gates[g].plans;
+    P.resize(b->nsets);
+    bool same = true, noop = true;
+    for (uint32_t y = 0; y < b->nsets; ++y) {
+      const double *m = matrices + y * b->set_doubles + b->moffs[g];
+      SQ_TRY(make_plan(m, t, k, b->nq, b->nq, &P[y]));
+      same = same && same_structure(P[0], P[y]);
+      noop = noop && P[y].kind == Kind::NOOP;
+    }
+    if (noop) {
+      P.clear();
+      continue;
+    }
+    if (!same)
+      for (uint32_t y = 0; y < b->nsets; ++y) {
+        const double *m = matrices + y * b->set_doubles + b->moffs[g];
+        SQ_TRY(make_plan(m, t, k, b->nq, b->nq, &P[y], kPlanDense));
+      }
+    std::vector<const GatePlan *> pp(b->nsets);
+    for (uint32_t y = 0; y < b->nsets; ++y) {
+      pp[y] = &P[y];
+      bytes += 2.0 * P[y].touched * (double)n * itemsize(b->dt);
+    }
+    tab.resize((tab.size() + 255) / 256 * 256);
+    gates[g].tab_off = tab.size();
+    SQ_TRY(batch_gate_table(pp.data(), b->nsets, b->dt, n, &tab));
+    if (has_block(P[0])) {
+      const size_t bb = block_bytes(P[0].kk, b->dt);
+      gates[g].blk_off = blk_total;
+      gates[g].gstride = bb / (4 * (b->dt == SQ_COMPLEX64 ? 4 : 8));
+      for (uint32_t y = 0; y < b->nsets; ++y) blocks.push_back({blk_total + y * bb, &P[y]});
+      blk_total += b->nsets * bb;
+    }
+    ++launches;
+  }
+  // image: tables, then blocks (256-byte aligned)
+  const size_t tab_bytes = (tab.size() + 255) / 256 * 256;
+  img->assign(tab_bytes + blk_total, 0);
+  std::memcpy(img->data(), tab.data(), tab.size());
+  std::vector<const GatePlan *> bp;
+  std::vector<size_t> bo;
+  for (auto &e : blocks) {
+    bp.push_back(e.second);
+    bo.push_back(tab_bytes + e.first);
+  }
+  block_image(bp, b->dt, bo, img->data());
+  for (auto &gt : gates)
+    if (gt.blk_off != (size_t)-1) gt.blk_off += tab_bytes;
+  b->gates.swap(gates);
+  b->launches = launches;
+  b->bytes = bytes;
+  return SQ_OK;
+}
+
+static int upload_batch(Batch *b, const std::vector<char> &img, hipStream_t st) {
+  if (img.size() > b->dev_cap) {
+    if (b->dev) SQ_HIP(hipFree(b->dev));
+    b->dev = nullptr;
+    b->dev_cap = 0;
+    if (hipMalloc(&b->dev, img.size()) != hipSuccess) {
+      (void)hipGetLastError();
+      b->dev = nullptr;
+      return fail(SQ_ERR_OOM, "batch tables: device allocation failed");
+    }
+    b->dev_cap = img.size();
+  }
+  if (!img.empty()) {
+    SQ_HIP(hipMemcpyAsync(b->dev, img.data(), img.size(), hipMemcpyHostToDevice, st));
+    SQ_HIP(hipStreamSynchronize(st));
+  }
+  return SQ_OK;
+}
+
+static int batch_launches(Batch *b, void *states, hipStream_t st) {
+  const uint64_t n = 1ull << b->nq;
+  for (auto &gt : b->gates) {
+    if (gt.plans.empty()) continue;
+    const char *d = (const char *)b->dev;
+    SQ_TRY(launch_batch_gate(gt.plans[0], b->dt, n, b->nsets, states, d + gt.tab_off,
+                             gt.blk_off == (size_t)-1 ? nullptr : d + gt.blk_off, gt.gstride,
+                             st));
+  }
+  return SQ_OK;
+}
+
+static int check_batch_states(const Batch *b, const void *states, uint64_t n) {
+  if (!states) return fail(SQ_ERR_INVALID, "null states");
+  if (n != (1ull << b->nq))
+    return fail(SQ_ERR_INVALID, "batch built for " + std::to_string(b->nq) + " qubits, states have " +
+                                    std::to_string(n) + " amplitudes");
+  return SQ_OK;
+}
+
+}  // namespace sq
+
+extern "C" {
+
+int sq_batch_create(void **batch, sq_dtype_t dt, uint32_t nq, uint32_t nsets, uint32_t ng,
+                    const uint32_t *num_targets, const uint32_t *targets,
+                    const double *matrices) {
+  if (!batch) return fail(SQ_ERR_INVALID, "null output pointer");
+  *batch = nullptr;
+  if (!valid_dtype(dt)) return fail(SQ_ERR_INVALID, "bad dtype");
+  if (nq < 1 || nq > 40) return fail(SQ_ERR_INVALID, "num_qubits must be in [1, 40]");
+  if (nsets < 1 || nsets > 65535) return fail(SQ_ERR_INVALID, "num_sets must be in [1, 65535]");
+  if (ng && (!num_targets || !matrices)) return fail(SQ_ERR_INVALID, "null gate arrays");
+  auto b = std::make_unique<Batch>();
+  b->dt = dt;
+  b->nq = (int)nq;
+  b->nsets = nsets;
+  SQ_TRY(current_device(&b->device));
+  size_t toff = 0, moff = 0;
+  for (uint32_t g = 0; g < ng; ++g) {
+    const int k = (int)num_targets[g];
+    if (k > kMaxK) return fail(SQ_ERR_UNSUPPORTED, "gate with more than 5 targets");
+    if (k && !targets) return fail(SQ_ERR_INVALID, "null targets");
+    b->nt.push_back(k);
+    b->toff.push_back(toff);
+    b->targets.insert(b->targets.end(), targets + toff, targets + toff + k);
+    b->moffs.push_back(moff);
+    toff += k;
+    moff += 2ull << (2 * k);
+  }
+  b->set_doubles = moff;
+  std::vector<char> img;
+  SQ_TRY(plan_batch(b.get(), matrices, &img));
+  SQ_TRY(upload_batch(b.get(), img, nullptr));
+  *batch = b.release();
+  return SQ_OK;
+}
+
+int sq_batch_set_matrices(void *batch, const double *matrices, void *stream) {
+  Batch *b = (Batch *)batch;
+  if (!b) return fail(SQ_ERR_INVALID, "null batch");
+  if (!matrices && !b->nt.empty()) return fail(SQ_ERR_INVALID, "null matrices");
+  std::vector<char> img;
+  SQ_TRY(plan_batch(b, matrices, &img));
+  drop_batch_graph(b);
+  // work already queued on `stream` may still read the old tables: the new
+  // image is copied behind it
+  return upload_batch(b, img, (hipStream_t)stream);
+}
+
+int sq_batch_run(void *batch, void *states, uint64_t n, void *stream) {
+  Batch *b = (Batch *)batch;
+  if (!b) return fail(SQ_ERR_INVALID, "null batch");
+  SQ_TRY(check_batch_states(b, states, n));
+  return batch_launches(b, states, (hipStream_t)stream);
+}
+
+int sq_batch_run_graph(void *batch, void *states, uint64_t n, void *stream) {
+  Batch *b = (Batch *)batch;
+  if (!b) return fail(SQ_ERR_INVALID, "null batch");
+  SQ_TRY(check_batch_states(b, states, n));
+  if (b->graph_sv != states) {
+    drop_batch_graph(b);
+    hipStream_t cs;
+    SQ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    int rc = SQ_OK;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      rc = fail(SQ_ERR_HIP, "hipStreamBeginCapture failed");
+    if (rc == SQ_OK) rc = batch_launches(b, states, cs);
+    hipGraph_t gr = nullptr;
+    if (hipStreamEndCapture(cs, &gr) != hipSuccess && rc == SQ_OK)
+      rc = fail(SQ_ERR_HIP, "hipStreamEndCapture failed");
+    (void)hipStreamDestroy(cs);
+    if (rc == SQ_OK && hipGraphInstantiate(&b->gexec, gr, nullptr, nullptr, 0) != hipSuccess)
+      rc = fail(SQ_ERR_HIP, "hipGraphInstantiate failed");
+    if (rc != SQ_OK) {
+      if (gr) (void)hipGraphDestroy(gr);
+      b->gexec = nullptr;
+      return rc;
+    }
+    b->graph = gr;
+    b->graph_sv = states;
+  }
+  SQ_HIP(hipGraphLaunch(b->gexec, (hipStream_t)stream));
+  return SQ_OK;
+}
+
+int sq_batch_stats(void *batch, uint64_t *num_launches, double *bytes) {
+  Batch *b = (Batch *)batch;
+  if (!b) return fail(SQ_ERR_INVALID, "null batch");
+  if (num_launches) *num_launches = b->launches;
+  if (bytes) *bytes = b->bytes;
+  return SQ_OK;
+}
+
+int sq_batch_destroy(void *batch) {
+  Batch *b = (Batch *)batch;
+  if (!b) return SQ_OK;
+  drop_batch_graph(b);
+  if (b->dev) SQ_HIP(hipFree(b->dev));
+  delete b;
+  return SQ_OK;
+}
+
+int sq_initialize_basis_z_batch(void *states, sq_dtype_t dt, uint64_t n, uint32_t nsets,
+                                void *stream) {
+  SQ_TRY(check_state(states, dt, n));
+  if (nsets < 1) return fail(SQ_ERR_INVALID, "num_sets must be >= 1");
+  return launch_init_batch(states, dt, n, nsets, (hipStream_t)stream);
+}
+
+int sq_expect_pauli_sum_batch(const void *states, sq_dtype_t dt, uint64_t n, uint32_t nsets,
+                              uint32_t nterms, const uint64_t *x, const uint64_t *z,
+                              const uint32_t *ny, const double *coeffs, double *out,
+                              double *term_values, void *stream) {
+  SQ_TRY(check_state(states, dt, n));
+  if (nsets < 1 || nsets > 65535) return fail(SQ_ERR_INVALID, "num_sets must be in [1, 65535]");
+  if (!out) return fail(SQ_ERR_INVALID, "null output");
+  if (nterms && (!x || !z || !ny)) return fail(SQ_ERR_INVALID, "null Pauli masks");
+  SQ_TRY(check_terms(nterms, x, z, ny, n));
+  if (nterms == 0) {
+    for (uint32_t s = 0; s < nsets; ++s) out[s] = 0.0;
+    return SQ_OK;
+  }
+  int dev;
+  SQ_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mutex[dev]);
+  // terms in launches of up to 64; sets in launches whose partials fit 16 MiB
+  const uint32_t tc = std::min<uint32_t>(64, nterms);
+  const size_t per_set = pauli_batch_partial_doubles(n, 1, (int)tc);
+  const uint32_t sc = (uint32_t)std::max<size_t>(1, std::min<size_t>(nsets, (2u << 20) / per_set));
+  const size_t part_b = per_set * sc * sizeof(double) + 256;
+  const size_t tb = nterms * pauli_term_bytes() + 256, ab = nterms * sizeof(double) + 256;
+  const size_t vb = (size_t)nsets * nterms * sizeof(double) + 256;
+  Workspace *w;
+  SQ_TRY(ws_reserve(dev, part_b + tb + ab + vb + 1024, tb + ab + vb + 1024, &w));
+  hipStream_t st = (hipStream_t)stream;
+  char *d = (char *)w->dev;
+  double *partial = (double *)d;
+  char *terms = al(d + part_b);
+  double *alpha = (double *)al(terms + tb);
+  double *vals = (double *)al((char *)alpha + ab);
+  char *h = (char *)w->host;
+  double *halpha = (double *)al(h + tb);
+  std::vector<double> ones(nterms, 1.0);
+  fill_pauli_terms((int)nterms, x, z, ny, ones.data(), h, halpha);
+  SQ_HIP(hipMemcpyAsync(terms, h, nterms * pauli_term_bytes(), hipMemcpyHostToDevice, st));
+  SQ_HIP(hipMemcpyAsync(alpha, halpha, nterms * sizeof(double), hipMemcpyHostToDevice, st));
+  for (uint32_t s0 = 0; s0 < nsets; s0 += sc) {
+    const uint32_t ns = std::min(sc, nsets - s0);
+    for (uint32_t t0 = 0; t0 < nterms; t0 += tc) {
+      const uint32_t c = std::min(tc, nterms - t0);
+      SQ_TRY(launch_pauli_terms_batch((const char *)states + s0 * n * itemsize(dt), dt, n, ns,
+                                      (int)c, terms + t0 * pauli_term_bytes(), alpha + t0,
+                                      partial, vals + (size_t)s0 * nterms + t0, nterms, st));
+    }
+  }
+  double *hv = (double *)al(h + tb + ab);
+  SQ_HIP(hipMemcpyAsync(hv, vals, (size_t)nsets * nterms * sizeof(double), hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  for (uint32_t s = 0; s < nsets; ++s) {
+    double acc = 0.0;
+    for (uint32_t t = 0; t < nterms; ++t) acc += (coeffs ? coeffs[t] : 1.0) * hv[(size_t)s * nterms + t];
+    out[s] = acc;
+  }
+  if (term_values) std::memcpy(term_values, hv, (size_t)nsets * nterms * sizeof(double));
+  return SQ_OK;
+}
+
+}  // extern "C"

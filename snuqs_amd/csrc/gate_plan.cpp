@@ -42,7 +42,7 @@ static inline int squeeze(int x, int j) {
 }
 
 int make_plan(const double *matrix, const uint32_t *targets, int k,
-              int num_qubits, int nbits, GatePlan *plan) {
+              int num_qubits, int nbits, GatePlan *plan, int flags) {
   if (k < 0 || k > kMaxK)
     return fail(SQ_ERR_UNSUPPORTED,
                 "gates on more than 5 targets are not supported (k=" +
@@ -92,8 +92,9 @@ int make_plan(const double *matrix, const uint32_t *targets, int k,
   for (int j = 0; j < k; ++j) tb[j] = bits[order[j]];
 
   // Control extraction.
+  const bool general = (flags & kPlanDense) != 0;
   int kk = k;
-  bool changed = true;
+  bool changed = !general;
   while (changed && kk > 0) {
     changed = false;
     const int dd = 1 << kk;
@@ -135,12 +136,12 @@ int make_plan(const double *matrix, const uint32_t *targets, int k,
 
   if (kk == 0) {
     p.m = {M[0]};
-    p.kind = is_one(M[0]) ? Kind::NOOP : Kind::PHASE;
+    p.kind = is_one(M[0]) && !general ? Kind::NOOP : Kind::PHASE;
     p.touched = p.kind == Kind::NOOP ? 0.0 : sub;
     *plan = std::move(p);
     return SQ_OK;
   }
-  bool diag = true, mono = true;
+  bool diag = !general, mono = !general;
   for (int r = 0; r < dd; ++r) {
     int nz = 0;
     for (int c = 0; c < dd; ++c) {

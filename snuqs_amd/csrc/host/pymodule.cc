@@ -523,6 +523,131 @@ class QaoaBatch {
   void *h_ = nullptr;
 };
 
+// ---- parameter batches (sq_batch_*): num_sets states of one size in one
+// device buffer, evolved by one gate structure with per-state matrices ------
+class StateBatch {
+ public:
+  StateBatch(size_t nsets, size_t nq, sq_dtype_t dt) : nsets_(nsets), nq_(nq), dt_(dt) {
+    if (nsets < 1 || nsets > 65535) throw py::value_error("num_sets must be in [1, 65535]");
+    if (nq < 1 || nq > 40) throw py::value_error("num_qubits must be in [1, 40]");
+    buffer_ = std::make_shared<Buffer>(DeviceType::CUDA, bytes(), false);
+  }
+  size_t num_sets() const { return nsets_; }
+  size_t num_qubits() const { return nq_; }
+  sq_dtype_t dtype() const { return dt_; }
+  size_t num_elems() const { return (size_t)1 << nq_; }
+  size_t bytes() const { return nsets_ * num_elems() * (dt_ == SQ_COMPLEX64 ? 8 : 16); }
+  void *ptr() const { return buffer_->ptr; }
+  // State y as a StateVector sharing the batch's memory.
+  std::shared_ptr<StateVector> state(size_t y) {
+    if (y >= nsets_) throw py::index_error("state index out of range");
+    return std::make_shared<StateVector>(DeviceType::CUDA, nq_, dt_, buffer_,
+                                         y * num_elems() * (dt_ == SQ_COMPLEX64 ? 8 : 16));
+  }
+  void initialize_basis_z(std::shared_ptr<Stream> stream) {
+    check(sq_initialize_basis_z_batch(ptr(), dt_, num_elems(), (uint32_t)nsets_, sraw(stream)));
+  }
+  py::array cpu(std::shared_ptr<Stream> stream) {
+    const std::vector<size_t> shape{nsets_, num_elems()};
+    py::array out = dt_ == SQ_COMPLEX64 ? (py::array)py::array_t<std::complex<float>>(shape)
+                                        : (py::array)py::array_t<std::complex<double>>(shape);
+    void *h = out.mutable_data();
+    {
+      py::gil_scoped_release nogil;
+      check(sq_memcpy_d2h(h, ptr(), bytes(), sraw(stream)));
+      check(sq_stream_synchronize(sraw(stream)));
+    }
+    return out;
+  }
+
+ private:
+  size_t nsets_, nq_;
+  sq_dtype_t dt_;
+  std::shared_ptr<Buffer> buffer_;
+};
+
+class BatchProgram {
+ public:
+  BatchProgram(sq_dtype_t dt, size_t nq,
+               const std::vector<std::vector<std::shared_ptr<GateOperation>>> &sets)
+      : dt_(dt), nq_(nq) {
+    if (sets.empty()) throw py::value_error("BatchProgram: no parameter sets");
+    ref_ = sets[0];
+    std::vector<uint32_t> nt, tg;
+    for (auto &op : ref_) {
+      auto t = op->get_targets();
+      if (t.size() != op->num_matrix_qubits())
+        throw py::value_error("gate " + op->formatted_string() + ": targets do not match matrix");
+      nt.push_back((uint32_t)t.size());
+      for (size_t q : t) tg.push_back((uint32_t)q);
+    }
+    std::vector<double> mats = matrices(sets);
+    check(sq_batch_create(&h_, dt, (uint32_t)nq, (uint32_t)sets.size(), (uint32_t)ref_.size(),
+                          nt.data(), tg.data(), mats.data()));
+    nsets_ = sets.size();
+  }
+  ~BatchProgram() {
+    if (h_) sq_batch_destroy(h_);
+  }
+  void set_gates(const std::vector<std::vector<std::shared_ptr<GateOperation>>> &sets,
+                 std::shared_ptr<Stream> stream) {
+    if (sets.size() != nsets_)
+      throw py::value_error("set_gates: " + std::to_string(sets.size()) + " sets given, the batch has " +
+                            std::to_string(nsets_));
+    std::vector<double> mats = matrices(sets);
+    py::gil_scoped_release nogil;
+    check(sq_batch_set_matrices(h_, mats.data(), sraw(stream)));
+  }
+  void run(StateBatch &b, bool use_graph, std::shared_ptr<Stream> stream) {
+    if (b.dtype() != dt_) throw py::value_error("state batch dtype differs from the program's");
+    if (b.num_qubits() != nq_) throw py::value_error("state batch has another qubit count");
+    if (b.num_sets() != nsets_) throw py::value_error("state batch has another number of states");
+    py::gil_scoped_release nogil;
+    if (use_graph)
+      check(sq_batch_run_graph(h_, b.ptr(), b.num_elems(), sraw(stream)));
+    else
+      check(sq_batch_run(h_, b.ptr(), b.num_elems(), sraw(stream)));
+  }
+  py::dict stats() const {
+    uint64_t l;
+    double by;
+    check(sq_batch_stats(h_, &l, &by));
+    py::dict d;
+    d["launches"] = l;
+    d["algorithmic_bytes"] = by;
+    return d;
+  }
+  size_t num_gates() const { return ref_.size(); }
+  size_t num_sets() const { return nsets_; }
+
+ private:
+  // every set's matrices, set-major, checked against set 0's targets
+  std::vector<double> matrices(const std::vector<std::vector<std::shared_ptr<GateOperation>>> &sets) const {
+    std::vector<double> mats;
+    for (size_t y = 0; y < sets.size(); ++y) {
+      if (sets[y].size() != ref_.size())
+        throw py::value_error("parameter set " + std::to_string(y) + " has " +
+                              std::to_string(sets[y].size()) + " gates, set 0 has " +
+                              std::to_string(ref_.size()));
+      for (size_t g = 0; g < ref_.size(); ++g) {
+        const auto &op = sets[y][g];
+        if (op->get_targets() != ref_[g]->get_targets() ||
+            op->num_matrix_qubits() != ref_[g]->num_matrix_qubits())
+          throw py::value_error("parameter set " + std::to_string(y) + ", gate " + std::to_string(g) +
+                                " (" + op->formatted_string() + ") has other targets than set 0's " +
+                                ref_[g]->formatted_string());
+        const double *m = mat_ptr(*op);
+        mats.insert(mats.end(), m, m + 2 * op->matrix().size());
+      }
+    }
+    return mats;
+  }
+  sq_dtype_t dt_;
+  size_t nq_, nsets_ = 0;
+  std::vector<std::shared_ptr<GateOperation>> ref_;
+  void *h_ = nullptr;
+};
+
 py::buffer_info gate_buffer(GateOperation &g) {
   return py::buffer_info(g.data(), sizeof(cplx), py::format_descriptor<cplx>::format(), 2,
                          g.shape(), g.stride());
@@ -847,6 +972,58 @@ PYBIND11_MODULE(_C, m) {
            py::arg("work1"), py::arg("checkpoints"), py::arg("param_map"),
            py::arg("num_params"), py::arg("paulis"), py::arg("qubits"), py::arg("coeffs"),
            py::kw_only(), py::arg("stream") = nullptr);
+
+  py::class_<StateBatch, std::shared_ptr<StateBatch>>(m_rt, "StateBatch")
+      .def(py::init([](size_t nsets, size_t n, std::string dtype) {
+             return std::make_shared<StateBatch>(nsets, n, parse_dtype(dtype));
+           }), py::arg("num_sets"), py::arg("num_qubits"), py::kw_only(),
+           py::arg("dtype") = "complex128",
+           "num_sets device states of 2^num_qubits amplitudes in one buffer.")
+      .def("state", &StateBatch::state, py::arg("index"),
+           "State `index` as a StateVector sharing the batch's memory.")
+      .def("initialize_basis_z", &StateBatch::initialize_basis_z, py::kw_only(),
+           py::arg("stream") = nullptr)
+      .def("cpu", &StateBatch::cpu, py::kw_only(), py::arg("stream") = nullptr,
+           "Host copy, shape (num_sets, 2^num_qubits).")
+      .def_property_readonly("num_sets", &StateBatch::num_sets)
+      .def_property_readonly("num_qubits", &StateBatch::num_qubits)
+      .def_property_readonly("dtype", [](const StateBatch &b) { return dtype_name(b.dtype()); })
+      .def_property_readonly("data_ptr", [](const StateBatch &b) { return (uintptr_t)b.ptr(); });
+
+  py::class_<BatchProgram, std::shared_ptr<BatchProgram>>(m_fn, "BatchProgram")
+      .def(py::init([](size_t nq,
+                       const std::vector<std::vector<std::shared_ptr<GateOperation>>> &sets,
+                       std::string dtype) {
+             return std::make_shared<BatchProgram>(parse_dtype(dtype), nq, sets);
+           }), py::arg("num_qubits"), py::arg("parameter_sets"), py::kw_only(),
+           py::arg("dtype") = "complex128",
+           "One gate list per parameter set (same gates and targets, own matrices).")
+      .def("run", &BatchProgram::run, py::arg("states"), py::kw_only(),
+           py::arg("use_graph") = false, py::arg("stream") = nullptr,
+           "Apply every gate to every state, one launch per gate for the whole batch.")
+      .def("set_gates", &BatchProgram::set_gates, py::arg("parameter_sets"), py::kw_only(),
+           py::arg("stream") = nullptr)
+      .def("stats", &BatchProgram::stats)
+      .def("__len__", &BatchProgram::num_gates)
+      .def_property_readonly("num_sets", &BatchProgram::num_sets);
+
+  m_fn.def("expectation_batch", [](StateBatch &b, const std::vector<std::string> &paulis,
+                                   const std::vector<std::vector<size_t>> &qubits,
+                                   const std::vector<double> &coeffs, std::shared_ptr<Stream> s) {
+    PauliSum ps = make_pauli_sum(b.num_qubits(), paulis, qubits, coeffs);
+    py::array_t<double> out(b.num_sets());
+    py::array_t<double> terms(std::vector<size_t>{b.num_sets(), ps.x.size()});
+    double *o = out.mutable_data(), *t = terms.mutable_data();
+    {
+      py::gil_scoped_release nogil;
+      check(sq_expect_pauli_sum_batch(b.ptr(), b.dtype(), b.num_elems(), (uint32_t)b.num_sets(),
+                                      (uint32_t)ps.x.size(), ps.x.data(), ps.z.data(),
+                                      ps.ny.data(), ps.c.data(), o, t, sraw(s)));
+    }
+    return py::make_tuple(out, terms);
+  }, py::arg("states"), py::arg("paulis"), py::arg("qubits"), py::arg("coeffs"), py::kw_only(),
+     py::arg("stream") = nullptr,
+     "Per-state energies sum_t c_t <psi_y|P_t|psi_y> (FP64) and per-term values.");
 
   py::class_<QaoaBatch, std::shared_ptr<QaoaBatch>>(m_fn, "QaoaBatch")
       .def(py::init([](const std::vector<uint32_t> &nq,

@@ -124,6 +124,20 @@ std::vector<double> fused_block_3m(const GatePlan &p);
 
 int launch_init(void *sv, sq_dtype_t dt, uint64_t nelems, bool basis,
                 hipStream_t stream);
+// |0...0> in each of nsets consecutive states of nelems amplitudes.
+int launch_init_batch(void *sv, sq_dtype_t dt, uint64_t nelems, uint32_t nsets,
+                      hipStream_t stream);
+
+// Parameter batches (one launch per gate over nsets states of nelems
+// amplitudes, state y at sv + y * nelems).  plans[y] is state y's plan of the
+// gate; all share their structure.  batch_gate_table appends the per-state
+// kernel arguments of the gate's launch (nothing for 4-5 target blocks,
+// whose matrices live in per-state block images gstride elements apart).
+int batch_gate_table(const GatePlan *const *plans, uint32_t nsets, sq_dtype_t dt,
+                     uint64_t nelems, std::vector<char> *tab);
+int launch_batch_gate(const GatePlan &p, sq_dtype_t dt, uint64_t nelems, uint32_t nsets,
+                      void *sv, const void *tab, const void *gdev, uint64_t gstride,
+                      hipStream_t stream);
 
 // Plain 16-byte-vector copy (bandwidth ceiling for the roofline).
 int launch_copy(void *dst, const void *src, size_t bytes, hipStream_t stream);

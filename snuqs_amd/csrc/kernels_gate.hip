@@ -27,7 +27,7 @@ namespace sq {
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    k_init(T *__restrict__ sv, uint64_t n, int basis) {
+    k_init(T *__restrict__ sv, uint64_t n, int basis, uint64_t bmask) {
   for (uint64_t i0 = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x; i0 < n;
        i0 += (uint64_t)gridDim.x * (kBlock * 4)) {
     uint64_t i = i0;
@@ -35,7 +35,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int u = 0; u < 4; ++u, i += kBlock) {
       if (i < n) {
         T z;
-        z.x = (basis && i == 0) ? 1 : 0;
+        z.x = (basis && (i & bmask) == 0) ? 1 : 0;  // bmask: all ones, or 2^n - 1 per state
         z.y = 0;
         st_nt(sv + i, z);
       }
@@ -43,17 +43,26 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-int launch_init(void *sv, sq_dtype_t dt, uint64_t n, bool basis,
-                hipStream_t stream) {
+static int launch_init_mask(void *sv, sq_dtype_t dt, uint64_t n, bool basis, uint64_t bmask,
+                            hipStream_t stream) {
   const dim3 blocks = sweep_grid(n, kBlock * 4);
   if (dt == SQ_COMPLEX64)
     hipLaunchKernelGGL(k_init<float2>, blocks, dim3(kBlock), 0, stream,
-                       (float2 *)sv, n, (int)basis);
+                       (float2 *)sv, n, (int)basis, bmask);
   else
     hipLaunchKernelGGL(k_init<double2>, blocks, dim3(kBlock), 0, stream,
-                       (double2 *)sv, n, (int)basis);
+                       (double2 *)sv, n, (int)basis, bmask);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
+}
+
+int launch_init(void *sv, sq_dtype_t dt, uint64_t n, bool basis, hipStream_t stream) {
+  return launch_init_mask(sv, dt, n, basis, ~0ull, stream);
+}
+
+int launch_init_batch(void *sv, sq_dtype_t dt, uint64_t nelems, uint32_t nsets,
+                      hipStream_t stream) {
+  return launch_init_mask(sv, dt, nelems * nsets, true, nelems - 1, stream);
 }
 
 Tuning &tuning() {
@@ -83,8 +92,8 @@ int launch_copy(void *dst, const void *src, size_t bytes, hipStream_t stream) {
 // PHASE: multiply the amplitudes whose control bits match by one scalar.
 // ---------------------------------------------------------------------------
 template <typename R, int U>
-__global__ void __launch_bounds__(kBlock)
-    k_phase(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
+__device__ __forceinline__ void
+    k_phase_body(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
             R pr, R pi) {
   using T = typename CV<R>::T;
   for (uint64_t f0 = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x; f0 < nfree;
@@ -109,6 +118,23 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+template <typename R, int U>
+__global__ void __launch_bounds__(kBlock)
+    k_phase(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im, R pr, R pi) {
+  k_phase_body<R, U>(sv, nfree, im, pr, pi);
+}
+// Batched forms (kb_*): one launch over a batch of states, blockIdx.y = the
+// state's index y in the batch; state y starts at sv + y * sstride and its
+// gate arguments are tab[y] (or its block matrix G + y * gstride).
+template <typename R, int U>
+__global__ void __launch_bounds__(kBlock)
+    kb_phase(typename CV<R>::T *__restrict__ sv, uint64_t sstride, uint64_t nfree, IdxMap im,
+             const typename CV<R>::T *__restrict__ tab) {
+  const typename CV<R>::T ph = tab[blockIdx.y];
+  k_phase_body<R, U>(sv + blockIdx.y * sstride, nfree, im, ph.x, ph.y);
+}
+
+
 // ---------------------------------------------------------------------------
 // DIAG: amplitude i (with matching control bits) times d[local index of i].
 // ---------------------------------------------------------------------------
@@ -120,8 +146,8 @@ struct DiagArgs {
 };
 
 template <typename R, int U>
-__global__ void __launch_bounds__(kBlock)
-    k_diag(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
+__device__ __forceinline__ void
+    k_diag_body(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
            DiagArgs<R> a) {
   using T = typename CV<R>::T;
   __shared__ R sdr[1 << kMaxK], sdi[1 << kMaxK];
@@ -155,6 +181,19 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+template <typename R, int U>
+__global__ void __launch_bounds__(kBlock)
+    k_diag(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im, DiagArgs<R> a) {
+  k_diag_body<R, U>(sv, nfree, im, a);
+}
+template <typename R, int U>
+__global__ void __launch_bounds__(kBlock)
+    kb_diag(typename CV<R>::T *__restrict__ sv, uint64_t sstride, uint64_t nfree, IdxMap im,
+            const DiagArgs<R> *__restrict__ tab) {
+  k_diag_body<R, U>(sv + blockIdx.y * sstride, nfree, im, tab[blockIdx.y]);
+}
+
+
 // ---------------------------------------------------------------------------
 // PERM: out[r] = ph[r] * in[perm[r]] for the moved local indices r.
 // src[r] / dst[r] are the amplitude offsets of perm[r] and r.
@@ -167,8 +206,8 @@ struct PermArgs {
 };
 
 template <typename R, int U>
-__global__ void __launch_bounds__(kBlock)
-    k_perm(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
+__device__ __forceinline__ void
+    k_perm_body(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
            PermArgs<R> a) {
   using T = typename CV<R>::T;
   for (uint64_t f0 = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x; f0 < nfree;
@@ -202,6 +241,19 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+template <typename R, int U>
+__global__ void __launch_bounds__(kBlock)
+    k_perm(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im, PermArgs<R> a) {
+  k_perm_body<R, U>(sv, nfree, im, a);
+}
+template <typename R, int U>
+__global__ void __launch_bounds__(kBlock)
+    kb_perm(typename CV<R>::T *__restrict__ sv, uint64_t sstride, uint64_t nfree, IdxMap im,
+            const PermArgs<R> *__restrict__ tab) {
+  k_perm_body<R, U>(sv + blockIdx.y * sstride, nfree, im, tab[blockIdx.y]);
+}
+
+
 // ---------------------------------------------------------------------------
 // DENSE, 1-3 targets: matrix in kernel arguments (scalar registers).
 // ---------------------------------------------------------------------------
@@ -212,8 +264,8 @@ struct DenseArgs {
 };
 
 template <typename R, int K, int U>
-__global__ void __launch_bounds__(kBlock)
-    k_dense(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
+__device__ __forceinline__ void
+    k_dense_body(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
             DenseArgs<R, K> a) {
   using T = typename CV<R>::T;
   constexpr int D = 1 << K;
@@ -252,6 +304,19 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+template <typename R, int K, int U>
+__global__ void __launch_bounds__(kBlock)
+    k_dense(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im, DenseArgs<R, K> a) {
+  k_dense_body<R, K, U>(sv, nfree, im, a);
+}
+template <typename R, int K, int U>
+__global__ void __launch_bounds__(kBlock)
+    kb_dense(typename CV<R>::T *__restrict__ sv, uint64_t sstride, uint64_t nfree, IdxMap im,
+             const DenseArgs<R, K> *__restrict__ tab) {
+  k_dense_body<R, K, U>(sv + blockIdx.y * sstride, nfree, im, tab[blockIdx.y]);
+}
+
+
 // ---------------------------------------------------------------------------
 // Dense 4-5 target matrices live in device memory in one layout shared by the
 // kernels below: entry (o, r) = {Re m[r][o], Im m[r][o], Re + Im, 0} at
@@ -275,8 +340,8 @@ struct OffArgs {
 // DENSE, 4-5 targets, VALU: matrix staged in LDS (broadcast reads).  Used for
 // states too small to fill MFMA tiles.
 template <typename R, int K>
-__global__ void __launch_bounds__(kBlock)
-    k_dense_lds(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
+__device__ __forceinline__ void
+    k_dense_lds_body(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
                 OffArgs off, const typename Q4<R>::T *__restrict__ G) {
   using T = typename CV<R>::T;
   constexpr int D = 1 << K;
@@ -313,6 +378,21 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+template <typename R, int K>
+__global__ void __launch_bounds__(kBlock)
+    k_dense_lds(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im, OffArgs off,
+                const typename Q4<R>::T *__restrict__ G) {
+  k_dense_lds_body<R, K>(sv, nfree, im, off, G);
+}
+template <typename R, int K>
+__global__ void __launch_bounds__(kBlock)
+    kb_dense_lds(typename CV<R>::T *__restrict__ sv, uint64_t sstride, uint64_t nfree,
+                 IdxMap im, OffArgs off, const typename Q4<R>::T *__restrict__ G,
+                 uint64_t gstride) {
+  k_dense_lds_body<R, K>(sv + blockIdx.y * sstride, nfree, im, off, G + blockIdx.y * gstride);
+}
+
+
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
@@ -336,8 +416,8 @@ __device__ __forceinline__ void stage_real_block16(R *As, const typename Q4<R>::
   }
 }
 
-__global__ void __launch_bounds__(kBlock)
-    k_mfma4_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, OffArgs off,
+__device__ __forceinline__ void
+    k_mfma4_c64_body(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, OffArgs off,
                 const float4 *__restrict__ G) {
   constexpr int D = 32, NS = 16;
   __shared__ float As[D * D];
@@ -374,7 +454,19 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_mfma4_c128(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, OffArgs off,
+    k_mfma4_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, OffArgs off,
+                const float4 *__restrict__ G) {
+  k_mfma4_c64_body(sv, ntiles, im, off, G);
+}
+__global__ void __launch_bounds__(kBlock)
+    kb_mfma4_c64(float2 *__restrict__ sv, uint64_t sstride, uint64_t ntiles, IdxMap im,
+                 OffArgs off, const float4 *__restrict__ G, uint64_t gstride) {
+  k_mfma4_c64_body(sv + blockIdx.y * sstride, ntiles, im, off, G + blockIdx.y * gstride);
+}
+
+
+__device__ __forceinline__ void
+    k_mfma4_c128_body(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, OffArgs off,
                  const typename Q4<double>::T *__restrict__ G) {
   constexpr int D = 32, NT = 2, NS = 8;
   __shared__ double As[D * D];
@@ -417,6 +509,18 @@ __global__ void __launch_bounds__(kBlock)
       }
   }
 }
+
+__global__ void __launch_bounds__(kBlock)
+    k_mfma4_c128(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, OffArgs off,
+                 const typename Q4<double>::T *__restrict__ G) {
+  k_mfma4_c128_body(sv, ntiles, im, off, G);
+}
+__global__ void __launch_bounds__(kBlock)
+    kb_mfma4_c128(double2 *__restrict__ sv, uint64_t sstride, uint64_t ntiles, IdxMap im,
+                  OffArgs off, const typename Q4<double>::T *__restrict__ G, uint64_t gstride) {
+  k_mfma4_c128_body(sv + blockIdx.y * sstride, ntiles, im, off, G + blockIdx.y * gstride);
+}
+
 
 // ---------------------------------------------------------------------------
 // DENSE, 5 targets on the matrix cores: three real GEMMs per complex product
@@ -485,8 +589,8 @@ __host__ __device__ constexpr uint32_t swz_c64(uint32_t x) {
 __host__ __device__ constexpr uint32_t swz_c128(uint32_t x) { return x ^ ((x >> 4) & 15u); }
 
 template <bool PIPE, bool BLK>
-__global__ void __launch_bounds__(kBlock)
-    k_mfma5_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
+__device__ __forceinline__ void
+    k_mfma5_c64_body(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
                 const float4 *__restrict__ G) {
   constexpr int D = 32, NS = 16;
   __shared__ float4 As[D * D];      // [o][r], 16 KB
@@ -578,6 +682,20 @@ __global__ void __launch_bounds__(kBlock)
 #undef SQ_KA
 #undef SQ_KW
 }
+
+template <bool PIPE, bool BLK>
+__global__ void __launch_bounds__(kBlock)
+    k_mfma5_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
+                const float4 *__restrict__ G) {
+  k_mfma5_c64_body<PIPE, BLK>(sv, ntiles, im, sa, G);
+}
+template <bool PIPE, bool BLK>
+__global__ void __launch_bounds__(kBlock)
+    kb_mfma5_c64(float2 *__restrict__ sv, uint64_t sstride, uint64_t ntiles, IdxMap im,
+                 StageArgs sa, const float4 *__restrict__ G, uint64_t gstride) {
+  k_mfma5_c64_body<PIPE, BLK>(sv + blockIdx.y * sstride, ntiles, im, sa, G + blockIdx.y * gstride);
+}
+
 
 // complex64, wide tile: 64 groups (the lowest 6 free positions) x the 32
 // amplitudes of the block per wave, so every global access is 32 lanes x 16 B
@@ -692,8 +810,8 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 template <bool PIPE, bool BLK>
-__global__ void __launch_bounds__(kBlock)
-    k_mfma5_c128(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
+__device__ __forceinline__ void
+    k_mfma5_c128_body(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
                  const typename Q4<double>::T *__restrict__ G) {
   using D4 = typename Q4<double>::T;
   constexpr int D = 32, NS = 8, NT = 2;
@@ -792,6 +910,21 @@ __global__ void __launch_bounds__(kBlock)
 #undef SQ_KA
 #undef SQ_KW
 }
+
+template <bool PIPE, bool BLK>
+__global__ void __launch_bounds__(kBlock)
+    k_mfma5_c128(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
+                 const typename Q4<double>::T *__restrict__ G) {
+  k_mfma5_c128_body<PIPE, BLK>(sv, ntiles, im, sa, G);
+}
+template <bool PIPE, bool BLK>
+__global__ void __launch_bounds__(kBlock)
+    kb_mfma5_c128(double2 *__restrict__ sv, uint64_t sstride, uint64_t ntiles, IdxMap im,
+                  StageArgs sa, const typename Q4<double>::T *__restrict__ G, uint64_t gstride) {
+  k_mfma5_c128_body<PIPE, BLK>(sv + blockIdx.y * sstride, ntiles, im, sa,
+                               G + blockIdx.y * gstride);
+}
+
 
 // ---------------------------------------------------------------------------
 // DENSE (1-3 targets) and PERM gates whose targets or controls sit on the
@@ -908,8 +1041,8 @@ __device__ __forceinline__ uint32_t small_group(int lane, const SmallTile &st) {
 }
 
 template <typename R, int K>
-__global__ void __launch_bounds__(kBlock)
-    k_small_dense(typename CV<R>::T *__restrict__ sv, uint64_t ntiles, IdxMap im,
+__device__ __forceinline__ void
+    k_small_dense_body(typename CV<R>::T *__restrict__ sv, uint64_t ntiles, IdxMap im,
                   SmallTile st, DenseArgs<R, K> a) {
   using T = typename CV<R>::T;
   constexpr int TB = K + 6, D = 1 << K;
@@ -947,6 +1080,20 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+template <typename R, int K>
+__global__ void __launch_bounds__(kBlock)
+    k_small_dense(typename CV<R>::T *__restrict__ sv, uint64_t ntiles, IdxMap im, SmallTile st,
+                  DenseArgs<R, K> a) {
+  k_small_dense_body<R, K>(sv, ntiles, im, st, a);
+}
+template <typename R, int K>
+__global__ void __launch_bounds__(kBlock)
+    kb_small_dense(typename CV<R>::T *__restrict__ sv, uint64_t sstride, uint64_t ntiles,
+                   IdxMap im, SmallTile st, const DenseArgs<R, K> *__restrict__ tab) {
+  k_small_dense_body<R, K>(sv + blockIdx.y * sstride, ntiles, im, st, tab[blockIdx.y]);
+}
+
+
 // PERM on the tile: out[dst[i]] = ph[i] * in[src[i]] for the moved local
 // indices; src/dst are tile-local offsets from the group's index.
 template <typename R>
@@ -957,8 +1104,8 @@ struct SmallPermArgs {
 };
 
 template <typename R, int K>
-__global__ void __launch_bounds__(kBlock)
-    k_small_perm(typename CV<R>::T *__restrict__ sv, uint64_t ntiles, IdxMap im,
+__device__ __forceinline__ void
+    k_small_perm_body(typename CV<R>::T *__restrict__ sv, uint64_t ntiles, IdxMap im,
                  SmallTile st, SmallPermArgs<R> a) {
   using T = typename CV<R>::T;
   constexpr int TB = K + 6, D = 1 << K;
@@ -991,6 +1138,20 @@ __global__ void __launch_bounds__(kBlock)
     wave_lds_sync();
   }
 }
+
+template <typename R, int K>
+__global__ void __launch_bounds__(kBlock)
+    k_small_perm(typename CV<R>::T *__restrict__ sv, uint64_t ntiles, IdxMap im, SmallTile st,
+                 SmallPermArgs<R> a) {
+  k_small_perm_body<R, K>(sv, ntiles, im, st, a);
+}
+template <typename R, int K>
+__global__ void __launch_bounds__(kBlock)
+    kb_small_perm(typename CV<R>::T *__restrict__ sv, uint64_t sstride, uint64_t ntiles,
+                  IdxMap im, SmallTile st, const SmallPermArgs<R> *__restrict__ tab) {
+  k_small_perm_body<R, K>(sv + blockIdx.y * sstride, ntiles, im, st, tab[blockIdx.y]);
+}
+
 
 // ---------------------------------------------------------------------------
 // host side
@@ -1147,6 +1308,60 @@ static inline dim3 grid_sweep1(uint64_t work) {
   return dim3((unsigned)b);
 }
 
+// Kernel arguments of a plan (shared by single-state and batched launches).
+template <typename R>
+static DiagArgs<R> diag_args(const GatePlan &p) {
+  DiagArgs<R> a{};
+  for (int r = 0; r < (1 << p.kk); ++r) {
+    a.dr[r] = (R)p.m[r].real();
+    a.di[r] = (R)p.m[r].imag();
+  }
+  for (int j = 0; j < p.kk; ++j) a.tbit[j] = p.tbit[j];
+  a.kk = p.kk;
+  return a;
+}
+
+template <typename R>
+static SmallPermArgs<R> small_perm_args(const GatePlan &p, const SmallTile &stile) {
+  SmallPermArgs<R> a{};
+  for (int r = 0; r < (1 << p.kk); ++r) {
+    if (!((p.moved >> r) & 1)) continue;
+    a.src[a.count] = stile.off[p.perm[r]];
+    a.dst[a.count] = stile.off[r];
+    a.pr[a.count] = (R)p.m[r].real();
+    a.pi[a.count] = (R)p.m[r].imag();
+    ++a.count;
+  }
+  return a;
+}
+
+template <typename R>
+static PermArgs<R> perm_args(const GatePlan &p) {
+  PermArgs<R> a{};
+  a.count = 0;
+  for (int r = 0; r < (1 << p.kk); ++r) {
+    if (!((p.moved >> r) & 1)) continue;
+    a.src[a.count] = local_offset(p, p.perm[r]);
+    a.dst[a.count] = local_offset(p, r);
+    a.pr[a.count] = (R)p.m[r].real();
+    a.pi[a.count] = (R)p.m[r].imag();
+    ++a.count;
+  }
+  return a;
+}
+
+template <typename R, int K>
+static DenseArgs<R, K> dense_args(const GatePlan &p) {
+  constexpr int d = 1 << K;
+  DenseArgs<R, K> a{};
+  for (int i = 0; i < d * d; ++i) {
+    a.mr[i] = (R)p.m[i].real();
+    a.mi[i] = (R)p.m[i].imag();
+  }
+  for (int r = 0; r < d; ++r) a.off[r] = local_offset(p, r);
+  return a;
+}
+
 template <typename R>
 static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
                          uint64_t nelems, hipStream_t st, const R *gdev) {
@@ -1167,13 +1382,7 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
       constexpr int U = 4;
       const uint64_t nfree = nelems >> nfixed;
       IdxMap im = make_idxmap(p.fixed_mask, p.fixed_val);
-      DiagArgs<R> a{};
-      for (int r = 0; r < (1 << p.kk); ++r) {
-        a.dr[r] = (R)p.m[r].real();
-        a.di[r] = (R)p.m[r].imag();
-      }
-      for (int j = 0; j < p.kk; ++j) a.tbit[j] = p.tbit[j];
-      a.kk = p.kk;
+      const DiagArgs<R> a = diag_args<R>(p);
       hipLaunchKernelGGL((k_diag<R, U>), grid_for(nfree, kBlock * U),
                          dim3(kBlock), 0, st, sv, nfree, im, a);
       break;
@@ -1184,15 +1393,7 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
       IdxMap sim;
       uint64_t ntiles;
       if (make_small_tile(p, nelems, sizeof(R) == 4, &stile, &sim, &ntiles)) {
-        SmallPermArgs<R> a{};
-        for (int r = 0; r < (1 << p.kk); ++r) {
-          if (!((p.moved >> r) & 1)) continue;
-          a.src[a.count] = stile.off[p.perm[r]];
-          a.dst[a.count] = stile.off[r];
-          a.pr[a.count] = (R)p.m[r].real();
-          a.pi[a.count] = (R)p.m[r].imag();
-          ++a.count;
-        }
+        const SmallPermArgs<R> a = small_perm_args<R>(p, stile);
         const dim3 grid = sweep_grid(ntiles, 4);
         if (p.kk == 1)
           hipLaunchKernelGGL((k_small_perm<R, 1>), grid, dim3(kBlock), 0, st, sv, ntiles, sim, stile, a);
@@ -1205,16 +1406,7 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
       constexpr int U = 2;
       const uint64_t nfree = nelems >> (nfixed + p.kk);
       IdxMap im = make_idxmap(p.fixed_mask | target_mask(p), p.fixed_val);
-      PermArgs<R> a{};
-      a.count = 0;
-      for (int r = 0; r < (1 << p.kk); ++r) {
-        if (!((p.moved >> r) & 1)) continue;
-        a.src[a.count] = local_offset(p, p.perm[r]);
-        a.dst[a.count] = local_offset(p, r);
-        a.pr[a.count] = (R)p.m[r].real();
-        a.pi[a.count] = (R)p.m[r].imag();
-        ++a.count;
-      }
+      const PermArgs<R> a = perm_args<R>(p);
       hipLaunchKernelGGL((k_perm<R, U>), grid_for(nfree, kBlock * U),
                          dim3(kBlock), 0, st, sv, nfree, im, a);
       break;
@@ -1230,12 +1422,7 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
         const bool small = make_small_tile(p, nelems, sizeof(R) == 4, &stile, &sim, &ntiles);
 #define SQ_DENSE_CASE(KV, UV)                                                   \
   case KV: {                                                                    \
-    DenseArgs<R, KV> a{};                                                       \
-    for (int i = 0; i < d * d; ++i) {                                           \
-      a.mr[i] = (R)p.m[i].real();                                               \
-      a.mi[i] = (R)p.m[i].imag();                                               \
-    }                                                                           \
-    for (int r = 0; r < d; ++r) a.off[r] = local_offset(p, r);                  \
+    const DenseArgs<R, KV> a = dense_args<R, KV>(p);                            \
     if (small)                                                                  \
       hipLaunchKernelGGL((k_small_dense<R, KV>), sweep_grid(ntiles, 4),         \
                          dim3(kBlock), 0, st, sv, ntiles, sim, stile, a);       \
@@ -1331,6 +1518,234 @@ int launch_plan(const GatePlan &p, void *sv, sq_dtype_t dt, uint64_t nelems,
                                 (const float *)dev_matrix);
   return launch_plan_t<double>(p, (double2 *)sv, nelems, stream,
                                (const double *)dev_matrix);
+}
+
+// ---------------------------------------------------------------------------
+// Parameter batches: one launch per gate over `nsets` states of `nelems`
+// amplitudes each (state y at sv + y * nelems), every state with its own
+// matrix for the gate.  The plans of one gate share their structure (kind,
+// target and control bits, permutation), so the kernel, grid and index maps
+// come from the first plan and only the numbers differ: they sit in a device
+// table with one kernel-argument entry per state (or one block matrix per
+// state for 4-5 target blocks), read by the kb_* kernels at blockIdx.y.
+// ---------------------------------------------------------------------------
+enum class BatchKernel : int { NONE, PHASE, DIAG, PERM, SPERM, DENSE, SDENSE, MFMA4, MFMA5, DLDS };
+
+struct BatchShape {
+  BatchKernel k = BatchKernel::NONE;
+  IdxMap im{};
+  SmallTile stile{};
+  StageArgs sa{};
+  OffArgs off{};
+  uint64_t work = 0;  // free indices (lane kernels) or tiles per state
+};
+
+template <typename R>
+static BatchShape batch_shape(const GatePlan &p, uint64_t nelems) {
+  BatchShape b;
+  const int nfixed = __builtin_popcountll(p.fixed_mask);
+  constexpr bool c64 = sizeof(R) == 4;
+  switch (p.kind) {
+    case Kind::NOOP:
+      break;
+    case Kind::PHASE:
+    case Kind::DIAG:
+      b.k = p.kind == Kind::PHASE ? BatchKernel::PHASE : BatchKernel::DIAG;
+      b.im = make_idxmap(p.fixed_mask, p.fixed_val);
+      b.work = nelems >> nfixed;
+      break;
+    case Kind::PERM:
+    case Kind::DENSE:
+      if (p.kk <= 3) {
+        const bool perm = p.kind == Kind::PERM;
+        if (make_small_tile(p, nelems, c64, &b.stile, &b.im, &b.work)) {
+          b.k = perm ? BatchKernel::SPERM : BatchKernel::SDENSE;
+        } else {
+          b.k = perm ? BatchKernel::PERM : BatchKernel::DENSE;
+          b.im = make_idxmap(p.fixed_mask | target_mask(p), p.fixed_val);
+          b.work = nelems >> (nfixed + p.kk);
+        }
+      } else {
+        const uint64_t nfree = nelems >> (nfixed + p.kk);
+        const uint64_t tg = c64 ? 32 : 16;
+        b.im = make_idxmap(p.fixed_mask | target_mask(p), p.fixed_val);
+        for (int r = 0; r < (1 << p.kk); ++r) b.off.o[r] = local_offset(p, r);
+        if (nfree >= tg && nfree % tg == 0 && (p.kk == 4 || make_stage(p, c64, &b.sa))) {
+          b.k = p.kk == 4 ? BatchKernel::MFMA4 : BatchKernel::MFMA5;
+          b.work = nfree / tg;
+        } else {
+          b.k = BatchKernel::DLDS;
+          b.work = nfree;
+        }
+      }
+      break;
+  }
+  return b;
+}
+
+template <typename R, typename A>
+static void put(std::vector<char> *tab, const A &a) {
+  const char *c = reinterpret_cast<const char *>(&a);
+  tab->insert(tab->end(), c, c + sizeof(A));
+}
+
+template <typename R>
+static int batch_table_t(const GatePlan *const *plans, uint32_t nsets, uint64_t nelems,
+                         std::vector<char> *tab) {
+  const BatchShape b = batch_shape<R>(*plans[0], nelems);
+  for (uint32_t y = 0; y < nsets; ++y) {
+    const GatePlan &p = *plans[y];
+    switch (b.k) {
+      case BatchKernel::PHASE: {
+        typename CV<R>::T ph;
+        ph.x = (R)p.m[0].real();
+        ph.y = (R)p.m[0].imag();
+        put<R>(tab, ph);
+        break;
+      }
+      case BatchKernel::DIAG:
+        put<R>(tab, diag_args<R>(p));
+        break;
+      case BatchKernel::PERM:
+        put<R>(tab, perm_args<R>(p));
+        break;
+      case BatchKernel::SPERM:
+        put<R>(tab, small_perm_args<R>(p, b.stile));
+        break;
+      case BatchKernel::DENSE:
+      case BatchKernel::SDENSE:
+        if (p.kk == 1) put<R>(tab, dense_args<R, 1>(p));
+        else if (p.kk == 2) put<R>(tab, dense_args<R, 2>(p));
+        else put<R>(tab, dense_args<R, 3>(p));
+        break;
+      default:  // blocks live in the batch's block images; no table
+        return SQ_OK;
+    }
+  }
+  return SQ_OK;
+}
+
+int batch_gate_table(const GatePlan *const *plans, uint32_t nsets, sq_dtype_t dt,
+                     uint64_t nelems, std::vector<char> *tab) {
+  return dt == SQ_COMPLEX64 ? batch_table_t<float>(plans, nsets, nelems, tab)
+                            : batch_table_t<double>(plans, nsets, nelems, tab);
+}
+
+template <typename R>
+static int launch_batch_t(const GatePlan &p, uint64_t nelems, uint32_t nsets,
+                          typename CV<R>::T *sv, const void *tab, const R *gdev,
+                          uint64_t gstride, hipStream_t st) {
+  using T = typename CV<R>::T;
+  using Q = typename Q4<R>::T;
+  const BatchShape b = batch_shape<R>(p, nelems);
+  if (b.k == BatchKernel::NONE) return SQ_OK;
+  if (nsets == 0 || nsets > 65535) return fail(SQ_ERR_INVALID, "batch: 1 to 65535 states");
+  const bool blocks = b.k == BatchKernel::MFMA4 || b.k == BatchKernel::MFMA5 ||
+                      b.k == BatchKernel::DLDS;
+  if (blocks ? gdev == nullptr : tab == nullptr)
+    return fail(SQ_ERR_INVALID, "internal: batch gate without its argument table");
+  const uint64_t ss = nelems;
+  switch (b.k) {
+    case BatchKernel::PHASE:
+      hipLaunchKernelGGL((kb_phase<R, 4>), dim3(grid_for(b.work, kBlock * 4).x, nsets),
+                         dim3(kBlock), 0, st, sv, ss, b.work, b.im, (const T *)tab);
+      break;
+    case BatchKernel::DIAG:
+      hipLaunchKernelGGL((kb_diag<R, 4>), dim3(grid_for(b.work, kBlock * 4).x, nsets),
+                         dim3(kBlock), 0, st, sv, ss, b.work, b.im, (const DiagArgs<R> *)tab);
+      break;
+    case BatchKernel::PERM:
+      hipLaunchKernelGGL((kb_perm<R, 2>), dim3(grid_for(b.work, kBlock * 2).x, nsets),
+                         dim3(kBlock), 0, st, sv, ss, b.work, b.im, (const PermArgs<R> *)tab);
+      break;
+    case BatchKernel::SPERM: {
+      const dim3 g(sweep_grid(b.work, 4).x, nsets);
+      const auto *a = (const SmallPermArgs<R> *)tab;
+      if (p.kk == 1)
+        hipLaunchKernelGGL((kb_small_perm<R, 1>), g, dim3(kBlock), 0, st, sv, ss, b.work, b.im, b.stile, a);
+      else if (p.kk == 2)
+        hipLaunchKernelGGL((kb_small_perm<R, 2>), g, dim3(kBlock), 0, st, sv, ss, b.work, b.im, b.stile, a);
+      else
+        hipLaunchKernelGGL((kb_small_perm<R, 3>), g, dim3(kBlock), 0, st, sv, ss, b.work, b.im, b.stile, a);
+      break;
+    }
+    case BatchKernel::DENSE: {
+      const dim3 g(grid_sweep1(b.work).x, nsets);
+      if (p.kk == 1)
+        hipLaunchKernelGGL((kb_dense<R, 1, 1>), g, dim3(kBlock), 0, st, sv, ss, b.work, b.im,
+                           (const DenseArgs<R, 1> *)tab);
+      else if (p.kk == 2)
+        hipLaunchKernelGGL((kb_dense<R, 2, 1>), g, dim3(kBlock), 0, st, sv, ss, b.work, b.im,
+                           (const DenseArgs<R, 2> *)tab);
+      else
+        hipLaunchKernelGGL((kb_dense<R, 3, 1>), g, dim3(kBlock), 0, st, sv, ss, b.work, b.im,
+                           (const DenseArgs<R, 3> *)tab);
+      break;
+    }
+    case BatchKernel::SDENSE: {
+      const dim3 g(sweep_grid(b.work, 4).x, nsets);
+      if (p.kk == 1)
+        hipLaunchKernelGGL((kb_small_dense<R, 1>), g, dim3(kBlock), 0, st, sv, ss, b.work, b.im,
+                           b.stile, (const DenseArgs<R, 1> *)tab);
+      else if (p.kk == 2)
+        hipLaunchKernelGGL((kb_small_dense<R, 2>), g, dim3(kBlock), 0, st, sv, ss, b.work, b.im,
+                           b.stile, (const DenseArgs<R, 2> *)tab);
+      else
+        hipLaunchKernelGGL((kb_small_dense<R, 3>), g, dim3(kBlock), 0, st, sv, ss, b.work, b.im,
+                           b.stile, (const DenseArgs<R, 3> *)tab);
+      break;
+    }
+    case BatchKernel::MFMA4:
+    case BatchKernel::MFMA5: {
+      const int di = sizeof(R) == 4 ? 0 : 1;
+      const Tuning &tu = tuning();
+      uint64_t nb = (b.work + 3) / 4;
+      const uint64_t cap = 256ull * (uint64_t)tu.mfma_blocks_per_cu[di];
+      if (nb > cap) nb = cap;
+      const dim3 g((unsigned)nb, nsets);
+      const Q *G = (const Q *)gdev;
+      if constexpr (sizeof(R) == 4) {
+        if (b.k == BatchKernel::MFMA4)
+          hipLaunchKernelGGL(kb_mfma4_c64, g, dim3(kBlock), 0, st, sv, ss, b.work, b.im, b.off, G,
+                             gstride);
+        else
+          hipLaunchKernelGGL((kb_mfma5_c64<true, false>), g, dim3(kBlock), 0, st, sv, ss, b.work,
+                             b.im, b.sa, G, gstride);
+      } else {
+        if (b.k == BatchKernel::MFMA4)
+          hipLaunchKernelGGL(kb_mfma4_c128, g, dim3(kBlock), 0, st, sv, ss, b.work, b.im, b.off,
+                             G, gstride);
+        else
+          hipLaunchKernelGGL((kb_mfma5_c128<true, false>), g, dim3(kBlock), 0, st, sv, ss, b.work,
+                             b.im, b.sa, G, gstride);
+      }
+      break;
+    }
+    case BatchKernel::DLDS: {
+      const dim3 g(grid_for(b.work, kBlock).x, nsets);
+      if (p.kk == 4)
+        hipLaunchKernelGGL((kb_dense_lds<R, 4>), g, dim3(kBlock), 0, st, sv, ss, b.work, b.im,
+                           b.off, (const Q *)gdev, gstride);
+      else
+        hipLaunchKernelGGL((kb_dense_lds<R, 5>), g, dim3(kBlock), 0, st, sv, ss, b.work, b.im,
+                           b.off, (const Q *)gdev, gstride);
+      break;
+    }
+    case BatchKernel::NONE:
+      break;
+  }
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+int launch_batch_gate(const GatePlan &p, sq_dtype_t dt, uint64_t nelems, uint32_t nsets,
+                      void *sv, const void *tab, const void *gdev, uint64_t gstride,
+                      hipStream_t st) {
+  if (dt == SQ_COMPLEX64)
+    return launch_batch_t<float>(p, nelems, nsets, (float2 *)sv, tab, (const float *)gdev,
+                                 gstride, st);
+  return launch_batch_t<double>(p, nelems, nsets, (double2 *)sv, tab, (const double *)gdev,
+                                gstride, st);
 }
 
 }  // namespace sq

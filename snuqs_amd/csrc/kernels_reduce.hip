@@ -131,6 +131,9 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_partial_pauli(const T *__restrict__ sv, uint64_t n,
                     const PauliTerm *__restrict__ terms, double *partial) {
+  // blockIdx.z: state z of a batch of consecutive n-amplitude states (0 for
+  // one state); partial row (z, term) at (z * gridDim.y + y) * gridDim.x
+  sv += (uint64_t)blockIdx.z * n;
   const PauliTerm pt = terms[blockIdx.y];
   double acc = 0.0, dummy = 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
@@ -156,7 +159,8 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   block_sum2(acc, dummy);
-  if (threadIdx.x == 0) partial[(uint64_t)blockIdx.y * gridDim.x + blockIdx.x] = acc;
+  if (threadIdx.x == 0)
+    partial[((uint64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = acc;
 }
 
 // out_j = sum_t c'_t (-1)^popcount((j^x_t) & z_t) in_{j^x_t},  c'_t = c_t i^ny_t
@@ -1010,6 +1014,42 @@ int launch_pauli_terms(const void *sv, sq_dtype_t dt, uint64_t n, int nterms,
                        (const double2 *)sv, n, (const PauliTerm *)terms_dev, partial);
   hipLaunchKernelGGL(k_finalize, dim3(nterms), dim3(kBlock), 0, st, partial, nb,
                      alpha_dev, 0.0, out_dev);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+// Batch: term values of nsets consecutive states; partial holds
+// nsets x nterms x red_blocks(n / 2) doubles; out_dev[z * out_stride + t].
+__global__ void __launch_bounds__(kBlock)
+    k_finalize_rows(const double *__restrict__ partial, int nblocks, int nterms,
+                    const double *__restrict__ alpha, double *out, uint64_t out_stride) {
+  const int v = blockIdx.x, z = v / nterms, t = v % nterms;
+  double acc = 0.0, dummy = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) acc += partial[(uint64_t)v * nblocks + b];
+  block_sum2(acc, dummy);
+  if (threadIdx.x == 0) out[(uint64_t)z * out_stride + t] = alpha[t] * acc;
+}
+
+size_t pauli_batch_partial_doubles(uint64_t n, uint32_t nsets, int nterms) {
+  return (size_t)red_blocks(n / 2) * nsets * (size_t)nterms;
+}
+
+int launch_pauli_terms_batch(const void *sv, sq_dtype_t dt, uint64_t n, uint32_t nsets,
+                             int nterms, const void *terms_dev, const double *alpha_dev,
+                             double *partial, double *out_dev, uint64_t out_stride,
+                             hipStream_t st) {
+  if (nsets == 0 || nsets > 65535 || nterms < 1 || nterms > 65535)
+    return fail(SQ_ERR_INVALID, "internal: batch expectation shape");
+  const int nb = red_blocks(n / 2);
+  const dim3 grid(nb, nterms, nsets);
+  if (dt == SQ_COMPLEX64)
+    hipLaunchKernelGGL(k_partial_pauli<float2>, grid, dim3(kBlock), 0, st,
+                       (const float2 *)sv, n, (const PauliTerm *)terms_dev, partial);
+  else
+    hipLaunchKernelGGL(k_partial_pauli<double2>, grid, dim3(kBlock), 0, st,
+                       (const double2 *)sv, n, (const PauliTerm *)terms_dev, partial);
+  hipLaunchKernelGGL(k_finalize_rows, dim3(nsets * nterms), dim3(kBlock), 0, st, partial, nb,
+                     nterms, alpha_dev, out_dev, out_stride);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
 }

@@ -76,6 +76,13 @@ int launch_gather(const void *sv, sq_dtype_t dt, const uint64_t *idx_dev,
                   uint64_t count, double *out_dev, hipStream_t st);
 
 size_t pauli_term_bytes();
+// Batch of nsets consecutive n-amplitude states: out_dev[z * out_stride + t] =
+// alpha[t] * <psi_z|P_t|psi_z>.  partial: pauli_batch_partial_doubles(...).
+int launch_pauli_terms_batch(const void *sv, sq_dtype_t dt, uint64_t n, uint32_t nsets,
+                             int nterms, const void *terms_dev, const double *alpha_dev,
+                             double *partial, double *out_dev, uint64_t out_stride,
+                             hipStream_t st);
+size_t pauli_batch_partial_doubles(uint64_t n, uint32_t nsets, int nterms);
 size_t pauli_apply_term_bytes();
 void fill_pauli_terms(int nterms, const uint64_t *x, const uint64_t *z,
                       const uint32_t *ny, const double *coeffs, void *terms_host,

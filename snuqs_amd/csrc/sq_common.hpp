@@ -79,8 +79,12 @@ struct GatePlan {
 // Build the plan of `matrix` (row-major, reference local order: targets[0] is
 // the MSB) on `targets` of an nbits-bit index space.  Returns SQ_OK or
 // SQ_ERR_INVALID (duplicate / out-of-range targets, k > kMaxK).
+// kPlanDense: no control extraction and no diagonal / permutation
+// classification: a DENSE plan on all k targets (PHASE for k = 0), the form
+// every matrix of a gate shares whatever its parameters.
+constexpr int kPlanDense = 1;
 int make_plan(const double *matrix, const uint32_t *targets, int k,
-              int num_qubits, int nbits, GatePlan *plan);
+              int num_qubits, int nbits, GatePlan *plan, int flags = 0);
 
 // Conjugate transpose of a row-major 2^k x 2^k complex128 matrix.
 std::vector<double> dagger(const double *matrix, int k);

@@ -50,23 +50,9 @@ class Simulation(ABC):
     @abstractmethod
     def marginal_probabilities(self, targets) -> np.ndarray:
         """Distribution over `targets` (targets[0] the most significant bit of
-        the outcome index), reduced on the GPU in FP64 (1..20 targets); a
-        permutation of all qubits reorders the full distribution."""
-        targets = [int(t) for t in targets]
-        n = self._qubit_count
-        if sorted(targets) == list(range(n)):
-            p = self.probabilities
-            if targets == list(range(n)):
-                return p
-            # outcome r reads qubit targets[j] at bit n-1-j: index of r in p
-            r = np.arange(p.size, dtype=np.int64)
-            src = np.zeros_like(r)
-            for j, q in enumerate(targets):
-                src |= ((r >> (n - 1 - j)) & 1) << (n - 1 - q)
-            return p[src]
-        m = _C.functionals.marginal_probabilities(self._state_vector, targets)
-        return m / m.sum() if self._normalize else m
+        the outcome index)."""
 
+    @abstractmethod
     def retrieve_samples(self) -> np.ndarray:
         """Sampled basis-state indices (one per shot)."""
 

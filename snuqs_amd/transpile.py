@@ -180,26 +180,45 @@ def _frontier_blocks(ops, max_qubits):
     return blocks
 
 
-def fuse_operations(operations, max_qubits: int = 5):
-    """Fuse gates into blocks of at most max_qubits qubits (see module doc)."""
-    ops = [op for op in operations if list(op.targets)]
-    phases = [op for op in operations if not list(op.targets)]
-    blocks = _greedy_blocks(ops, max_qubits)
-    if ops:
-        alt = _frontier_blocks(ops, max_qubits)
+def fusion_plan(operations, max_qubits: int = 5):
+    """The grouping ``fuse_operations`` uses, as gate indices: a list of
+    (block qubits, [indices of its gates in order]) and the indices of the
+    zero-target gates (folded into one global phase).  It depends on the
+    gates' targets only, so circuits that differ in parameters alone share
+    it (parameter batches)."""
+    ops = list(operations)
+    idx = [i for i, op in enumerate(ops) if list(op.targets)]
+    phases = [i for i, op in enumerate(ops) if not list(op.targets)]
+    sub = [ops[i] for i in idx]
+    blocks = _greedy_blocks(sub, max_qubits)
+    if sub:
+        alt = _frontier_blocks(sub, max_qubits)
         if len(alt) < len(blocks):
             blocks = alt
+    pos = {id(op): i for op, i in zip(sub, idx)}
+    return [(list(b.qubits), [pos[id(op)] for op in b.gates]) for b in blocks], phases
+
+
+def fuse_with_plan(operations, plan):
+    """Apply a ``fusion_plan`` to ``operations`` (the circuit it was made
+    from, or one with the same targets and other parameters)."""
+    ops = list(operations)
+    blocks, phases = plan
     out = []
-    for blk in blocks:
-        if len(blk.gates) == 1:
-            out.append(blk.gates[0])
+    for qubits, members in blocks:
+        if len(members) == 1:
+            out.append(ops[members[0]])
             continue
-        k = len(blk.qubits)
-        mat = np.eye(1 << k, dtype=np.complex128)
-        for op in blk.gates:
-            mat = _apply_to_block(mat, blk.qubits, np.array(op.matrix()), list(op.targets))
-        out.append(g.Unitary(blk.qubits, mat))
+        mat = np.eye(1 << len(qubits), dtype=np.complex128)
+        for i in members:
+            mat = _apply_to_block(mat, qubits, np.array(ops[i].matrix()), list(ops[i].targets))
+        out.append(g.Unitary(qubits, mat))
     if phases:
-        ph = complex(np.prod([np.array(op.matrix())[0, 0] for op in phases]))
+        ph = complex(np.prod([np.array(ops[i].matrix())[0, 0] for i in phases]))
         out.append(g.GPhase([], float(np.angle(ph))))
     return out
+
+
+def fuse_operations(operations, max_qubits: int = 5):
+    """Fuse gates into blocks of at most max_qubits qubits (see module doc)."""
+    return fuse_with_plan(operations, fusion_plan(operations, max_qubits))

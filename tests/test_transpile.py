@@ -54,3 +54,23 @@ def test_pseudo_sort_hand_worked_orders_product():
     for name, ops, want in _PSEUDO_SORT_CASES:
         got = pseudo_sort_operations_descending([_Op(nm, t) for nm, t in ops])
         assert [op.name for op in got] == want, name
+
+
+def test_fusion_plan_is_shared_by_parameter_sets():
+    """Sets that differ only in angles fuse into the same blocks (same
+    targets, gate by gate), each with its own matrices: equal to fusing the
+    set alone."""
+    from snuqs_amd.batch import fuse_operations_batch
+    from snuqs_amd.circuits import hea_ansatz
+    from snuqs_amd.transpile import fuse_operations
+
+    n, layers = 9, 3
+    rng = np.random.default_rng(1)
+    sets = [hea_ansatz(n, layers, rng.uniform(0, 2 * np.pi, 2 * n * layers))[0] for _ in range(4)]
+    fused = fuse_operations_batch(sets, 5)
+    for s, f in zip(sets, fused):
+        alone = fuse_operations(s, 5)
+        assert [list(op.targets) for op in f] == [list(op.targets) for op in fused[0]]
+        assert [list(op.targets) for op in f] == [list(op.targets) for op in alone]
+        for a, b in zip(f, alone):
+            assert np.array_equal(np.array(a.matrix()), np.array(b.matrix()))
