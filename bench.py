@@ -322,6 +322,47 @@ def run_batch_bench(C, n=20, layers=8, nsets=64, nterms=64, weight=4, reps=3):
     }
 
 
+def host_memory_gib():
+    info = {}
+    with open("/proc/meminfo") as f:
+        for line in f:
+            k, v = line.split(":", 1)
+            info[k] = int(v.split()[0]) * 1024 / 2**30
+    return info.get("MemTotal"), info.get("MemAvailable")
+
+
+def run_host_tier_bench(C, n=32, m=30, k=22, depth=10):
+    """States larger than the device slice: a 32-qubit complex64 state (32 GiB)
+    in pinned host memory, evolved through 30-qubit device slices (three
+    8 GiB buffers) by the random layered circuit of depth 10, pass by pass
+    over PCIe (each pass reads and writes the whole host state)."""
+    from snuqs_amd.circuits import random_layered
+    from snuqs_amd.host_tier import HostStateSimulation
+
+    total, avail = host_memory_gib()
+    need = (1 << n) * 8 / 2**30
+    if avail is None or avail < need * 2 + 16:
+        return f"skipped: {avail:.0f} GiB of host memory available, {need:.0f} GiB state"
+    ops = random_layered(n, depth, seed=4242)
+    t0 = time.perf_counter()
+    sim = HostStateSimulation(n, m, dtype="complex64", contiguous_bits=k, buffers=3)
+    t_alloc = time.perf_counter() - t0
+    plan = sim.plan(ops)
+    t0 = time.perf_counter()
+    sim.evolve(ops)
+    dt = time.perf_counter() - t0
+    moved = 2.0 * need * 2**30 * (len(plan) - 1) + 2.0 * (1 << m) * 8  # first pass: slice 0 only
+    log(f"host tier: {n} q c64 through {m} q slices, {len(plan)} passes in {dt:.1f} s")
+    return {
+        "config": f"random layered circuit, {n} qubits, depth {depth}, complex64 ({need:.0f} GiB "
+                  f"in pinned host memory), {m}-qubit device slices gathered in chunks of 2^{k} "
+                  f"amplitudes, 3 device buffers",
+        "passes": len(plan), "gates": len(ops), "seconds": dt, "alloc_init_seconds": t_alloc,
+        "pcie_GBps": moved / dt / 1e9,
+        "host_mem_total_GiB": total, "host_mem_available_GiB": avail,
+    }
+
+
 def run_qaoa_bench(C, reps=2):
     """Config 4: 256 QAOA MaxCut instances, qubits uniform over the even values
     12..26 (3-regular graphs need an even count), p = 4, complex64 with FP64
@@ -380,6 +421,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-qaoa", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-host-tier", action="store_true")
     ap.add_argument("--skip-configs", action="store_true",
                     help="skip the config 1, config 2 complex128 and config 5 legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -468,6 +510,7 @@ def main():
     others = None if args.skip_configs else run_other_configs(C, args.depth)
     qaoa = None if args.skip_qaoa else run_qaoa_bench(C)
     batch = None if args.skip_batch else run_batch_bench(C)
+    host_tier = None if args.skip_host_tier else run_host_tier_bench(C)
     cpu = None if args.skip_cpu else cpu_baseline(n, args.depth, args.cpu_seconds)
 
     out = {
@@ -517,6 +560,7 @@ def main():
         "adjoint": adj,
         "qaoa_batch": qaoa,
         "parameter_batch": batch,
+        "host_tier": host_tier,
         "other_configs": others,
         "cpu_baseline": cpu,
     }

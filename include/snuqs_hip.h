@@ -215,6 +215,30 @@ int sq_expect_pauli_sum_batch(const void *states, sq_dtype_t dtype,
                               const double *coeffs, double *out,
                               double *term_values, void *stream);
 
+/* ---- states larger than HBM: the reference's hybrid and CPU-offload tiers
+ * (simulation.py:286-351, transpile.py:184-298) on one GPU.  The state is
+ * num_elems amplitudes in pinned host memory (sq_host_alloc_pinned).
+ *
+ * sq_host_tier_pass: one streaming pass.  local_bits[0..num_local-1] are the
+ * host index bits that form a device slice (device bit j = host bit
+ * local_bits[j]); they must start with 0, 1, ..., k-1 for some k >= 1 (the
+ * slice is gathered as 2^(num_local-k) contiguous chunks of 2^k
+ * amplitudes).  For every assignment of the other bits, the slice is copied
+ * into one of the num_buffers device buffers (each 2^num_local amplitudes),
+ * `program` (built for num_local qubits, gates on device bits) runs on it,
+ * and it is copied back; copies in, compute and copies out overlap on three
+ * streams.  zero_slices != 0 skips every slice but the first (correct when
+ * the state is |0...0>: the other slices are zero and stay zero).  Ordered
+ * after the work queued on `stream`; returns when the host state is
+ * written. */
+int sq_host_tier_pass(void *program, void *host_state, sq_dtype_t dtype,
+                      uint64_t num_elems, uint32_t num_local,
+                      const uint32_t *local_bits, void *const *buffers,
+                      uint32_t num_buffers, int zero_slices, void *stream);
+/* |0...0> in a host state, written by `threads` host threads. */
+int sq_host_initialize_basis_z(void *host_state, sq_dtype_t dtype,
+                               uint64_t num_elems, uint32_t threads);
+
 /* ---- results: simulation.py:196-220 (state_vector, probabilities,
  *      retrieve_samples) and the expectation/inner-product reductions that
  *      the reference lacks (result_types.py:149-151 registers StateVector

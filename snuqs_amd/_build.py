@@ -30,6 +30,7 @@ HIP_SOURCES = [
     "kernels_sample.hip",
     "kernels_qaoa.hip",
     "capi.hip",
+    "host_tier.cpp",
 ]
 HOST_SOURCES = ["host/pymodule.cc", "host/gates.cc"]
 

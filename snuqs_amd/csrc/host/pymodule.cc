@@ -407,6 +407,32 @@ class Program {
     d["algorithmic_bytes"] = b;
     return d;
   }
+  // One streaming pass of the host-memory tier (sq_host_tier_pass): this
+  // program (on local_bits.size() qubits) over every slice of `host`.
+  void run_host_pass(StateVector &host, const std::vector<uint32_t> &local_bits,
+                     const std::vector<std::shared_ptr<StateVector>> &buffers, bool zero_slices,
+                     std::shared_ptr<Stream> stream) {
+    if (host.device() != DeviceType::CPU)
+      throw py::value_error("run_host_pass: the state must be a host StateVector (pinned)");
+    if (host.dtype() != dt_) throw py::value_error("run_host_pass: state dtype differs from the program's");
+    if (local_bits.size() != nq_)
+      throw py::value_error("run_host_pass: the program has " + std::to_string(nq_) +
+                            " qubits, " + std::to_string(local_bits.size()) + " local bits given");
+    if (buffers.empty()) throw py::value_error("run_host_pass: no device buffers");
+    std::vector<void *> bp;
+    for (auto &b : buffers) {
+      if (!b) throw py::value_error("run_host_pass: buffer is None");
+      b->require_device("run_host_pass buffer");
+      if (b->dtype() != dt_ || b->num_qubits() != nq_)
+        throw py::value_error("run_host_pass: buffers must be device states of the program's "
+                              "qubit count and dtype");
+      bp.push_back(b->ptr());
+    }
+    py::gil_scoped_release nogil;
+    check(sq_host_tier_pass(prog_, host.ptr(), dt_, host.num_elems(), (uint32_t)local_bits.size(),
+                            local_bits.data(), bp.data(), (uint32_t)bp.size(), zero_slices ? 1 : 0,
+                            sraw(stream)));
+  }
   // Energy and gradient w.r.t. `num_params` parameters; param_map[g] lists
   // (angle index, parameter index) pairs of gate g.
   py::tuple adjoint(StateVector &sv, StateVector &w0, StateVector &w1,
@@ -858,6 +884,12 @@ PYBIND11_MODULE(_C, m) {
            py::arg("stream") = nullptr);
   m_fn.def("initialize_basis_z", &initialize_basis_z, py::arg("state_vector"), py::kw_only(),
            py::arg("stream") = nullptr);
+  m_fn.def("host_initialize_basis_z", [](StateVector &sv, uint32_t threads) {
+    if (sv.device() != DeviceType::CPU) throw py::value_error("host_initialize_basis_z: host states only");
+    py::gil_scoped_release nogil;
+    check(sq_host_initialize_basis_z(sv.ptr(), sv.dtype(), sv.num_elems(), threads));
+  }, py::arg("state_vector"), py::kw_only(), py::arg("threads") = 16,
+     "|0...0> in a host state (parallel fill).");
   m_fn.def("probabilities", &probabilities, py::arg("state_vector"), py::kw_only(),
            py::arg("normalize") = false, py::arg("stream") = nullptr,
            "|sv|^2 in float64; normalize=True divides by the FP64 norm^2.");
@@ -968,6 +1000,11 @@ PYBIND11_MODULE(_C, m) {
       .def("stats", &Program::stats)
       .def("set_gates", &Program::set_gates, py::arg("ops"), py::kw_only(),
            py::arg("stream") = nullptr)
+      .def("run_host_pass", &Program::run_host_pass, py::arg("host_state"), py::arg("local_bits"),
+           py::arg("buffers"), py::kw_only(), py::arg("zero_slices") = false,
+           py::arg("stream") = nullptr,
+           "Stream a pinned host state through the device buffers, running this program on "
+           "every slice whose device bit j is host bit local_bits[j].")
       .def("adjoint", &Program::adjoint, py::arg("state_vector"), py::arg("work0"),
            py::arg("work1"), py::arg("checkpoints"), py::arg("param_map"),
            py::arg("num_params"), py::arg("paulis"), py::arg("qubits"), py::arg("coeffs"),

@@ -165,3 +165,78 @@ class StateVectorSimulation(Simulation):
     def expectation(self, paulis, qubits, coeffs) -> float:
         return _C.functionals.expectation(self._state_vector, list(paulis),
                                           [list(q) for q in qubits], list(coeffs))[0]
+
+
+class HostStateVectorSimulation(Simulation):
+    """The state in pinned host memory, evolved through device slices (the
+    reference's hybrid accelerator and CPU-offload tiers,
+    simulation.py:286-351): for states larger than HBM.  ``local_qubits``
+    (the reference's qubit_count_cuda) is the device slice size;
+    ``contiguous_bits`` (the reference's qubit_count_slice plays this part)
+    the size of the chunks slices are gathered from.  Results come from the
+    host state: state_vector, amplitudes, probabilities (FP64, divided by the
+    FP64 norm^2 for complex64) and samples drawn by numpy's choice exactly as
+    simulation.py:217-220 does; expectations of Pauli and Hermitian
+    observables need the state on the device and are not offered."""
+
+    def __init__(self, qubit_count: int, shots: int, subcircuit: Subcircuit, *,
+                 precision: str = "complex128", fuse: int = 5, local_qubits: int | None = None,
+                 contiguous_bits: int | None = None, buffers: int = 3):
+        super().__init__(qubit_count=qubit_count, shots=shots)
+        from .host_tier import HostStateSimulation
+
+        item = 8 if precision == "complex64" else 16
+        if local_qubits is None:
+            # three device buffers within half of the free HBM
+            free = _C.core.cuda.mem_info()[0]
+            local_qubits = int(np.floor(np.log2(max(free // 2 // (3 * item), 2))))
+        m = max(1, min(int(local_qubits), qubit_count))
+        self._precision = precision
+        self._host = HostStateSimulation(qubit_count, m, dtype=precision,
+                                         contiguous_bits=contiguous_bits, buffers=buffers,
+                                         fuse=fuse)
+
+    @property
+    def precision(self) -> str:
+        return self._precision
+
+    def evolve(self, subcircuit: Subcircuit) -> None:
+        self._host.evolve(list(subcircuit.operations))
+
+    @property
+    def state_vector(self) -> np.ndarray:
+        return self._host.state_vector.astype(np.complex128 if self._precision == "complex128"
+                                              else np.complex64)
+
+    def amplitudes(self, indices) -> np.ndarray:
+        a = self._host.state_vector
+        return np.array([a[int(i)] for i in indices], dtype=np.complex128)
+
+    @property
+    def probabilities(self) -> np.ndarray:
+        p = self._host.probabilities
+        return p / p.sum() if self._precision == "complex64" else p
+
+    def marginal_probabilities(self, targets) -> np.ndarray:
+        targets = [int(t) for t in targets]
+        n = self._qubit_count
+        p = self.probabilities.reshape([2] * n)
+        rest = tuple(q for q in range(n) if q not in targets)
+        m = p.sum(axis=rest) if rest else p
+        # axes left in ascending qubit order; reorder to `targets`
+        order = sorted(targets)
+        return np.transpose(m, [order.index(q) for q in targets]).reshape(-1)
+
+    def retrieve_samples(self) -> np.ndarray:
+        if self._shots == 0:
+            return np.array([], dtype=np.int64)
+        p = self.probabilities
+        return np.random.choice(p.size, p=p, size=self._shots)
+
+    def expectation(self, paulis, qubits, coeffs) -> float:
+        raise NotImplementedError("expectation values need the device-resident state: run "
+                                  "without offload='cpu' / accelerator='hybrid'")
+
+    def diagonalized_moments(self, rotations, targets, eigenvalues) -> tuple:
+        raise NotImplementedError("observables need the device-resident state: run without "
+                                  "offload='cpu' / accelerator='hybrid'")

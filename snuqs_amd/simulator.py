@@ -13,9 +13,12 @@ with ``source`` and ``inputs``, or the source text; read by
 ("complex64" / "complex128"), ``fuse`` (largest fused block, 0 = off) and
 ``exact_sampling``.
 
-Differences from the reference, by design: the engine runs on the GPU only
-(accelerator None / "cuda" / "hip"; "cpu" / "hybrid" and the offload tiers
-raise), and besides StateVector the result types Amplitude, Probability
+Differences from the reference, by design: the engine runs on the GPU
+(accelerator None / "cuda" / "hip"); accelerator "hybrid" or offload "cpu"
+keeps the state in pinned host memory and streams it through device slices
+of ``qubit_count_cuda`` qubits (``HostStateVectorSimulation``, for states
+larger than HBM; StateVector, Amplitude, Probability and shots only);
+"cpu" and the storage tier raise.  Besides StateVector the result types Amplitude, Probability
 (marginals over up to 20 targets reduced on the GPU), Expectation and Variance
 (Pauli products with h and i factors, and Hermitian matrices through their
 diagonalizing gates and eigenvalues; an observable without targets gives one
@@ -35,7 +38,7 @@ import numpy as np
 
 from . import _C, openqasm
 from .gate_operations import BRAKET_GATES, from_braket_instruction
-from .simulation import StateVectorSimulation
+from .simulation import HostStateVectorSimulation, StateVectorSimulation
 from .subcircuit import Subcircuit
 from .transpile import pseudo_sort_operations_descending
 from .types import AcceleratorType, OffloadType, PrefetchType
@@ -63,10 +66,11 @@ class BaseSimulator:
     @staticmethod
     def _validate_simulation_type(accelerator: Optional[str]) -> AcceleratorType:
         supported = {None: AcceleratorType.CUDA, "cuda": AcceleratorType.CUDA,
-                     "hip": AcceleratorType.HIP, "gpu": AcceleratorType.CUDA}
-        if accelerator in ("cpu", "hybrid"):
+                     "hip": AcceleratorType.HIP, "gpu": AcceleratorType.CUDA,
+                     "hybrid": AcceleratorType.HYBRID}
+        if accelerator == "cpu":
             raise TypeError(f"Accelerator {accelerator} is not supported: this engine runs "
-                            "on the GPU (accelerator='cuda' or 'hip')")
+                            "on the GPU (accelerator='cuda', 'hip' or 'hybrid')")
         if accelerator not in supported:
             raise TypeError(f"Accelerator {accelerator} is not supported")
         return supported[accelerator]
@@ -75,9 +79,11 @@ class BaseSimulator:
     def _validate_offload_type(offload: Optional[str]) -> OffloadType:
         if offload in (None, "none"):
             return OffloadType.NONE
-        if offload in ("cpu", "storage"):
-            raise TypeError(f"Offload {offload} is not supported: the state lives in one "
-                            "GPU's HBM (up to 33 qubits complex128, 34 qubits complex64)")
+        if offload == "cpu":
+            return OffloadType.CPU
+        if offload == "storage":
+            raise TypeError("Offload storage is not supported: the state lives in one GPU's "
+                            "HBM or, with offload='cpu', in pinned host memory")
         raise TypeError(f"Offload {offload} is not supported")
 
     @staticmethod
@@ -189,11 +195,19 @@ class BaseSimulator:
         acc = self._validate_simulation_type(accelerator)
         self._validate_prefetch_type(prefetch)
         off = self._validate_offload_type(offload)
-        self._validate_qubit_count(qubit_count, precision)
+        if acc != AcceleratorType.HYBRID and off != OffloadType.CPU:
+            self._validate_qubit_count(qubit_count, precision)
         ops = pseudo_sort_operations_descending(ops)
-        sub = Subcircuit(qubit_count=qubit_count, accelerator=acc, offload=off, operations=ops)
-        sim = StateVectorSimulation(qubit_count, shots, sub, precision=precision, fuse=fuse,
-                                    exact_sampling=exact_sampling)
+        sub = Subcircuit(qubit_count=qubit_count, accelerator=acc, offload=off, operations=ops,
+                         qubit_count_cuda=qubit_count_cuda, qubit_count_slice=qubit_count_slice)
+        if acc == AcceleratorType.HYBRID or off == OffloadType.CPU:
+            # the state in pinned host memory, streamed through device slices
+            sim = HostStateVectorSimulation(qubit_count, shots, sub, precision=precision,
+                                            fuse=fuse, local_qubits=qubit_count_cuda,
+                                            contiguous_bits=qubit_count_slice)
+        else:
+            sim = StateVectorSimulation(qubit_count, shots, sub, precision=precision, fuse=fuse,
+                                        exact_sampling=exact_sampling)
         sim.evolve(sub)
         values = []
         kinds = [str(getattr(_get(r, "type"), "value", _get(r, "type"))).lower() for r in results]

@@ -222,3 +222,71 @@ def fuse_with_plan(operations, plan):
 def fuse_operations(operations, max_qubits: int = 5):
     """Fuse gates into blocks of at most max_qubits qubits (see module doc)."""
     return fuse_with_plan(operations, fusion_plan(operations, max_qubits))
+
+
+def plan_host_passes(operations, qubit_count: int, local_qubits: int, contiguous_bits: int):
+    """Passes of the host-memory tier (the position of the reference's
+    localize_operations / partition_operations, transpile.py:209-298, which
+    insert Swap gates and split the circuit at every non-local gate).
+
+    A pass makes ``local_qubits`` index bits local: the lowest
+    ``contiguous_bits`` bits (each slice is gathered in contiguous chunks of
+    2^contiguous_bits amplitudes) plus up to ``local_qubits -
+    contiguous_bits`` others, chosen greedily: gates are taken in an order
+    that respects every qubit's gate sequence, each gate whose high bits fit
+    the pass's set is taken (gates that do not grow the set first), and a
+    new pass starts when no ready gate fits.  Returns a list of
+    (local_bits, [operation indices]) with local_bits in device order
+    (0..contiguous_bits-1, then the chosen bits ascending, topped up with
+    unused bits to local_qubits bits)."""
+    n, m, k = qubit_count, local_qubits, contiguous_bits
+    if not 1 <= k <= m <= n:
+        raise ValueError("need 1 <= contiguous_bits <= local_qubits <= qubit_count")
+    ops = list(operations)
+    high = [frozenset(b for b in (n - 1 - q for q in op.targets) if b >= k) for op in ops]
+    for i, h in enumerate(high):
+        if len(h) > m - k:
+            raise ValueError(f"gate {i} has {len(h)} targets above bit {k}: more than the "
+                             f"{m - k} free local bits")
+    perq: dict[int, list] = {}
+    for i, op in enumerate(ops):
+        for q in op.targets:
+            perq.setdefault(q, []).append(i)
+    ptr = {q: 0 for q in perq}
+    placed = [False] * len(ops)
+    order = list(range(len(ops)))
+    passes = []
+    left = len(ops)
+    while left:
+        hset: set = set()
+        taken = []
+        progress = True
+        while progress:
+            progress = False
+            best = None
+            for i in order:
+                if placed[i] or any(perq[q][ptr[q]] != i for q in ops[i].targets):
+                    continue
+                grow = len(high[i] - hset)
+                if len(hset) + grow > m - k:
+                    continue
+                if grow == 0:  # absorb at once, in program order
+                    best = i
+                    break
+                if best is None:
+                    best = i
+            if best is not None:
+                placed[best] = True
+                taken.append(best)
+                hset |= high[best]
+                for q in ops[best].targets:
+                    ptr[q] += 1
+                left -= 1
+                progress = True
+        if not taken:
+            raise RuntimeError("host-tier planning made no progress")
+        bits = sorted(hset)
+        fill = [b for b in range(k, n) if b not in hset][: m - k - len(bits)]
+        passes.append((list(range(k)) + sorted(bits + fill), taken))
+        order = [i for i in order if not placed[i]]
+    return passes
