@@ -75,9 +75,13 @@ int sq_copy(void *dst, const void *src, size_t bytes, void *stream);
  * MFMA kernels, default 256), "mfma5_pipeline_c64" / "_c128" (0/1, prefetch
  * the next tile in the 5-target MFMA kernels, default 1) and
  * "mfma5_order_c64" / "_c128" (0 tiles dealt cyclically over the grid,
- * 1 contiguous tile ranges per workgroup; default 0) and "mfma5_wide_c64"
+ * 1 contiguous tile ranges per workgroup, 2 cyclic over the bit-reversed
+ * tile index; default 0) and "mfma5_wide_c64"
  * (0/1, 64-group instead of 32-group complex64 tiles: 512 B instead of 256 B
- * runs per access, half the resident waves; default 0).  A gate tuned by
+ * runs per access, half the resident waves; default 0) and
+ * "qaoa_mid_scatter" (0/1, QAOA batches: mid passes take their chunks in
+ * bit-reversed order; applies to batches run or captured afterwards) and
+ * "tune_scatter" (0/1, sq_program_tune also times tile order 2; default 1).  A gate tuned by
  * sq_program_tune uses its own shape instead.  No reference counterpart. */
 int sq_tune_set(const char *key, int value);
 int sq_tune_get(const char *key, int *value);

@@ -24,7 +24,7 @@ DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.
 SPACE = {
     "mfma_blocks_per_cu": [32, 64, 128, 256, 512, 1024],
     "mfma5_pipeline": [0, 1],
-    "mfma5_order": [0, 1],
+    "mfma5_order": [0, 1, 2],
 }
 SUFFIX = {"complex64": "c64", "complex128": "c128"}
 

@@ -160,6 +160,8 @@ static int *tune_slot(const std::string &k) {
       {"mfma5_pipeline_c128", [] { return &tuning().mfma5_pipeline[1]; }},
       {"mfma5_order_c64", [] { return &tuning().mfma5_order[0]; }},
       {"mfma5_order_c128", [] { return &tuning().mfma5_order[1]; }},
+      {"qaoa_mid_scatter", [] { return &tuning().qaoa_mid_scatter; }},
+      {"tune_scatter", [] { return &tuning().tune_scatter; }},
       {"mfma5_wide_c64", [] { return &tuning().mfma5_wide; }},
   };
   for (auto &e : table)
@@ -173,8 +175,10 @@ int sq_tune_set(const char *key, int value) {
   if (!slot) return fail(SQ_ERR_INVALID, "unknown tuning key '" + k + "'");
   if (k.rfind("mfma_blocks_per_cu", 0) == 0 && (value < 1 || value > 4096))
     return fail(SQ_ERR_INVALID, "mfma_blocks_per_cu must be in [1, 4096]");
-  const bool flag = k.rfind("mfma5_pipeline", 0) == 0 || k.rfind("mfma5_order", 0) == 0 ||
-                    k.rfind("mfma5_wide", 0) == 0;
+  if (k.rfind("mfma5_order", 0) == 0 && (value < 0 || value > 2))
+    return fail(SQ_ERR_INVALID, "mfma5_order must be 0 (cyclic), 1 (blocked) or 2 (scattered)");
+  const bool flag = k.rfind("mfma5_pipeline", 0) == 0 || k.rfind("mfma5_wide", 0) == 0 ||
+                    k == "qaoa_mid_scatter" || k == "tune_scatter";
   *slot = flag ? (value ? 1 : 0) : value;
   return SQ_OK;
 }
@@ -615,7 +619,7 @@ int sq_program_tune(void *program, void *sv, uint64_t n, int reps, void *stream,
   // costs depends on the launches around it: timing a gate alone, launch
   // after launch, mispredicts it); every gate then keeps its fastest shape.
   std::vector<GatePlan::Shape> cands;
-  for (int order = 0; order < 2; ++order) {
+  for (int order = 0; order < (tuning().tune_scatter ? 3 : 2); ++order) {
     for (int bpc : {64, 256, 1024}) cands.push_back({bpc, order, 1, 0});
     for (int bpc : {256, 1024}) cands.push_back({bpc, order, 0, 0});
     if (p->dt == SQ_COMPLEX64)  // 64-group tiles: longer runs, half the waves per CU

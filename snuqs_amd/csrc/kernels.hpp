@@ -97,8 +97,10 @@ __device__ __forceinline__ void st_nt(uint4 *p, uint4 v) {
 struct Tuning {
   int mfma_blocks_per_cu[2] = {256, 256};  // grid cap of the MFMA kernels [c64, c128]
   int mfma5_pipeline[2] = {1, 1};        // 5-target MFMA: prefetch the next tile
-  int mfma5_order[2] = {0, 0};           // 5-target MFMA: 0 cyclic tiles, 1 blocked
+  int mfma5_order[2] = {0, 0};           // 5-target MFMA: 0 cyclic tiles, 1 blocked, 2 scattered
   int mfma5_wide = 0;                    // complex64 5-target MFMA: 64-group tiles
+  int qaoa_mid_scatter = 0;              // QAOA mid passes: bit-reversed chunk order
+  int tune_scatter = 1;                  // sq_program_tune also tries tile order 2
 };
 Tuning &tuning();
 

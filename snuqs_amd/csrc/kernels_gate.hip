@@ -544,17 +544,27 @@ __global__ void __launch_bounds__(kBlock)
 // need one address constant plus uniform per-k terms.  Loads of the next
 // tile are issued before the current tile's MFMAs (PIPE).
 // ---------------------------------------------------------------------------
-// Tiles of wave `wave` of this workgroup: cyclic (tile 4b + w, then + 4 x grid)
-// or blocked (workgroup b owns the contiguous range [b per, (b + 1) per),
-// per a multiple of 4, its waves interleaved).  Which order keeps HBM busier
-// depends on the target bits; sq_program_tune picks per gate.
+// Tiles of wave `wave` of this workgroup, by tile order ORD:
+//   0 cyclic    tile 4b + w, then + 4 x grid (concurrent waves on neighbouring
+//               tiles);
+//   1 blocked   workgroup b owns the contiguous range [b per, (b + 1) per),
+//               per a multiple of 4, its waves interleaved;
+//   2 scattered the cyclic walk over the bit-reversed tile index (ntiles a
+//               power of two): concurrent waves on tiles far apart in the
+//               free index space.
+// Which order keeps HBM busiest depends on the target bits (a tile's runs
+// that differ only in high address bits compete for the same DRAM banks;
+// spreading the concurrent tiles helps those), so sq_program_tune picks per
+// gate.
 struct TileRange {
   uint64_t first, end, step;
+  int rbits;  // ORD 2: log2(ntiles), else 0
 };
-template <bool BLK>
+template <int ORD>
 __device__ __forceinline__ TileRange tile_range(uint64_t ntiles, int wave) {
   TileRange r;
-  if (BLK) {
+  r.rbits = 0;
+  if (ORD == 1) {
     const uint64_t per = ((ntiles + gridDim.x - 1) / gridDim.x + 3) / 4 * 4;
     r.first = (uint64_t)blockIdx.x * per + wave;
     const uint64_t e = (uint64_t)(blockIdx.x + 1) * per;
@@ -564,8 +574,15 @@ __device__ __forceinline__ TileRange tile_range(uint64_t ntiles, int wave) {
     r.first = (uint64_t)blockIdx.x * 4 + wave;
     r.end = ntiles;
     r.step = (uint64_t)gridDim.x * 4;
+    if (ORD == 2) r.rbits = __builtin_ctzll(ntiles);
   }
   return r;
+}
+// The tile a walk position stands for.
+template <int ORD>
+__device__ __forceinline__ uint64_t tile_at(uint64_t t, const TileRange &r) {
+  if (ORD == 2) return r.rbits ? __builtin_bitreverse64(t) >> (64 - r.rbits) : t;
+  return t;
 }
 
 struct StageArgs {
@@ -588,7 +605,7 @@ __host__ __device__ constexpr uint32_t swz_c64(uint32_t x) {
 // complex128: element (c, r) at slot 16r + c, swizzled by XOR (r & 15).
 __host__ __device__ constexpr uint32_t swz_c128(uint32_t x) { return x ^ ((x >> 4) & 15u); }
 
-template <bool PIPE, bool BLK>
+template <bool PIPE, int ORD>
 __device__ __forceinline__ void
     k_mfma5_c64_body(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
                 const float4 *__restrict__ G) {
@@ -600,7 +617,7 @@ __device__ __forceinline__ void
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 31, half = lane >> 5;
   float2 *X = Xs[wave];
-  TileRange tr = tile_range<BLK>(ntiles, wave);
+  TileRange tr = tile_range<ORD>(ntiles, wave);
   uint64_t tile = tr.first;
   if (tile >= tr.end) return;
   // j = 2 lane + 128 k + e: lane bits are tile bits 1-6, k bits tile bits 7-9
@@ -618,7 +635,7 @@ __device__ __forceinline__ void
   ((((k) & 1) ? sa.w[7] : 0u) ^ (((k) & 2) ? sa.w[8] : 0u) ^ (((k) & 4) ? sa.w[9] : 0u))
   const uint32_t w0 = sa.w[0];
   float4 cur[8], nxt[8];
-  uint64_t base = deposit(tile * 32, im) + la;
+  uint64_t base = deposit(tile_at<ORD>(tile, tr) * 32, im) + la;
 #pragma unroll
   for (int k = 0; k < 8; ++k) cur[k] = ld_nt(reinterpret_cast<const float4 *>(sv + base + SQ_KA(k)));
   while (true) {
@@ -626,7 +643,7 @@ __device__ __forceinline__ void
     const bool more = ntile < tr.end;
     uint64_t nbase = 0;
     if (PIPE && more) {
-      nbase = deposit(ntile * 32, im) + la;
+      nbase = deposit(tile_at<ORD>(ntile, tr) * 32, im) + la;
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         nxt[k] = ld_nt(reinterpret_cast<const float4 *>(sv + nbase + SQ_KA(k)));
@@ -673,7 +690,7 @@ __device__ __forceinline__ void
 #pragma unroll
       for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
     } else {
-      base = deposit(tile * 32, im) + la;
+      base = deposit(tile_at<ORD>(tile, tr) * 32, im) + la;
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         cur[k] = ld_nt(reinterpret_cast<const float4 *>(sv + base + SQ_KA(k)));
@@ -683,17 +700,17 @@ __device__ __forceinline__ void
 #undef SQ_KW
 }
 
-template <bool PIPE, bool BLK>
+template <bool PIPE, int ORD>
 __global__ void __launch_bounds__(kBlock)
     k_mfma5_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
                 const float4 *__restrict__ G) {
-  k_mfma5_c64_body<PIPE, BLK>(sv, ntiles, im, sa, G);
+  k_mfma5_c64_body<PIPE, ORD>(sv, ntiles, im, sa, G);
 }
-template <bool PIPE, bool BLK>
+template <bool PIPE, int ORD>
 __global__ void __launch_bounds__(kBlock)
     kb_mfma5_c64(float2 *__restrict__ sv, uint64_t sstride, uint64_t ntiles, IdxMap im,
                  StageArgs sa, const float4 *__restrict__ G, uint64_t gstride) {
-  k_mfma5_c64_body<PIPE, BLK>(sv + blockIdx.y * sstride, ntiles, im, sa, G + blockIdx.y * gstride);
+  k_mfma5_c64_body<PIPE, ORD>(sv + blockIdx.y * sstride, ntiles, im, sa, G + blockIdx.y * gstride);
 }
 
 
@@ -709,7 +726,7 @@ struct StageArgsW {
   uint32_t w[11];  // LDS slot weight of tile bit i
 };
 
-template <bool PIPE, bool BLK>
+template <bool PIPE, int ORD>
 __global__ void __launch_bounds__(kBlock)
     k_mfma5w_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgsW sa,
                  const float4 *__restrict__ G) {
@@ -721,7 +738,7 @@ __global__ void __launch_bounds__(kBlock)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 31, half = lane >> 5;
   float2 *X = Xs[wave];
-  TileRange tr = tile_range<BLK>(ntiles, wave);
+  TileRange tr = tile_range<ORD>(ntiles, wave);
   uint64_t tile = tr.first;
   if (tile >= tr.end) return;
   // j = 2 lane + 128 k + e: lane bits are tile bits 1-6, k bits tile bits 7-10
@@ -741,7 +758,7 @@ __global__ void __launch_bounds__(kBlock)
    (((k) & 8) ? sa.w[10] : 0u))
   const uint32_t w0 = sa.w[0];
   float4 cur[NK], nxt[NK];
-  uint64_t base = deposit(tile * 64, im) + la;
+  uint64_t base = deposit(tile_at<ORD>(tile, tr) * 64, im) + la;
 #pragma unroll
   for (int k = 0; k < NK; ++k)
     cur[k] = ld_nt(reinterpret_cast<const float4 *>(sv + base + SQ_KA(k)));
@@ -750,7 +767,7 @@ __global__ void __launch_bounds__(kBlock)
     const bool more = ntile < tr.end;
     uint64_t nbase = 0;
     if (PIPE && more) {
-      nbase = deposit(ntile * 64, im) + la;
+      nbase = deposit(tile_at<ORD>(ntile, tr) * 64, im) + la;
 #pragma unroll
       for (int k = 0; k < NK; ++k)
         nxt[k] = ld_nt(reinterpret_cast<const float4 *>(sv + nbase + SQ_KA(k)));
@@ -799,7 +816,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
       for (int k = 0; k < NK; ++k) cur[k] = nxt[k];
     } else {
-      base = deposit(tile * 64, im) + la;
+      base = deposit(tile_at<ORD>(tile, tr) * 64, im) + la;
 #pragma unroll
       for (int k = 0; k < NK; ++k)
         cur[k] = ld_nt(reinterpret_cast<const float4 *>(sv + base + SQ_KA(k)));
@@ -809,7 +826,7 @@ __global__ void __launch_bounds__(kBlock)
 #undef SQ_KW
 }
 
-template <bool PIPE, bool BLK>
+template <bool PIPE, int ORD>
 __device__ __forceinline__ void
     k_mfma5_c128_body(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
                  const typename Q4<double>::T *__restrict__ G) {
@@ -822,7 +839,7 @@ __device__ __forceinline__ void
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, q = lane >> 4;
   double2 *X = Xs[wave];
-  TileRange tr = tile_range<BLK>(ntiles, wave);
+  TileRange tr = tile_range<ORD>(ntiles, wave);
   uint64_t tile = tr.first;
   if (tile >= tr.end) return;
   // j = lane + 64 k: lane bits are tile bits 0-5, k bits tile bits 6-8
@@ -839,7 +856,7 @@ __device__ __forceinline__ void
 #define SQ_KW(k) \
   ((((k) & 1) ? sa.w[6] : 0u) ^ (((k) & 2) ? sa.w[7] : 0u) ^ (((k) & 4) ? sa.w[8] : 0u))
   double2 cur[8], nxt[8];
-  uint64_t base = deposit(tile * 16, im) + la;
+  uint64_t base = deposit(tile_at<ORD>(tile, tr) * 16, im) + la;
 #pragma unroll
   for (int k = 0; k < 8; ++k) cur[k] = ld_nt(sv + base + SQ_KA(k));
   while (true) {
@@ -847,7 +864,7 @@ __device__ __forceinline__ void
     const bool more = ntile < tr.end;
     uint64_t nbase = 0;
     if (PIPE && more) {
-      nbase = deposit(ntile * 16, im) + la;
+      nbase = deposit(tile_at<ORD>(ntile, tr) * 16, im) + la;
 #pragma unroll
       for (int k = 0; k < 8; ++k) nxt[k] = ld_nt(sv + nbase + SQ_KA(k));
     }
@@ -902,7 +919,7 @@ __device__ __forceinline__ void
 #pragma unroll
       for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
     } else {
-      base = deposit(tile * 16, im) + la;
+      base = deposit(tile_at<ORD>(tile, tr) * 16, im) + la;
 #pragma unroll
       for (int k = 0; k < 8; ++k) cur[k] = ld_nt(sv + base + SQ_KA(k));
     }
@@ -911,17 +928,17 @@ __device__ __forceinline__ void
 #undef SQ_KW
 }
 
-template <bool PIPE, bool BLK>
+template <bool PIPE, int ORD>
 __global__ void __launch_bounds__(kBlock)
     k_mfma5_c128(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
                  const typename Q4<double>::T *__restrict__ G) {
-  k_mfma5_c128_body<PIPE, BLK>(sv, ntiles, im, sa, G);
+  k_mfma5_c128_body<PIPE, ORD>(sv, ntiles, im, sa, G);
 }
-template <bool PIPE, bool BLK>
+template <bool PIPE, int ORD>
 __global__ void __launch_bounds__(kBlock)
     kb_mfma5_c128(double2 *__restrict__ sv, uint64_t sstride, uint64_t ntiles, IdxMap im,
                   StageArgs sa, const typename Q4<double>::T *__restrict__ G, uint64_t gstride) {
-  k_mfma5_c128_body<PIPE, BLK>(sv + blockIdx.y * sstride, ntiles, im, sa,
+  k_mfma5_c128_body<PIPE, ORD>(sv + blockIdx.y * sstride, ntiles, im, sa,
                                G + blockIdx.y * gstride);
 }
 
@@ -1049,7 +1066,7 @@ __device__ __forceinline__ void
   __shared__ T Xs[4][1 << TB];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   T *X = Xs[wave];
-  const TileRange tr = tile_range<false>(ntiles, wave);
+  const TileRange tr = tile_range<0>(ntiles, wave);
   const uint32_t cb = small_group(lane, st);
   for (uint64_t t = tr.first; t < tr.end; t += tr.step) {
     const uint64_t base = small_tile_base(t, im, st);
@@ -1112,7 +1129,7 @@ __device__ __forceinline__ void
   __shared__ T Xs[4][1 << TB];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   T *X = Xs[wave];
-  const TileRange tr = tile_range<false>(ntiles, wave);
+  const TileRange tr = tile_range<0>(ntiles, wave);
   const uint32_t cb = small_group(lane, st);
   for (uint64_t t = tr.first; t < tr.end; t += tr.step) {
     const uint64_t base = small_tile_base(t, im, st);
@@ -1308,6 +1325,18 @@ static inline dim3 grid_sweep1(uint64_t work) {
   return dim3((unsigned)b);
 }
 
+// The 5-target kernel of a launch shape: prefetch or not, tile order 0-2.
+template <typename K>
+static K mfma5_kernel(K p0, K p1, K p2, K n0, K n1, K n2, bool pipe, int ord) {
+  if (pipe) return ord == 1 ? p1 : ord == 2 ? p2 : p0;
+  return ord == 1 ? n1 : ord == 2 ? n2 : n0;
+}
+// The scattered order walks a bit-reversed tile index: ntiles must be a power
+// of two (it is for every full state; otherwise the cyclic order runs).
+static int scatter_ok(int ord, uint64_t ntiles) {
+  return ord == 2 && (ntiles & (ntiles - 1)) != 0 ? 0 : ord;
+}
+
 // Kernel arguments of a plan (shared by single-state and batched launches).
 template <typename R>
 static DiagArgs<R> diag_args(const GatePlan &p) {
@@ -1460,7 +1489,8 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
           const int bpc = p.shape.blocks_per_cu > 0 ? p.shape.blocks_per_cu
                                                     : tu.mfma_blocks_per_cu[di];
           const bool pipe = (p.shape.pipe >= 0 ? p.shape.pipe : tu.mfma5_pipeline[di]) != 0;
-          const bool blk = (p.shape.order >= 0 ? p.shape.order : tu.mfma5_order[di]) != 0;
+          int ord = p.shape.order >= 0 ? p.shape.order : tu.mfma5_order[di];
+          if (ord < 0 || ord > 2) ord = 0;
           const uint64_t cap = 256ull * (uint64_t)bpc;
           if (blocks > cap) blocks = cap;
           using Q = typename Q4<R>::T;
@@ -1473,13 +1503,17 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
             } else if (wide && ntiles % 2 == 0 && make_stage_wide(p, &sw)) {
               const uint64_t wt = ntiles / 2;  // 64-group tiles
               const uint64_t wb = (wt + 3) / 4 < cap ? (wt + 3) / 4 : cap;
-              auto k = pipe ? (blk ? k_mfma5w_c64<true, true> : k_mfma5w_c64<true, false>)
-                            : (blk ? k_mfma5w_c64<false, true> : k_mfma5w_c64<false, false>);
+              auto k = mfma5_kernel(k_mfma5w_c64<true, 0>, k_mfma5w_c64<true, 1>,
+                                    k_mfma5w_c64<true, 2>, k_mfma5w_c64<false, 0>,
+                                    k_mfma5w_c64<false, 1>, k_mfma5w_c64<false, 2>, pipe,
+                                    scatter_ok(ord, wt));
               hipLaunchKernelGGL(k, dim3(wb), dim3(kBlock), 0, st, (float2 *)sv, wt, im, sw,
                                  (const Q *)gdev);
             } else {
-              auto k = pipe ? (blk ? k_mfma5_c64<true, true> : k_mfma5_c64<true, false>)
-                            : (blk ? k_mfma5_c64<false, true> : k_mfma5_c64<false, false>);
+              auto k = mfma5_kernel(k_mfma5_c64<true, 0>, k_mfma5_c64<true, 1>,
+                                    k_mfma5_c64<true, 2>, k_mfma5_c64<false, 0>,
+                                    k_mfma5_c64<false, 1>, k_mfma5_c64<false, 2>, pipe,
+                                    scatter_ok(ord, ntiles));
               hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), 0, st, (float2 *)sv, ntiles, im,
                                  sa, (const Q *)gdev);
             }
@@ -1488,8 +1522,10 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
               hipLaunchKernelGGL(k_mfma4_c128, dim3(blocks), dim3(kBlock), 0, st,
                                  (double2 *)sv, ntiles, im, off, (const Q *)gdev);
             } else {
-              auto k = pipe ? (blk ? k_mfma5_c128<true, true> : k_mfma5_c128<true, false>)
-                            : (blk ? k_mfma5_c128<false, true> : k_mfma5_c128<false, false>);
+              auto k = mfma5_kernel(k_mfma5_c128<true, 0>, k_mfma5_c128<true, 1>,
+                                    k_mfma5_c128<true, 2>, k_mfma5_c128<false, 0>,
+                                    k_mfma5_c128<false, 1>, k_mfma5_c128<false, 2>, pipe,
+                                    scatter_ok(ord, ntiles));
               hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), 0, st, (double2 *)sv, ntiles,
                                  im, sa, (const Q *)gdev);
             }
@@ -1709,14 +1745,14 @@ static int launch_batch_t(const GatePlan &p, uint64_t nelems, uint32_t nsets,
           hipLaunchKernelGGL(kb_mfma4_c64, g, dim3(kBlock), 0, st, sv, ss, b.work, b.im, b.off, G,
                              gstride);
         else
-          hipLaunchKernelGGL((kb_mfma5_c64<true, false>), g, dim3(kBlock), 0, st, sv, ss, b.work,
+          hipLaunchKernelGGL((kb_mfma5_c64<true, 0>), g, dim3(kBlock), 0, st, sv, ss, b.work,
                              b.im, b.sa, G, gstride);
       } else {
         if (b.k == BatchKernel::MFMA4)
           hipLaunchKernelGGL(kb_mfma4_c128, g, dim3(kBlock), 0, st, sv, ss, b.work, b.im, b.off,
                              G, gstride);
         else
-          hipLaunchKernelGGL((kb_mfma5_c128<true, false>), g, dim3(kBlock), 0, st, sv, ss, b.work,
+          hipLaunchKernelGGL((kb_mfma5_c128<true, 0>), g, dim3(kBlock), 0, st, sv, ss, b.work,
                              b.im, b.sa, G, gstride);
       }
       break;

@@ -556,6 +556,13 @@ struct QMid {
   int W;
 };
 
+// Chunk of a mid-pass workgroup: its index within the instance's launch, or
+// (scatter) that index bit-reversed over the instance's n - 12 chunk bits, so
+// concurrent workgroups take chunks far apart instead of neighbouring ones.
+__device__ __forceinline__ uint32_t qmid_chunk(uint64_t rel, int n, int scatter) {
+  const uint32_t c = (uint32_t)rel;
+  return scatter && n > kQL ? __builtin_bitreverse32(c) >> (32 - (n - kQL)) : c;
+}
 __device__ __forceinline__ QMid qmid_setup(uint32_t chunk, int n, int h0, int t) {
   QMid m;
   m.W = min(8, n - h0);
@@ -623,14 +630,14 @@ template <typename R>
 __global__ void __launch_bounds__(kBlock)
     k_qaoa_mid_fwd(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
                    const typename CV<R>::T *__restrict__ ph, int p,
-                   const QEntry *__restrict__ tab, int count, int layer, int h0) {
+                   const QEntry *__restrict__ tab, int count, int layer, int h0, int scatter) {
   using T = typename CV<R>::T;
   __shared__ T X[kQSlots];
   const QFound fe = find_entry_u(tab, count, blockIdx.x);
   const int ii = fe.inst;
   const QInstDev q = inst[ii];
   const int t = threadIdx.x;
-  const QMid m = qmid_setup((uint32_t)(blockIdx.x - fe.first), q.n, h0, t);
+  const QMid m = qmid_setup(qmid_chunk(blockIdx.x - fe.first, q.n, scatter), q.n, h0, t);
   T *s = st + q.psi + m.base;
   T v[16];
 #pragma unroll
@@ -649,14 +656,14 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
     k_qaoa_mid_rev(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
                    const typename CV<R>::T *__restrict__ ph, int p,
                    const QEntry *__restrict__ tab, int count, int layer, int h0,
-                   double *__restrict__ partial) {
+                   double *__restrict__ partial, int scatter) {
   using T = typename CV<R>::T;
   __shared__ T X[kQSlots];
   const QFound fe = find_entry_u(tab, count, blockIdx.x);
   const int ii = fe.inst;
   const QInstDev q = inst[ii];
   const int t = threadIdx.x;
-  const uint32_t chunk = (uint32_t)(blockIdx.x - fe.first);
+  const uint32_t chunk = qmid_chunk(blockIdx.x - fe.first, q.n, scatter);
   const QMid m = qmid_setup(chunk, q.n, h0, t);
   T *sa = st + q.psi + m.base, *sl = st + q.lam + m.base;
   T a[16], l[16];
@@ -1006,11 +1013,11 @@ static int enqueue_t(QaoaBatch *B, hipStream_t st) {
         break;
       case 5:
         hipLaunchKernelGGL(k_qaoa_mid_fwd<R>, g, bl, 0, st, s, B->inst, ph, p, tab, L.count,
-                           L.layer, L.bit);
+                           L.layer, L.bit, tuning().qaoa_mid_scatter);
         break;
       case 6:
         hipLaunchKernelGGL(k_qaoa_mid_rev<R>, g, bl, 0, st, s, B->inst, ph, p, tab, L.count,
-                           L.layer, L.bit, B->partial);
+                           L.layer, L.bit, B->partial, tuning().qaoa_mid_scatter);
         break;
       case 4:
         if (L.nb)

@@ -70,7 +70,7 @@ struct GatePlan {
   // sq_program_tune; a negative field defers to the process-wide tuning.
   struct Shape {
     int blocks_per_cu = -1;  // grid cap / 256 CUs
-    int order = -1;          // 0: tiles dealt cyclically, 1: contiguous per block
+    int order = -1;          // 0: tiles dealt cyclically, 1: contiguous per block, 2: cyclic over the bit-reversed tile index
     int pipe = -1;           // prefetch the next tile
     int wide = -1;           // complex64: 64-group tiles (512 B runs)
   } shape;

@@ -78,8 +78,11 @@ def test_unitary_k_targets(C, dtype, k, n):
         assert err <= TOL[dtype], (t, err)
 
 
-_NARROW = [(256, 1, 0, 0), (1, 1, 0, 0), (1, 0, 0, 0), (1, 1, 1, 0), (3, 0, 1, 0), (1024, 1, 1, 0)]
-_WIDE = [(256, 1, 0, 1), (1, 1, 0, 1), (1, 0, 1, 1), (1024, 0, 1, 1)]  # complex64 only
+# (grid cap, prefetch, tile order 0 cyclic / 1 blocked / 2 scattered, wide)
+_NARROW = [(256, 1, 0, 0), (1, 1, 0, 0), (1, 0, 0, 0), (1, 1, 1, 0), (3, 0, 1, 0), (1024, 1, 1, 0),
+           (256, 1, 2, 0), (1, 1, 2, 0), (3, 0, 2, 0)]
+_WIDE = [(256, 1, 0, 1), (1, 1, 0, 1), (1, 0, 1, 1), (1024, 0, 1, 1), (1, 1, 2, 1),
+         (256, 0, 2, 1)]  # complex64 only
 
 
 @pytest.mark.parametrize("dtype,grid,pipe,order,wide",
@@ -152,6 +155,32 @@ def test_wide_tiles_are_bitwise_narrow_tiles(C):
             assert np.array_equal(out[0], out[1]), t
     finally:
         C.core.tune_set("mfma5_wide_c64", saved)
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_tile_orders_are_bitwise_equal(C, dtype):
+    """Cyclic, blocked and scattered (bit-reversed) tile orders only change
+    which wave takes which tile: outputs are bitwise equal."""
+    from snuqs_amd.circuits import haar_unitary
+
+    g = C.operation.gate_operations
+    n = 22
+    rng = np.random.default_rng(81)
+    init = random_state(n, rng, np.complex64 if dtype == "complex64" else np.complex128)
+    key = "mfma5_order_" + ("c64" if dtype == "complex64" else "c128")
+    saved = C.core.tune_get(key)
+    try:
+        for t in ([21, 3, 9, 15, 0], [5, 6, 7, 8, 9], [20, 17, 12, 1, 10]):
+            op = g.Unitary(t, haar_unitary(32, rng))
+            out = []
+            for order in (0, 1, 2):
+                C.core.tune_set(key, order)
+                out.append(run_engine(C, n, [op], dtype, init))
+            assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2]), t
+        with pytest.raises(ValueError):
+            C.core.tune_set(key, 3)
+    finally:
+        C.core.tune_set(key, saved)
 
 
 def test_ghz20_closed_form_bitwise(C):
