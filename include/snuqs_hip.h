@@ -81,7 +81,9 @@ int sq_copy(void *dst, const void *src, size_t bytes, void *stream);
  * runs per access, half the resident waves; default 0) and
  * "qaoa_mid_scatter" (0/1, QAOA batches: mid passes take their chunks in
  * bit-reversed order; applies to batches run or captured afterwards) and
- * "tune_scatter" (0/1, sq_program_tune also times tile order 2; default 1).  A gate tuned by
+ * "tune_scatter" (0/1, sq_program_tune also times tile order 2; default 1) and
+ * "adjoint_layers" (0/1, sq_adjoint_gradient applies layers of one-target
+ * gates on distinct qubits in passes of up to 12 bits; default 1).  A gate tuned by
  * sq_program_tune uses its own shape instead.  No reference counterpart. */
 int sq_tune_set(const char *key, int value);
 int sq_tune_get(const char *key, int *value);

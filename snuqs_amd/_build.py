@@ -29,6 +29,7 @@ HIP_SOURCES = [
     "kernels_reduce.hip",
     "kernels_sample.hip",
     "kernels_qaoa.hip",
+    "kernels_layer.hip",
     "capi.hip",
     "host_tier.cpp",
 ]

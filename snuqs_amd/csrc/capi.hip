@@ -162,6 +162,7 @@ static int *tune_slot(const std::string &k) {
       {"mfma5_order_c128", [] { return &tuning().mfma5_order[1]; }},
       {"qaoa_mid_scatter", [] { return &tuning().qaoa_mid_scatter; }},
       {"tune_scatter", [] { return &tuning().tune_scatter; }},
+      {"adjoint_layers", [] { return &tuning().adjoint_layers; }},
       {"mfma5_wide_c64", [] { return &tuning().mfma5_wide; }},
   };
   for (auto &e : table)
@@ -178,7 +179,7 @@ int sq_tune_set(const char *key, int value) {
   if (k.rfind("mfma5_order", 0) == 0 && (value < 0 || value > 2))
     return fail(SQ_ERR_INVALID, "mfma5_order must be 0 (cyclic), 1 (blocked) or 2 (scattered)");
   const bool flag = k.rfind("mfma5_pipeline", 0) == 0 || k.rfind("mfma5_wide", 0) == 0 ||
-                    k == "qaoa_mid_scatter" || k == "tune_scatter";
+                    k == "qaoa_mid_scatter" || k == "tune_scatter" || k == "adjoint_layers";
   *slot = flag ? (value ? 1 : 0) : value;
   return SQ_OK;
 }
@@ -1136,15 +1137,46 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   const size_t dblk_b = block_layout(dptr, dt, &dblk_off);
   std::vector<const void *> dmat_ptr(num_derivs, nullptr);
 
-  // checkpoint c holds the state before gate bound[c]
+  // checkpoint c holds the state before gate bound[c]; once the reverse runs
+  // and layers are planned, each bound moves down to the lowest gate of the
+  // run or layer holding it, so no pass is split by a checkpoint
   std::vector<uint32_t> bound(num_checkpoints);
   for (uint32_t c = 0; c < num_checkpoints; ++c)
     bound[c] = (uint32_t)(((uint64_t)(c + 1) * ng) / (num_checkpoints + 1));
+  auto gbit = [&](uint32_t g) { return p->nq - 1 - (int)p->targets[p->toff[g]]; };
+  auto is_diag = [](const double *m, int k) {
+    const int d = 1 << k;
+    for (int r = 0; r < d; ++r)
+      for (int c = 0; c < d; ++c)
+        if (r != c && (m[2 * (r * d + c)] != 0.0 || m[2 * (r * d + c) + 1] != 0.0)) return false;
+    return true;
+  };
+  // G^dagger and dG G^dagger of a one-target gate (row-major complex 2x2)
+  auto adj_mats = [](const double *G, const double *D, double *gi, double *dg) {
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c) {  // G^dagger[r][c] = conj(G[c][r])
+        gi[2 * (2 * r + c)] = G[2 * (2 * c + r)];
+        gi[2 * (2 * r + c) + 1] = -G[2 * (2 * c + r) + 1];
+      }
+    if (!D) return;
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c) {  // (dG G^dagger)[r][c]
+        double re = 0, im = 0;
+        for (int t = 0; t < 2; ++t) {
+          const double ar = D[2 * (2 * r + t)], ai = D[2 * (2 * r + t) + 1];
+          const double br = gi[2 * (2 * t + c)], bi = gi[2 * (2 * t + c) + 1];
+          re += ar * br - ai * bi;
+          im += ar * bi + ai * br;
+        }
+        dg[2 * (2 * r + c)] = re;
+        dg[2 * (2 * r + c) + 1] = im;
+      }
+  };
 
   // Runs of consecutive diagonal gates (1-2 targets, at most one derivative
   // each, diagonal derivative) are undone in one pass in the reverse sweep
-  // (launch_adjoint_diag_run).  A gate where a checkpoint is restored stays
-  // out of every run.  run_top[g] = index of the run whose highest gate is g.
+  // (launch_adjoint_diag_run).  run_top[g] = index of the run whose highest
+  // gate is g.
   std::vector<DiagRunGate> rgates;
   std::vector<int> rslots;
   struct Run {
@@ -1152,20 +1184,11 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   };
   std::vector<Run> runs;
   std::vector<int> run_top(ng, -1);
-  std::vector<char> is_bound(ng, 0);
-  for (uint32_t c = 0; c < num_checkpoints; ++c)
-    if (bound[c] < ng) is_bound[bound[c]] = 1;
+  std::vector<char> in_run(ng, 0);
   if ((n % 1024) == 0) {
-    auto is_diag = [](const double *m, int k) {
-      const int d = 1 << k;
-      for (int r = 0; r < d; ++r)
-        for (int c = 0; c < d; ++c)
-          if (r != c && (m[2 * (r * d + c)] != 0.0 || m[2 * (r * d + c) + 1] != 0.0)) return false;
-      return true;
-    };
     auto diag_ok = [&](uint32_t g) {
       const int k = p->nt[g];
-      if (is_bound[g] || k < 1 || k > 2 || by_gate[g].size() > 1) return false;
+      if (k < 1 || k > 2 || by_gate[g].size() > 1) return false;
       if (!is_diag(p->mats.data() + p->moffs[g], k)) return false;
       return by_gate[g].empty() || is_diag(dmatrices + doffs[by_gate[g][0]], k);
     };
@@ -1181,6 +1204,7 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
         Run r{(uint32_t)lo, (uint32_t)rgates.size(), (uint32_t)(g - lo + 1),
               (uint32_t)rslots.size(), 0};
         for (int q = g; q >= lo; --q) {
+          in_run[q] = 1;
           const int k = p->nt[q];
           const int d = 1 << k;
           const double *G = p->mats.data() + p->moffs[q];
@@ -1215,11 +1239,128 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       }
     }
   }
+
+  // Layers: consecutive one-target gates on distinct qubits (at most one
+  // derivative each in the reverse sweep), applied by layer passes
+  // (launch_adjoint_layer): the gates on bits 0-11 in one pass, higher bits in
+  // windows of up to 8 bits, one pass each.  A layer is taken where it needs
+  // fewer passes than the register groups below would.
+  struct Win {
+    int h0, W;
+    uint64_t own;  // the layer's bits this window applies
+  };
+  auto layer_windows = [&](uint64_t bits) {
+    std::vector<Win> w;
+    if (bits & 0xFFFull) w.push_back({12, 0, bits & 0xFFFull});
+    uint64_t hi = bits & ~0xFFFull;
+    while (hi) {
+      const int b = __builtin_ctzll(hi);
+      const int h0 = std::max(12, std::min(b, (int)p->nq - 8));
+      const int W = std::min(8, (int)p->nq - h0);
+      const uint64_t cover = ((1ull << W) - 1) << h0;
+      w.push_back({h0, W, hi & cover});
+      hi &= ~cover;
+    }
+    return w;
+  };
+  struct LPass {
+    int h0, W, count;
+    int pos[12], slot[12];
+    double gm[96], dm[96];
+    uint32_t first_slot, nslots;
+  };
+  struct Layer {
+    uint32_t lo, m, first_pass, npass;
+  };
+  // one pass per window: reverse (G^dagger, dG G^dagger, slots) or forward (G)
+  auto make_passes = [&](uint32_t lo, uint32_t m, uint64_t used, bool rev,
+                         std::vector<LPass> *out) {
+    for (const Win &w : layer_windows(used)) {
+      LPass ps{};
+      ps.h0 = w.h0;
+      ps.W = w.W;
+      ps.first_slot = (uint32_t)rslots.size();
+      for (uint32_t q = lo; q < lo + m; ++q) {
+        const int b = gbit(q);
+        if (!((w.own >> b) & 1)) continue;
+        const int j = ps.count++;
+        ps.pos[j] = w.W == 0 ? b : 4 + (b - w.h0);
+        ps.slot[j] = -1;
+        const double *G = p->mats.data() + p->moffs[q];
+        if (!rev) {
+          for (int e = 0; e < 8; ++e) ps.gm[8 * j + e] = G[e];
+          continue;
+        }
+        const double *D = nullptr;
+        if (!by_gate[q].empty()) {
+          const uint32_t di = by_gate[q][0];
+          D = dmatrices + doffs[di];
+          ps.slot[j] = (int)ps.nslots++;
+          rslots.push_back((int)deriv_param[di]);
+        }
+        adj_mats(G, D, ps.gm + 8 * j, ps.dm + 8 * j);
+      }
+      out->push_back(ps);
+    }
+  };
+  const bool use_layers = tuning().adjoint_layers && n >= 4096 && (n % 4096) == 0;
+  std::vector<LPass> lpasses;
+  std::vector<Layer> layers;
+  std::vector<int> layer_top(ng, -1);
+  std::vector<char> in_layer(ng, 0);
+  if (use_layers) {
+    auto ok = [&](int q) { return !in_run[q] && p->nt[q] == 1 && by_gate[q].size() <= 1; };
+    for (int g = (int)ng - 1; g >= 0;) {
+      if (!ok(g)) {
+        --g;
+        continue;
+      }
+      int lo = g;
+      uint64_t used = 0;
+      int hi1 = 0, lo1 = 0, none = 0;
+      while (lo >= 0 && ok(lo) && !((used >> gbit(lo)) & 1)) {
+        used |= 1ull << gbit(lo);
+        if (by_gate[lo].empty())
+          ++none;
+        else if (gbit(lo) >= 4)
+          ++hi1;
+        else
+          ++lo1;
+        --lo;
+      }
+      ++lo;
+      const uint32_t m = (uint32_t)(g - lo + 1);
+      const int est = (hi1 + 3) / 4 + (lo1 + 3) / 4 + none;  // register groups + single steps
+      if (m >= 2 && (int)layer_windows(used).size() < est) {
+        Layer L{(uint32_t)lo, m, (uint32_t)lpasses.size(), 0};
+        make_passes((uint32_t)lo, m, used, true, &lpasses);
+        L.npass = (uint32_t)lpasses.size() - L.first_pass;
+        for (int q = lo; q <= g; ++q) in_layer[q] = 1;
+        layer_top[g] = (int)layers.size();
+        layers.push_back(L);
+      }
+      g = lo - 1;
+    }
+  }
+  // bounds to the lowest gate of their run or layer
+  {
+    std::vector<uint32_t> seg_lo(ng);
+    for (uint32_t g = 0; g < ng; ++g) seg_lo[g] = g;
+    for (const Run &r : runs)
+      for (uint32_t q = r.lo; q < r.lo + r.m; ++q) seg_lo[q] = r.lo;
+    for (const Layer &L : layers)
+      for (uint32_t q = L.lo; q < L.lo + L.m; ++q) seg_lo[q] = L.lo;
+    for (uint32_t c = 0; c < num_checkpoints; ++c) bound[c] = seg_lo[bound[c]];
+  }
+  std::vector<char> is_bound(ng, 0);
+  for (uint32_t c = 0; c < num_checkpoints; ++c) is_bound[bound[c]] = 1;
+
   // Groups of 2-4 consecutive one-target gates with one derivative each, on
-  // distinct amplitude bits, outside the runs, take one reverse pass
-  // (launch_adjoint_step_group): bits all above 3 (a lane holds each group of
-  // amplitudes) or all below 4 (pairs across the lanes of a wave; states of
-  // at least 64 amplitudes).  grp_top[g] = group whose highest gate is g.
+  // distinct amplitude bits, outside the runs and layers and not at a
+  // checkpoint, take one reverse pass (launch_adjoint_step_group): bits all
+  // above 3 (a lane holds each group of amplitudes) or all below 4 (pairs
+  // across the lanes of a wave; states of at least 64 amplitudes).
+  // grp_top[g] = group whose highest gate is g.
   struct Group {
     uint32_t lo, k, first_slot;
     int bits[5];
@@ -1228,13 +1369,9 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   std::vector<Group> groups;
   std::vector<int> grp_top(ng, -1);
   {
-    std::vector<char> in_run(ng, 0);
-    for (const Run &r : runs)
-      for (uint32_t q = r.lo; q < r.lo + r.m; ++q) in_run[q] = 1;
-    auto gbit = [&](int g) { return p->nq - 1 - (int)p->targets[p->toff[g]]; };
     auto grp_ok = [&](int g) {
-      return !in_run[g] && !is_bound[g] && p->nt[g] == 1 && by_gate[g].size() == 1 &&
-             (gbit(g) >= 4 || p->nq >= 6);
+      return !in_run[g] && !in_layer[g] && !is_bound[g] && p->nt[g] == 1 &&
+             by_gate[g].size() == 1 && (gbit(g) >= 4 || p->nq >= 6);
     };
     for (int g = (int)ng - 1; g >= 0;) {
       int lo = g;
@@ -1258,28 +1395,10 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       for (int j = 0; j < (int)gr.k; ++j) {
         const int q = g - j;
         const uint32_t di = by_gate[q][0];
-        gr.bits[j] = p->nq - 1 - (int)p->targets[p->toff[q]];
+        gr.bits[j] = gbit(q);
         rslots.push_back((int)deriv_param[di]);
-        const double *G = p->mats.data() + p->moffs[q];
-        const double *D = dmatrices + doffs[di];
-        double *gi = gr.ginv + 8 * j, *dg = gr.dgg + 8 * j;
-        for (int r = 0; r < 2; ++r)
-          for (int c = 0; c < 2; ++c) {  // G^dagger[r][c] = conj(G[c][r])
-            gi[2 * (2 * r + c)] = G[2 * (2 * c + r)];
-            gi[2 * (2 * r + c) + 1] = -G[2 * (2 * c + r) + 1];
-          }
-        for (int r = 0; r < 2; ++r)
-          for (int c = 0; c < 2; ++c) {  // (dG G^dagger)[r][c]
-            double re = 0, im = 0;
-            for (int t = 0; t < 2; ++t) {
-              const double ar = D[2 * (2 * r + t)], ai = D[2 * (2 * r + t) + 1];
-              const double br = gi[2 * (2 * t + c)], bi = gi[2 * (2 * t + c) + 1];
-              re += ar * br - ai * bi;
-              im += ar * bi + ai * br;
-            }
-            dg[2 * (2 * r + c)] = re;
-            dg[2 * (2 * r + c) + 1] = im;
-          }
+        adj_mats(p->mats.data() + p->moffs[q], dmatrices + doffs[di], gr.ginv + 8 * j,
+                 gr.dgg + 8 * j);
       }
       grp_top[g] = (int)groups.size();
       groups.push_back(gr);
@@ -1287,27 +1406,26 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     }
   }
 
-  // forward runs: the same gates, multiplied by their own diagonals, so the
-  // forward sweep also takes one pass per run; fwd_start[g] = run starting at g
+  // Forward passes never hold a checkpoint bound but as their first gate
+  // (the copy is taken before the pass).  Forward runs: the same gates,
+  // multiplied by their own diagonals; fwd_start[g] = run starting at g.
   std::vector<int> fwd_start(ng, -1);
   std::vector<Run> fruns;
+  std::vector<char> in_frun(ng, 0);
   if ((n % 1024) == 0) {
     auto is_diag_g = [&](uint32_t g) {
       const int k = p->nt[g];
-      if (is_bound[g] || k < 1 || k > 2) return false;
-      const int d = 1 << k;
-      const double *m = p->mats.data() + p->moffs[g];
-      for (int r = 0; r < d; ++r)
-        for (int c = 0; c < d; ++c)
-          if (r != c && (m[2 * (r * d + c)] != 0.0 || m[2 * (r * d + c) + 1] != 0.0)) return false;
-      return true;
+      return k >= 1 && k <= 2 && is_diag(p->mats.data() + p->moffs[g], k);
     };
     for (uint32_t g = 0; g < ng;) {
       uint32_t hi = g;
-      while (hi < ng && is_diag_g(hi) && hi - g < (uint32_t)kMaxDiagRun) ++hi;
+      while (hi < ng && is_diag_g(hi) && (hi == g || !is_bound[hi]) &&
+             hi - g < (uint32_t)kMaxDiagRun)
+        ++hi;
       if (hi - g >= 2) {
         Run r{g, (uint32_t)rgates.size(), hi - g, 0, 0};
         for (uint32_t q = g; q < hi; ++q) {
+          in_frun[q] = 1;
           const int k = p->nt[q];
           const int d = 1 << k;
           const double *G = p->mats.data() + p->moffs[q];
@@ -1330,27 +1448,57 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       }
     }
   }
+  // forward layers (as the reverse ones; gates with any number of
+  // derivatives), taken where they need fewer passes than forward groups
+  std::vector<LPass> fpasses;
+  std::vector<Layer> flayers;
+  std::vector<int> flayer_start(ng, -1);
+  std::vector<char> in_flayer(ng, 0);
+  if (use_layers) {
+    auto ok = [&](uint32_t q) { return !in_frun[q] && p->nt[q] == 1; };
+    for (uint32_t g = 0; g < ng;) {
+      if (!ok(g)) {
+        ++g;
+        continue;
+      }
+      uint32_t hi = g;
+      uint64_t used = 0;
+      int nhi = 0, nlo = 0;
+      while (hi < ng && ok(hi) && (hi == g || !is_bound[hi]) && !((used >> gbit(hi)) & 1)) {
+        used |= 1ull << gbit(hi);
+        (gbit(hi) >= 4 ? nhi : nlo)++;
+        ++hi;
+      }
+      const uint32_t m = hi - g;
+      const int est = (nhi + 4) / 5 + (nlo + 4) / 5;  // forward groups of up to 5
+      if (m >= 2 && (int)layer_windows(used).size() < est) {
+        Layer L{g, m, (uint32_t)fpasses.size(), 0};
+        make_passes(g, m, used, false, &fpasses);
+        L.npass = (uint32_t)fpasses.size() - L.first_pass;
+        for (uint32_t q = g; q < hi; ++q) in_flayer[q] = 1;
+        flayer_start[g] = (int)flayers.size();
+        flayers.push_back(L);
+      }
+      g = hi;
+    }
+  }
   // forward groups: 2-5 consecutive one-target gates on distinct bits, all
-  // above 3 or all below 4 as in the reverse groups (outside forward runs, not
-  // at a checkpoint), applied in one pass (5: psi alone leaves the registers
-  // for 32 amplitudes per lane; the reverse holds psi and lambda and stops at 4)
+  // above 3 or all below 4 as in the reverse groups (outside forward runs and
+  // layers), applied in one pass (5: psi alone leaves the registers for 32
+  // amplitudes per lane; the reverse holds psi and lambda and stops at 4)
   std::vector<Group> fgroups;
   std::vector<int> fgrp_start(ng, -1);
   {
-    std::vector<char> in_frun(ng, 0);
-    for (const Run &r : fruns)
-      for (uint32_t q = r.lo; q < r.lo + r.m; ++q) in_frun[q] = 1;
-    auto fbit = [&](uint32_t g) { return p->nq - 1 - (int)p->targets[p->toff[g]]; };
     auto fok = [&](uint32_t g) {
-      return !in_frun[g] && !is_bound[g] && p->nt[g] == 1 && (fbit(g) >= 4 || p->nq >= 6);
+      return !in_frun[g] && !in_flayer[g] && p->nt[g] == 1 && (gbit(g) >= 4 || p->nq >= 6);
     };
     for (uint32_t g = 0; g < ng;) {
       uint32_t hi = g;
       uint64_t used = 0;
-      while (hi < ng && hi - g < 5 && fok(hi)) {
-        const int b = fbit(hi);
+      while (hi < ng && hi - g < 5 && fok(hi) && (hi == g || !is_bound[hi])) {
+        const int b = gbit(hi);
         if ((used >> b) & 1) break;
-        if (hi > g && (b >= 4) != (fbit(g) >= 4)) break;  // one kind per group
+        if (hi > g && (b >= 4) != (gbit(g) >= 4)) break;  // one kind per group
         used |= 1ull << b;
         ++hi;
       }
@@ -1363,7 +1511,7 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       gr.k = hi - g;
       for (uint32_t j = 0; j < gr.k; ++j) {
         const uint32_t q = g + j;
-        gr.bits[j] = p->nq - 1 - (int)p->targets[p->toff[q]];
+        gr.bits[j] = gbit(q);
         const double *G = p->mats.data() + p->moffs[q];
         for (int e = 0; e < 8; ++e) gr.ginv[8 * j + e] = G[e];  // the gate itself
       }
@@ -1372,7 +1520,7 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       g = hi;
     }
   }
-  const bool any_runs = !runs.empty() || !fruns.empty() || !groups.empty();
+  const bool any_runs = !runs.empty() || !fruns.empty() || !groups.empty() || !layers.empty();
   auto distinct_params = [&](uint32_t first, uint32_t count) {
     for (uint32_t a = first; a < first + count; ++a)
       for (uint32_t b = a + 1; b < first + count; ++b)
@@ -1433,6 +1581,16 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       g = r.lo + r.m - 1;  // the loop steps past the run
       continue;
     }
+    if (flayer_start[g] >= 0) {
+      const Layer &L = flayers[flayer_start[g]];
+      for (uint32_t i = L.first_pass; i < L.first_pass + L.npass; ++i) {
+        const LPass &ps = fpasses[i];
+        SQ_TRY(launch_adjoint_layer(sv, nullptr, dt, n, ps.h0, ps.W, ps.count, ps.pos, ps.gm,
+                                    nullptr, nullptr, 0, nullptr, false, nullptr, nullptr, st));
+      }
+      g = L.lo + L.m - 1;
+      continue;
+    }
     if (fgrp_start[g] >= 0) {
       const Group &gr = fgroups[fgrp_start[g]];
       SQ_TRY(launch_adjoint_step_group(sv, nullptr, dt, n, (int)gr.k, gr.bits, gr.ginv, gr.dgg,
@@ -1457,6 +1615,15 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   SQ_TRY(launch_inner(sv, work0, dt, n, partial, eslot, nullptr, 0.0, st));
   // reverse sweep
   int cidx = (int)num_checkpoints - 1;
+  // after a run or layer whose lowest gate is a checkpoint bound, psi is
+  // reloaded from the checkpoint (the state before that gate)
+  auto restore_below = [&](uint32_t lo) -> int {
+    if (cidx >= 0 && bound[cidx] == lo) {
+      SQ_HIP(hipMemcpyAsync(sv, checkpoints[cidx], n * itemsize(dt), hipMemcpyDeviceToDevice, st));
+      while (cidx >= 0 && bound[cidx] == lo) --cidx;
+    }
+    return SQ_OK;
+  };
   for (int g = (int)ng - 1; g >= 0; --g) {
     if (run_top[g] >= 0) {
       const Run &r = runs[run_top[g]];
@@ -1464,7 +1631,21 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
                                      (int)r.nslots, rslots_dev + r.first_slot,
                                      distinct_params(r.first_slot, r.nslots), rpartial, gslots,
                                      st));
+      SQ_TRY(restore_below(r.lo));
       g = (int)r.lo;  // the loop steps below the run
+      continue;
+    }
+    if (layer_top[g] >= 0) {
+      const Layer &L = layers[layer_top[g]];
+      for (uint32_t i = L.first_pass; i < L.first_pass + L.npass; ++i) {
+        const LPass &ps = lpasses[i];
+        SQ_TRY(launch_adjoint_layer(sv, work0, dt, n, ps.h0, ps.W, ps.count, ps.pos, ps.gm, ps.dm,
+                                    ps.slot, (int)ps.nslots, rslots_dev + ps.first_slot,
+                                    distinct_params(ps.first_slot, ps.nslots), rpartial, gslots,
+                                    st));
+      }
+      SQ_TRY(restore_below(L.lo));
+      g = (int)L.lo;
       continue;
     }
     if (grp_top[g] >= 0) {

@@ -93,6 +93,29 @@ __device__ __forceinline__ void st_nt(uint4 *p, uint4 v) {
   __builtin_nontemporal_store(w, reinterpret_cast<nt_u4 *>(p));
 }
 
+// A 2x2 complex matrix in the state's precision, row-major; y = m x on the
+// amplitude pair (x0, x1) whose target bit is 0 / 1.
+template <typename R>
+struct Mat2 {
+  R r[4], i[4];
+};
+
+template <typename T, typename R>
+__device__ __forceinline__ void mat2_apply(const Mat2<R> &m, const T &x0, const T &x1, T &y0,
+                                           T &y1) {
+  y0.x = m.r[0] * x0.x - m.i[0] * x0.y + m.r[1] * x1.x - m.i[1] * x1.y;
+  y0.y = m.r[0] * x0.y + m.i[0] * x0.x + m.r[1] * x1.y + m.i[1] * x1.x;
+  y1.x = m.r[2] * x0.x - m.i[2] * x0.y + m.r[3] * x1.x - m.i[3] * x1.y;
+  y1.y = m.r[2] * x0.y + m.i[2] * x0.x + m.r[3] * x1.y + m.i[3] * x1.x;
+}
+
+// Sum over the 64 lanes of a wave (every lane gets the total).
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
 // Launch-shape parameters chosen by the autotuner (sq_tune_set).
 struct Tuning {
   int mfma_blocks_per_cu[2] = {256, 256};  // grid cap of the MFMA kernels [c64, c128]
@@ -101,6 +124,7 @@ struct Tuning {
   int mfma5_wide = 0;                    // complex64 5-target MFMA: 64-group tiles
   int qaoa_mid_scatter = 0;              // QAOA mid passes: bit-reversed chunk order
   int tune_scatter = 1;                  // sq_program_tune also tries tile order 2
+  int adjoint_layers = 1;                // adjoint sweeps: layer passes (12 / 8 bits a pass)
 };
 Tuning &tuning();
 

@@ -1,0 +1,411 @@
+// kernels_layer.hip -- adjoint layer passes: a layer of one-target gates on
+// distinct qubits, applied to psi (and lambda) in one pass per 12 bits.
+//
+// The reference has no gradient code (it simulates circuits and reads
+// results); these kernels serve sq_adjoint_gradient's forward and reverse
+// sweeps.  A workgroup holds 4096 amplitudes of psi (and of lambda) in
+// registers, 16 per thread, whose local index j maps to the global index as
+// in the batched QAOA mid passes: local bits 0-3 -> global bits 0-3, local
+// bits 4..3+W -> h0..h0+W-1, local bits 4+W..11 -> global bits 4..11-W
+// (W = 0: the identity, the "low" pass over bits 0-11).  The 16 registers hold
+// local bits 8-11 (layout L2, the load layout), 4-7 (L1) or 0-3 (L0); a gate
+// whose local bit is in the register index is applied in registers, and the
+// layouts change through LDS.  So one pass applies up to 12 gates (all of
+// them on bits 0-11, or up to 8 on high bits) where the register-group passes
+// apply 4-5.
+//
+// Reverse (LAM): the gates commute, so every gate's derivative term
+//   Re <lam| dG G^dagger |psi>
+// may be taken at any point inside the layer; each is taken just before the
+// gate is undone on both states (psi <- G^dagger psi, lam <- G^dagger lam).
+// Forward: psi <- G psi.  Gate arithmetic in the state's precision, the
+// derivative sums in FP64, reduced over a fixed grid in a fixed order.
+#include "chunk_layout.hpp"
+#include "kernels.hpp"
+#include "kernels_reduce.hpp"
+
+namespace sq {
+
+// Per local position k (0..11) of the pass: the gate's matrix (G^dagger in
+// the reverse sweep, G forward), its dG G^dagger and the partial row of its
+// derivative (-1: none).
+template <typename R>
+struct LayerArgs {
+  Mat2<R> g[12];
+  Mat2<R> d[12];
+  int slot[12];
+  int has;  // positions that hold a gate
+};
+
+// Global offsets of a chunk's amplitudes (see the file comment): base is the
+// chunk's part (elements), t2 / t1 the thread's part in layouts L2 / L1;
+// layer_r gives the register's part (lo = 8: L2, lo = 4: L1).  Within a
+// chunk the parts are byte offsets of type O: 32-bit when every offset of the
+// pass fits (the loads then take the uniform chunk pointer plus one 32-bit
+// offset, shared by psi and lambda), else 64-bit.
+template <typename O>
+struct LayerGeom {
+  uint64_t base;
+  O t2, t1;
+};
+
+__device__ __forceinline__ int layer_pos(int k, int W, int h0) {
+  return k < 4 ? k : k < 4 + W ? h0 + (k - 4) : k - W;
+}
+
+template <typename O>
+__device__ __forceinline__ O layer_r(int r, int lo, int W, int h0, int esh) {
+  O o = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) o |= (O)((r >> k) & 1) << (layer_pos(lo + k, W, h0) + esh);
+  return o;
+}
+
+template <typename O>
+__device__ __forceinline__ LayerGeom<O> layer_geom(uint64_t chunk, int W, int h0, int t,
+                                                   int esh) {
+  LayerGeom<O> m;
+  const int A = h0 - 12 + W;  // chunk bits at 12-W .. h0-1, the rest from h0+W up
+  m.base = ((chunk & ((1ull << A) - 1)) << (12 - W)) | ((chunk >> A) << (h0 + W));
+  m.t2 = (O)(t & 15) << esh;
+  m.t1 = (O)(t & 15) << esh;
+#pragma unroll
+  for (int k = 4; k < 8; ++k) m.t2 |= (O)((t >> k) & 1) << (layer_pos(k, W, h0) + esh);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) m.t1 |= (O)((t >> (4 + k)) & 1) << (layer_pos(8 + k, W, h0) + esh);
+  return m;
+}
+
+// Derivative term of one amplitude pair, Re(conj(l0) d0 + conj(l1) d1):
+// complex64 forms the pair's sum in FP32 and adds it to the FP64 total (one
+// rounding per pair, the order of the state's own error), complex128 in FP64.
+__device__ __forceinline__ double pair_term(const float2 &l0, const float2 &d0, const float2 &l1,
+                                            const float2 &d1) {
+  return (double)(l0.x * d0.x + l0.y * d0.y + l1.x * d1.x + l1.y * d1.y);
+}
+__device__ __forceinline__ double pair_term(const double2 &l0, const double2 &d0,
+                                            const double2 &l1, const double2 &d1) {
+  return l0.x * d0.x + l0.y * d0.y + l1.x * d1.x + l1.y * d1.y;
+}
+
+// y = m x for a real matrix m (its imaginary parts are zero): half the
+// multiplications of mat2_apply.
+template <typename T, typename R>
+__device__ __forceinline__ void mat2_apply_real(const Mat2<R> &m, const T &x0, const T &x1,
+                                                T &y0, T &y1) {
+  y0.x = m.r[0] * x0.x + m.r[1] * x1.x;
+  y0.y = m.r[0] * x0.y + m.r[1] * x1.y;
+  y1.x = m.r[2] * x0.x + m.r[3] * x1.x;
+  y1.y = m.r[2] * x0.y + m.r[3] * x1.y;
+}
+
+template <bool REAL, typename T, typename R>
+__device__ __forceinline__ void mat2_apply_k(const Mat2<R> &m, const T &x0, const T &x1, T &y0,
+                                             T &y1) {
+  if (REAL)
+    mat2_apply_real(m, x0, x1, y0, y1);
+  else
+    mat2_apply(m, x0, x1, y0, y1);
+}
+
+// The gates at local positions k0..k0+3 (register bits 0-3 of the current
+// layout), matrices read from LDS.  Every position is applied, without
+// branches: an empty position holds the identity (and a zero derivative
+// matrix), which keeps the 16 registers of each state in place instead of
+// duplicating them across per-gate branches.  Reverse: derivative terms,
+// then G^dagger on both states.  REAL: every matrix of the pass is real
+// (rotations about Y, Hadamards, their derivatives).
+template <typename T, typename R, bool LAM, bool REAL>
+__device__ __forceinline__ void layer_phase(T (&a)[16], T (&l)[16], const Mat2<R> *sg,
+                                            const Mat2<R> *sd, int k0, double (&acc)[12]) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int k = k0 + b;
+    const int bb = 1 << b;
+    if (LAM) {
+      const Mat2<R> d = sd[k];
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        if (r & bb) continue;
+        T d0, d1;
+        mat2_apply_k<REAL>(d, a[r], a[r | bb], d0, d1);
+        s += pair_term(l[r], d0, l[r | bb], d1);
+      }
+      acc[k] += s;
+    }
+    const Mat2<R> g = sg[k];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      if (r & bb) continue;
+      T y0, y1;
+      mat2_apply_k<REAL>(g, a[r], a[r | bb], y0, y1);
+      a[r] = y0;
+      a[r | bb] = y1;
+      if (LAM) {
+        mat2_apply_k<REAL>(g, l[r], l[r | bb], y0, y1);
+        l[r] = y0;
+        l[r | bb] = y1;
+      }
+    }
+  }
+}
+
+// Both states from layout F to layout TO through X.  A thread's slots of one
+// layout are the same for writing and reading, so writes in layout F need no
+// barrier after reads in layout F (FRONT = false: the last reads of X were in
+// layout F); lambda's writes wait until psi's reads in layout TO are done.
+template <int F, int TO, bool FRONT, typename T, bool LAM>
+__device__ __forceinline__ void layer_xpose(T (&a)[16], T (&l)[16], T *X, int t) {
+  if (FRONT) __syncthreads();
+  qtranspose(a, X, t, F, TO);
+  if (LAM) {
+    __syncthreads();
+    qtranspose(l, X, t, F, TO);
+  }
+}
+
+// Chunk access.  32-bit offsets: raw buffer loads / stores on a resource at
+// the chunk's base, the thread's part as the VGPR offset and the register's
+// part (uniform) as the SGPR offset, so a thread keeps one offset register
+// for all 16 amplitudes of both states.  64-bit offsets: flat global
+// addresses.  Both non-temporal, as the other sweeps.
+constexpr int kBufNT = 2;                // cache policy: non-temporal
+constexpr int kBufDword3 = 0x00020000;  // gfx9 raw buffer resource word 3
+
+template <typename T, typename O>
+struct ChunkIO;
+
+template <typename T>
+struct ChunkIO<T, uint32_t> {
+  __amdgpu_buffer_rsrc_t rs;
+  __device__ __forceinline__ explicit ChunkIO(T *base)
+      : rs(__builtin_amdgcn_make_buffer_rsrc(base, 0, -1, kBufDword3)) {}
+  __device__ __forceinline__ T load(uint32_t vo, uint32_t so) const {
+    if constexpr (sizeof(T) == 8)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)vo, (int)so, kBufNT));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)vo, (int)so, kBufNT));
+  }
+  __device__ __forceinline__ void store(uint32_t vo, uint32_t so, const T &v) const {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    if constexpr (sizeof(T) == 8)
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), rs, (int)vo, (int)so, kBufNT);
+    else
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), rs, (int)vo, (int)so, kBufNT);
+  }
+};
+
+template <typename T>
+struct ChunkIO<T, uint64_t> {
+  char *p;
+  __device__ __forceinline__ explicit ChunkIO(T *base) : p(reinterpret_cast<char *>(base)) {}
+  __device__ __forceinline__ T load(uint64_t vo, uint64_t so) const {
+    return ld_nt(reinterpret_cast<const T *>(p + (vo | so)));
+  }
+  __device__ __forceinline__ void store(uint64_t vo, uint64_t so, const T &v) const {
+    st_nt(reinterpret_cast<T *>(p + (vo | so)), v);
+  }
+};
+
+// PH: the phases the pass runs, bit 2: local positions 8-11 (layout L2),
+// bit 1: 4-7 (L1), bit 0: 0-3 (L0).  Layout path: L2 (load) -> [L1] ->
+// [L0 -> L2]; stores go out in L1 when that is the last layout, else in L2.
+// O: offset type (see LayerGeom).  REAL: see layer_phase.
+template <typename T, typename R, typename O, bool LAM, int PH, bool REAL>
+__global__ void __launch_bounds__(kBlock)
+    k_adjoint_layer(T *__restrict__ psi, T *__restrict__ lam, uint64_t nchunks, int W, int h0,
+                    LayerArgs<R> A, double *__restrict__ partial) {
+  constexpr bool P2 = PH & 4, P1 = PH & 2, P0 = PH & 1;
+  constexpr int NW = kBlock / 64;
+  constexpr int esh = sizeof(T) == 8 ? 3 : 4;
+  __shared__ T X[kQSlots];
+  __shared__ Mat2<R> sg[12], sd[12];
+  __shared__ double sred[12][NW];
+  const int t = threadIdx.x;
+  const int has = A.has;
+  int dmask = 0;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) dmask |= (A.slot[k] >= 0 ? 1 : 0) << k;
+  dmask &= has;
+  if (t < 24) {  // matrices to LDS: identity / zero at empty positions
+    const int k = t % 12;
+    const bool dv = t >= 12;
+    Mat2<R> m{};
+    if (dv ? ((dmask >> k) & 1) : ((has >> k) & 1)) {
+      m = dv ? A.d[k] : A.g[k];
+    } else if (!dv) {
+      m.r[0] = 1;
+      m.r[3] = 1;
+    }
+    (dv ? sd : sg)[k] = m;
+  }
+  __syncthreads();
+  double acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+  for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const LayerGeom<O> m = layer_geom<O>(c, W, h0, t, esh);
+    const ChunkIO<T, O> ia(psi + m.base), il(LAM ? lam + m.base : psi);
+    T a[16], l[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const O so = layer_r<O>(r, 8, W, h0, esh);
+      a[r] = ia.load(m.t2, so);
+      if (LAM) l[r] = il.load(m.t2, so);
+    }
+    if (P2) layer_phase<T, R, LAM, REAL>(a, l, sg, sd, 8, acc);
+    // the previous chunk's last reads of X: layout L1 (P1 only) or L2 (P0)
+    constexpr bool FRONT = P1 && !P0;
+    if (P1) {
+      layer_xpose<2, 1, FRONT, T, LAM>(a, l, X, t);
+      layer_phase<T, R, LAM, REAL>(a, l, sg, sd, 4, acc);
+    }
+    if (P0) {
+      if (P1)
+        layer_xpose<1, 0, false, T, LAM>(a, l, X, t);
+      else
+        layer_xpose<2, 0, false, T, LAM>(a, l, X, t);
+      layer_phase<T, R, LAM, REAL>(a, l, sg, sd, 0, acc);
+      layer_xpose<0, 2, false, T, LAM>(a, l, X, t);
+    }
+    constexpr bool L1OUT = P1 && !P0;
+    const O vo = L1OUT ? m.t1 : m.t2;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const O so = L1OUT ? layer_r<O>(r, 4, W, h0, esh) : layer_r<O>(r, 8, W, h0, esh);
+      ia.store(vo, so, a[r]);
+      if (LAM) il.store(vo, so, l[r]);
+    }
+  }
+  if (!LAM) return;
+  const int lane = t & 63, wave = t >> 6;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    if (!((dmask >> k) & 1)) continue;
+    const double v = wave_sum_d(acc[k]);
+    if (lane == 0) sred[k][wave] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {  // thread k writes position k's row
+    if (t != k || !((dmask >> k) & 1)) continue;
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) v += sred[k][w];
+    partial[(uint64_t)A.slot[k] * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+// Instantiated phase sets: L1 alone (windows of up to 4 high bits), L2 + L1
+// (windows of 5-8 bits) and all three (the low pass); a pass runs the
+// smallest set that holds its gates, empty positions as identities.
+template <typename T, typename R, typename O, bool LAM, bool REAL>
+static void launch_layer_r(void *psi, void *lam, uint64_t nchunks, int W, int h0,
+                           const LayerArgs<R> &a, double *partial, int nb, hipStream_t st) {
+  const int need = (((a.has >> 8) & 15) ? 4 : 0) | (((a.has >> 4) & 15) ? 2 : 0) |
+                   ((a.has & 15) ? 1 : 0);
+  void (*k)(T *, T *, uint64_t, int, int, LayerArgs<R>, double *) =
+      (need & ~2) == 0   ? k_adjoint_layer<T, R, O, LAM, 2, REAL>
+      : (need & ~6) == 0 ? k_adjoint_layer<T, R, O, LAM, 6, REAL>
+                         : k_adjoint_layer<T, R, O, LAM, 7, REAL>;
+  hipLaunchKernelGGL(k, dim3(nb), dim3(kBlock), 0, st, (T *)psi, (T *)lam, nchunks, W, h0, a,
+                     partial);
+}
+
+template <typename T, typename R, typename O, bool LAM>
+static void launch_layer_o(void *psi, void *lam, uint64_t nchunks, int W, int h0,
+                           const LayerArgs<R> &a, double *partial, int nb, hipStream_t st) {
+  bool real = true;
+  for (int k = 0; k < 12; ++k) {
+    if (!((a.has >> k) & 1)) continue;
+    for (int e = 0; e < 4; ++e)
+      real = real && a.g[k].i[e] == 0 && (!LAM || a.slot[k] < 0 || a.d[k].i[e] == 0);
+  }
+  if (real)
+    launch_layer_r<T, R, O, LAM, true>(psi, lam, nchunks, W, h0, a, partial, nb, st);
+  else
+    launch_layer_r<T, R, O, LAM, false>(psi, lam, nchunks, W, h0, a, partial, nb, st);
+}
+
+// 32-bit byte offsets when every byte of the pass lies below 2^31 from the
+// chunk base (well inside the resource's 2^32 - 1 byte range).
+template <typename T, typename R, bool LAM>
+static void launch_layer_t(void *psi, void *lam, uint64_t nchunks, int W, int h0,
+                           const LayerArgs<R> &a, double *partial, int nb, hipStream_t st) {
+  const int top = W == 0 ? 12 : h0 + W;  // offsets below 2^top elements
+  if (top + (sizeof(T) == 8 ? 3 : 4) <= 31)
+    launch_layer_o<T, R, uint32_t, LAM>(psi, lam, nchunks, W, h0, a, partial, nb, st);
+  else
+    launch_layer_o<T, R, uint64_t, LAM>(psi, lam, nchunks, W, h0, a, partial, nb, st);
+}
+
+template <typename R>
+static LayerArgs<R> layer_args(int count, const int *pos, const int *slot, const double *gm,
+                               const double *dm) {
+  LayerArgs<R> a{};
+  for (int k = 0; k < 12; ++k) a.slot[k] = -1;
+  for (int j = 0; j < count; ++j) {
+    const int k = pos[j];
+    a.has |= 1 << k;
+    a.slot[k] = slot ? slot[j] : -1;
+    for (int e = 0; e < 4; ++e) {
+      a.g[k].r[e] = (R)gm[8 * j + 2 * e];
+      a.g[k].i[e] = (R)gm[8 * j + 2 * e + 1];
+      if (dm) {
+        a.d[k].r[e] = (R)dm[8 * j + 2 * e];
+        a.d[k].i[e] = (R)dm[8 * j + 2 * e + 1];
+      }
+    }
+  }
+  return a;
+}
+
+int layer_pass_blocks(uint64_t n) {
+  const uint64_t nchunks = n >> 12;
+  return (int)(nchunks < kLayerMaxBlocks ? nchunks : kLayerMaxBlocks);
+}
+
+int launch_adjoint_layer(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int h0, int W,
+                         int count, const int *pos, const double *gm, const double *dm,
+                         const int *slot, int nslots, const int *slot_param_dev, bool distinct,
+                         double *partial, double *out_dev, hipStream_t st) {
+  if (n < 4096 || (n & 4095) || count < 1 || count > 12 || W < 0 || W > 8)
+    return fail(SQ_ERR_INVALID, "internal: layer pass shape");
+  const int nq = __builtin_ctzll(n);
+  if (W == 0 ? h0 != 12 : (h0 < 12 || h0 + W > nq))
+    return fail(SQ_ERR_INVALID, "internal: layer pass window");
+  int seen = 0;
+  for (int j = 0; j < count; ++j) {
+    if (pos[j] < 0 || pos[j] > 11 || ((seen >> pos[j]) & 1))
+      return fail(SQ_ERR_INVALID, "internal: layer pass position");
+    seen |= 1 << pos[j];
+    if (lam && slot && (slot[j] < -1 || slot[j] >= nslots))
+      return fail(SQ_ERR_INVALID, "internal: layer pass slot");
+  }
+  if (lam && nslots > 12) return fail(SQ_ERR_INVALID, "internal: layer pass slots");
+  const uint64_t nchunks = n >> 12;
+  const int nb = layer_pass_blocks(n);
+  if (dt == SQ_COMPLEX64) {
+    const LayerArgs<float> a = layer_args<float>(count, pos, lam ? slot : nullptr, gm,
+                                                 lam ? dm : nullptr);
+    if (lam)
+      launch_layer_t<float2, float, true>(psi, lam, nchunks, W, h0, a, partial, nb, st);
+    else
+      launch_layer_t<float2, float, false>(psi, nullptr, nchunks, W, h0, a, nullptr, nb, st);
+  } else {
+    const LayerArgs<double> a = layer_args<double>(count, pos, lam ? slot : nullptr, gm,
+                                                   lam ? dm : nullptr);
+    if (lam)
+      launch_layer_t<double2, double, true>(psi, lam, nchunks, W, h0, a, partial, nb, st);
+    else
+      launch_layer_t<double2, double, false>(psi, nullptr, nchunks, W, h0, a, nullptr, nb, st);
+  }
+  SQ_HIP(hipGetLastError());
+  if (lam && nslots > 0)
+    SQ_TRY(launch_finalize_slots(partial, nb, nslots, slot_param_dev, distinct, 2.0, out_dev, st));
+  return SQ_OK;
+}
+
+}  // namespace sq

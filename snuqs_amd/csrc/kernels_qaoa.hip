@@ -28,11 +28,10 @@
 
 #include "kernels.hpp"
 #include "kernels_qaoa.hpp"
+#include "chunk_layout.hpp"
 
 namespace sq {
 
-constexpr int kQL = 12;            // log2 chunk size
-constexpr int kQChunk = 1 << kQL;  // amplitudes per chunk
 constexpr int kQMaxEdges = 128;
 constexpr int kQGM = 8;            // high bits per mid pass
 
@@ -135,37 +134,6 @@ __device__ __forceinline__ uint64_t slot_gamma(const QInstDev &q, int l) {
 }
 __device__ __forceinline__ uint64_t slot_beta(const QInstDev &q, int l, int part) {
   return slot_gamma(q, l) + (uint64_t)q.nchunks * (1 + part);
-}
-
-// ---- full 4096-amplitude chunks (n >= 12): register phases ----------------
-// Thread t holds 16 amplitudes of the chunk in one of three layouts, each
-// putting 4 index bits in the register index r:
-//   L2: j = t + 256 r (bits 8-11 in r; the global load / store layout, coalesced)
-//   L1: j = (t & 15) | r << 4 | (t >> 4) << 8      (bits 4-7 in r)
-//   L0: j = r | t << 4                             (bits 0-3 in r)
-// RX on the r bits is applied in registers; layouts change through LDS with
-// slot swz(j) = j + j / 16 (one pad slot per 16), which makes every slot a
-// per-thread base plus a multiple of r -- 17 r (L0, L1: the pads shift the
-// 16-element rows) or 272 r (L2) -- so the addresses fold into instruction
-// offsets, and keeps the three patterns free of bank conflicts for 8 B and
-// 16 B elements (L2 with 8 B elements: one 2-way pair per 32 lanes).
-constexpr int kQSlots = kQChunk + kQChunk / 16;
-__device__ __forceinline__ int qswz(int j) { return j + (j >> 4); }
-__device__ __forceinline__ int qlay(int layout, int t, int r) {
-  return layout == 2 ? t + 256 * r
-         : layout == 1 ? (t & 15) | (r << 4) | ((t >> 4) << 8)
-                       : r | (t << 4);
-}
-
-// Own 16 slots out in layout `from`, back in layout `to` (one barrier; the
-// caller separates successive uses of X by a barrier).
-template <typename T>
-__device__ __forceinline__ void qtranspose(T (&v)[16], T *X, int t, int from, int to) {
-#pragma unroll
-  for (int r = 0; r < 16; ++r) X[qswz(qlay(from, t, r))] = v[r];
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < 16; ++r) v[r] = X[qswz(qlay(to, t, r))];
 }
 
 // RX(2b) (dir = +1) or RX(-2b) (dir = -1) on the 4 register-index bits.
@@ -338,11 +306,7 @@ __global__ void __launch_bounds__(kBlock)
   qrx16(v, c0, s0);                // bits 8-11
   qtranspose(v, X, t, 2, 1);
   qrx16(v, c0, s0);                // bits 4-7
-#pragma unroll
-  for (int r = 0; r < 16; ++r) X[qswz(qlay(1, t, r))] = v[r];
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < 16; ++r) v[r] = X[qswz(qlay(0, t, r))];
+  qtranspose(v, X, t, 1, 0);       // writes the L1 slots this thread just read
   qrx16(v, c0, s0);                // bits 0-3
   qtranspose(v, X, t, 0, 2);
 #pragma unroll
@@ -537,19 +501,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
 }
 
 // ---- mid pass: RX(2b) (forward) or RX(-2b) and the beta partial (reverse) on
-// W = min(8, n - h0) high bits h0 .. h0+W-1 -----------------------------------
-// A workgroup holds 2^12 amplitudes whose local index j maps to the global
-// index bit by bit: local bits 0-3 -> global bits 0-3 (16 consecutive
-// amplitudes, 128 B of complex64), local bits 4..3+W -> h0..h0+W-1 (the bits
-// rotated), local bits 4+W..11 -> global bits 4..11-W (carried unchanged);
-// the chunk number fills the other n - 12 bits.  Loads use layout L2 (local
-// bits 8-11 in the register index), then one LDS transpose to L1 (bits 4-7),
-// and the stores go out in L1.  Eight rotated bits per pass, against the 4-5
-// a lane can hold in registers without LDS, halves the passes over high bits.
-__device__ __forceinline__ int qmid_pos(int k, int W, int h0) {
-  return k < 4 ? k : k < 4 + W ? h0 + (k - 4) : k - W;
-}
-
+// Mid-pass geometry: see qmid_pos (chunk_layout.hpp).
 struct QMid {
   uint32_t base;     // global offset of the chunk (uniform)
   uint32_t t2, t1;   // thread part of the L2 / L1 offsets

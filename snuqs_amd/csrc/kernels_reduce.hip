@@ -202,20 +202,6 @@ __global__ void __launch_bounds__(kBlock)
 // over both states instead of separate undo, copy, derivative and inner
 // product sweeps.  Gate arithmetic in the state's precision, the inner
 // product in FP64.
-template <typename R>
-struct Mat2 {
-  R r[4], i[4];  // row-major 2x2
-};
-
-template <typename T, typename R>
-__device__ __forceinline__ void mat2_apply(const Mat2<R> &m, const T &x0, const T &x1, T &y0,
-                                           T &y1) {
-  y0.x = m.r[0] * x0.x - m.i[0] * x0.y + m.r[1] * x1.x - m.i[1] * x1.y;
-  y0.y = m.r[0] * x0.y + m.i[0] * x0.x + m.r[1] * x1.y + m.i[1] * x1.x;
-  y1.x = m.r[2] * x0.x - m.i[2] * x0.y + m.r[3] * x1.x - m.i[3] * x1.y;
-  y1.y = m.r[2] * x0.y + m.i[2] * x0.x + m.r[3] * x1.y + m.i[3] * x1.x;
-}
-
 template <typename T, typename R>
 __global__ void __launch_bounds__(kBlock)
     k_adjoint_step1(T *__restrict__ psi, T *__restrict__ lam, uint64_t npair, int bit,
@@ -267,12 +253,6 @@ __global__ void __launch_bounds__(kBlock)
 __device__ __forceinline__ int diag_loc(const DiagRunGate &g, uint64_t i) {
   return g.b1 < 0 ? (int)((i >> g.b0) & 1)
                   : (int)((((i >> g.b0) & 1) << 1) | ((i >> g.b1) & 1));
-}
-
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
 }
 
 template <typename T, typename R, bool LAM>
@@ -470,6 +450,15 @@ __global__ void __launch_bounds__(kBlock)
     if (threadIdx.x == 0) out[2 * slot_param[v]] += alpha * acc;
     __syncthreads();  // block_sum2's LDS is reused by the next slot
   }
+}
+
+int launch_finalize_slots(const double *partial, int nblocks, int nslots,
+                          const int *slot_param_dev, bool distinct, double alpha, double *out,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize_slots, dim3(distinct ? nslots : 1), dim3(kBlock), 0, st,
+                     partial, nblocks, nslots, slot_param_dev, alpha, out);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
 }
 
 // ---- reverse step over a group of one-target gates on distinct bits ------------

@@ -1,0 +1,134 @@
+"""Adjoint layer passes (launch_adjoint_layer): layers of one-target gates on
+distinct qubits applied 12 or up to 8 bits a pass in both sweeps, checked
+against the CPU oracle and against the same gradient with the layer passes
+switched off (register groups and single steps instead)."""
+import numpy as np
+import pytest
+
+from helpers import device_state
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _energy_oracle(n, ops, paulis, qubits, coeffs):
+    st = O.basis_state(n)
+    for op in ops:
+        O.apply(st, np.array(op.matrix()), op.targets, n)
+    return sum(c * O.expect_pauli(st, p, qs, n).real for p, qs, c in zip(paulis, qubits, coeffs))
+
+
+def _layer_circuit(n, thetas):
+    """Layers of every kind: RotY on all qubits (own parameters), H on every
+    other qubit (no derivative), RotX on a scattered subset with one shared
+    parameter, a U layer (three parameters: not a layer gate) and CNOT
+    ladders between them."""
+    from snuqs_amd import _C as C
+
+    g = C.operation.gate_operations
+    ops, pmap = [], []
+
+    def add(op, params=()):
+        ops.append(op)
+        pmap.append(list(params))
+
+    k = 0
+    for rep in range(2):
+        for q in range(n):
+            add(g.RotY([q], thetas[k]), [(0, k)])
+            k += 1
+        for q in range(0, n, 2):
+            add(g.Hadamard([q]))
+        for q in range(n - 1, -1, -3):
+            add(g.RotX([q], thetas[k]), [(0, k)])
+        k += 1
+        for q in range(n - 1):
+            add(g.CX([q, q + 1]))
+        for q in range(1, n, 4):
+            add(g.U([q], thetas[k], thetas[k + 1], thetas[k + 2]),
+                [(0, k), (1, k + 1), (2, k + 2)])
+        k += 3
+        for q in range(n):
+            add(g.RotZ([q], thetas[k]), [(0, k)])
+            k += 1
+    return ops, pmap, k
+
+
+def _nparams(n):
+    return _layer_circuit(n, np.zeros(10 * n))[2]
+
+
+def _adjoint(C, n, ops, pmap, nparams, obs, dtype, checkpoints, layers):
+    saved = C.core.tune_get("adjoint_layers")
+    C.core.tune_set("adjoint_layers", layers)
+    try:
+        prog = C.functionals.Program(n, ops, dtype=dtype)
+        sv, w0, w1 = (device_state(C, n, dtype) for _ in range(3))
+        cps = [device_state(C, n, dtype) for _ in range(checkpoints)]
+        return prog.adjoint(sv, w0, w1, cps, pmap, nparams, *obs)
+    finally:
+        C.core.tune_set("adjoint_layers", saved)
+
+
+@pytest.mark.parametrize("dtype,etol,gtol", [("complex128", 1e-10, 1e-7),
+                                             ("complex64", 1e-5, 1e-4)])
+@pytest.mark.parametrize("n,checkpoints", [(12, 0), (13, 3), (14, 2)])
+def test_layer_passes_vs_oracle_finite_differences(C, n, checkpoints, dtype, etol, gtol):
+    """12-14 qubits: the low pass (bits 0-11) and a 1-2 bit mid window, with
+    checkpoint bounds moved to layer starts; energy and gradients against
+    central differences of the FP64 oracle."""
+    from snuqs_amd.circuits import random_pauli_sum
+
+    nparams = _nparams(n)
+    thetas = np.random.default_rng(n).uniform(0, 2 * np.pi, nparams)
+    ops, pmap, _ = _layer_circuit(n, thetas)
+    obs = random_pauli_sum(n, 6, 3, np.random.default_rng(5))
+    e, grads = _adjoint(C, n, ops, pmap, nparams, obs, dtype, checkpoints, 1)
+    assert abs(e - _energy_oracle(n, ops, *obs)) < etol
+    h = 1e-6
+    for p in range(nparams):
+        up, dn = thetas.copy(), thetas.copy()
+        up[p] += h
+        dn[p] -= h
+        fd = (_energy_oracle(n, _layer_circuit(n, up)[0], *obs) -
+              _energy_oracle(n, _layer_circuit(n, dn)[0], *obs)) / (2 * h)
+        assert abs(grads[p] - fd) < gtol, (p, grads[p], fd)
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-10), ("complex64", 5e-5)])
+@pytest.mark.parametrize("n,checkpoints", [(16, 0), (20, 2), (22, 3), (23, 1)])
+def test_layer_passes_match_groups(C, n, checkpoints, dtype, tol):
+    """16-23 qubits (mid windows of 4 and 8 bits; at 22 and 23 qubits the two
+    high windows overlap and each gate goes to one of them): layer passes on
+    and off give the same energy and gradients."""
+    from snuqs_amd.circuits import random_pauli_sum
+
+    nparams = _nparams(n)
+    thetas = np.random.default_rng(2 * n).uniform(0, 2 * np.pi, nparams)
+    ops, pmap, _ = _layer_circuit(n, thetas)
+    obs = random_pauli_sum(n, 8, 4, np.random.default_rng(n))
+    e1, g1 = _adjoint(C, n, ops, pmap, nparams, obs, dtype, checkpoints, 1)
+    e0, g0 = _adjoint(C, n, ops, pmap, nparams, obs, dtype, checkpoints, 0)
+    print(f"{n}q {dtype}: energy diff {abs(e1 - e0):.2e}, max grad diff "
+          f"{np.abs(g1 - g0).max():.2e}")
+    assert abs(e1 - e0) < tol
+    assert np.abs(g1 - g0).max() < tol
+
+
+def test_layer_passes_64bit_offsets_28q(C):
+    """28 qubits in complex128 (4 GiB a state): the window over bits 20-27
+    addresses past 2^31 bytes from a chunk base and takes the 64-bit path."""
+    from snuqs_amd import _C
+
+    g = _C.operation.gate_operations
+    n = 28
+    rng = np.random.default_rng(28)
+    th = rng.uniform(0, 2 * np.pi, n)
+    ops = [g.RotY([q], th[q]) for q in range(n)] + [g.CX([q, q + 1]) for q in range(0, n - 1, 2)]
+    pmap = [[(0, q)] for q in range(n)] + [[] for _ in range(0, n - 1, 2)]
+    obs = (["ZZ", "XZ", "YY"], [[0, n - 1], [3, 20], [11, 25]], [0.7, -1.1, 0.4])
+    e1, g1 = _adjoint(C, n, ops, pmap, n, obs, "complex128", 0, 1)
+    e0, g0 = _adjoint(C, n, ops, pmap, n, obs, "complex128", 0, 0)
+    assert abs(e1 - e0) < 1e-11
+    assert np.abs(g1 - g0).max() < 1e-11
+    assert np.abs(g1).max() > 1e-3  # a non-trivial gradient
