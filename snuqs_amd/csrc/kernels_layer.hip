@@ -212,9 +212,12 @@ struct ChunkIO<T, uint64_t> {
 // PH: the phases the pass runs, bit 2: local positions 8-11 (layout L2),
 // bit 1: 4-7 (L1), bit 0: 0-3 (L0).  Layout path: L2 (load) -> [L1] ->
 // [L0 -> L2]; stores go out in L1 when that is the last layout, else in L2.
-// O: offset type (see LayerGeom).  REAL: see layer_phase.
+// O: offset type (see LayerGeom).  REAL: see layer_phase.  complex128 asks
+// for 2 waves per SIMD (otherwise 1: 256 VGPRs and AGPR spills; 5% faster
+// measured on the 24-qubit ansatz gradient); complex64 keeps the compiler's
+// 3 (4 would spill to scratch and measured no faster).
 template <typename T, typename R, typename O, bool LAM, int PH, bool REAL>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 1 : 2)
     k_adjoint_layer(T *__restrict__ psi, T *__restrict__ lam, uint64_t nchunks, int W, int h0,
                     LayerArgs<R> A, double *__restrict__ partial) {
   constexpr bool P2 = PH & 4, P1 = PH & 2, P0 = PH & 1;
@@ -298,16 +301,19 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Instantiated phase sets: L1 alone (windows of up to 4 high bits), L2 + L1
-// (windows of 5-8 bits) and all three (the low pass); a pass runs the
-// smallest set that holds its gates, empty positions as identities.
+// Instantiated phase sets: L2 alone (a window whose gates sit on its top 4
+// bits: no transpose at all), L1 alone (up to 4 bits at the window's
+// bottom), L2 + L1 (windows of 5-8 bits) and all three (the low pass); a
+// pass runs the smallest set that holds its gates, empty positions as
+// identities.
 template <typename T, typename R, typename O, bool LAM, bool REAL>
 static void launch_layer_r(void *psi, void *lam, uint64_t nchunks, int W, int h0,
                            const LayerArgs<R> &a, double *partial, int nb, hipStream_t st) {
   const int need = (((a.has >> 8) & 15) ? 4 : 0) | (((a.has >> 4) & 15) ? 2 : 0) |
                    ((a.has & 15) ? 1 : 0);
   void (*k)(T *, T *, uint64_t, int, int, LayerArgs<R>, double *) =
-      (need & ~2) == 0   ? k_adjoint_layer<T, R, O, LAM, 2, REAL>
+      need == 4          ? k_adjoint_layer<T, R, O, LAM, 4, REAL>
+      : (need & ~2) == 0 ? k_adjoint_layer<T, R, O, LAM, 2, REAL>
       : (need & ~6) == 0 ? k_adjoint_layer<T, R, O, LAM, 6, REAL>
                          : k_adjoint_layer<T, R, O, LAM, 7, REAL>;
   hipLaunchKernelGGL(k, dim3(nb), dim3(kBlock), 0, st, (T *)psi, (T *)lam, nchunks, W, h0, a,
