@@ -69,4 +69,50 @@ __device__ __forceinline__ int qmid_pos(int k, int W, int h0) {
   return k < 4 ? k : k < 4 + W ? h0 + (k - 4) : k - W;
 }
 
+// Chunk access.  32-bit offsets: raw buffer loads / stores on a resource at
+// the chunk's base (wave-uniform), the thread's part of the byte offset as
+// the VGPR offset and the register's part (uniform) as the SGPR offset, so a
+// thread keeps one offset register for all 16 amplitudes of a state.  The
+// caller keeps every byte it touches below 2^31 from the base (well inside
+// the resource's 2^32 - 1 byte range).  64-bit offsets: flat global
+// addresses.  Both non-temporal, as the other sweeps.
+constexpr int kBufNT = 2;                // cache policy: non-temporal
+constexpr int kBufDword3 = 0x00020000;  // gfx9 raw buffer resource word 3
+
+template <typename T, typename O>
+struct ChunkIO;
+
+template <typename T>
+struct ChunkIO<T, uint32_t> {
+  __amdgpu_buffer_rsrc_t rs;
+  __device__ __forceinline__ explicit ChunkIO(T *base)
+      : rs(__builtin_amdgcn_make_buffer_rsrc(base, 0, -1, kBufDword3)) {}
+  __device__ __forceinline__ T load(uint32_t vo, uint32_t so) const {
+    if constexpr (sizeof(T) == 8)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)vo, (int)so, kBufNT));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)vo, (int)so, kBufNT));
+  }
+  __device__ __forceinline__ void store(uint32_t vo, uint32_t so, const T &v) const {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    if constexpr (sizeof(T) == 8)
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), rs, (int)vo, (int)so, kBufNT);
+    else
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), rs, (int)vo, (int)so, kBufNT);
+  }
+};
+
+template <typename T>
+struct ChunkIO<T, uint64_t> {
+  char *p;
+  __device__ __forceinline__ explicit ChunkIO(T *base) : p(reinterpret_cast<char *>(base)) {}
+  __device__ __forceinline__ T load(uint64_t vo, uint64_t so) const {
+    return ld_nt(reinterpret_cast<const T *>(p + (vo | so)));
+  }
+  __device__ __forceinline__ void store(uint64_t vo, uint64_t so, const T &v) const {
+    st_nt(reinterpret_cast<T *>(p + (vo | so)), v);
+  }
+};
+
 }  // namespace sq
