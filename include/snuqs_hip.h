@@ -83,7 +83,10 @@ int sq_copy(void *dst, const void *src, size_t bytes, void *stream);
  * bit-reversed order; applies to batches run or captured afterwards) and
  * "tune_scatter" (0/1, sq_program_tune also times tile order 2; default 1) and
  * "adjoint_layers" (0/1, sq_adjoint_gradient applies layers of one-target
- * gates on distinct qubits in passes of up to 12 bits; default 1).  A gate tuned by
+ * gates on distinct qubits in passes of up to 12 bits; default 1) and
+ * "adjoint_wide_windows" (0/1, those passes take windows of up to 6 high bits
+ * with 512 B runs where that needs no more passes than 8-bit windows with
+ * 128 B runs; default 1).  A gate tuned by
  * sq_program_tune uses its own shape instead.  No reference counterpart. */
 int sq_tune_set(const char *key, int value);
 int sq_tune_get(const char *key, int *value);

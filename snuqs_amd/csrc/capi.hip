@@ -163,6 +163,7 @@ static int *tune_slot(const std::string &k) {
       {"qaoa_mid_scatter", [] { return &tuning().qaoa_mid_scatter; }},
       {"tune_scatter", [] { return &tuning().tune_scatter; }},
       {"adjoint_layers", [] { return &tuning().adjoint_layers; }},
+      {"adjoint_wide_windows", [] { return &tuning().adjoint_wide_windows; }},
       {"mfma5_wide_c64", [] { return &tuning().mfma5_wide; }},
   };
   for (auto &e : table)
@@ -179,7 +180,8 @@ int sq_tune_set(const char *key, int value) {
   if (k.rfind("mfma5_order", 0) == 0 && (value < 0 || value > 2))
     return fail(SQ_ERR_INVALID, "mfma5_order must be 0 (cyclic), 1 (blocked) or 2 (scattered)");
   const bool flag = k.rfind("mfma5_pipeline", 0) == 0 || k.rfind("mfma5_wide", 0) == 0 ||
-                    k == "qaoa_mid_scatter" || k == "tune_scatter" || k == "adjoint_layers";
+                    k == "qaoa_mid_scatter" || k == "tune_scatter" || k == "adjoint_layers" ||
+                    k == "adjoint_wide_windows";
   *slot = flag ? (value ? 1 : 0) : value;
   return SQ_OK;
 }
@@ -1243,28 +1245,34 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   // Layers: consecutive one-target gates on distinct qubits (at most one
   // derivative each in the reverse sweep), applied by layer passes
   // (launch_adjoint_layer): the gates on bits 0-11 in one pass, higher bits in
-  // windows of up to 8 bits, one pass each.  A layer is taken where it needs
-  // fewer passes than the register groups below would.
+  // windows, one pass each: windows of up to 6 bits beside 6 contiguous low
+  // bits (512 B runs) where they need no more passes than windows of up to
+  // 8 bits beside 4 (128 B runs).  A layer is taken where it needs fewer
+  // passes than the register groups below would.
   struct Win {
-    int h0, W;
+    int h0, W, C;
     uint64_t own;  // the layer's bits this window applies
   };
-  auto layer_windows = [&](uint64_t bits) {
-    std::vector<Win> w;
-    if (bits & 0xFFFull) w.push_back({12, 0, bits & 0xFFFull});
-    uint64_t hi = bits & ~0xFFFull;
+  auto windows_of = [&](uint64_t hi, int wmax, int C, std::vector<Win> *w) {
     while (hi) {
       const int b = __builtin_ctzll(hi);
-      const int h0 = std::max(12, std::min(b, (int)p->nq - 8));
-      const int W = std::min(8, (int)p->nq - h0);
+      const int h0 = std::max(12, std::min(b, (int)p->nq - wmax));
+      const int W = std::min(wmax, (int)p->nq - h0);
       const uint64_t cover = ((1ull << W) - 1) << h0;
-      w.push_back({h0, W, hi & cover});
+      w->push_back({h0, W, C, hi & cover});
       hi &= ~cover;
     }
-    return w;
+  };
+  auto layer_windows = [&](uint64_t bits) {
+    std::vector<Win> w, w6;
+    if (bits & 0xFFFull) w.push_back({12, 0, 4, bits & 0xFFFull});
+    w6 = w;
+    windows_of(bits & ~0xFFFull, 8, 4, &w);
+    windows_of(bits & ~0xFFFull, 6, 6, &w6);
+    return tuning().adjoint_wide_windows && w6.size() <= w.size() ? w6 : w;
   };
   struct LPass {
-    int h0, W, count;
+    int h0, W, C, count;
     int pos[12], slot[12];
     double gm[96], dm[96];
     uint32_t first_slot, nslots;
@@ -1279,12 +1287,13 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       LPass ps{};
       ps.h0 = w.h0;
       ps.W = w.W;
+      ps.C = w.C;
       ps.first_slot = (uint32_t)rslots.size();
       for (uint32_t q = lo; q < lo + m; ++q) {
         const int b = gbit(q);
         if (!((w.own >> b) & 1)) continue;
         const int j = ps.count++;
-        ps.pos[j] = w.W == 0 ? b : 4 + (b - w.h0);
+        ps.pos[j] = w.W == 0 ? b : w.C + (b - w.h0);
         ps.slot[j] = -1;
         const double *G = p->mats.data() + p->moffs[q];
         if (!rev) {
@@ -1585,7 +1594,7 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       const Layer &L = flayers[flayer_start[g]];
       for (uint32_t i = L.first_pass; i < L.first_pass + L.npass; ++i) {
         const LPass &ps = fpasses[i];
-        SQ_TRY(launch_adjoint_layer(sv, nullptr, dt, n, ps.h0, ps.W, ps.count, ps.pos, ps.gm,
+        SQ_TRY(launch_adjoint_layer(sv, nullptr, dt, n, ps.h0, ps.W, ps.C, ps.count, ps.pos, ps.gm,
                                     nullptr, nullptr, 0, nullptr, false, nullptr, nullptr, st));
       }
       g = L.lo + L.m - 1;
@@ -1639,7 +1648,8 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       const Layer &L = layers[layer_top[g]];
       for (uint32_t i = L.first_pass; i < L.first_pass + L.npass; ++i) {
         const LPass &ps = lpasses[i];
-        SQ_TRY(launch_adjoint_layer(sv, work0, dt, n, ps.h0, ps.W, ps.count, ps.pos, ps.gm, ps.dm,
+        SQ_TRY(launch_adjoint_layer(sv, work0, dt, n, ps.h0, ps.W, ps.C, ps.count, ps.pos, ps.gm,
+                                    ps.dm,
                                     ps.slot, (int)ps.nslots, rslots_dev + ps.first_slot,
                                     distinct_params(ps.first_slot, ps.nslots), rpartial, gslots,
                                     st));

@@ -125,6 +125,7 @@ struct Tuning {
   int qaoa_mid_scatter = 0;              // QAOA mid passes: bit-reversed chunk order
   int tune_scatter = 1;                  // sq_program_tune also tries tile order 2
   int adjoint_layers = 1;                // adjoint sweeps: layer passes (12 / 8 bits a pass)
+  int adjoint_wide_windows = 1;          // layer passes: 6-bit windows with 512 B runs
 };
 Tuning &tuning();
 

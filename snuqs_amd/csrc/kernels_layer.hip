@@ -4,10 +4,13 @@
 // The reference has no gradient code (it simulates circuits and reads
 // results); these kernels serve sq_adjoint_gradient's forward and reverse
 // sweeps.  A workgroup holds 4096 amplitudes of psi (and of lambda) in
-// registers, 16 per thread, whose local index j maps to the global index as
-// in the batched QAOA mid passes: local bits 0-3 -> global bits 0-3, local
-// bits 4..3+W -> h0..h0+W-1, local bits 4+W..11 -> global bits 4..11-W
-// (W = 0: the identity, the "low" pass over bits 0-11).  The 16 registers hold
+// registers, 16 per thread, whose local index j maps to the global index
+// with C = 4 or 6 contiguous low bits: local bits 0..C-1 -> global bits
+// 0..C-1, local bits C..C+W-1 -> a window h0..h0+W-1 of higher bits, local
+// bits C+W..11 -> global bits C..11-W (W = 0: the identity, the "low" pass
+// over bits 0-11).  C = 4 (windows of up to 8 bits, the geometry of the
+// batched QAOA mid passes) loads 128 B runs (16 complex64 amplitudes); C = 6
+// (windows of up to 6 bits) loads and stores 512 B runs.  The 16 registers hold
 // local bits 8-11 (layout L2, the load layout), 4-7 (L1) or 0-3 (L0); a gate
 // whose local bit is in the register index is applied in registers, and the
 // layouts change through LDS.  So one pass applies up to 12 gates (all of
@@ -49,20 +52,20 @@ struct LayerGeom {
   O t2, t1;
 };
 
-__device__ __forceinline__ int layer_pos(int k, int W, int h0) {
-  return k < 4 ? k : k < 4 + W ? h0 + (k - 4) : k - W;
+__device__ __forceinline__ int layer_pos(int k, int W, int h0, int C) {
+  return k < C ? k : k < C + W ? h0 + (k - C) : k - W;
 }
 
 template <typename O>
-__device__ __forceinline__ O layer_r(int r, int lo, int W, int h0, int esh) {
+__device__ __forceinline__ O layer_r(int r, int lo, int W, int h0, int C, int esh) {
   O o = 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) o |= (O)((r >> k) & 1) << (layer_pos(lo + k, W, h0) + esh);
+  for (int k = 0; k < 4; ++k) o |= (O)((r >> k) & 1) << (layer_pos(lo + k, W, h0, C) + esh);
   return o;
 }
 
 template <typename O>
-__device__ __forceinline__ LayerGeom<O> layer_geom(uint64_t chunk, int W, int h0, int t,
+__device__ __forceinline__ LayerGeom<O> layer_geom(uint64_t chunk, int W, int h0, int C, int t,
                                                    int esh) {
   LayerGeom<O> m;
   const int A = h0 - 12 + W;  // chunk bits at 12-W .. h0-1, the rest from h0+W up
@@ -70,9 +73,10 @@ __device__ __forceinline__ LayerGeom<O> layer_geom(uint64_t chunk, int W, int h0
   m.t2 = (O)(t & 15) << esh;
   m.t1 = (O)(t & 15) << esh;
 #pragma unroll
-  for (int k = 4; k < 8; ++k) m.t2 |= (O)((t >> k) & 1) << (layer_pos(k, W, h0) + esh);
+  for (int k = 4; k < 8; ++k) m.t2 |= (O)((t >> k) & 1) << (layer_pos(k, W, h0, C) + esh);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) m.t1 |= (O)((t >> (4 + k)) & 1) << (layer_pos(8 + k, W, h0) + esh);
+  for (int k = 0; k < 4; ++k)
+    m.t1 |= (O)((t >> (4 + k)) & 1) << (layer_pos(8 + k, W, h0, C) + esh);
   return m;
 }
 
@@ -172,10 +176,12 @@ __device__ __forceinline__ void layer_xpose(T (&a)[16], T (&l)[16], T *X, int t)
 // for 2 waves per SIMD (otherwise 1: 256 VGPRs and AGPR spills; 5% faster
 // measured on the 24-qubit ansatz gradient); complex64 keeps the compiler's
 // 3 (4 would spill to scratch and measured no faster).
-template <typename T, typename R, typename O, bool LAM, int PH, bool REAL>
+// WIDE (C = 6): the L1 phase transposes back to L2 before the stores, so
+// they go out as 512 B runs too.
+template <typename T, typename R, typename O, bool LAM, int PH, bool REAL, bool WIDE>
 __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 1 : 2)
     k_adjoint_layer(T *__restrict__ psi, T *__restrict__ lam, uint64_t nchunks, int W, int h0,
-                    LayerArgs<R> A, double *__restrict__ partial) {
+                    int C, LayerArgs<R> A, double *__restrict__ partial) {
   constexpr bool P2 = PH & 4, P1 = PH & 2, P0 = PH & 1;
   constexpr int NW = kBlock / 64;
   constexpr int esh = sizeof(T) == 8 ? 3 : 4;
@@ -205,18 +211,19 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 1 : 2)
 #pragma unroll
   for (int k = 0; k < 12; ++k) acc[k] = 0.0;
   for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    const LayerGeom<O> m = layer_geom<O>(c, W, h0, t, esh);
+    const LayerGeom<O> m = layer_geom<O>(c, W, h0, C, t, esh);
     const ChunkIO<T, O> ia(psi + m.base), il(LAM ? lam + m.base : psi);
     T a[16], l[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const O so = layer_r<O>(r, 8, W, h0, esh);
+      const O so = layer_r<O>(r, 8, W, h0, C, esh);
       a[r] = ia.load(m.t2, so);
       if (LAM) l[r] = il.load(m.t2, so);
     }
     if (P2) layer_phase<T, R, LAM, REAL>(a, l, sg, sd, 8, acc);
-    // the previous chunk's last reads of X: layout L1 (P1 only) or L2 (P0)
-    constexpr bool FRONT = P1 && !P0;
+    // the previous chunk's last reads of X: layout L1 (P1 only, not WIDE)
+    // or L2 (P0, or WIDE: transposed back)
+    constexpr bool FRONT = P1 && !P0 && !WIDE;
     if (P1) {
       layer_xpose<2, 1, FRONT, T, LAM>(a, l, X, t);
       layer_phase<T, R, LAM, REAL>(a, l, sg, sd, 4, acc);
@@ -229,11 +236,12 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 1 : 2)
       layer_phase<T, R, LAM, REAL>(a, l, sg, sd, 0, acc);
       layer_xpose<0, 2, false, T, LAM>(a, l, X, t);
     }
-    constexpr bool L1OUT = P1 && !P0;
+    if (WIDE && P1 && !P0) layer_xpose<1, 2, false, T, LAM>(a, l, X, t);
+    constexpr bool L1OUT = P1 && !P0 && !WIDE;
     const O vo = L1OUT ? m.t1 : m.t2;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const O so = L1OUT ? layer_r<O>(r, 4, W, h0, esh) : layer_r<O>(r, 8, W, h0, esh);
+      const O so = L1OUT ? layer_r<O>(r, 4, W, h0, C, esh) : layer_r<O>(r, 8, W, h0, C, esh);
       ia.store(vo, so, a[r]);
       if (LAM) il.store(vo, so, l[r]);
     }
@@ -263,21 +271,24 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 1 : 2)
 // pass runs the smallest set that holds its gates, empty positions as
 // identities.
 template <typename T, typename R, typename O, bool LAM, bool REAL>
-static void launch_layer_r(void *psi, void *lam, uint64_t nchunks, int W, int h0,
+static void launch_layer_r(void *psi, void *lam, uint64_t nchunks, int W, int h0, int C,
                            const LayerArgs<R> &a, double *partial, int nb, hipStream_t st) {
   const int need = (((a.has >> 8) & 15) ? 4 : 0) | (((a.has >> 4) & 15) ? 2 : 0) |
                    ((a.has & 15) ? 1 : 0);
-  void (*k)(T *, T *, uint64_t, int, int, LayerArgs<R>, double *) =
-      need == 4          ? k_adjoint_layer<T, R, O, LAM, 4, REAL>
-      : (need & ~2) == 0 ? k_adjoint_layer<T, R, O, LAM, 2, REAL>
-      : (need & ~6) == 0 ? k_adjoint_layer<T, R, O, LAM, 6, REAL>
-                         : k_adjoint_layer<T, R, O, LAM, 7, REAL>;
-  hipLaunchKernelGGL(k, dim3(nb), dim3(kBlock), 0, st, (T *)psi, (T *)lam, nchunks, W, h0, a,
-                     partial);
+  const bool wide = C == 6;  // only phase sets 2 and 6 store differently
+  void (*k)(T *, T *, uint64_t, int, int, int, LayerArgs<R>, double *) =
+      need == 4          ? k_adjoint_layer<T, R, O, LAM, 4, REAL, false>
+      : (need & ~2) == 0 ? (wide ? k_adjoint_layer<T, R, O, LAM, 2, REAL, true>
+                                 : k_adjoint_layer<T, R, O, LAM, 2, REAL, false>)
+      : (need & ~6) == 0 ? (wide ? k_adjoint_layer<T, R, O, LAM, 6, REAL, true>
+                                 : k_adjoint_layer<T, R, O, LAM, 6, REAL, false>)
+                         : k_adjoint_layer<T, R, O, LAM, 7, REAL, false>;
+  hipLaunchKernelGGL(k, dim3(nb), dim3(kBlock), 0, st, (T *)psi, (T *)lam, nchunks, W, h0, C,
+                     a, partial);
 }
 
 template <typename T, typename R, typename O, bool LAM>
-static void launch_layer_o(void *psi, void *lam, uint64_t nchunks, int W, int h0,
+static void launch_layer_o(void *psi, void *lam, uint64_t nchunks, int W, int h0, int C,
                            const LayerArgs<R> &a, double *partial, int nb, hipStream_t st) {
   bool real = true;
   for (int k = 0; k < 12; ++k) {
@@ -286,21 +297,21 @@ static void launch_layer_o(void *psi, void *lam, uint64_t nchunks, int W, int h0
       real = real && a.g[k].i[e] == 0 && (!LAM || a.slot[k] < 0 || a.d[k].i[e] == 0);
   }
   if (real)
-    launch_layer_r<T, R, O, LAM, true>(psi, lam, nchunks, W, h0, a, partial, nb, st);
+    launch_layer_r<T, R, O, LAM, true>(psi, lam, nchunks, W, h0, C, a, partial, nb, st);
   else
-    launch_layer_r<T, R, O, LAM, false>(psi, lam, nchunks, W, h0, a, partial, nb, st);
+    launch_layer_r<T, R, O, LAM, false>(psi, lam, nchunks, W, h0, C, a, partial, nb, st);
 }
 
 // 32-bit byte offsets when every byte of the pass lies below 2^31 from the
 // chunk base (well inside the resource's 2^32 - 1 byte range).
 template <typename T, typename R, bool LAM>
-static void launch_layer_t(void *psi, void *lam, uint64_t nchunks, int W, int h0,
+static void launch_layer_t(void *psi, void *lam, uint64_t nchunks, int W, int h0, int C,
                            const LayerArgs<R> &a, double *partial, int nb, hipStream_t st) {
   const int top = W == 0 ? 12 : h0 + W;  // offsets below 2^top elements
   if (top + (sizeof(T) == 8 ? 3 : 4) <= 31)
-    launch_layer_o<T, R, uint32_t, LAM>(psi, lam, nchunks, W, h0, a, partial, nb, st);
+    launch_layer_o<T, R, uint32_t, LAM>(psi, lam, nchunks, W, h0, C, a, partial, nb, st);
   else
-    launch_layer_o<T, R, uint64_t, LAM>(psi, lam, nchunks, W, h0, a, partial, nb, st);
+    launch_layer_o<T, R, uint64_t, LAM>(psi, lam, nchunks, W, h0, C, a, partial, nb, st);
 }
 
 template <typename R>
@@ -329,11 +340,12 @@ int layer_pass_blocks(uint64_t n) {
   return (int)(nchunks < kLayerMaxBlocks ? nchunks : kLayerMaxBlocks);
 }
 
-int launch_adjoint_layer(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int h0, int W,
+int launch_adjoint_layer(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int h0, int W, int C,
                          int count, const int *pos, const double *gm, const double *dm,
                          const int *slot, int nslots, const int *slot_param_dev, bool distinct,
                          double *partial, double *out_dev, hipStream_t st) {
-  if (n < 4096 || (n & 4095) || count < 1 || count > 12 || W < 0 || W > 8)
+  if (n < 4096 || (n & 4095) || count < 1 || count > 12 || (C != 4 && C != 6) || W < 0 ||
+      W > 12 - C)
     return fail(SQ_ERR_INVALID, "internal: layer pass shape");
   const int nq = __builtin_ctzll(n);
   if (W == 0 ? h0 != 12 : (h0 < 12 || h0 + W > nq))
@@ -353,16 +365,16 @@ int launch_adjoint_layer(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int h0
     const LayerArgs<float> a = layer_args<float>(count, pos, lam ? slot : nullptr, gm,
                                                  lam ? dm : nullptr);
     if (lam)
-      launch_layer_t<float2, float, true>(psi, lam, nchunks, W, h0, a, partial, nb, st);
+      launch_layer_t<float2, float, true>(psi, lam, nchunks, W, h0, C, a, partial, nb, st);
     else
-      launch_layer_t<float2, float, false>(psi, nullptr, nchunks, W, h0, a, nullptr, nb, st);
+      launch_layer_t<float2, float, false>(psi, nullptr, nchunks, W, h0, C, a, nullptr, nb, st);
   } else {
     const LayerArgs<double> a = layer_args<double>(count, pos, lam ? slot : nullptr, gm,
                                                    lam ? dm : nullptr);
     if (lam)
-      launch_layer_t<double2, double, true>(psi, lam, nchunks, W, h0, a, partial, nb, st);
+      launch_layer_t<double2, double, true>(psi, lam, nchunks, W, h0, C, a, partial, nb, st);
     else
-      launch_layer_t<double2, double, false>(psi, nullptr, nchunks, W, h0, a, nullptr, nb, st);
+      launch_layer_t<double2, double, false>(psi, nullptr, nchunks, W, h0, C, a, nullptr, nb, st);
   }
   SQ_HIP(hipGetLastError());
   if (lam && nslots > 0)

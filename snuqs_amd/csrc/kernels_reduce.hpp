@@ -72,9 +72,10 @@ int launch_finalize_slots(const double *partial, int nblocks, int nslots,
                           hipStream_t st);
 // Layer pass over a state of n = 2^nq amplitudes (n >= 4096): count one-target
 // gates at distinct local positions pos[j] (0..11) of 4096-amplitude chunks
-// whose local bits map to global bits as: 0-3 -> 0-3, 4..3+W -> h0..h0+W-1,
-// 4+W..11 -> 4..11-W (W = 0, h0 = 12: local = global, bits 0-11; else
-// 12 <= h0, h0 + W <= nq, W <= 8).  gm: each gate's 2x2 (row-major complex128).
+// whose local bits map to global bits as: 0..C-1 -> 0..C-1, C..C+W-1 ->
+// h0..h0+W-1, C+W..11 -> C..11-W, with C = 4 or 6 contiguous low bits
+// (W = 0, h0 = 12: local = global, bits 0-11; else 12 <= h0, h0 + W <= nq,
+// W <= 12 - C).  gm: each gate's 2x2 (row-major complex128).
 // lam == nullptr (forward): psi <- gm_j psi for every j.  Reverse: gm_j =
 // G_j^dagger, dm_j = dG_j G_j^dagger; for slot[j] >= 0,
 // out_dev[2 slot_param[slot[j]]] += 2 Re <lam| dm_j |psi> at the layer's end
@@ -82,7 +83,7 @@ int launch_finalize_slots(const double *partial, int nblocks, int nslots,
 // layer_pass_blocks(n) doubles.
 constexpr int kLayerMaxBlocks = 8192;
 int layer_pass_blocks(uint64_t n);
-int launch_adjoint_layer(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int h0, int W,
+int launch_adjoint_layer(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int h0, int W, int C,
                          int count, const int *pos, const double *gm, const double *dm,
                          const int *slot, int nslots, const int *slot_param_dev, bool distinct,
                          double *partial, double *out_dev, hipStream_t st);

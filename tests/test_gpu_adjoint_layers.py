@@ -115,20 +115,47 @@ def test_layer_passes_match_groups(C, n, checkpoints, dtype, tol):
     assert np.abs(g1 - g0).max() < tol
 
 
-def test_layer_passes_64bit_offsets_28q(C):
-    """28 qubits in complex128 (4 GiB a state): the window over bits 20-27
-    addresses past 2^31 bytes from a chunk base and takes the 64-bit path."""
+@pytest.mark.parametrize("n,dtype,tol", [(28, "complex128", 1e-11), (30, "complex64", 2e-5)])
+def test_layer_passes_64bit_offsets(C, n, dtype, tol):
+    """28 qubits complex128 (windows of 8 bits beside 4 low bits) and 30
+    qubits complex64 (windows of 6 bits beside 6 low bits), 4 GiB and 8 GiB a
+    state: the top window addresses past 2^31 bytes from a chunk base and
+    takes the 64-bit path."""
     from snuqs_amd import _C
 
     g = _C.operation.gate_operations
-    n = 28
-    rng = np.random.default_rng(28)
+    rng = np.random.default_rng(n)
     th = rng.uniform(0, 2 * np.pi, n)
     ops = [g.RotY([q], th[q]) for q in range(n)] + [g.CX([q, q + 1]) for q in range(0, n - 1, 2)]
     pmap = [[(0, q)] for q in range(n)] + [[] for _ in range(0, n - 1, 2)]
     obs = (["ZZ", "XZ", "YY"], [[0, n - 1], [3, 20], [11, 25]], [0.7, -1.1, 0.4])
-    e1, g1 = _adjoint(C, n, ops, pmap, n, obs, "complex128", 0, 1)
-    e0, g0 = _adjoint(C, n, ops, pmap, n, obs, "complex128", 0, 0)
-    assert abs(e1 - e0) < 1e-11
-    assert np.abs(g1 - g0).max() < 1e-11
+    e1, g1 = _adjoint(C, n, ops, pmap, n, obs, dtype, 0, 1)
+    e0, g0 = _adjoint(C, n, ops, pmap, n, obs, dtype, 0, 0)
+    assert abs(e1 - e0) < tol
+    assert np.abs(g1 - g0).max() < tol
     assert np.abs(g1).max() > 1e-3  # a non-trivial gradient
+
+
+@pytest.mark.parametrize("n", [16, 22, 24])
+def test_wide_windows_match_narrow(C, n):
+    """Windows of up to 6 bits beside 6 contiguous low bits (512 B runs)
+    against windows of up to 8 bits beside 4 (adjoint_wide_windows off) and
+    the register groups (adjoint_layers off), complex128."""
+    from snuqs_amd.circuits import random_pauli_sum
+
+    nparams = _nparams(n)
+    thetas = np.random.default_rng(3 * n).uniform(0, 2 * np.pi, nparams)
+    ops, pmap, _ = _layer_circuit(n, thetas)
+    obs = random_pauli_sum(n, 8, 4, np.random.default_rng(n + 1))
+    saved = C.core.tune_get("adjoint_wide_windows")
+    try:
+        res = {}
+        for wide in (1, 0):
+            C.core.tune_set("adjoint_wide_windows", wide)
+            res[wide] = _adjoint(C, n, ops, pmap, nparams, obs, "complex128", 2, 1)
+    finally:
+        C.core.tune_set("adjoint_wide_windows", saved)
+    e0, g0 = _adjoint(C, n, ops, pmap, nparams, obs, "complex128", 2, 0)
+    for e, g in res.values():
+        assert abs(e - e0) < 1e-10
+        assert np.abs(g - g0).max() < 1e-10
