@@ -164,6 +164,7 @@ static int *tune_slot(const std::string &k) {
       {"tune_scatter", [] { return &tuning().tune_scatter; }},
       {"adjoint_layers", [] { return &tuning().adjoint_layers; }},
       {"adjoint_wide_windows", [] { return &tuning().adjoint_wide_windows; }},
+      {"adjoint_fuse_runs", [] { return &tuning().adjoint_fuse_runs; }},
       {"mfma5_wide_c64", [] { return &tuning().mfma5_wide; }},
   };
   for (auto &e : table)
@@ -182,6 +183,8 @@ int sq_tune_set(const char *key, int value) {
   const bool flag = k.rfind("mfma5_pipeline", 0) == 0 || k.rfind("mfma5_wide", 0) == 0 ||
                     k == "qaoa_mid_scatter" || k == "tune_scatter" || k == "adjoint_layers" ||
                     k == "adjoint_wide_windows";
+  if (k == "adjoint_fuse_runs" && (value < 0 || value > 3))
+    return fail(SQ_ERR_INVALID, "adjoint_fuse_runs must be in [0, 3]");
   *slot = flag ? (value ? 1 : 0) : value;
   return SQ_OK;
 }
@@ -1351,12 +1354,55 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       g = lo - 1;
     }
   }
-  // bounds to the lowest gate of their run or layer
+  // Diagonal runs folded into the low pass of the layer right below them
+  // (launch_adjoint_layer with a FuseDesc).  Reverse: the run's derivative
+  // terms by a run pass that leaves both states unchanged, then the layer's
+  // low pass undoes the run's phases before the layer's gates and goes first
+  // among the layer's passes; forward: the low pass applies the run's phases
+  // after the layer's gates and goes last.  Needs a low pass with gates in
+  // all three register layouts and at most kFuseMix run gates with one bit
+  // on each side of bit 12.
+  auto low_pass_of = [&](const std::vector<LPass> &ps, const Layer &L) -> int {
+    for (uint32_t i = L.first_pass; i < L.first_pass + L.npass; ++i) {
+      if (ps[i].W != 0) continue;
+      int seen = 0;
+      for (int j = 0; j < ps[i].count; ++j) seen |= 1 << ps[i].pos[j];
+      return ((seen & 0xF) && (seen & 0xF0) && (seen & 0xF00)) ? (int)i : -1;
+    }
+    return -1;
+  };
+  auto run_mixed = [&](const Run &r) {
+    std::vector<DiagRunGate> mix;
+    for (uint32_t k = r.first; k < r.first + r.m; ++k) {
+      const DiagRunGate &e = rgates[k];
+      if (e.b1 >= 0 && (e.b0 < 12) != (e.b1 < 12)) mix.push_back(e);
+    }
+    return mix;
+  };
+  std::vector<int> rfuse(runs.size(), -1), rfuse_low(runs.size(), -1);
+  if (use_layers && (tuning().adjoint_fuse_runs & 2)) {
+    std::vector<int> layer_below(ng + 1, -1);  // layer whose last gate is key - 1
+    for (size_t L = 0; L < layers.size(); ++L)
+      layer_below[layers[L].lo + layers[L].m] = (int)L;
+    for (size_t ri = 0; ri < runs.size(); ++ri) {
+      const int L = layer_below[runs[ri].lo];
+      if (L < 0) continue;
+      const int lp = low_pass_of(lpasses, layers[L]);
+      if (lp < 0 || run_mixed(runs[ri]).size() > (size_t)kFuseMix) continue;
+      rfuse[ri] = L;
+      rfuse_low[ri] = lp;
+    }
+  }
+  // bounds to the lowest gate of their run or layer (of the layer for a run
+  // folded into it)
   {
     std::vector<uint32_t> seg_lo(ng);
     for (uint32_t g = 0; g < ng; ++g) seg_lo[g] = g;
-    for (const Run &r : runs)
-      for (uint32_t q = r.lo; q < r.lo + r.m; ++q) seg_lo[q] = r.lo;
+    for (size_t ri = 0; ri < runs.size(); ++ri) {
+      const Run &r = runs[ri];
+      const uint32_t lo = rfuse[ri] >= 0 ? layers[rfuse[ri]].lo : r.lo;
+      for (uint32_t q = r.lo; q < r.lo + r.m; ++q) seg_lo[q] = lo;
+    }
     for (const Layer &L : layers)
       for (uint32_t q = L.lo; q < L.lo + L.m; ++q) seg_lo[q] = L.lo;
     for (uint32_t c = 0; c < num_checkpoints; ++c) bound[c] = seg_lo[bound[c]];
@@ -1491,6 +1537,18 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       g = hi;
     }
   }
+  // forward: a run right after a layer (no checkpoint bound at its start)
+  std::vector<int> ffuse(flayers.size(), -1), ffuse_low(flayers.size(), -1);
+  if (use_layers && (tuning().adjoint_fuse_runs & 1)) {
+    for (size_t fi = 0; fi < flayers.size(); ++fi) {
+      const uint32_t next = flayers[fi].lo + flayers[fi].m;
+      if (next >= ng || fwd_start[next] < 0 || is_bound[next]) continue;
+      const int lp = low_pass_of(fpasses, flayers[fi]);
+      if (lp < 0 || run_mixed(fruns[fwd_start[next]]).size() > (size_t)kFuseMix) continue;
+      ffuse[fi] = fwd_start[next];
+      ffuse_low[fi] = lp;
+    }
+  }
   // forward groups: 2-5 consecutive one-target gates on distinct bits, all
   // above 3 or all below 4 as in the reverse groups (outside forward runs and
   // layers), applied in one pass (5: psi alone leaves the registers for 32
@@ -1539,6 +1597,23 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   const size_t rg_b = (rgates.size() * sizeof(DiagRunGate) + 255) / 256 * 256;
   const size_t rs_b = (rslots.size() * sizeof(int) + 255) / 256 * 256;
   const size_t rp_b = any_runs ? (size_t)kMaxDiagSlots * 4096 * sizeof(double) : 0;
+  // fused runs: a FuseDesc and a FuseTableJob each, and their phase tables
+  std::vector<int> rfuse_id(runs.size(), -1), ffuse_id(flayers.size(), -1);
+  std::vector<const Run *> fused_runs;
+  for (size_t ri = 0; ri < runs.size(); ++ri)
+    if (rfuse[ri] >= 0) {
+      rfuse_id[ri] = (int)fused_runs.size();
+      fused_runs.push_back(&runs[ri]);
+    }
+  for (size_t fi = 0; fi < flayers.size(); ++fi)
+    if (ffuse[fi] >= 0) {
+      ffuse_id[fi] = (int)fused_runs.size();
+      fused_runs.push_back(&fruns[ffuse[fi]]);
+    }
+  const size_t nfuse = fused_runs.size();
+  const size_t fz_b = (nfuse * (sizeof(FuseDesc) + sizeof(FuseTableJob)) + 255) / 256 * 256;
+  const size_t tab_stride = ((4096 + (n >> 12)) * itemsize(dt) + 255) / 256 * 256;
+  const size_t tab_b = nfuse ? nfuse * tab_stride : 0;
 
   int dev;
   SQ_TRY(current_device(&dev));
@@ -1547,8 +1622,8 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   const size_t edev = expect_ws_dev(nterms);
   const size_t apb = nterms * pauli_apply_term_bytes() + 256;
   Workspace *w;
-  SQ_TRY(ws_reserve(dev, edev + grad_b + apb + rg_b + rs_b + rp_b + dblk_b + 4096,
-                    nterms * 64 + grad_b + apb + rg_b + rs_b + dblk_b + 4096, &w));
+  SQ_TRY(ws_reserve(dev, edev + grad_b + apb + rg_b + rs_b + rp_b + dblk_b + fz_b + tab_b + 4096,
+                    nterms * 64 + grad_b + apb + rg_b + rs_b + dblk_b + fz_b + 4096, &w));
   hipStream_t st = (hipStream_t)stream;
   char *dbase = (char *)w->dev;
   double *partial = (double *)dbase;                      // reduction partials
@@ -1560,6 +1635,9 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   int *rslots_dev = (int *)al((char *)rgates_dev + rg_b);
   double *rpartial = (double *)al((char *)rslots_dev + rs_b);
   char *dblk_dev = al((char *)rpartial + rp_b);           // derivative blocks
+  FuseDesc *fdesc_dev = (FuseDesc *)al(dblk_dev + dblk_b);
+  FuseTableJob *fjob_dev = (FuseTableJob *)(fdesc_dev + nfuse);
+  char *tab_dev = al((char *)fdesc_dev + fz_b);
   // staged through the pinned host buffer, after the gradient slots
   char *hs = (char *)al((char *)w->host + apb + grad_b);
   if (any_runs) {
@@ -1573,6 +1651,31 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     SQ_HIP(hipMemcpyAsync(dblk_dev, hb, dblk_b, hipMemcpyHostToDevice, st));
     for (uint32_t d = 0; d < num_derivs; ++d)
       if (dblk_off[d] != (size_t)-1) dmat_ptr[d] = dblk_dev + dblk_off[d];
+  }
+  if (nfuse) {
+    char *hf = al(hs + rg_b + rs_b + dblk_b);
+    FuseDesc *hd = (FuseDesc *)hf;
+    FuseTableJob *hj = (FuseTableJob *)(hd + nfuse);
+    for (size_t f = 0; f < nfuse; ++f) {
+      const Run &r = *fused_runs[f];
+      char *tl = tab_dev + f * tab_stride;
+      char *tc = tl + 4096 * itemsize(dt);
+      FuseDesc d{};
+      d.tl = tl;
+      d.tc = tc;
+      const std::vector<DiagRunGate> mix = run_mixed(r);
+      d.nmix = (int)mix.size();
+      for (size_t k = 0; k < mix.size(); ++k) d.mix[k] = mix[k];
+      std::memcpy(hd + f, &d, sizeof d);
+      FuseTableJob j{};
+      j.g = rgates_dev + r.first;
+      j.m = (int)r.m;
+      j.tl = tl;
+      j.tc = tc;
+      std::memcpy(hj + f, &j, sizeof j);
+    }
+    SQ_HIP(hipMemcpyAsync(fdesc_dev, hf, fz_b, hipMemcpyHostToDevice, st));
+    SQ_TRY(launch_fuse_tables(fjob_dev, (int)nfuse, dt, n, st));
   }
 
   // forward pass with checkpoints
@@ -1591,13 +1694,24 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       continue;
     }
     if (flayer_start[g] >= 0) {
-      const Layer &L = flayers[flayer_start[g]];
+      const int fi = flayer_start[g];
+      const Layer &L = flayers[fi];
       for (uint32_t i = L.first_pass; i < L.first_pass + L.npass; ++i) {
+        if ((int)i == ffuse_low[fi]) continue;  // the fused low pass goes last
         const LPass &ps = fpasses[i];
         SQ_TRY(launch_adjoint_layer(sv, nullptr, dt, n, ps.h0, ps.W, ps.C, ps.count, ps.pos, ps.gm,
                                     nullptr, nullptr, 0, nullptr, false, nullptr, nullptr, st));
       }
-      g = L.lo + L.m - 1;
+      if (ffuse[fi] >= 0) {
+        const LPass &ps = fpasses[ffuse_low[fi]];
+        SQ_TRY(launch_adjoint_layer(sv, nullptr, dt, n, ps.h0, ps.W, ps.C, ps.count, ps.pos, ps.gm,
+                                    nullptr, nullptr, 0, nullptr, false, nullptr, nullptr, st,
+                                    fdesc_dev + ffuse_id[fi]));
+        const Run &r = fruns[ffuse[fi]];
+        g = r.lo + r.m - 1;  // the loop steps past the run
+      } else {
+        g = L.lo + L.m - 1;
+      }
       continue;
     }
     if (fgrp_start[g] >= 0) {
@@ -1634,6 +1748,29 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     return SQ_OK;
   };
   for (int g = (int)ng - 1; g >= 0; --g) {
+    if (run_top[g] >= 0 && rfuse[run_top[g]] >= 0) {
+      const int ri = run_top[g];
+      const Run &r = runs[ri];
+      if (r.nslots)  // derivative terms only; the low pass undoes the phases
+        SQ_TRY(launch_adjoint_diag_run(sv, work0, dt, n, rgates_dev + r.first, (int)r.m,
+                                       (int)r.nslots, rslots_dev + r.first_slot,
+                                       distinct_params(r.first_slot, r.nslots), rpartial, gslots,
+                                       st, false));
+      const Layer &L = layers[rfuse[ri]];
+      auto pass = [&](uint32_t i, const FuseDesc *fz) {
+        const LPass &ps = lpasses[i];
+        return launch_adjoint_layer(sv, work0, dt, n, ps.h0, ps.W, ps.C, ps.count, ps.pos, ps.gm,
+                                    ps.dm, ps.slot, (int)ps.nslots, rslots_dev + ps.first_slot,
+                                    distinct_params(ps.first_slot, ps.nslots), rpartial, gslots,
+                                    st, fz);
+      };
+      SQ_TRY(pass((uint32_t)rfuse_low[ri], fdesc_dev + rfuse_id[ri]));
+      for (uint32_t i = L.first_pass; i < L.first_pass + L.npass; ++i)
+        if ((int)i != rfuse_low[ri]) SQ_TRY(pass(i, nullptr));
+      SQ_TRY(restore_below(L.lo));
+      g = (int)L.lo;
+      continue;
+    }
     if (run_top[g] >= 0) {
       const Run &r = runs[run_top[g]];
       SQ_TRY(launch_adjoint_diag_run(sv, work0, dt, n, rgates_dev + r.first, (int)r.m,

@@ -126,6 +126,7 @@ struct Tuning {
   int tune_scatter = 1;                  // sq_program_tune also tries tile order 2
   int adjoint_layers = 1;                // adjoint sweeps: layer passes (12 / 8 bits a pass)
   int adjoint_wide_windows = 1;          // layer passes: 6-bit windows with 512 B runs
+  int adjoint_fuse_runs = 3;             // diagonal runs folded into layer low passes (1 fwd, 2 rev)
 };
 Tuning &tuning();
 

@@ -169,6 +169,42 @@ __device__ __forceinline__ void layer_xpose(T (&a)[16], T (&l)[16], T *X, int t)
   }
 }
 
+// Multiply the thread's 16 amplitudes (layout L2 of a low pass: index
+// chunk << 12 | t | r << 8) by a fused diagonal run's phases: the tables'
+// products were formed in FP64 and rounded to the state's precision once;
+// here one product of the two table entries and the mixed gates' entries,
+// in the state's precision.
+template <typename T, typename R, bool LAM>
+__device__ __forceinline__ void fuse_phases(T (&a)[16], T (&l)[16], const FuseDesc *fz,
+                                            uint64_t chunk, int t) {
+  const T *tl = reinterpret_cast<const T *>(fz->tl);
+  const T cf = reinterpret_cast<const T *>(fz->tc)[chunk];
+  const int nmix = fz->nmix;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const T lf = tl[t + 256 * r];
+    R pr = cf.x * lf.x - cf.y * lf.y, pi = cf.x * lf.y + cf.y * lf.x;
+    const uint64_t i = (chunk << 12) | (uint64_t)(t + 256 * r);
+    for (int k = 0; k < nmix; ++k) {
+      const DiagRunGate &g = fz->mix[k];
+      const int loc = (int)((((i >> g.b0) & 1) << 1) | ((i >> g.b1) & 1));
+      const R gr = (R)g.ginv[loc][0], gi = (R)g.ginv[loc][1];
+      const R tr = pr * gr - pi * gi;
+      pi = pr * gi + pi * gr;
+      pr = tr;
+    }
+    T o;
+    o.x = a[r].x * pr - a[r].y * pi;
+    o.y = a[r].x * pi + a[r].y * pr;
+    a[r] = o;
+    if (LAM) {
+      o.x = l[r].x * pr - l[r].y * pi;
+      o.y = l[r].x * pi + l[r].y * pr;
+      l[r] = o;
+    }
+  }
+}
+
 // PH: the phases the pass runs, bit 2: local positions 8-11 (layout L2),
 // bit 1: 4-7 (L1), bit 0: 0-3 (L0).  Layout path: L2 (load) -> [L1] ->
 // [L0 -> L2]; stores go out in L1 when that is the last layout, else in L2.
@@ -177,11 +213,14 @@ __device__ __forceinline__ void layer_xpose(T (&a)[16], T (&l)[16], T *X, int t)
 // measured on the 24-qubit ansatz gradient); complex64 keeps the compiler's
 // 3 (4 would spill to scratch and measured no faster).
 // WIDE (C = 6): the L1 phase transposes back to L2 before the stores, so
-// they go out as 512 B runs too.
-template <typename T, typename R, typename O, bool LAM, int PH, bool REAL, bool WIDE>
+// they go out as 512 B runs too.  FUSE (low pass, PH = 7): a diagonal run's
+// phases (fz) before the layer's gates (reverse) or after them (forward).
+template <typename T, typename R, typename O, bool LAM, int PH, bool REAL, bool WIDE, bool FUSE>
 __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 1 : 2)
     k_adjoint_layer(T *__restrict__ psi, T *__restrict__ lam, uint64_t nchunks, int W, int h0,
-                    int C, LayerArgs<R> A, double *__restrict__ partial) {
+                    int C, LayerArgs<R> A, double *__restrict__ partial,
+                    const FuseDesc *__restrict__ fz) {
+  static_assert(!FUSE || PH == 7, "a fused run rides on the low pass");
   constexpr bool P2 = PH & 4, P1 = PH & 2, P0 = PH & 1;
   constexpr int NW = kBlock / 64;
   constexpr int esh = sizeof(T) == 8 ? 3 : 4;
@@ -220,6 +259,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 1 : 2)
       a[r] = ia.load(m.t2, so);
       if (LAM) l[r] = il.load(m.t2, so);
     }
+    if (FUSE && LAM) fuse_phases<T, R, true>(a, l, fz, c, t);
     if (P2) layer_phase<T, R, LAM, REAL>(a, l, sg, sd, 8, acc);
     // the previous chunk's last reads of X: layout L1 (P1 only, not WIDE)
     // or L2 (P0, or WIDE: transposed back)
@@ -237,6 +277,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 1 : 2)
       layer_xpose<0, 2, false, T, LAM>(a, l, X, t);
     }
     if (WIDE && P1 && !P0) layer_xpose<1, 2, false, T, LAM>(a, l, X, t);
+    if (FUSE && !LAM) fuse_phases<T, R, false>(a, l, fz, c, t);
     constexpr bool L1OUT = P1 && !P0 && !WIDE;
     const O vo = L1OUT ? m.t1 : m.t2;
 #pragma unroll
@@ -272,24 +313,27 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 1 : 2)
 // identities.
 template <typename T, typename R, typename O, bool LAM, bool REAL>
 static void launch_layer_r(void *psi, void *lam, uint64_t nchunks, int W, int h0, int C,
-                           const LayerArgs<R> &a, double *partial, int nb, hipStream_t st) {
+                           const LayerArgs<R> &a, double *partial, int nb, hipStream_t st,
+                           const FuseDesc *fz) {
   const int need = (((a.has >> 8) & 15) ? 4 : 0) | (((a.has >> 4) & 15) ? 2 : 0) |
                    ((a.has & 15) ? 1 : 0);
   const bool wide = C == 6;  // only phase sets 2 and 6 store differently
-  void (*k)(T *, T *, uint64_t, int, int, int, LayerArgs<R>, double *) =
-      need == 4          ? k_adjoint_layer<T, R, O, LAM, 4, REAL, false>
-      : (need & ~2) == 0 ? (wide ? k_adjoint_layer<T, R, O, LAM, 2, REAL, true>
-                                 : k_adjoint_layer<T, R, O, LAM, 2, REAL, false>)
-      : (need & ~6) == 0 ? (wide ? k_adjoint_layer<T, R, O, LAM, 6, REAL, true>
-                                 : k_adjoint_layer<T, R, O, LAM, 6, REAL, false>)
-                         : k_adjoint_layer<T, R, O, LAM, 7, REAL, false>;
+  void (*k)(T *, T *, uint64_t, int, int, int, LayerArgs<R>, double *, const FuseDesc *) =
+      fz                 ? k_adjoint_layer<T, R, O, LAM, 7, REAL, false, true>
+      : need == 4        ? k_adjoint_layer<T, R, O, LAM, 4, REAL, false, false>
+      : (need & ~2) == 0 ? (wide ? k_adjoint_layer<T, R, O, LAM, 2, REAL, true, false>
+                                 : k_adjoint_layer<T, R, O, LAM, 2, REAL, false, false>)
+      : (need & ~6) == 0 ? (wide ? k_adjoint_layer<T, R, O, LAM, 6, REAL, true, false>
+                                 : k_adjoint_layer<T, R, O, LAM, 6, REAL, false, false>)
+                         : k_adjoint_layer<T, R, O, LAM, 7, REAL, false, false>;
   hipLaunchKernelGGL(k, dim3(nb), dim3(kBlock), 0, st, (T *)psi, (T *)lam, nchunks, W, h0, C,
-                     a, partial);
+                     a, partial, fz);
 }
 
 template <typename T, typename R, typename O, bool LAM>
 static void launch_layer_o(void *psi, void *lam, uint64_t nchunks, int W, int h0, int C,
-                           const LayerArgs<R> &a, double *partial, int nb, hipStream_t st) {
+                           const LayerArgs<R> &a, double *partial, int nb, hipStream_t st,
+                           const FuseDesc *fz) {
   bool real = true;
   for (int k = 0; k < 12; ++k) {
     if (!((a.has >> k) & 1)) continue;
@@ -297,21 +341,22 @@ static void launch_layer_o(void *psi, void *lam, uint64_t nchunks, int W, int h0
       real = real && a.g[k].i[e] == 0 && (!LAM || a.slot[k] < 0 || a.d[k].i[e] == 0);
   }
   if (real)
-    launch_layer_r<T, R, O, LAM, true>(psi, lam, nchunks, W, h0, C, a, partial, nb, st);
+    launch_layer_r<T, R, O, LAM, true>(psi, lam, nchunks, W, h0, C, a, partial, nb, st, fz);
   else
-    launch_layer_r<T, R, O, LAM, false>(psi, lam, nchunks, W, h0, C, a, partial, nb, st);
+    launch_layer_r<T, R, O, LAM, false>(psi, lam, nchunks, W, h0, C, a, partial, nb, st, fz);
 }
 
 // 32-bit byte offsets when every byte of the pass lies below 2^31 from the
 // chunk base (well inside the resource's 2^32 - 1 byte range).
 template <typename T, typename R, bool LAM>
 static void launch_layer_t(void *psi, void *lam, uint64_t nchunks, int W, int h0, int C,
-                           const LayerArgs<R> &a, double *partial, int nb, hipStream_t st) {
+                           const LayerArgs<R> &a, double *partial, int nb, hipStream_t st,
+                           const FuseDesc *fz) {
   const int top = W == 0 ? 12 : h0 + W;  // offsets below 2^top elements
   if (top + (sizeof(T) == 8 ? 3 : 4) <= 31)
-    launch_layer_o<T, R, uint32_t, LAM>(psi, lam, nchunks, W, h0, C, a, partial, nb, st);
+    launch_layer_o<T, R, uint32_t, LAM>(psi, lam, nchunks, W, h0, C, a, partial, nb, st, fz);
   else
-    launch_layer_o<T, R, uint64_t, LAM>(psi, lam, nchunks, W, h0, C, a, partial, nb, st);
+    launch_layer_o<T, R, uint64_t, LAM>(psi, lam, nchunks, W, h0, C, a, partial, nb, st, fz);
 }
 
 template <typename R>
@@ -343,7 +388,8 @@ int layer_pass_blocks(uint64_t n) {
 int launch_adjoint_layer(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int h0, int W, int C,
                          int count, const int *pos, const double *gm, const double *dm,
                          const int *slot, int nslots, const int *slot_param_dev, bool distinct,
-                         double *partial, double *out_dev, hipStream_t st) {
+                         double *partial, double *out_dev, hipStream_t st,
+                         const FuseDesc *fuse_dev) {
   if (n < 4096 || (n & 4095) || count < 1 || count > 12 || (C != 4 && C != 6) || W < 0 ||
       W > 12 - C)
     return fail(SQ_ERR_INVALID, "internal: layer pass shape");
@@ -359,26 +405,77 @@ int launch_adjoint_layer(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int h0
       return fail(SQ_ERR_INVALID, "internal: layer pass slot");
   }
   if (lam && nslots > 12) return fail(SQ_ERR_INVALID, "internal: layer pass slots");
+  if (fuse_dev && (W != 0 || (seen & 0x00F) == 0 || (seen & 0x0F0) == 0 || (seen & 0xF00) == 0))
+    return fail(SQ_ERR_INVALID, "internal: a fused run needs a low pass with gates in every layout");
   const uint64_t nchunks = n >> 12;
   const int nb = layer_pass_blocks(n);
   if (dt == SQ_COMPLEX64) {
     const LayerArgs<float> a = layer_args<float>(count, pos, lam ? slot : nullptr, gm,
                                                  lam ? dm : nullptr);
     if (lam)
-      launch_layer_t<float2, float, true>(psi, lam, nchunks, W, h0, C, a, partial, nb, st);
+      launch_layer_t<float2, float, true>(psi, lam, nchunks, W, h0, C, a, partial, nb, st, fuse_dev);
     else
-      launch_layer_t<float2, float, false>(psi, nullptr, nchunks, W, h0, C, a, nullptr, nb, st);
+      launch_layer_t<float2, float, false>(psi, nullptr, nchunks, W, h0, C, a, nullptr, nb, st, fuse_dev);
   } else {
     const LayerArgs<double> a = layer_args<double>(count, pos, lam ? slot : nullptr, gm,
                                                    lam ? dm : nullptr);
     if (lam)
-      launch_layer_t<double2, double, true>(psi, lam, nchunks, W, h0, C, a, partial, nb, st);
+      launch_layer_t<double2, double, true>(psi, lam, nchunks, W, h0, C, a, partial, nb, st, fuse_dev);
     else
-      launch_layer_t<double2, double, false>(psi, nullptr, nchunks, W, h0, C, a, nullptr, nb, st);
+      launch_layer_t<double2, double, false>(psi, nullptr, nchunks, W, h0, C, a, nullptr, nb, st, fuse_dev);
   }
   SQ_HIP(hipGetLastError());
   if (lam && nslots > 0)
     SQ_TRY(launch_finalize_slots(partial, nb, nslots, slot_param_dev, distinct, 2.0, out_dev, st));
+  return SQ_OK;
+}
+
+// ---- tables of fused diagonal runs ---------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_fuse_tables(const FuseTableJob *__restrict__ jobs, uint64_t nchunks) {
+  using R = decltype(T{}.x);
+  const FuseTableJob jb = jobs[blockIdx.y];
+  const uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool do_l = x < 4096, do_c = x < nchunks;
+  double lr = 1.0, li = 0.0, cr = 1.0, ci = 0.0;
+  for (int k = 0; k < jb.m; ++k) {
+    const DiagRunGate &g = jb.g[k];
+    const bool low0 = g.b0 < 12, low1 = g.b1 < 12;  // b1 < 0: one target
+    if (low0 && low1 && do_l) {
+      const int loc = g.b1 < 0 ? (int)((x >> g.b0) & 1)
+                               : (int)((((x >> g.b0) & 1) << 1) | ((x >> g.b1) & 1));
+      const double gr = g.ginv[loc][0], gi = g.ginv[loc][1];
+      const double tr = lr * gr - li * gi;
+      li = lr * gi + li * gr;
+      lr = tr;
+    }
+    if (!low0 && (g.b1 < 0 || !low1) && do_c) {
+      const uint64_t i = x << 12;
+      const int loc = g.b1 < 0 ? (int)((i >> g.b0) & 1)
+                               : (int)((((i >> g.b0) & 1) << 1) | ((i >> g.b1) & 1));
+      const double gr = g.ginv[loc][0], gi = g.ginv[loc][1];
+      const double tr = cr * gr - ci * gi;
+      ci = cr * gi + ci * gr;
+      cr = tr;
+    }
+  }
+  if (do_l) reinterpret_cast<T *>(jb.tl)[x] = T{(R)lr, (R)li};
+  if (do_c) reinterpret_cast<T *>(jb.tc)[x] = T{(R)cr, (R)ci};
+}
+
+int launch_fuse_tables(const FuseTableJob *jobs_dev, int njobs, sq_dtype_t dt, uint64_t n,
+                       hipStream_t st) {
+  if (njobs <= 0) return SQ_OK;
+  if (njobs > 65535 || n < 4096) return fail(SQ_ERR_INVALID, "internal: fuse tables");
+  const uint64_t nchunks = n >> 12;
+  const uint64_t width = nchunks > 4096 ? nchunks : 4096;
+  const dim3 g((unsigned)((width + kBlock - 1) / kBlock), njobs);
+  if (dt == SQ_COMPLEX64)
+    hipLaunchKernelGGL(k_fuse_tables<float2>, g, dim3(kBlock), 0, st, jobs_dev, nchunks);
+  else
+    hipLaunchKernelGGL(k_fuse_tables<double2>, g, dim3(kBlock), 0, st, jobs_dev, nchunks);
+  SQ_HIP(hipGetLastError());
   return SQ_OK;
 }
 

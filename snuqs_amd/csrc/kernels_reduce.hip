@@ -250,12 +250,14 @@ __global__ void __launch_bounds__(kBlock)
 // uniform gate takes dgg(chunk) times the wave's sum of w in that chunk (one
 // wave sum per chunk for all of them, each lane adding its own gate's term:
 // slots are distinct per gate); a mixed gate is summed per chunk.
+// APPLY = false (with LAM): the derivative terms only, psi and lambda read and
+// left unchanged (the run's phases are then applied by another pass).
 __device__ __forceinline__ int diag_loc(const DiagRunGate &g, uint64_t i) {
   return g.b1 < 0 ? (int)((i >> g.b0) & 1)
                   : (int)((((i >> g.b0) & 1) << 1) | ((i >> g.b1) & 1));
 }
 
-template <typename T, typename R, bool LAM>
+template <typename T, typename R, bool LAM, bool APPLY>
 __global__ void __launch_bounds__(kBlock)
     k_adjoint_diag_run(T *__restrict__ psi, T *__restrict__ lam, uint64_t n,
                        const DiagRunGate *__restrict__ gates, int m, int nslots,
@@ -301,7 +303,7 @@ __global__ void __launch_bounds__(kBlock)
     lr[u] = 1.0;
     li[u] = 0.0;
   }
-  for (int k = 0; k < m; ++k) {
+  for (int k = 0; APPLY && k < m; ++k) {
     const DiagRunGate &gk = sg[k];
     if (!is_local(gk)) continue;
 #pragma unroll
@@ -351,15 +353,17 @@ __global__ void __launch_bounds__(kBlock)
       cr = sg[myk].ginv[myloc][0];
       ci = sg[myk].ginv[myloc][1];
     }
+    if (APPLY) {
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const double orr = __shfl_xor(cr, off), oi = __shfl_xor(ci, off);
-      const double tr = cr * orr - ci * oi;
-      ci = cr * oi + ci * orr;
-      cr = tr;
+      for (int off = 1; off < 64; off <<= 1) {
+        const double orr = __shfl_xor(cr, off), oi = __shfl_xor(ci, off);
+        const double tr = cr * orr - ci * oi;
+        ci = cr * oi + ci * orr;
+        cr = tr;
+      }
+      cr = __shfl(cr, 0);
+      ci = __shfl(ci, 0);
     }
-    cr = __shfl(cr, 0);
-    ci = __shfl(ci, 0);
     if (LAM && uslots) {  // uniform slot gates: dgg(chunk) times the wave's sum of w
       double tr2 = 0.0, ti2 = 0.0;
 #pragma unroll
@@ -377,10 +381,12 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int loc = diag_loc(gk, base + threadIdx.x + (uint64_t)kBlock * u);
-        const double gr = gk.ginv[loc][0], gi = gk.ginv[loc][1];
-        const double tr = pr[u] * gr - pi[u] * gi;
-        pi[u] = pr[u] * gi + pi[u] * gr;
-        pr[u] = tr;
+        if (APPLY) {
+          const double gr = gk.ginv[loc][0], gi = gk.ginv[loc][1];
+          const double tr = pr[u] * gr - pi[u] * gi;
+          pi[u] = pr[u] * gi + pi[u] * gr;
+          pr[u] = tr;
+        }
         if (LAM && gk.slot >= 0) acc += wr[u] * gk.dgg[loc][0] - wi[u] * gk.dgg[loc][1];
       }
       if (LAM && gk.slot >= 0) {  // uniform over the block
@@ -389,7 +395,7 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; APPLY && u < U; ++u) {
       const uint64_t i = base + threadIdx.x + (uint64_t)kBlock * u;
       const double t = pr[u] * cr - pi[u] * ci;
       pi[u] = pr[u] * ci + pi[u] * cr;
@@ -821,7 +827,7 @@ int launch_adjoint_step1(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int bi
 int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
                             const DiagRunGate *gates_dev, int m, int nslots,
                             const int *slot_param_dev, bool distinct, double *partial,
-                            double *out_dev, hipStream_t st) {
+                            double *out_dev, hipStream_t st, bool apply) {
   if (m < 1 || m > kMaxDiagRun || nslots < 0 || nslots > kMaxDiagSlots)
     return fail(SQ_ERR_INVALID, "internal: diagonal run out of range");
   const uint64_t chunk = (uint64_t)kBlock * 4;
@@ -831,13 +837,17 @@ int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
   uint64_t nb = n / chunk;
   if (nb > 1536) nb = 1536;
   if (!lam && nslots) return fail(SQ_ERR_INVALID, "internal: derivative slots need lambda");
+  if (!lam && !apply) return fail(SQ_ERR_INVALID, "internal: a forward run applies its phases");
   if (dt == SQ_COMPLEX64) {
-    auto k = lam ? k_adjoint_diag_run<float2, float, true> : k_adjoint_diag_run<float2, float, false>;
+    auto k = !lam   ? k_adjoint_diag_run<float2, float, false, true>
+             : apply ? k_adjoint_diag_run<float2, float, true, true>
+                     : k_adjoint_diag_run<float2, float, true, false>;
     hipLaunchKernelGGL(k, dim3((unsigned)nb), dim3(kBlock), 0, st, (float2 *)psi, (float2 *)lam,
                        n, gates_dev, m, nslots, partial);
   } else {
-    auto k = lam ? k_adjoint_diag_run<double2, double, true>
-                 : k_adjoint_diag_run<double2, double, false>;
+    auto k = !lam   ? k_adjoint_diag_run<double2, double, false, true>
+             : apply ? k_adjoint_diag_run<double2, double, true, true>
+                     : k_adjoint_diag_run<double2, double, true, false>;
     hipLaunchKernelGGL(k, dim3((unsigned)nb), dim3(kBlock), 0, st, (double2 *)psi, (double2 *)lam,
                        n, gates_dev, m, nslots, partial);
   }

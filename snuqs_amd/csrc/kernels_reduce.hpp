@@ -38,7 +38,8 @@ int launch_adjoint_step1(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int bi
 // derivative term).  gates_dev: m DiagRunGate records; slot_param_dev: the
 // parameter of each of the nslots slots.  lam == nullptr (and nslots == 0):
 // psi alone is multiplied by the records' `ginv` diagonals (used with the
-// gates' own diagonals for a forward run).
+// gates' own diagonals for a forward run).  apply == false (reverse only):
+// the derivative terms alone, both states left unchanged.
 constexpr int kMaxDiagRun = 64;
 constexpr int kMaxDiagSlots = 32;
 struct DiagRunGate {
@@ -53,7 +54,7 @@ struct DiagRunGate {
 int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
                             const DiagRunGate *gates_dev, int m, int nslots,
                             const int *slot_param_dev, bool distinct, double *partial,
-                            double *out_dev, hipStream_t st);
+                            double *out_dev, hipStream_t st, bool apply = true);
 // Reverse step over K = 2..4 one-target gates on distinct amplitude bits
 // (bits[j] = bit of gate j, none of the lowest 4): psi and lambda undone by
 // every gate's G^dagger (ginv, row-major complex128 2x2 each), and for gate j
@@ -82,11 +83,35 @@ int launch_finalize_slots(const double *partial, int nblocks, int nslots,
 // point, and both states are undone by every gm_j.  partial needs nslots x
 // layer_pass_blocks(n) doubles.
 constexpr int kLayerMaxBlocks = 8192;
+// A diagonal run folded into a low pass (h0 = 12, W = 0): every amplitude i
+// is multiplied by the product over the run's gates of their diagonal entry
+// at i, as tl[i & 4095] (gates on bits 0-11) x tc[i >> 12] (gates on bits 12
+// and above) x the nmix gates with one bit on each side (records as the run's
+// DiagRunGate: ginv holds the entries applied).  The tables hold complex
+// values of the state's precision.  Reverse: before the layer's gates;
+// forward: after them.
+constexpr int kFuseMix = 4;
+struct FuseDesc {
+  const void *tl, *tc;
+  int nmix, pad;
+  DiagRunGate mix[kFuseMix];
+};
+// Tables of fused runs: job j fills tl (4096 entries) and tc (n >> 12
+// entries) from its m gate records, products in FP64 rounded once.
+struct FuseTableJob {
+  const DiagRunGate *g;
+  int m, pad;
+  void *tl, *tc;
+};
+int launch_fuse_tables(const FuseTableJob *jobs_dev, int njobs, sq_dtype_t dt, uint64_t n,
+                       hipStream_t st);
 int layer_pass_blocks(uint64_t n);
+// fuse_dev: a FuseDesc in device memory (low pass only), or nullptr.
 int launch_adjoint_layer(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int h0, int W, int C,
                          int count, const int *pos, const double *gm, const double *dm,
                          const int *slot, int nslots, const int *slot_param_dev, bool distinct,
-                         double *partial, double *out_dev, hipStream_t st);
+                         double *partial, double *out_dev, hipStream_t st,
+                         const FuseDesc *fuse_dev = nullptr);
 int launch_pauli_terms(const void *sv, sq_dtype_t dt, uint64_t n, int nterms,
                        const void *terms_dev, const double *alpha_dev,
                        double *partial, double *out_dev, hipStream_t st);

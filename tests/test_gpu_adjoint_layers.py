@@ -159,3 +159,67 @@ def test_wide_windows_match_narrow(C, n):
     for e, g in res.values():
         assert abs(e - e0) < 1e-10
         assert np.abs(g - g0).max() < 1e-10
+
+
+def _fuse_adjoint(C, n, ops, pmap, nparams, obs, dtype, checkpoints, fuse):
+    saved = C.core.tune_get("adjoint_fuse_runs")
+    C.core.tune_set("adjoint_fuse_runs", fuse)
+    try:
+        return _adjoint(C, n, ops, pmap, nparams, obs, dtype, checkpoints, 1)
+    finally:
+        C.core.tune_set("adjoint_fuse_runs", saved)
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-10), ("complex64", 5e-5)])
+@pytest.mark.parametrize("n,checkpoints", [(12, 0), (16, 3), (22, 2)])
+def test_fused_runs_match_unfused(C, n, checkpoints, dtype, tol):
+    """The ansatz's RotZ layer and CZ ring (a diagonal run) folded into the
+    low pass of the RotY layer below it, forward, reverse and both (at 16 and
+    22 qubits two CZs cross bit 12), against the unfused sweep and the
+    register groups."""
+    from snuqs_amd.circuits import hea_ansatz, random_pauli_sum
+
+    layers = 3
+    thetas = np.random.default_rng(n).uniform(0, 2 * np.pi, 2 * n * layers)
+    ops, pmap = hea_ansatz(n, layers, thetas)
+    obs = random_pauli_sum(n, 8, 4, np.random.default_rng(n + 7))
+    e0, g0 = _adjoint(C, n, ops, pmap, thetas.size, obs, dtype, checkpoints, 0)
+    for fuse in (0, 1, 2, 3):
+        e, g = _fuse_adjoint(C, n, ops, pmap, thetas.size, obs, dtype, checkpoints, fuse)
+        assert abs(e - e0) < tol, fuse
+        assert np.abs(g - g0).max() < tol, fuse
+
+
+def test_fused_runs_vs_oracle_and_declined(C):
+    """13 qubits against central differences of the FP64 oracle; a run with
+    more than four gates across bit 12 (CZ(q, q + 6) on 14 qubits) is not
+    folded and still matches."""
+    from snuqs_amd import _C
+    from snuqs_amd.circuits import hea_ansatz, random_pauli_sum
+
+    n, layers = 13, 2
+    thetas = np.random.default_rng(13).uniform(0, 2 * np.pi, 2 * n * layers)
+    ops, pmap = hea_ansatz(n, layers, thetas)
+    obs = random_pauli_sum(n, 6, 3, np.random.default_rng(3))
+    e, grads = _fuse_adjoint(C, n, ops, pmap, thetas.size, obs, "complex128", 2, 3)
+    assert abs(e - _energy_oracle(n, ops, *obs)) < 1e-10
+    h = 1e-6
+    for p in range(thetas.size):
+        up, dn = thetas.copy(), thetas.copy()
+        up[p] += h
+        dn[p] -= h
+        fd = (_energy_oracle(n, hea_ansatz(n, layers, up)[0], *obs) -
+              _energy_oracle(n, hea_ansatz(n, layers, dn)[0], *obs)) / (2 * h)
+        assert abs(grads[p] - fd) < 1e-7, (p, grads[p], fd)
+    g = _C.operation.gate_operations
+    n = 14
+    th = np.random.default_rng(14).uniform(0, 2 * np.pi, 2 * n)
+    ops = [g.RotY([q], th[q]) for q in range(n)]
+    ops += [g.RotZ([q], th[n + q]) for q in range(n)]
+    ops += [g.CZ([q, q + 6]) for q in range(n - 6)]
+    ops += [g.RotY([q], 0.3 * q) for q in range(n)]
+    pmap = [[(0, q)] for q in range(2 * n)] + [[] for _ in range(2 * n - 6)]
+    obs = random_pauli_sum(n, 6, 3, np.random.default_rng(4))
+    e0, g0 = _fuse_adjoint(C, n, ops, pmap, 2 * n, obs, "complex128", 0, 0)
+    e1, g1 = _fuse_adjoint(C, n, ops, pmap, 2 * n, obs, "complex128", 0, 3)
+    assert abs(e1 - e0) < 1e-12 and np.abs(g1 - g0).max() < 1e-12
