@@ -86,7 +86,9 @@ int sq_copy(void *dst, const void *src, size_t bytes, void *stream);
  * gates on distinct qubits in passes of up to 12 bits; default 1) and
  * "adjoint_wide_windows" (0/1, those passes take windows of up to 6 high bits
  * with 512 B runs where that needs no more passes than 8-bit windows with
- * 128 B runs; default 1).  A gate tuned by
+ * 128 B runs; default 1) and "adjoint_fuse_runs" (0-3, diagonal runs folded
+ * into the low pass of the layer next to them: bit 0 forward, bit 1 reverse;
+ * default 3).  A gate tuned by
  * sq_program_tune uses its own shape instead.  No reference counterpart. */
 int sq_tune_set(const char *key, int value);
 int sq_tune_get(const char *key, int *value);
