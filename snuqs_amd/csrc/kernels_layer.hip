@@ -171,24 +171,54 @@ __device__ __forceinline__ void layer_xpose(T (&a)[16], T (&l)[16], T *X, int t)
 
 // Multiply the thread's 16 amplitudes (layout L2 of a low pass: index
 // chunk << 12 | t | r << 8) by a fused diagonal run's phases: the tables'
-// products were formed in FP64 and rounded to the state's precision once;
-// here one product of the two table entries and the mixed gates' entries,
-// in the state's precision.
+// products were formed in FP64 and rounded to the state's precision once.
+// Per chunk, the chunk's entry and the mixed gates whose local bit lies in
+// the thread part (bits 0-7) make one factor per thread; a mixed gate on a
+// register bit (8-11) leaves a pair of entries, selected per register.
 template <typename T, typename R, bool LAM>
 __device__ __forceinline__ void fuse_phases(T (&a)[16], T (&l)[16], const FuseDesc *fz,
                                             uint64_t chunk, int t) {
   const T *tl = reinterpret_cast<const T *>(fz->tl);
   const T cf = reinterpret_cast<const T *>(fz->tc)[chunk];
   const int nmix = fz->nmix;
+  R sr = cf.x, si = cf.y;
+  R f0r[kFuseMix], f0i[kFuseMix], f1r[kFuseMix], f1i[kFuseMix];
+  int rb[kFuseMix];
+#pragma unroll
+  for (int k = 0; k < kFuseMix; ++k) {
+    rb[k] = -1;
+    f0r[k] = f1r[k] = 1;
+    f0i[k] = f1i[k] = 0;
+    if (k >= nmix) continue;
+    const DiagRunGate &g = fz->mix[k];
+    // loc = (bit b0) << 1 | (bit b1); one of b0, b1 is below 12
+    const bool l0 = g.b0 < 12;
+    const int lb = l0 ? g.b0 : g.b1, cb = l0 ? g.b1 : g.b0;
+    const int cv = (int)((chunk >> (cb - 12)) & 1);
+    const int loc0 = l0 ? cv : cv << 1, loc1 = l0 ? (2 | cv) : ((cv << 1) | 1);
+    if (lb < 8) {  // thread part: one factor per thread
+      const int loc = ((t >> lb) & 1) ? loc1 : loc0;
+      const R gr = (R)g.ginv[loc][0], gi = (R)g.ginv[loc][1];
+      const R tr = sr * gr - si * gi;
+      si = sr * gi + si * gr;
+      sr = tr;
+    } else {
+      rb[k] = lb - 8;
+      f0r[k] = (R)g.ginv[loc0][0];
+      f0i[k] = (R)g.ginv[loc0][1];
+      f1r[k] = (R)g.ginv[loc1][0];
+      f1i[k] = (R)g.ginv[loc1][1];
+    }
+  }
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const T lf = tl[t + 256 * r];
-    R pr = cf.x * lf.x - cf.y * lf.y, pi = cf.x * lf.y + cf.y * lf.x;
-    const uint64_t i = (chunk << 12) | (uint64_t)(t + 256 * r);
-    for (int k = 0; k < nmix; ++k) {
-      const DiagRunGate &g = fz->mix[k];
-      const int loc = (int)((((i >> g.b0) & 1) << 1) | ((i >> g.b1) & 1));
-      const R gr = (R)g.ginv[loc][0], gi = (R)g.ginv[loc][1];
+    R pr = sr * lf.x - si * lf.y, pi = sr * lf.y + si * lf.x;
+#pragma unroll
+    for (int k = 0; k < kFuseMix; ++k) {
+      if (rb[k] < 0) continue;
+      const bool one = (r >> rb[k]) & 1;
+      const R gr = one ? f1r[k] : f0r[k], gi = one ? f1i[k] : f0i[k];
       const R tr = pr * gr - pi * gi;
       pi = pr * gi + pi * gr;
       pr = tr;
@@ -216,7 +246,7 @@ __device__ __forceinline__ void fuse_phases(T (&a)[16], T (&l)[16], const FuseDe
 // they go out as 512 B runs too.  FUSE (low pass, PH = 7): a diagonal run's
 // phases (fz) before the layer's gates (reverse) or after them (forward).
 template <typename T, typename R, typename O, bool LAM, int PH, bool REAL, bool WIDE, bool FUSE>
-__global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 1 : 2)
+__global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? (FUSE && LAM ? 3 : 1) : 2)
     k_adjoint_layer(T *__restrict__ psi, T *__restrict__ lam, uint64_t nchunks, int W, int h0,
                     int C, LayerArgs<R> A, double *__restrict__ partial,
                     const FuseDesc *__restrict__ fz) {

@@ -169,25 +169,39 @@ struct PauliApplyTerm {
   double cr, ci;
 };
 
+// y_j += c_t (-1)^{popc(i & z_t)} in_i, i = j ^ x_t, over the terms in order
+template <typename T>
+__device__ __forceinline__ void pauli_acc(const T *__restrict__ in, uint64_t j,
+                                          const PauliApplyTerm &pt, const double2 &v,
+                                          double &yr, double &yi) {
+  double cr = pt.cr, ci = pt.ci;
+  if (__popcll((j ^ pt.x) & pt.z) & 1) {
+    cr = -cr;
+    ci = -ci;
+  }
+  yr += cr * v.x - ci * v.y;
+  yi += cr * v.y + ci * v.x;
+}
+
+// The gathers of 8 terms go out before their sums (the same order of
+// additions as one term at a time, so the result is unchanged).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_apply_pauli_sum(const T *__restrict__ in, T *__restrict__ out, uint64_t n,
                       const PauliApplyTerm *__restrict__ terms, int nterms) {
+  constexpr int U = 8;
   for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < n;
        j += (uint64_t)gridDim.x * kBlock) {
     double yr = 0.0, yi = 0.0;
-    for (int t = 0; t < nterms; ++t) {
-      const PauliApplyTerm pt = terms[t];
-      const uint64_t i = j ^ pt.x;
-      const double2 v = ld2(in, i);
-      double cr = pt.cr, ci = pt.ci;
-      if (__popcll(i & pt.z) & 1) {
-        cr = -cr;
-        ci = -ci;
-      }
-      yr += cr * v.x - ci * v.y;
-      yi += cr * v.y + ci * v.x;
+    int t = 0;
+    for (; t + U <= nterms; t += U) {
+      double2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = ld2(in, j ^ terms[t + u].x);
+#pragma unroll
+      for (int u = 0; u < U; ++u) pauli_acc(in, j, terms[t + u], v[u], yr, yi);
     }
+    for (; t < nterms; ++t) pauli_acc(in, j, terms[t], ld2(in, j ^ terms[t].x), yr, yi);
     T o;
     o.x = yr;
     o.y = yi;
