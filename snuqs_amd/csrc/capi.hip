@@ -1611,6 +1611,29 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       fused_runs.push_back(&fruns[ffuse[fi]]);
     }
   const size_t nfuse = fused_runs.size();
+  // Deferred finalisation: the reverse passes with derivative slots (runs,
+  // layer passes, groups) write their partial rows to their own region of
+  // one buffer, and one launch at the end adds each parameter's rows in pass
+  // order (instead of a finalisation launch per pass).
+  size_t drow_doubles = 0, nrows = 0;
+  for (const Run &r : runs) {
+    drow_doubles += (size_t)r.nslots * diag_run_blocks(n);
+    nrows += r.nslots;
+  }
+  for (const LPass &ps : lpasses) {
+    drow_doubles += (size_t)ps.nslots * layer_pass_blocks(n);
+    nrows += ps.nslots;
+  }
+  for (const Group &gr : groups) {
+    drow_doubles += (size_t)gr.k * step_group_blocks(n, (int)gr.k, gr.bits);
+    nrows += gr.k;
+  }
+  const bool defer = nrows > 0 && drow_doubles <= (64ull << 20);  // rows of at most 512 MB
+  const size_t dp_b = defer ? (drow_doubles * sizeof(double) + 255) / 256 * 256 : 0;
+  const size_t rowmeta_b =
+      defer ? (nrows * (sizeof(uint64_t) + sizeof(int)) + (num_params + 1) * sizeof(int) + 1024 +
+               255) / 256 * 256
+            : 0;
   const size_t fz_b = (nfuse * (sizeof(FuseDesc) + sizeof(FuseTableJob)) + 255) / 256 * 256;
   const size_t tab_stride = ((4096 + (n >> 12)) * itemsize(dt) + 255) / 256 * 256;
   const size_t tab_b = nfuse ? nfuse * tab_stride : 0;
@@ -1622,8 +1645,10 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   const size_t edev = expect_ws_dev(nterms);
   const size_t apb = nterms * pauli_apply_term_bytes() + 256;
   Workspace *w;
-  SQ_TRY(ws_reserve(dev, edev + grad_b + apb + rg_b + rs_b + rp_b + dblk_b + fz_b + tab_b + 4096,
-                    nterms * 64 + grad_b + apb + rg_b + rs_b + dblk_b + fz_b + 4096, &w));
+  SQ_TRY(ws_reserve(dev,
+                    edev + grad_b + apb + rg_b + rs_b + rp_b + dblk_b + fz_b + tab_b + dp_b +
+                        rowmeta_b + 4096,
+                    nterms * 64 + grad_b + apb + rg_b + rs_b + dblk_b + fz_b + rowmeta_b + 4096, &w));
   hipStream_t st = (hipStream_t)stream;
   char *dbase = (char *)w->dev;
   double *partial = (double *)dbase;                      // reduction partials
@@ -1638,6 +1663,8 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
   FuseDesc *fdesc_dev = (FuseDesc *)al(dblk_dev + dblk_b);
   FuseTableJob *fjob_dev = (FuseTableJob *)(fdesc_dev + nfuse);
   char *tab_dev = al((char *)fdesc_dev + fz_b);
+  double *dpart_dev = (double *)al(tab_dev + tab_b);       // deferred partial rows
+  char *rowmeta_dev = al((char *)dpart_dev + dp_b);
   // staged through the pinned host buffer, after the gradient slots
   char *hs = (char *)al((char *)w->host + apb + grad_b);
   if (any_runs) {
@@ -1747,22 +1774,38 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     }
     return SQ_OK;
   };
+  // where the next slot-carrying pass writes its rows, and the rows so far
+  size_t dcur = 0;
+  std::vector<uint64_t> row_off;
+  std::vector<int> row_nb, row_param;
+  auto rows_at = [&](uint32_t nslots, int nb, uint32_t first_slot) -> double * {
+    if (!defer) return rpartial;
+    double *at = dpart_dev + dcur;
+    for (uint32_t sl = 0; sl < nslots; ++sl) {
+      row_off.push_back(dcur + (size_t)sl * nb);
+      row_nb.push_back(nb);
+      row_param.push_back(rslots[first_slot + sl]);
+    }
+    dcur += (size_t)nslots * nb;
+    return at;
+  };
+  double *fin = defer ? nullptr : gslots;  // per-pass finalisation target
   for (int g = (int)ng - 1; g >= 0; --g) {
     if (run_top[g] >= 0 && rfuse[run_top[g]] >= 0) {
       const int ri = run_top[g];
       const Run &r = runs[ri];
       if (r.nslots)  // derivative terms only; the low pass undoes the phases
-        SQ_TRY(launch_adjoint_diag_run(sv, work0, dt, n, rgates_dev + r.first, (int)r.m,
-                                       (int)r.nslots, rslots_dev + r.first_slot,
-                                       distinct_params(r.first_slot, r.nslots), rpartial, gslots,
-                                       st, false));
+        SQ_TRY(launch_adjoint_diag_run(
+            sv, work0, dt, n, rgates_dev + r.first, (int)r.m, (int)r.nslots,
+            rslots_dev + r.first_slot, distinct_params(r.first_slot, r.nslots),
+            rows_at(r.nslots, diag_run_blocks(n), r.first_slot), fin, st, false));
       const Layer &L = layers[rfuse[ri]];
       auto pass = [&](uint32_t i, const FuseDesc *fz) {
         const LPass &ps = lpasses[i];
-        return launch_adjoint_layer(sv, work0, dt, n, ps.h0, ps.W, ps.C, ps.count, ps.pos, ps.gm,
-                                    ps.dm, ps.slot, (int)ps.nslots, rslots_dev + ps.first_slot,
-                                    distinct_params(ps.first_slot, ps.nslots), rpartial, gslots,
-                                    st, fz);
+        return launch_adjoint_layer(
+            sv, work0, dt, n, ps.h0, ps.W, ps.C, ps.count, ps.pos, ps.gm, ps.dm, ps.slot,
+            (int)ps.nslots, rslots_dev + ps.first_slot, distinct_params(ps.first_slot, ps.nslots),
+            rows_at(ps.nslots, layer_pass_blocks(n), ps.first_slot), fin, st, fz);
       };
       SQ_TRY(pass((uint32_t)rfuse_low[ri], fdesc_dev + rfuse_id[ri]));
       for (uint32_t i = L.first_pass; i < L.first_pass + L.npass; ++i)
@@ -1775,7 +1818,8 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       const Run &r = runs[run_top[g]];
       SQ_TRY(launch_adjoint_diag_run(sv, work0, dt, n, rgates_dev + r.first, (int)r.m,
                                      (int)r.nslots, rslots_dev + r.first_slot,
-                                     distinct_params(r.first_slot, r.nslots), rpartial, gslots,
+                                     distinct_params(r.first_slot, r.nslots),
+                                     rows_at(r.nslots, diag_run_blocks(n), r.first_slot), fin,
                                      st));
       SQ_TRY(restore_below(r.lo));
       g = (int)r.lo;  // the loop steps below the run
@@ -1785,11 +1829,10 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       const Layer &L = layers[layer_top[g]];
       for (uint32_t i = L.first_pass; i < L.first_pass + L.npass; ++i) {
         const LPass &ps = lpasses[i];
-        SQ_TRY(launch_adjoint_layer(sv, work0, dt, n, ps.h0, ps.W, ps.C, ps.count, ps.pos, ps.gm,
-                                    ps.dm,
-                                    ps.slot, (int)ps.nslots, rslots_dev + ps.first_slot,
-                                    distinct_params(ps.first_slot, ps.nslots), rpartial, gslots,
-                                    st));
+        SQ_TRY(launch_adjoint_layer(
+            sv, work0, dt, n, ps.h0, ps.W, ps.C, ps.count, ps.pos, ps.gm, ps.dm, ps.slot,
+            (int)ps.nslots, rslots_dev + ps.first_slot, distinct_params(ps.first_slot, ps.nslots),
+            rows_at(ps.nslots, layer_pass_blocks(n), ps.first_slot), fin, st));
       }
       SQ_TRY(restore_below(L.lo));
       g = (int)L.lo;
@@ -1797,10 +1840,10 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     }
     if (grp_top[g] >= 0) {
       const Group &gr = groups[grp_top[g]];
-      SQ_TRY(launch_adjoint_step_group(sv, work0, dt, n, (int)gr.k, gr.bits, gr.ginv, gr.dgg,
-                                       rslots_dev + gr.first_slot,
-                                       distinct_params(gr.first_slot, gr.k), rpartial, gslots,
-                                       st));
+      SQ_TRY(launch_adjoint_step_group(
+          sv, work0, dt, n, (int)gr.k, gr.bits, gr.ginv, gr.dgg, rslots_dev + gr.first_slot,
+          distinct_params(gr.first_slot, gr.k),
+          rows_at(gr.k, step_group_blocks(n, (int)gr.k, gr.bits), gr.first_slot), fin, st));
       g = (int)gr.lo;
       continue;
     }
@@ -1835,6 +1878,32 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
                           1.0, st));
     }
     if (g > 0) SQ_TRY(launch_plan(p->inv[g], work0, dt, n, st, p->inv_dev[g]));
+  }
+  if (defer && !row_off.empty()) {
+    if (dcur > drow_doubles || row_off.size() > nrows)
+      return fail(SQ_ERR_INVALID, "internal: deferred rows exceed their plan");
+    // rows grouped by parameter, pass order kept inside each group
+    const size_t nr = row_off.size();
+    std::vector<int> ptr(num_params + 1, 0), order(nr);
+    for (size_t i = 0; i < nr; ++i) ++ptr[row_param[i] + 1];
+    for (uint32_t q = 0; q < num_params; ++q) ptr[q + 1] += ptr[q];
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (size_t i = 0; i < nr; ++i) order[fill[row_param[i]]++] = (int)i;
+    char *hm = al(hs + rg_b + rs_b + dblk_b + fz_b);
+    uint64_t *h_off = (uint64_t *)hm;
+    int *h_nb = (int *)(h_off + nr);
+    int *h_ptr = h_nb + nr;
+    for (size_t i = 0; i < nr; ++i) {
+      h_off[i] = row_off[order[i]];
+      h_nb[i] = row_nb[order[i]];
+    }
+    std::memcpy(h_ptr, ptr.data(), (num_params + 1) * sizeof(int));
+    const size_t mb = nr * (sizeof(uint64_t) + sizeof(int)) + (num_params + 1) * sizeof(int);
+    SQ_HIP(hipMemcpyAsync(rowmeta_dev, hm, mb, hipMemcpyHostToDevice, st));
+    const uint64_t *d_off = (const uint64_t *)rowmeta_dev;
+    const int *d_nb = (const int *)(d_off + nr);
+    SQ_TRY(launch_finalize_rows(dpart_dev, d_off, d_nb, d_nb + nr, (int)num_params, 2.0, gslots,
+                                st));
   }
   double *hg = (double *)al((char *)w->host + apb);
   SQ_HIP(hipMemcpyAsync(hg, gslots, grad_b, hipMemcpyDeviceToHost, st));

@@ -455,7 +455,7 @@ int launch_adjoint_layer(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int h0
       launch_layer_t<double2, double, false>(psi, nullptr, nchunks, W, h0, C, a, nullptr, nb, st, fuse_dev);
   }
   SQ_HIP(hipGetLastError());
-  if (lam && nslots > 0)
+  if (lam && nslots > 0 && out_dev)
     SQ_TRY(launch_finalize_slots(partial, nb, nslots, slot_param_dev, distinct, 2.0, out_dev, st));
   return SQ_OK;
 }

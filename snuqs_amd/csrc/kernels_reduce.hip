@@ -838,6 +838,11 @@ int launch_adjoint_step1(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int bi
   return SQ_OK;
 }
 
+int diag_run_blocks(uint64_t n) {
+  const uint64_t nb = n / ((uint64_t)kBlock * 4);
+  return (int)(nb > 1536 ? 1536 : (nb < 1 ? 1 : nb));
+}
+
 int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
                             const DiagRunGate *gates_dev, int m, int nslots,
                             const int *slot_param_dev, bool distinct, double *partial,
@@ -848,8 +853,7 @@ int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
   if (n % chunk) return fail(SQ_ERR_INVALID, "internal: diagonal run needs n % 1024 == 0");
   // at most 1536 workgroups (2 rounds of 3 per CU): each then covers several
   // chunks, over which its local phase products and sums of w are shared
-  uint64_t nb = n / chunk;
-  if (nb > 1536) nb = 1536;
+  const uint64_t nb = (uint64_t)diag_run_blocks(n);
   if (!lam && nslots) return fail(SQ_ERR_INVALID, "internal: derivative slots need lambda");
   if (!lam && !apply) return fail(SQ_ERR_INVALID, "internal: a forward run applies its phases");
   if (dt == SQ_COMPLEX64) {
@@ -865,7 +869,7 @@ int launch_adjoint_diag_run(void *psi, void *lam, sq_dtype_t dt, uint64_t n,
     hipLaunchKernelGGL(k, dim3((unsigned)nb), dim3(kBlock), 0, st, (double2 *)psi, (double2 *)lam,
                        n, gates_dev, m, nslots, partial);
   }
-  if (nslots)
+  if (nslots && out_dev)
     hipLaunchKernelGGL(k_finalize_slots, dim3(distinct ? nslots : 1), dim3(kBlock), 0, st,
                        partial, (int)nb, nslots, slot_param_dev, 2.0, out_dev);
   SQ_HIP(hipGetLastError());
@@ -927,6 +931,42 @@ static void launch_group_t(void *psi, void *lam, uint64_t n, const int *bits, co
                      (T *)psi, (T *)nullptr, n >> K, make_idxmap(ins, 0), a, partial);
 }
 
+int step_group_blocks(uint64_t n, int k, const int *bits) {
+  int hi = -1;
+  for (int j = 0; j < k; ++j) hi = std::max(hi, bits[j]);
+  return red_blocks(hi < 6 ? n / 4 : n >> k);
+}
+
+// Deferred finalisation: block p adds, in row order, the sums of its rows
+// (row_off[i], row_nb[i] for i in [param_ptr[p], param_ptr[p + 1])), then
+// out[2p] += alpha * total.
+__global__ void __launch_bounds__(kBlock)
+    k_finalize_rows(const double *__restrict__ partial, const uint64_t *__restrict__ row_off,
+                    const int *__restrict__ row_nb, const int *__restrict__ param_ptr,
+                    double alpha, double *out) {
+  const int p = blockIdx.x;
+  double total = 0.0;
+  for (int i = param_ptr[p]; i < param_ptr[p + 1]; ++i) {
+    const double *row = partial + row_off[i];
+    double acc = 0.0, dummy = 0.0;
+    for (int b = threadIdx.x; b < row_nb[i]; b += kBlock) acc += row[b];
+    block_sum2(acc, dummy);
+    total += acc;  // every thread holds the block's sum
+    __syncthreads();  // block_sum2's LDS is reused by the next row
+  }
+  if (threadIdx.x == 0 && param_ptr[p + 1] > param_ptr[p]) out[2 * p] += alpha * total;
+}
+
+int launch_finalize_rows(const double *partial, const uint64_t *row_off_dev,
+                         const int *row_nb_dev, const int *param_ptr_dev, int nparams,
+                         double alpha, double *out, hipStream_t st) {
+  if (nparams <= 0) return SQ_OK;
+  hipLaunchKernelGGL(k_finalize_rows, dim3(nparams), dim3(kBlock), 0, st, partial, row_off_dev,
+                     row_nb_dev, param_ptr_dev, alpha, out);
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
 int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, int k,
                               const int *bits, const double *ginv, const double *dgg,
                               const int *slot_param_dev, bool distinct, double *partial,
@@ -968,7 +1008,7 @@ int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, i
     SQ_GROUP(5)
   }
 #undef SQ_GROUP
-  if (lam)
+  if (lam && out_dev)
     hipLaunchKernelGGL(k_finalize_slots, dim3(distinct ? k : 1), dim3(kBlock), 0, st, partial,
                        nb, k, slot_param_dev, 2.0, out_dev);
   SQ_HIP(hipGetLastError());

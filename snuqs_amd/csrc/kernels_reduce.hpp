@@ -66,6 +66,16 @@ int launch_adjoint_step_group(void *psi, void *lam, sq_dtype_t dt, uint64_t n, i
                               const int *bits, const double *ginv, const double *dgg,
                               const int *slot_param_dev, bool distinct, double *partial,
                               double *out_dev, hipStream_t st);
+// Partial rows per slot of the adjoint passes (nslots x this many doubles).
+// With out_dev == nullptr the passes leave their rows unfinalised (see
+// launch_finalize_rows).
+int diag_run_blocks(uint64_t n);
+int step_group_blocks(uint64_t n, int k, const int *bits);
+// Deferred finalisation of rows: out[2p] += alpha * sum over parameter p's
+// rows (in the order given) of the row's nb doubles at partial + row_off.
+int launch_finalize_rows(const double *partial, const uint64_t *row_off_dev,
+                         const int *row_nb_dev, const int *param_ptr_dev, int nparams,
+                         double alpha, double *out, hipStream_t st);
 // out[2 slot_param[v]] += alpha * sum_b partial[v * nblocks + b], v < nslots
 // (distinct: the slots' parameters differ and are summed in parallel).
 int launch_finalize_slots(const double *partial, int nblocks, int nslots,
