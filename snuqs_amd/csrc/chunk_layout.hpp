@@ -56,17 +56,14 @@ __device__ __forceinline__ void qtranspose(T (&v)[16], T *X, int t, int from, in
   for (int r = 0; r < 16; ++r) v[r] = q[qs * r];
 }
 
-// W = min(8, n - h0) high bits h0 .. h0+W-1 -----------------------------------
+// Mid-pass geometry: W high bits h0 .. h0+W-1 beside C contiguous low bits.
 // A workgroup holds 2^12 amplitudes whose local index j maps to the global
-// index bit by bit: local bits 0-3 -> global bits 0-3 (16 consecutive
-// amplitudes, 128 B of complex64), local bits 4..3+W -> h0..h0+W-1 (the bits
-// rotated), local bits 4+W..11 -> global bits 4..11-W (carried unchanged);
-// the chunk number fills the other n - 12 bits.  Loads use layout L2 (local
-// bits 8-11 in the register index), then one LDS transpose to L1 (bits 4-7),
-// and the stores go out in L1.  Eight rotated bits per pass, against the 4-5
-// a lane can hold in registers without LDS, halves the passes over high bits.
-__device__ __forceinline__ int qmid_pos(int k, int W, int h0) {
-  return k < 4 ? k : k < 4 + W ? h0 + (k - 4) : k - W;
+// index bit by bit: local bits 0..C-1 -> global bits 0..C-1 (2^C consecutive
+// amplitudes), local bits C..C+W-1 -> h0..h0+W-1 (the bits rotated), local
+// bits C+W..11 -> global bits C..11-W (carried unchanged); the chunk number
+// fills the other n - 12 bits.
+__device__ __forceinline__ int qmid_pos(int k, int W, int h0, int C) {
+  return k < C ? k : k < C + W ? h0 + (k - C) : k - W;
 }
 
 // Chunk access.  32-bit offsets: raw buffer loads / stores on a resource at

@@ -12,8 +12,9 @@
 // Work unit: a chunk of 2^L amplitudes (L = min(12, n)); one workgroup per
 // (instance, chunk).  Qubits on bits below L ("low") are applied inside the
 // chunk in LDS together with the cost layer; the higher bits go in groups of
-// up to kQGM bits per "mid" pass over the state, whose workgroups hold 16
-// consecutive amplitudes for each of the 2^8 values of the group's bits.
+// up to kQGM = 7 bits per "mid" pass over the state, whose workgroups hold 32
+// consecutive amplitudes (256 B of complex64 per access) for each of the 2^7
+// values of the group's bits.
 // Grouping is exact for the gradients too: every X_q commutes with the
 // whole mixer layer, so <lambda| X_q |psi> has the same value anywhere inside
 // it and a group's partial can be taken before the group is undone.  Each
@@ -33,7 +34,8 @@
 namespace sq {
 
 constexpr int kQMaxEdges = 128;
-constexpr int kQGM = 8;            // high bits per mid pass
+constexpr int kQGM = 7;            // high bits per mid pass
+constexpr int kQC = 5;             // contiguous low bits beside them
 
 struct QInstDev {
   uint64_t psi, lam;  // amplitude offsets into the batch state buffer
@@ -501,10 +503,17 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
 }
 
 // ---- mid pass: RX(2b) (forward) or RX(-2b) and the beta partial (reverse) on
-// Mid-pass geometry: see qmid_pos (chunk_layout.hpp).
+// the rotated bits h0 .. h0+W-1 ----------------------------------------------
+// Geometry (qmid_pos, chunk_layout.hpp) with C = kQC = 5 contiguous
+// low bits and W = min(7, n - h0) rotated bits: loads in layout L2 (local
+// bits 8-11 in the register index: rotated bits h0+3..), one LDS transpose to
+// L1 (local bits 4-7: bit 4 carried, 5-7 rotated), and back to L2 for the
+// stores, so every global access is 256 B runs (32 consecutive complex64
+// amplitudes); 7 rotated bits still take the 14 high bits of a 26-qubit
+// instance in two passes.
 struct QMid {
   uint32_t base;     // global offset of the chunk (uniform)
-  uint32_t t2, t1;   // thread part of the L2 / L1 offsets
+  uint32_t t2;       // thread part of the L2 offset
   int W;
 };
 
@@ -517,23 +526,20 @@ __device__ __forceinline__ uint32_t qmid_chunk(uint64_t rel, int n, int scatter)
 }
 __device__ __forceinline__ QMid qmid_setup(uint32_t chunk, int n, int h0, int t) {
   QMid m;
-  m.W = min(8, n - h0);
+  m.W = min(kQGM, n - h0);
   const int A = h0 - 12 + m.W;  // chunk bits below h0, at 12-W ..
   m.base = ((chunk & ((1u << A) - 1)) << (12 - m.W)) | ((chunk >> A) << (h0 + m.W));
   m.t2 = (uint32_t)(t & 15);
-  m.t1 = (uint32_t)(t & 15);
 #pragma unroll
-  for (int k = 4; k < 8; ++k) m.t2 |= (uint32_t)((t >> k) & 1) << qmid_pos(k, m.W, h0);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) m.t1 |= (uint32_t)((t >> (4 + k)) & 1) << qmid_pos(8 + k, m.W, h0);
+  for (int k = 4; k < 8; ++k) m.t2 |= (uint32_t)((t >> k) & 1) << qmid_pos(k, m.W, h0, kQC);
   return m;
 }
 
-// register part of the offset: L2 holds local bits 8-11 in r, L1 bits 4-7
-__device__ __forceinline__ uint32_t qmid_r(int r, int lo, int W, int h0) {
+// register part of the L2 offset: local bits 8-11 in r
+__device__ __forceinline__ uint32_t qmid_r(int r, int W, int h0) {
   uint32_t o = 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) o |= (uint32_t)((r >> k) & 1) << qmid_pos(lo + k, W, h0);
+  for (int k = 0; k < 4; ++k) o |= (uint32_t)((r >> k) & 1) << qmid_pos(8 + k, W, h0, kQC);
   return o;
 }
 
@@ -573,10 +579,12 @@ __device__ __forceinline__ double qxpart16m(const T (&a)[16], const T (&l)[16], 
   return s0 + s1;
 }
 
-// register-bit masks of the rotated local bits 4..3+W: L2 phase (local 8-11),
-// L1 phase (local 4-7)
-__device__ __forceinline__ int qmid_mask2(int W) { return (((1 << W) - 1) << 4) >> 8; }
-__device__ __forceinline__ int qmid_mask1(int W) { return ((1 << W) - 1) & 15; }
+// register-bit masks of the rotated local bits kQC..kQC+W-1: L2 phase (local
+// 8-11), L1 phase (local 4-7)
+__device__ __forceinline__ int qmid_mask2(int W) {
+  return ((((1 << W) - 1) << kQC) >> 8) & 15;
+}
+__device__ __forceinline__ int qmid_mask1(int W) { return ((((1 << W) - 1) << kQC) >> 4) & 15; }
 
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
@@ -593,14 +601,15 @@ __global__ void __launch_bounds__(kBlock)
   T *s = st + q.psi + m.base;
   T v[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) v[r] = ld_nt(s + (m.t2 | qmid_r(r, 8, m.W, h0)));
+  for (int r = 0; r < 16; ++r) v[r] = ld_nt(s + (m.t2 | qmid_r(r, m.W, h0)));
   const T cb = ph[(size_t)(ii * p + layer) * kQPh + kQPh - 1];
   const R c0 = cb.x, s0 = cb.y;  // RX(2b) = [[c, -i s], [-i s, c]]
   qrx16m(v, c0, s0, qmid_mask2(m.W));
   qtranspose(v, X, t, 2, 1);
   qrx16m(v, c0, s0, qmid_mask1(m.W));
+  qtranspose(v, X, t, 1, 2);  // writes the L1 slots this thread just read
 #pragma unroll
-  for (int r = 0; r < 16; ++r) st_nt(s + (m.t1 | qmid_r(r, 4, m.W, h0)), v[r]);
+  for (int r = 0; r < 16; ++r) st_nt(s + (m.t2 | qmid_r(r, m.W, h0)), v[r]);
 }
 
 template <typename R>
@@ -621,7 +630,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
   T a[16], l[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    const uint32_t o = m.t2 | qmid_r(r, 8, m.W, h0);
+    const uint32_t o = m.t2 | qmid_r(r, m.W, h0);
     a[r] = ld_nt(sa + o);
     l[r] = ld_nt(sl + o);
   }
@@ -637,9 +646,12 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
   acc += qxpart16m(a, l, m1);
   qrx16m(a, c0, s0, m1);
   qrx16m(l, c0, s0, m1);
+  qtranspose(a, X, t, 1, 2);  // writes the L1 slots this thread just read
+  __syncthreads();
+  qtranspose(l, X, t, 1, 2);
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    const uint32_t o = m.t1 | qmid_r(r, 4, m.W, h0);
+    const uint32_t o = m.t2 | qmid_r(r, m.W, h0);
     st_nt(sa + o, a[r]);
     st_nt(sl + o, l[r]);
   }
