@@ -79,8 +79,9 @@ int sq_copy(void *dst, const void *src, size_t bytes, void *stream);
  * tile index; default 0) and "mfma5_wide_c64"
  * (0/1, 64-group instead of 32-group complex64 tiles: 512 B instead of 256 B
  * runs per access, half the resident waves; default 0) and
- * "qaoa_mid_scatter" (0/1, QAOA batches: mid passes take their chunks in
- * bit-reversed order; applies to batches run or captured afterwards) and
+ * "qaoa_mid_scatter" (0/1, QAOA batches: window passes above bit 11 take
+ * their chunks in bit-reversed order; applies to batches run or captured
+ * afterwards) and
  * "tune_scatter" (0/1, sq_program_tune also times tile order 2; default 1) and
  * "adjoint_layers" (0/1, sq_adjoint_gradient applies layers of one-target
  * gates on distinct qubits in passes of up to 12 bits; default 1) and

@@ -62,7 +62,7 @@ __device__ __forceinline__ void qtranspose(T (&v)[16], T *X, int t, int from, in
 // amplitudes), local bits C..C+W-1 -> h0..h0+W-1 (the bits rotated), local
 // bits C+W..11 -> global bits C..11-W (carried unchanged); the chunk number
 // fills the other n - 12 bits.
-__device__ __forceinline__ int qmid_pos(int k, int W, int h0, int C) {
+__host__ __device__ __forceinline__ int qmid_pos(int k, int W, int h0, int C) {
   return k < C ? k : k < C + W ? h0 + (k - C) : k - W;
 }
 

@@ -122,7 +122,7 @@ struct Tuning {
   int mfma5_pipeline[2] = {1, 1};        // 5-target MFMA: prefetch the next tile
   int mfma5_order[2] = {0, 0};           // 5-target MFMA: 0 cyclic tiles, 1 blocked, 2 scattered
   int mfma5_wide = 0;                    // complex64 5-target MFMA: 64-group tiles
-  int qaoa_mid_scatter = 0;              // QAOA mid passes: bit-reversed chunk order
+  int qaoa_mid_scatter = 0;              // QAOA window passes above bit 11: bit-reversed chunk order
   int tune_scatter = 1;                  // sq_program_tune also tries tile order 2
   int adjoint_layers = 1;                // adjoint sweeps: layer passes (12 / 8 bits a pass)
   int adjoint_wide_windows = 1;          // layer passes: 6-bit windows with 512 B runs

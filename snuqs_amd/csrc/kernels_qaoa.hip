@@ -9,17 +9,27 @@
 //   dE/dg_l = 2 Im <lambda| S |psi>,  dE/db_l = 2 Im <lambda| sum_q X_q |psi>
 // evaluated on the adjoint sweep where lambda = C psi is carried backwards.
 //
-// Work unit: a chunk of 2^L amplitudes (L = min(12, n)); one workgroup per
-// (instance, chunk).  Qubits on bits below L ("low") are applied inside the
-// chunk in LDS together with the cost layer; the higher bits go in groups of
-// up to kQGM = 7 bits per "mid" pass over the state, whose workgroups hold 32
-// consecutive amplitudes (256 B of complex64 per access) for each of the 2^7
-// values of the group's bits.
-// Grouping is exact for the gradients too: every X_q commutes with the
-// whole mixer layer, so <lambda| X_q |psi> has the same value anywhere inside
-// it and a group's partial can be taken before the group is undone.  Each
-// launch runs the same pass for every instance that has it, so one sequence
-// of launches serves the whole batch (and is captured in one hipGraph).
+// Instances of n >= 12 qubits are swept in window passes.  A workgroup holds
+// 4096 amplitudes: the 32 consecutive ones of bits 0-4 for each value of 7
+// more bits, the window (bits 5-11 for window 0, then 12-18, 19-25, ...; see
+// qmid_pos), so every global access moves 256-byte runs of complex64.  An
+// instance has k = 1 + ceil((n - 12) / 7) windows.  The mixer of a layer is
+// split over the windows (the X_q commute), and the pass that ends a layer on
+// its window also applies the next layer's cost (diagonal: it needs only each
+// amplitude's global index) and starts the next layer's mixer on that window
+// and on bits 0-4.  A forward sweep is then p (k - 1) + 1 passes instead of
+// p k (p + 1 for k = 1).  Its last pass (the "turn") also forms the energy
+// and lambda = C psi and undoes its own window on lambda, so it writes lambda
+// only: psi in memory is already its value before that pass.  The reverse
+// sweep undoes the other passes in reverse order with the gradient partials;
+// its last pass (the first forward pass undone) only reads.  Instances below
+// 12 qubits take one workgroup per instance and layer, in LDS.
+// Grouping is exact for the gradients too: every X_q commutes with the whole
+// mixer layer, so <lambda| X_q |psi> has the same value anywhere inside it and
+// a window's partial can be taken wherever psi and lambda stand at the same
+// point of the layer.  Each launch runs one pass (per instance, from its
+// launch-table entry) for every instance that has one, so one sequence of
+// launches serves the whole batch (and is captured in one hipGraph).
 // Reductions write one FP64 partial per (pass, chunk) and are summed in a
 // fixed order by the finaliser: results are bitwise reproducible.
 #include <algorithm>
@@ -34,21 +44,34 @@
 namespace sq {
 
 constexpr int kQMaxEdges = 128;
-constexpr int kQGM = 7;            // high bits per mid pass
+constexpr int kQMaxLayers = 256;
+constexpr int kQGM = 7;            // window bits
 constexpr int kQC = 5;             // contiguous low bits beside them
 
 struct QInstDev {
   uint64_t psi, lam;  // amplitude offsets into the batch state buffer
   uint64_t slot;      // first partial slot
-  int n, nedges, edge_off, lc;  // lc = min(L, n)
-  int nchunks, ngroups;         // ngroups = ceil((n - lc) / kQGM) mid passes per layer
-  int pad1, pad2;
+  int n, nedges, edge_off, lc;  // lc = min(12, n)
+  int nchunks, nwin;  // nwin: windows (n >= 12), beta partial sets per layer; 1 below
+  int win, pad;       // first window (QWin) of the instance
 };
 
-// Launch table entry: instance and its first block index in this launch.
+// Window-pass descriptor (launch-table entry): window g, flags, the layer lp
+// whose mixer the pass applies on the window bits first ("pre"), and the layer
+// lc whose cost it applies next, followed by lc's mixer on the window bits
+// and bits 0-4 ("bnd").  The reverse pass undoes them in the opposite order.
+constexpr uint32_t kQInit = 1;     // forward: start from |+> (nothing read)
+constexpr uint32_t kQPre = 2;      // RX(2 b_lp) on the window bits
+constexpr uint32_t kQBnd = 4;      // cost(lc), then RX(2 b_lc) on window and bits 0-4
+constexpr uint32_t kQNoStore = 8;  // reverse: the sweep's last pass, partials only
+__host__ __device__ constexpr uint32_t qpass(int g, uint32_t flags, int lp, int lc) {
+  return (uint32_t)g | flags << 4 | (uint32_t)lp << 8 | (uint32_t)lc << 16;
+}
+
+// Launch table entry: instance, its pass, and its first block index in this launch.
 struct QEntry {
   int inst;
-  int pad;
+  uint32_t pass;
   uint64_t first_block;
 };
 
@@ -61,25 +84,29 @@ __device__ __forceinline__ int find_entry_global(const QEntry *tab, int count, u
   return lo;
 }
 
-// The launch's entry (instance) whose block range holds workgroup b, and that
-// instance's index.  With at most kBlock entries the table's first blocks are
-// staged in LDS by one load per thread and searched there: one global round
-// trip instead of a chain of dependent loads before the chunk's own loads can
-// start.  Every thread of the workgroup must call it (it synchronises).
+// The launch's entry (instance) whose block range holds workgroup b, that
+// instance's index and pass.  With at most kBlock entries the table's first
+// blocks are staged in LDS by one load per thread and searched there: one
+// global round trip instead of a chain of dependent loads before the chunk's
+// own loads can start.  Every thread of the workgroup must call it (it
+// synchronises).
 struct QFound {
   int entry, inst;
+  uint32_t pass;
   uint64_t first;
 };
 __device__ __forceinline__ QFound find_entry(const QEntry *tab, int count, uint64_t b) {
   __shared__ uint64_t sfb[kBlock];
   __shared__ int sinst[kBlock];
+  __shared__ uint32_t spass[kBlock];
   if (count > kBlock) {
     const int e = find_entry_global(tab, count, b);
-    return {e, tab[e].inst, tab[e].first_block};
+    return {e, tab[e].inst, tab[e].pass, tab[e].first_block};
   }
   if ((int)threadIdx.x < count) {
     sfb[threadIdx.x] = tab[threadIdx.x].first_block;
     sinst[threadIdx.x] = tab[threadIdx.x].inst;
+    spass[threadIdx.x] = tab[threadIdx.x].pass;
   }
   __syncthreads();
   int lo = 0, hi = count - 1;
@@ -87,7 +114,7 @@ __device__ __forceinline__ QFound find_entry(const QEntry *tab, int count, uint6
     const int mid = (lo + hi + 1) >> 1;
     if (sfb[mid] <= b) lo = mid; else hi = mid - 1;
   }
-  return {lo, sinst[lo], sfb[lo]};
+  return {lo, sinst[lo], spass[lo], sfb[lo]};
 }
 
 // find_entry with its results moved to scalar registers: every lane holds the
@@ -98,7 +125,7 @@ __device__ __forceinline__ QFound find_entry_u(const QEntry *tab, int count, uin
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)f.first);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(f.first >> 32));
   return {__builtin_amdgcn_readfirstlane(f.entry), __builtin_amdgcn_readfirstlane(f.inst),
-          (uint64_t)hi << 32 | lo};
+          (uint32_t)__builtin_amdgcn_readfirstlane(f.pass), (uint64_t)hi << 32 | lo};
 }
 
 __device__ __forceinline__ int cut_of(uint64_t i, const uint8_t *e, int ne) {
@@ -129,16 +156,17 @@ __device__ double qblock_sum(double v) {
 }
 
 // Slot layout of instance j (nch = nchunks): energy [nch], then per layer l:
-// gamma [nch], beta [(1 + ngroups) * nch] (low pass first, then mid groups).
+// gamma [nch], beta [nwin * nch] (one set per window).
 __device__ __forceinline__ uint64_t slot_energy(const QInstDev &q) { return q.slot; }
 __device__ __forceinline__ uint64_t slot_gamma(const QInstDev &q, int l) {
-  return q.slot + (uint64_t)q.nchunks * (1 + (uint64_t)l * (2 + q.ngroups));
+  return q.slot + (uint64_t)q.nchunks * (1 + (uint64_t)l * (1 + q.nwin));
 }
 __device__ __forceinline__ uint64_t slot_beta(const QInstDev &q, int l, int part) {
   return slot_gamma(q, l) + (uint64_t)q.nchunks * (1 + part);
 }
 
-// RX(2b) (dir = +1) or RX(-2b) (dir = -1) on the 4 register-index bits.
+// RX(2b) on the 4 register-index bits, with (c0, s0) = (cos b, sin b); the
+// inverse takes -s0.
 template <typename T, typename R>
 __device__ __forceinline__ void qrx16(T (&v)[16], R c0, R s0) {
 #pragma unroll
@@ -154,84 +182,51 @@ __device__ __forceinline__ void qrx16(T (&v)[16], R c0, R s0) {
     }
 }
 
+// RX on the register-index bits set in `mask` (uniform), as qrx16
+template <typename T, typename R>
+__device__ __forceinline__ void qrx16m(T (&v)[16], R c0, R s0, int mask) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    if (!((mask >> b) & 1)) continue;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      if (r & (1 << b)) continue;
+      const T x = v[r], y = v[r | (1 << b)];
+      v[r].x = c0 * x.x + s0 * y.y;
+      v[r].y = c0 * x.y - s0 * y.x;
+      v[r | (1 << b)].x = c0 * y.x + s0 * x.y;
+      v[r | (1 << b)].y = c0 * y.y - s0 * x.x;
+    }
+  }
+}
+
 // Im(conj(l) a) of one amplitude pair in the state's precision: for complex64
 // one FMA after one product (|error| <= 2 ulp of |l||a|, the state's own
-// rounding scale), so only the per-pair result is widened to FP64 and the
+// rounding scale), so only per-pair results are widened to FP64 and the
 // amplitudes stay in 32-bit registers.
 template <typename T>
 __device__ __forceinline__ auto qimcl(const T &l, const T &a) {
   return fma(l.x, a.y, -(l.y * a.x));
 }
 
-// sum over the 4 register bits b and pairs (r, r^b) of
-// Im(conj(l_r) a_{r^b} + conj(l_{r^b}) a_r)  (= Im <l| X_b |a> over the group),
-// accumulated in FP64
+// sum over the register bits b set in `mask` (uniform) and pairs (r, r^b) of
+// Im(conj(l_r) a_{r^b} + conj(l_{r^b}) a_r)  (= Im <l| X_b |a> over the pairs),
+// the two terms of a pair added in the state's precision (one more rounding
+// at the same scale), the pairs accumulated in FP64
 template <typename T>
-__device__ __forceinline__ double qxpart16(const T (&a)[16], const T (&l)[16]) {
-  double s0 = 0.0, s1 = 0.0;
+__device__ __forceinline__ double qxpart16m(const T (&a)[16], const T (&l)[16], int mask) {
+  double s = 0.0;
 #pragma unroll
-  for (int b = 0; b < 4; ++b)
+  for (int b = 0; b < 4; ++b) {
+    if (!((mask >> b) & 1)) continue;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       if (r & (1 << b)) continue;
       const int u = r | (1 << b);
-      s0 += (double)qimcl(l[r], a[u]);
-      s1 += (double)qimcl(l[u], a[r]);
+      s += (double)(qimcl(l[r], a[u]) + qimcl(l[u], a[r]));
     }
-  return s0 + s1;
-}
-
-// Cut counts of full chunks (n >= 12).  With i = base + j, j = t + 256 r < 2^12,
-// split the edges by where their bits lie: both below 12 ("lo"), both at 12 or
-// above ("hi"), or one of each ("cross", low bit a, high bit b):
-//   cut(i) = lo(j) + hi(base) + sum_cross (j_a xor base_b)
-//          = lo(j) + C + sum_{a < 12} j_a w_a,
-//   C   = hi(base) + sum_a popc(base & M_a),   w_a = popc(M_a) - 2 popc(base & M_a),
-// with M_a the mask of bit a's neighbours at bits >= 12.  lo(j) is a 4096-byte
-// table per instance, built at batch creation and stored [t][r] so a thread's 16
-// values are one 16-byte load; C and w_a depend on the chunk only and are
-// computed once per wave in scalar registers.
-struct QCutTab {
-  uint32_t m[kQL];  // low bit a: its neighbours at bits >= 12
-  uint32_t nh[32];  // high bit b: its neighbours at bits above b
-};
-
-struct QCutF {
-  int K;                  // C + sum_{a < 8} t_a w_a
-  int w8, w9, w10, w11;   // w_a of the register bits (uniform)
-  uint4 lo;               // lo(t + 256 r) in byte r & 3 of word r >> 2
-};
-
-// `base` and `ii` must be wave-uniform.
-__device__ __forceinline__ QCutF qcut_full(const QCutTab *__restrict__ cuts,
-                                           const uint4 *__restrict__ lotab, int ii, int n,
-                                           uint32_t base, int t) {
-  QCutF f;
-  f.lo = lotab[(size_t)ii * 256 + t];
-  const QCutTab &c = cuts[ii];
-  int C = 0, w[kQL];
-#pragma unroll
-  for (int a = 0; a < kQL; ++a) {
-    const int pm = __popc(base & c.m[a]);
-    C += pm;
-    w[a] = __popc(c.m[a]) - 2 * pm;
   }
-  for (int b = kQL; b < n; ++b) C += __popc(c.nh[b] & (((base >> b) & 1) ? ~base : base));
-  int K = C;
-#pragma unroll
-  for (int a = 0; a < 8; ++a) K += ((t >> a) & 1) * w[a];
-  f.K = K;
-  f.w8 = w[8];
-  f.w9 = w[9];
-  f.w10 = w[10];
-  f.w11 = w[11];
-  return f;
-}
-
-__device__ __forceinline__ int qcut_r(const QCutF &f, int r) {
-  const uint32_t word = r < 4 ? f.lo.x : r < 8 ? f.lo.y : r < 12 ? f.lo.z : f.lo.w;
-  return f.K + (int)((word >> (8 * (r & 3))) & 0xffu) + (r & 1) * f.w8 + ((r >> 1) & 1) * f.w9 +
-         ((r >> 2) & 1) * f.w10 + ((r >> 3) & 1) * f.w11;
+  return s;
 }
 
 // Per (instance, layer): the cost-layer phases exp(-i g s_c), s_c = |E| - 2c for
@@ -260,59 +255,390 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// ---- window passes (n >= 12) -------------------------------------------------
+// Geometry (qmid_pos, chunk_layout.hpp) with C = kQC = 5 contiguous low bits
+// and W = min(7, n - h0) window bits h0 .. h0+W-1: loads and stores in layout
+// L2 (local bits 8-11 in the register index), the cost and the window's local
+// bits 5-7 in layout L1 (local bits 4-7), bits 0-3 in layout L0.
+struct QMid {
+  uint32_t base;  // global offset of the chunk (uniform)
+  uint32_t t2;    // thread part of the L2 offset
+  int W;
+};
+
+// Chunk of a window workgroup: its index within the instance's launch, or
+// (scatter) that index bit-reversed over the instance's n - 12 chunk bits, so
+// concurrent workgroups take chunks far apart instead of neighbouring ones.
+__device__ __forceinline__ uint32_t qmid_chunk(uint64_t rel, int n, int scatter) {
+  const uint32_t c = (uint32_t)rel;
+  return scatter && n > kQL ? __builtin_bitreverse32(c) >> (32 - (n - kQL)) : c;
+}
+__device__ __forceinline__ QMid qmid_setup(uint32_t chunk, int n, int h0, int t) {
+  QMid m;
+  m.W = min(kQGM, n - h0);
+  const int A = h0 - 12 + m.W;  // chunk bits below h0, at 12-W ..
+  m.base = ((chunk & ((1u << A) - 1)) << (12 - m.W)) | ((chunk >> A) << (h0 + m.W));
+  m.t2 = (uint32_t)(t & 15);
+#pragma unroll
+  for (int k = 4; k < 8; ++k) m.t2 |= (uint32_t)((t >> k) & 1) << qmid_pos(k, m.W, h0, kQC);
+  return m;
+}
+
+// A copy of x that the compiler cannot identify with x: store addresses
+// formed from it are recomputed (one VALU op each) instead of being kept live
+// in registers since the loads.
+__device__ __forceinline__ uint32_t qfresh(uint32_t x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+// register part of the L2 offset: local bits 8-11 in r
+__device__ __forceinline__ uint32_t qmid_r(int r, int W, int h0) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) o |= (uint32_t)((r >> k) & 1) << qmid_pos(8 + k, W, h0, kQC);
+  return o;
+}
+
+// register-bit masks of the window's local bits kQC..kQC+W-1: L2 phase (local
+// 8-11), L1 phase (local 4-7)
+__device__ __forceinline__ int qmid_mask2(int W) {
+  return ((((1 << W) - 1) << kQC) >> 8) & 15;
+}
+__device__ __forceinline__ int qmid_mask1(int W) { return ((((1 << W) - 1) << kQC) >> 4) & 15; }
+
+// Window g of an instance: its first bit h0 and width W, and the cut terms of
+// its workgroups (see qcut_win): m[k], the neighbours of local bit k's global
+// bit outside the 12 local bits; nh[b], for a bit b outside them, its
+// neighbours above b that are outside too (0 for local bits).
+struct QWin {
+  uint32_t m[kQL];
+  uint32_t nh[32];
+  int h0, W, pad0, pad1;
+};
+
+// Cut counts of a window workgroup in layout L1.  With i = base | (local bits
+// placed by qmid_pos), split the edges by where their bits lie: both local
+// ("lo", a function of the local index j), both outside ("hi", uniform), or
+// one of each (local bit a, outside bit b: j_a xor base_b):
+//   cut(i) = lo(j) + C + sum_a j_a w_a,
+//   C   = hi(base) + sum_a popc(base & m_a),   w_a = popc(m_a) - 2 popc(base & m_a).
+// lo(j) is a 4096-byte table per window stored [t][r] in layout L1, so a
+// thread's 16 values are one 16-byte load; C and w_a depend on the workgroup
+// only and are computed once per wave in scalar registers.
+struct QCut1 {
+  int K;                // C + sum over the thread's local bits (0-3, 8-11) of j_a w_a
+  int w4, w5, w6, w7;   // w_a of the register bits (uniform)
+  uint4 lo;             // lo(j(t, r)) in byte r & 3 of word r >> 2
+};
+
+// `c` and `base` must be wave-uniform.
+__device__ __forceinline__ QCut1 qcut_win(const QWin &c, const uint4 *__restrict__ lo, int n,
+                                          uint32_t base, int t) {
+  QCut1 f;
+  f.lo = lo[t];
+  int C = 0, w[kQL];
+#pragma unroll
+  for (int a = 0; a < kQL; ++a) {
+    const int pm = __popc(base & c.m[a]);
+    C += pm;
+    w[a] = __popc(c.m[a]) - 2 * pm;
+  }
+  for (int b = 0; b < n; ++b) C += __popc(c.nh[b] & (((base >> b) & 1) ? ~base : base));
+  int K = C;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) K += ((t >> a) & 1) * w[a] + ((t >> (4 + a)) & 1) * w[8 + a];
+  f.K = K;
+  f.w4 = w[4];
+  f.w5 = w[5];
+  f.w6 = w[6];
+  f.w7 = w[7];
+  return f;
+}
+
+__device__ __forceinline__ int qcut_r1(const QCut1 &f, int r) {
+  const uint32_t word = r < 4 ? f.lo.x : r < 8 ? f.lo.y : r < 12 ? f.lo.z : f.lo.w;
+  return f.K + (int)((word >> (8 * (r & 3))) & 0xffu) + (r & 1) * f.w4 + ((r >> 1) & 1) * f.w5 +
+         ((r >> 2) & 1) * f.w6 + ((r >> 3) & 1) * f.w7;
+}
+
+// Fields of a window-pass descriptor (qpass).
+struct QPassF {
+  int g, fl, lp, lc;
+};
+__device__ __forceinline__ QPassF qpass_fields(uint32_t v) {
+  return {(int)(v & 15), (int)((v >> 4) & 15), (int)((v >> 8) & 255), (int)((v >> 16) & 255)};
+}
+
+// Forward window pass: [RX(2 b_lp) on the window] [cost(lc), RX(2 b_lc) on the
+// window and bits 0-4], on psi.
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
-    k_qaoa_low_fwd_full(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                        const QCutTab *__restrict__ cuts, const uint4 *__restrict__ lotab,
-                        const typename CV<R>::T *__restrict__ ph, int p,
-                        const QEntry *__restrict__ tab, int count, int layer) {
+    k_qaoa_win_fwd(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
+                   const QWin *__restrict__ wins, const uint4 *__restrict__ wlo,
+                   const typename CV<R>::T *__restrict__ ph, int p,
+                   const QEntry *__restrict__ tab, int count, int scatter) {
   using T = typename CV<R>::T;
   __shared__ T X[kQSlots];
   __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
   const QFound fe = find_entry_u(tab, count, blockIdx.x);
   const int ii = fe.inst;
   const QInstDev q = inst[ii];
-  const uint32_t chunk = (uint32_t)(blockIdx.x - fe.first);
-  const uint64_t base = (uint64_t)chunk << kQL;
-  const T *phl = ph + (size_t)(ii * p + layer) * kQPh;
+  const QPassF P = qpass_fields(fe.pass);
+  const int wi = q.win + P.g;
+  const int h0 = wins[wi].h0;
   const int t = threadIdx.x;
+  const uint32_t chunk = qmid_chunk(blockIdx.x - fe.first, q.n, scatter && P.g > 0);
+  const QMid m = qmid_setup(chunk, q.n, h0, t);
+  const int m2 = qmid_mask2(m.W), m1 = qmid_mask1(m.W);
+  T *s = st + q.psi + m.base;
   // the chunk's loads go out first; the phase table and cut terms overlap them
   T v[16];
-  const R amp0 = (R)(1.0 / sqrt((double)(1ull << q.n)));
+  if (P.fl & kQInit) {
+    const R amp0 = (R)(1.0 / sqrt((double)(1ull << q.n)));
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    if (layer == 0) {
+    for (int r = 0; r < 16; ++r) {
       v[r].x = amp0;
       v[r].y = 0;
-    } else {
-      v[r] = ld_nt(st + q.psi + base + t + 256 * r);
     }
+  } else {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = ld_nt(s + (m.t2 | qmid_r(r, m.W, h0)));
   }
-  if (t <= q.nedges) {
-    const T z = phl[t];
-    ph_r[t] = z.x;
-    ph_i[t] = z.y;
+  QCut1 cc{};
+  if (P.fl & kQBnd) {
+    const T *phl = ph + (size_t)(ii * p + P.lc) * kQPh;
+    if (t <= q.nedges) {
+      const T z = phl[t];
+      ph_r[t] = z.x;
+      ph_i[t] = z.y;
+    }
+    cc = qcut_win(wins[wi], wlo + (size_t)wi * kBlock, q.n, m.base, t);
   }
-  const QCutF cc = qcut_full(cuts, lotab, ii, q.n, (uint32_t)base, t);
-  __syncthreads();
+  if (P.fl & kQPre) {
+    const T cb = ph[(size_t)(ii * p + P.lp) * kQPh + kQPh - 1];
+    qrx16m(v, cb.x, cb.y, m2);  // RX(2b) = [[c, -i s], [-i s, c]]
+    qtranspose(v, X, t, 2, 1);
+    qrx16m(v, cb.x, cb.y, m1);
+  } else if (!(P.fl & kQInit)) {
+    qtranspose(v, X, t, 2, 1);
+  }  // (|+> is the same in every layout)
+  if (P.fl & kQBnd) {
+    if (P.fl & kQInit) __syncthreads();  // the phase table (no transpose since)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      R x = v[r].x, y = v[r].y;
+      const int c = qcut_r1(cc, r);
+      cmul(x, y, ph_r[c], ph_i[c]);
+      v[r].x = x;
+      v[r].y = y;
+    }
+    const T cb = ph[(size_t)(ii * p + P.lc) * kQPh + kQPh - 1];
+    qrx16m(v, cb.x, cb.y, m1 | 1);  // local bit 4 and the window's 5-7
+    qtranspose(v, X, t, 1, 0);      // writes the L1 slots this thread just read
+    qrx16(v, cb.x, cb.y);           // bits 0-3
+    qtranspose(v, X, t, 0, 2);
+    qrx16m(v, cb.x, cb.y, m2);
+  } else {
+    qtranspose(v, X, t, 1, 2);  // writes the L1 slots this thread just read
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) st_nt(s + (m.t2 | qmid_r(r, m.W, h0)), v[r]);
+}
+
+// The turn: the forward sweep's last pass [RX(2 b_lp) on the window], then the
+// energy partial and lambda = C psi, then the window undone on lambda with the
+// beta partial of layer lp; writes lambda only (psi in memory is its value
+// before the pass, which the reverse sweep continues from).  The window's bits
+// 8-11 take their partial against psi with only those bits rotated (kept in
+// registers): there psi and the half-undone lambda stand at the same point.
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_qaoa_win_turn(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
+                    const QWin *__restrict__ wins, const uint4 *__restrict__ wlo,
+                    const typename CV<R>::T *__restrict__ ph, int p,
+                    const QEntry *__restrict__ tab, int count, double *__restrict__ partial,
+                    int scatter) {
+  using T = typename CV<R>::T;
+  __shared__ T X[kQSlots];
+  const QFound fe = find_entry_u(tab, count, blockIdx.x);
+  const int ii = fe.inst;
+  const QInstDev q = inst[ii];
+  const QPassF P = qpass_fields(fe.pass);
+  const int wi = q.win + P.g;
+  const int h0 = wins[wi].h0;
+  const int t = threadIdx.x;
+  const uint32_t chunk = qmid_chunk(blockIdx.x - fe.first, q.n, scatter && P.g > 0);
+  const QMid m = qmid_setup(chunk, q.n, h0, t);
+  const int m2 = qmid_mask2(m.W), m1 = qmid_mask1(m.W);
+  const bool pre = (P.fl & kQPre) != 0;
+  T v[16], b[16], l[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = ld_nt(st + q.psi + m.base + (m.t2 | qmid_r(r, m.W, h0)));
+  const QCut1 cc = qcut_win(wins[wi], wlo + (size_t)wi * kBlock, q.n, m.base, t);
+  R c0 = 1, s0 = 0;
+  if (pre) {
+    const T cb = ph[(size_t)(ii * p + P.lp) * kQPh + kQPh - 1];
+    c0 = cb.x;
+    s0 = cb.y;
+    qrx16m(v, c0, s0, m2);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) b[r] = v[r];
+  qtranspose(v, X, t, 2, 1);
+  if (pre) qrx16m(v, c0, s0, m1);
+  double acce = 0.0, accb = 0.0;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    R x = v[r].x, y = v[r].y;
-    const int c = qcut_r(cc, r);
-    cmul(x, y, ph_r[c], ph_i[c]);
-    v[r].x = x;
-    v[r].y = y;
+    const int c = qcut_r1(cc, r);
+    acce += (double)c * ((double)v[r].x * v[r].x + (double)v[r].y * v[r].y);
+    l[r].x = (R)c * v[r].x;
+    l[r].y = (R)c * v[r].y;
   }
-  const T cb = phl[kQPh - 1];
-  const R c0 = cb.x, s0 = cb.y;    // RX(2b) = [[c, -i s], [-i s, c]]
-  qrx16(v, c0, s0);                // bits 8-11
-  qtranspose(v, X, t, 2, 1);
-  qrx16(v, c0, s0);                // bits 4-7
-  qtranspose(v, X, t, 1, 0);       // writes the L1 slots this thread just read
-  qrx16(v, c0, s0);                // bits 0-3
-  qtranspose(v, X, t, 0, 2);
+  if (pre) {
+    accb += qxpart16m(v, l, m1);
+    qrx16m(l, c0, -s0, m1);
+  }
+  qtranspose(l, X, t, 1, 2);  // writes the L1 slots this thread read for psi
+  if (pre) {
+    accb += qxpart16m(b, l, m2);
+    qrx16m(l, c0, -s0, m2);
+  }
 #pragma unroll
-  for (int r = 0; r < 16; ++r) st_nt(st + q.psi + base + t + 256 * r, v[r]);
+  for (int r = 0; r < 16; ++r) st_nt(st + q.lam + m.base + (m.t2 | qmid_r(r, m.W, h0)), l[r]);
+  acce = qblock_sum(acce);
+  if (t == 0) partial[slot_energy(q) + chunk] = acce;
+  if (pre) {
+    accb = qblock_sum(accb);
+    if (t == 0) partial[slot_beta(q, P.lp, P.g) + chunk] = accb;
+  }
+}
+
+// Reverse window pass on psi and lambda: the forward pass undone in reverse
+// order, each part's partial taken before it is undone (beta of lc over the
+// window and bits 0-4, gamma of lc at the cost, beta of lp over the window).
+// kQNoStore: the sweep's last pass; it stops at the gamma partial.  BND
+// selects the passes with kQBnd (their own instantiation: the cost and the
+// twelve-bit undo need far more registers than a window-only pass).
+template <typename R, bool BND>
+__global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
+    k_qaoa_win_rev(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
+                   const QWin *__restrict__ wins, const uint4 *__restrict__ wlo,
+                   const typename CV<R>::T *__restrict__ ph, int p,
+                   const QEntry *__restrict__ tab, int count, double *__restrict__ partial,
+                   int scatter) {
+  using T = typename CV<R>::T;
+  __shared__ T X[kQSlots];
+  __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
+  const QFound fe = find_entry_u(tab, count, blockIdx.x);
+  const int ii = fe.inst;
+  const QInstDev q = inst[ii];
+  const QPassF P = qpass_fields(fe.pass);
+  const int wi = q.win + P.g;
+  const int h0 = wins[wi].h0;
+  const int t = threadIdx.x;
+  const uint32_t chunk = qmid_chunk(blockIdx.x - fe.first, q.n, scatter && P.g > 0);
+  const QMid m = qmid_setup(chunk, q.n, h0, t);
+  const int m2 = qmid_mask2(m.W), m1 = qmid_mask1(m.W);
+  constexpr bool bnd = BND;
+  const bool pre = (P.fl & kQPre) != 0;
+  const bool store = !(P.fl & kQNoStore);
+  T *sa = st + q.psi + m.base, *sl = st + q.lam + m.base;
+  T a[16], l[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const uint32_t o = m.t2 | qmid_r(r, m.W, h0);
+    a[r] = ld_nt(sa + o);
+    l[r] = ld_nt(sl + o);
+  }
+  R pc = 1, ps = 0;  // RX(-2 b_lp)
+  if (pre) {
+    const T cb = ph[(size_t)(ii * p + P.lp) * kQPh + kQPh - 1];
+    pc = cb.x;
+    ps = -cb.y;
+  }
+  double accp = 0.0, accq = 0.0, accg = 0.0;  // beta of lc, beta of lp, gamma of lc
+  if (bnd) {
+    const T *phl = ph + (size_t)(ii * p + P.lc) * kQPh;
+    if (t <= q.nedges) {  // undo: exp(+i g s), the conjugate phases
+      const T z = phl[t];
+      ph_r[t] = z.x;
+      ph_i[t] = -z.y;
+    }
+    const T cb = phl[kQPh - 1];
+    const R c0 = cb.x, s0 = -cb.y;  // RX(-2 b_lc)
+    accp += qxpart16m(a, l, m2);
+    qrx16m(a, c0, s0, m2);
+    qrx16m(l, c0, s0, m2);
+    qtranspose(a, X, t, 2, 0);
+    __syncthreads();
+    qtranspose(l, X, t, 2, 0);
+    accp += qxpart16m(a, l, 15);  // bits 0-3
+    qrx16(a, c0, s0);
+    qrx16(l, c0, s0);
+    qtranspose(a, X, t, 0, 1);  // writes the L0 slots this thread just read
+    __syncthreads();
+    qtranspose(l, X, t, 0, 1);
+    accp += qxpart16m(a, l, m1 | 1);
+    qrx16m(a, c0, s0, m1 | 1);
+    qrx16m(l, c0, s0, m1 | 1);
+    // gamma partial Im(conj(lambda) s psi) and undo the cost layer
+    const QCut1 cc = qcut_win(wins[wi], wlo + (size_t)wi * kBlock, q.n, m.base, t);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int c = qcut_r1(cc, r);
+      const double sv = (double)(q.nedges - 2 * c);
+      R x = a[r].x, y = a[r].y, u = l[r].x, w = l[r].y;
+      accg += sv * ((double)u * y - (double)w * x);
+      if (store) {
+        cmul(x, y, ph_r[c], ph_i[c]);
+        cmul(u, w, ph_r[c], ph_i[c]);
+        a[r].x = x;
+        a[r].y = y;
+        l[r].x = u;
+        l[r].y = w;
+      }
+    }
+  } else {  // window only: its L2 part first
+    accq += qxpart16m(a, l, m2);
+    qrx16m(a, pc, ps, m2);
+    qrx16m(l, pc, ps, m2);
+    qtranspose(a, X, t, 2, 1);
+    __syncthreads();
+    qtranspose(l, X, t, 2, 1);
+  }
+  if (pre) {
+    accq += qxpart16m(a, l, m1);
+    qrx16m(a, pc, ps, m1);
+    qrx16m(l, pc, ps, m1);
+  }
+  if (store) {
+    qtranspose(a, X, t, 1, 2);  // writes the L1 slots this thread just read
+    __syncthreads();
+    qtranspose(l, X, t, 1, 2);
+    if (bnd && pre) {
+      accq += qxpart16m(a, l, m2);
+      qrx16m(a, pc, ps, m2);
+      qrx16m(l, pc, ps, m2);
+    }
+    const uint32_t t2 = qfresh(m.t2);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const uint32_t o = t2 | qmid_r(r, m.W, h0);
+      st_nt(sa + o, a[r]);
+      st_nt(sl + o, l[r]);
+    }
+  }
+  if (bnd) {
+    accp = qblock_sum(accp);
+    if (t == 0) partial[slot_beta(q, P.lc, P.g) + chunk] = accp;
+    accg = qblock_sum(accg);
+    if (t == 0) partial[slot_gamma(q, P.lc) + chunk] = accg;
+  }
+  if (pre) {
+    accq = qblock_sum(accq);
+    if (t == 0) partial[slot_beta(q, P.lp, P.g) + chunk] = accq;
+  }
 }
 
 // ---- low pass, forward, chunks below 2^12 (n < 12): (init |+>) cost(l), RX on
@@ -378,12 +704,12 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// ---- energy: lambda = C psi, partial sum of cut(i) |psi_i|^2 ----------------
+// ---- energy, instances below 12 qubits: lambda = C psi, partial sum of
+// cut(i) |psi_i|^2 (larger instances form both in the turn) -------------------
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
     k_qaoa_energy(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                  const uint8_t *__restrict__ edges, const QCutTab *__restrict__ cuts,
-                  const uint4 *__restrict__ lotab, const QEntry *__restrict__ tab, int count,
+                  const uint8_t *__restrict__ edges, const QEntry *__restrict__ tab, int count,
                   double *__restrict__ partial) {
   using T = typename CV<R>::T;
   const QFound fe = find_entry_u(tab, count, blockIdx.x);
@@ -394,269 +720,17 @@ __global__ void __launch_bounds__(kBlock)
   const uint64_t base = chunk << q.lc;
   const uint8_t *e = edges + 2 * q.edge_off;
   double acc = 0.0;
-  if (q.lc == kQL) {  // uniform per workgroup
-    // all 16 loads of the thread in flight at once (m = 16 x kBlock)
-    const int t = threadIdx.x;
-    T v[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = ld_nt(st + q.psi + base + t + kBlock * r);
-    const QCutF cc = qcut_full(cuts, lotab, ii, q.n, (uint32_t)base, t);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int c = qcut_r(cc, r);
-      acc += (double)c * ((double)v[r].x * v[r].x + (double)v[r].y * v[r].y);
-      T o;
-      o.x = (R)c * v[r].x;
-      o.y = (R)c * v[r].y;
-      st_nt(st + q.lam + base + t + kBlock * r, o);
-    }
-  } else {
-    for (int j = threadIdx.x; j < m; j += kBlock) {
-      const T v = ld_nt(st + q.psi + base + j);
-      const int c = cut_of(base + j, e, q.nedges);
-      acc += (double)c * ((double)v.x * v.x + (double)v.y * v.y);
-      T o;
-      o.x = (R)c * v.x;
-      o.y = (R)c * v.y;
-      st_nt(st + q.lam + base + j, o);
-    }
+  for (int j = threadIdx.x; j < m; j += kBlock) {
+    const T v = ld_nt(st + q.psi + base + j);
+    const int c = cut_of(base + j, e, q.nedges);
+    acc += (double)c * ((double)v.x * v.x + (double)v.y * v.y);
+    T o;
+    o.x = (R)c * v.x;
+    o.y = (R)c * v.y;
+    st_nt(st + q.lam + base + j, o);
   }
   acc = qblock_sum(acc);
   if (threadIdx.x == 0) partial[slot_energy(q) + chunk] = acc;
-}
-
-template <typename R>
-__global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
-    k_qaoa_low_rev_full(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                        const QCutTab *__restrict__ cuts, const uint4 *__restrict__ lotab,
-                        const typename CV<R>::T *__restrict__ ph, int p,
-                        const QEntry *__restrict__ tab, int count, int layer,
-                        double *__restrict__ partial) {
-  using T = typename CV<R>::T;
-  __shared__ T X[kQSlots];
-  __shared__ R ph_r[kQMaxEdges + 1], ph_i[kQMaxEdges + 1];
-  const QFound fe = find_entry_u(tab, count, blockIdx.x);
-  const int ii = fe.inst;
-  const QInstDev q = inst[ii];
-  const uint32_t chunk = (uint32_t)(blockIdx.x - fe.first);
-  const uint64_t base = (uint64_t)chunk << kQL;
-  const T *phl = ph + (size_t)(ii * p + layer) * kQPh;
-  const int t = threadIdx.x;
-  // the chunk's loads go out first; the phase table and cut terms overlap them
-  T a[16], l[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    a[r] = ld_nt(st + q.psi + base + t + 256 * r);
-    l[r] = ld_nt(st + q.lam + base + t + 256 * r);
-  }
-  if (t <= q.nedges) {  // undo: exp(+i g s), the conjugate phases
-    const T z = phl[t];
-    ph_r[t] = z.x;
-    ph_i[t] = -z.y;
-  }
-  const QCutF cc = qcut_full(cuts, lotab, ii, q.n, (uint32_t)base, t);
-  const T cb = phl[kQPh - 1];
-  const R c0 = cb.x, s0 = -cb.y;  // RX(-2b)
-  // beta partial (any point inside the mixer layer) and undo, 4 bits at a time
-  double accb = qxpart16(a, l);  // bits 8-11
-  qrx16(a, c0, s0);
-  qrx16(l, c0, s0);
-  qtranspose(a, X, t, 2, 1);
-  __syncthreads();
-  qtranspose(l, X, t, 2, 1);
-  accb += qxpart16(a, l);  // bits 4-7
-  qrx16(a, c0, s0);
-  qrx16(l, c0, s0);
-  __syncthreads();
-  qtranspose(a, X, t, 1, 0);
-  __syncthreads();
-  qtranspose(l, X, t, 1, 0);
-  accb += qxpart16(a, l);  // bits 0-3
-  qrx16(a, c0, s0);
-  qrx16(l, c0, s0);
-  __syncthreads();
-  qtranspose(a, X, t, 0, 2);
-  __syncthreads();
-  qtranspose(l, X, t, 0, 2);
-  // gamma partial Im(conj(lambda) s psi) and undo the cost layer
-  double accg = 0.0;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int c = qcut_r(cc, r);
-    const double sv = (double)(q.nedges - 2 * c);
-    R x = a[r].x, y = a[r].y, u = l[r].x, w = l[r].y;
-    accg += sv * ((double)u * y - (double)w * x);
-    cmul(x, y, ph_r[c], ph_i[c]);
-    cmul(u, w, ph_r[c], ph_i[c]);
-    T o, m;
-    o.x = x;
-    o.y = y;
-    m.x = u;
-    m.y = w;
-    st_nt(st + q.psi + base + t + 256 * r, o);
-    st_nt(st + q.lam + base + t + 256 * r, m);
-  }
-  accb = qblock_sum(accb);
-  if (t == 0) partial[slot_beta(q, layer, 0) + chunk] = accb;
-  accg = qblock_sum(accg);
-  if (t == 0) partial[slot_gamma(q, layer) + chunk] = accg;
-}
-
-// ---- mid pass: RX(2b) (forward) or RX(-2b) and the beta partial (reverse) on
-// the rotated bits h0 .. h0+W-1 ----------------------------------------------
-// Geometry (qmid_pos, chunk_layout.hpp) with C = kQC = 5 contiguous
-// low bits and W = min(7, n - h0) rotated bits: loads in layout L2 (local
-// bits 8-11 in the register index: rotated bits h0+3..), one LDS transpose to
-// L1 (local bits 4-7: bit 4 carried, 5-7 rotated), and back to L2 for the
-// stores, so every global access is 256 B runs (32 consecutive complex64
-// amplitudes); 7 rotated bits still take the 14 high bits of a 26-qubit
-// instance in two passes.
-struct QMid {
-  uint32_t base;     // global offset of the chunk (uniform)
-  uint32_t t2;       // thread part of the L2 offset
-  int W;
-};
-
-// Chunk of a mid-pass workgroup: its index within the instance's launch, or
-// (scatter) that index bit-reversed over the instance's n - 12 chunk bits, so
-// concurrent workgroups take chunks far apart instead of neighbouring ones.
-__device__ __forceinline__ uint32_t qmid_chunk(uint64_t rel, int n, int scatter) {
-  const uint32_t c = (uint32_t)rel;
-  return scatter && n > kQL ? __builtin_bitreverse32(c) >> (32 - (n - kQL)) : c;
-}
-__device__ __forceinline__ QMid qmid_setup(uint32_t chunk, int n, int h0, int t) {
-  QMid m;
-  m.W = min(kQGM, n - h0);
-  const int A = h0 - 12 + m.W;  // chunk bits below h0, at 12-W ..
-  m.base = ((chunk & ((1u << A) - 1)) << (12 - m.W)) | ((chunk >> A) << (h0 + m.W));
-  m.t2 = (uint32_t)(t & 15);
-#pragma unroll
-  for (int k = 4; k < 8; ++k) m.t2 |= (uint32_t)((t >> k) & 1) << qmid_pos(k, m.W, h0, kQC);
-  return m;
-}
-
-// register part of the L2 offset: local bits 8-11 in r
-__device__ __forceinline__ uint32_t qmid_r(int r, int W, int h0) {
-  uint32_t o = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) o |= (uint32_t)((r >> k) & 1) << qmid_pos(8 + k, W, h0, kQC);
-  return o;
-}
-
-// RX on the register-index bits set in `mask` (uniform), as qrx16
-template <typename T, typename R>
-__device__ __forceinline__ void qrx16m(T (&v)[16], R c0, R s0, int mask) {
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    if (!((mask >> b) & 1)) continue;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      if (r & (1 << b)) continue;
-      const T x = v[r], y = v[r | (1 << b)];
-      v[r].x = c0 * x.x + s0 * y.y;
-      v[r].y = c0 * x.y - s0 * y.x;
-      v[r | (1 << b)].x = c0 * y.x + s0 * x.y;
-      v[r | (1 << b)].y = c0 * y.y - s0 * x.x;
-    }
-  }
-}
-
-// qxpart16 over the register-index bits set in `mask` (uniform)
-template <typename T>
-__device__ __forceinline__ double qxpart16m(const T (&a)[16], const T (&l)[16], int mask) {
-  double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    if (!((mask >> b) & 1)) continue;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      if (r & (1 << b)) continue;
-      const int u = r | (1 << b);
-      s0 += (double)qimcl(l[r], a[u]);
-      s1 += (double)qimcl(l[u], a[r]);
-    }
-  }
-  return s0 + s1;
-}
-
-// register-bit masks of the rotated local bits kQC..kQC+W-1: L2 phase (local
-// 8-11), L1 phase (local 4-7)
-__device__ __forceinline__ int qmid_mask2(int W) {
-  return ((((1 << W) - 1) << kQC) >> 8) & 15;
-}
-__device__ __forceinline__ int qmid_mask1(int W) { return ((((1 << W) - 1) << kQC) >> 4) & 15; }
-
-template <typename R>
-__global__ void __launch_bounds__(kBlock)
-    k_qaoa_mid_fwd(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                   const typename CV<R>::T *__restrict__ ph, int p,
-                   const QEntry *__restrict__ tab, int count, int layer, int h0, int scatter) {
-  using T = typename CV<R>::T;
-  __shared__ T X[kQSlots];
-  const QFound fe = find_entry_u(tab, count, blockIdx.x);
-  const int ii = fe.inst;
-  const QInstDev q = inst[ii];
-  const int t = threadIdx.x;
-  const QMid m = qmid_setup(qmid_chunk(blockIdx.x - fe.first, q.n, scatter), q.n, h0, t);
-  T *s = st + q.psi + m.base;
-  T v[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) v[r] = ld_nt(s + (m.t2 | qmid_r(r, m.W, h0)));
-  const T cb = ph[(size_t)(ii * p + layer) * kQPh + kQPh - 1];
-  const R c0 = cb.x, s0 = cb.y;  // RX(2b) = [[c, -i s], [-i s, c]]
-  qrx16m(v, c0, s0, qmid_mask2(m.W));
-  qtranspose(v, X, t, 2, 1);
-  qrx16m(v, c0, s0, qmid_mask1(m.W));
-  qtranspose(v, X, t, 1, 2);  // writes the L1 slots this thread just read
-#pragma unroll
-  for (int r = 0; r < 16; ++r) st_nt(s + (m.t2 | qmid_r(r, m.W, h0)), v[r]);
-}
-
-template <typename R>
-__global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
-    k_qaoa_mid_rev(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
-                   const typename CV<R>::T *__restrict__ ph, int p,
-                   const QEntry *__restrict__ tab, int count, int layer, int h0,
-                   double *__restrict__ partial, int scatter) {
-  using T = typename CV<R>::T;
-  __shared__ T X[kQSlots];
-  const QFound fe = find_entry_u(tab, count, blockIdx.x);
-  const int ii = fe.inst;
-  const QInstDev q = inst[ii];
-  const int t = threadIdx.x;
-  const uint32_t chunk = qmid_chunk(blockIdx.x - fe.first, q.n, scatter);
-  const QMid m = qmid_setup(chunk, q.n, h0, t);
-  T *sa = st + q.psi + m.base, *sl = st + q.lam + m.base;
-  T a[16], l[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const uint32_t o = m.t2 | qmid_r(r, m.W, h0);
-    a[r] = ld_nt(sa + o);
-    l[r] = ld_nt(sl + o);
-  }
-  const T cb = ph[(size_t)(ii * p + layer) * kQPh + kQPh - 1];
-  const R c0 = cb.x, s0 = -cb.y;  // RX(-2b)
-  const int m2 = qmid_mask2(m.W), m1 = qmid_mask1(m.W);
-  double acc = qxpart16m(a, l, m2);
-  qrx16m(a, c0, s0, m2);
-  qrx16m(l, c0, s0, m2);
-  qtranspose(a, X, t, 2, 1);
-  __syncthreads();
-  qtranspose(l, X, t, 2, 1);
-  acc += qxpart16m(a, l, m1);
-  qrx16m(a, c0, s0, m1);
-  qrx16m(l, c0, s0, m1);
-  qtranspose(a, X, t, 1, 2);  // writes the L1 slots this thread just read
-  __syncthreads();
-  qtranspose(l, X, t, 1, 2);
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const uint32_t o = m.t2 | qmid_r(r, m.W, h0);
-    st_nt(sa + o, a[r]);
-    st_nt(sl + o, l[r]);
-  }
-  acc = qblock_sum(acc);
-  if (t == 0) partial[slot_beta(q, layer, 1 + (h0 - kQL) / kQGM) + chunk] = acc;
 }
 
 // ---- low pass, reverse, chunks below 2^12 (n < 12): beta partials and undo RX
@@ -757,7 +831,7 @@ __global__ void __launch_bounds__(kBlock)
       scale = 2.0;
     } else {
       first = slot_beta(q, v - 1 - p, 0);
-      len = nch * (uint64_t)(1 + q.ngroups);
+      len = nch * (uint64_t)q.nwin;
       scale = 2.0;
     }
     double acc = 0.0;
@@ -777,16 +851,17 @@ struct QaoaBatch {
   void *states = nullptr;
   QInstDev *inst = nullptr;
   uint8_t *edges = nullptr;
-  QCutTab *cuts = nullptr;   // per instance
-  uint8_t *lotab = nullptr;  // per instance, 4096 bytes
+  QWin *wins = nullptr;      // windows of the instances of n >= 12
+  uint8_t *wlo = nullptr;    // per window, 4096 bytes (qcut_win)
   void *phase = nullptr;     // per (instance, layer), kQPh amplitudes
   double *params = nullptr;  // gam [ninst*p], bet [ninst*p]
   double *partial = nullptr;
   double *out = nullptr;
   QEntry *tables = nullptr;
   struct Launch {
-    int kind, layer, bit;  // kind: 0 low_fwd, 2 energy, 4 low_rev, 5 mid_fwd, 6 mid_rev
-    int nb;                // low passes: 1 for the full-chunk kernel
+    // kind: n < 12, per layer: 0 low_fwd, 2 energy, 4 low_rev;
+    //       n >= 12: 1 win_fwd, 3 win_turn, 6 win_rev of kQBnd passes, 7 of the others
+    int kind, layer;
     size_t tab_off;
     int count;
     uint64_t blocks;
@@ -800,15 +875,36 @@ struct QaoaBatch {
 static void free_batch(QaoaBatch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  for (void *ptr : {(void *)b->states, (void *)b->inst, (void *)b->edges, (void *)b->cuts,
-                    (void *)b->lotab, (void *)b->phase, (void *)b->params, (void *)b->partial,
+  for (void *ptr : {(void *)b->states, (void *)b->inst, (void *)b->edges, (void *)b->wins,
+                    (void *)b->wlo, (void *)b->phase, (void *)b->params, (void *)b->partial,
                     (void *)b->out, (void *)b->tables})
     if (ptr) (void)hipFree(ptr);
+}
+
+// Window-pass schedule of an instance of n >= 12 with k windows: forward
+// passes s = 0 .. F-1 (F = p (k - 1), or p for k = 1), then the turn; the
+// reverse sweep undoes s = F-1 .. 0.  Pass s takes window s mod k, so the k
+// passes of a layer (from the one applying its cost to the one applying the
+// next cost) see every window once; for k = 1 every pass is a whole layer.
+static int qsched_nfwd(int k, int p) { return k == 1 ? p : p * (k - 1); }
+static uint32_t qsched_fwd(int k, int s) {
+  if (k == 1) return qpass(0, (s == 0 ? kQInit : 0) | kQBnd, 0, s);
+  const int g = s % k, per = k - 1;
+  if (s == 0) return qpass(g, kQInit | kQBnd, 0, 0);
+  if (s % per == 0) return qpass(g, kQPre | kQBnd, s / per - 1, s / per);
+  return qpass(g, kQPre, s / per, 0);
+}
+static uint32_t qsched_turn(int k, int p) {
+  return k == 1 ? qpass(0, 0, 0, 0) : qpass((p * (k - 1)) % k, kQPre, p - 1, 0);
+}
+static uint32_t qsched_rev(int k, int s) {
+  return qsched_fwd(k, s) | (s == 0 ? kQNoStore << 4 : 0u);
 }
 
 int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq,
                 const uint32_t *nedges, const uint32_t *edges, uint32_t p) {
   if (ninst == 0 || p == 0) return fail(SQ_ERR_INVALID, "qaoa: empty batch");
+  if (p > (uint32_t)kQMaxLayers) return fail(SQ_ERR_INVALID, "qaoa: at most 256 layers");
   auto B = std::make_unique<QaoaBatch>();
   B->dt = dt;
   B->ninst = (int)ninst;
@@ -818,8 +914,8 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
   uint64_t amp = 0, slot = 0;
   size_t eoff = 0;
   std::vector<uint8_t> hedges;
-  std::vector<QCutTab> hcuts(ninst);
-  std::vector<uint8_t> hlo((size_t)ninst * kQChunk, 0);
+  std::vector<QWin> hwins;
+  std::vector<uint8_t> hlo;
   for (uint32_t j = 0; j < ninst; ++j) {
     const int n = (int)nq[j];
     if (n < 2 || n > 30) return fail(SQ_ERR_INVALID, "qaoa: qubit counts must be in [2, 30]");
@@ -828,10 +924,12 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
     q.n = n;
     q.lc = std::min(n, kQL);
     q.nchunks = 1 << (n - q.lc);
-    q.ngroups = (n - q.lc + kQGM - 1) / kQGM;
+    q.nwin = n >= kQL ? 1 + (n - kQL + kQGM - 1) / kQGM : 1;
+    q.win = (int)hwins.size();
     q.nedges = (int)nedges[j];
     q.edge_off = (int)(hedges.size() / 2);
     std::vector<uint32_t> seen(n, 0);  // seen[a]: bit b set if edge {a, b} was given
+    std::vector<std::pair<int, int>> eb;  // edges as bit positions, lower first
     for (uint32_t k = 0; k < nedges[j]; ++k) {
       const uint32_t a = edges[2 * (eoff + k)], b = edges[2 * (eoff + k) + 1];
       if (a >= (uint32_t)n || b >= (uint32_t)n || a == b)
@@ -840,26 +938,35 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
       if ((seen[a] >> b) & 1) return fail(SQ_ERR_INVALID, "qaoa: repeated edge");
       seen[a] |= 1u << b;
       seen[b] |= 1u << a;
-      hedges.push_back((uint8_t)(n - 1 - a));  // qubit -> bit position
-      hedges.push_back((uint8_t)(n - 1 - b));
+      const int u = n - 1 - (int)a, v = n - 1 - (int)b;  // qubit -> bit position
+      hedges.push_back((uint8_t)u);
+      hedges.push_back((uint8_t)v);
+      eb.emplace_back(std::min(u, v), std::max(u, v));
     }
-    if (q.lc == kQL) {  // cut tables of full chunks (see qcut_full)
-      QCutTab &ct = hcuts[j];
+    for (int g = 0; n >= kQL && g < q.nwin; ++g) {  // window tables (see QWin, qcut_win)
+      QWin w{};
+      w.h0 = g == 0 ? kQC : kQL + kQGM * (g - 1);
+      w.W = std::min(kQGM, n - w.h0);
+      int loc[32];
+      std::fill(loc, loc + 32, -1);
+      for (int k = 0; k < kQL; ++k) loc[qmid_pos(k, w.W, w.h0, kQC)] = k;
       std::vector<std::pair<int, int>> lo_edges;
-      for (uint32_t k = 0; k < nedges[j]; ++k) {
-        const int u = n - 1 - (int)edges[2 * (eoff + k)], v = n - 1 - (int)edges[2 * (eoff + k) + 1];
-        const int x = std::min(u, v), y = std::max(u, v);
-        if (y < kQL) lo_edges.emplace_back(x, y);
-        else if (x < kQL) ct.m[x] |= 1u << y;
-        else ct.nh[x] |= 1u << y;
+      for (auto &e : eb) {
+        const int x = e.first, y = e.second;
+        if (loc[x] >= 0 && loc[y] >= 0) lo_edges.emplace_back(loc[x], loc[y]);
+        else if (loc[x] >= 0) w.m[loc[x]] |= 1u << y;
+        else if (loc[y] >= 0) w.m[loc[y]] |= 1u << x;
+        else w.nh[x] |= 1u << y;
       }
-      uint8_t *lo = hlo.data() + (size_t)j * kQChunk;
+      hwins.push_back(w);
+      const size_t off = hlo.size();
+      hlo.resize(off + kQChunk);
       for (int t = 0; t < 256; ++t)
         for (int r = 0; r < 16; ++r) {
-          const int jj = t + 256 * r;
+          const int jj = (t & 15) | (r << 4) | ((t >> 4) << 8);  // layout L1
           int c = 0;
           for (auto &e : lo_edges) c += ((jj >> e.first) ^ (jj >> e.second)) & 1;
-          lo[t * 16 + r] = (uint8_t)c;
+          hlo[off + (size_t)t * 16 + r] = (uint8_t)c;
         }
     }
     eoff += nedges[j];
@@ -867,12 +974,16 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
     q.lam = amp + (1ull << n);
     amp += 2ull << n;
     q.slot = slot;
-    slot += (uint64_t)q.nchunks * (1 + (uint64_t)p * (2 + q.ngroups));
+    slot += (uint64_t)q.nchunks * (1 + (uint64_t)p * (1 + q.nwin));
     B->host_inst.push_back(q);
     B->maxn = std::max(B->maxn, n);
     B->state_bytes += 2.0 * std::ldexp(1.0, n) * item;
   }
   if (hedges.empty()) hedges.push_back(0);
+  if (hwins.empty()) {
+    hwins.emplace_back();
+    hlo.resize(kQChunk);
+  }
   auto dmalloc = [](void **ptr, size_t bytes) -> int {
     if (hipMalloc(ptr, bytes) != hipSuccess) {
       (void)hipGetLastError();
@@ -885,42 +996,53 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
   if (!rc) rc = dmalloc(&B->states, amp * item);
   if (!rc) rc = dmalloc((void **)&B->inst, ninst * sizeof(QInstDev));
   if (!rc) rc = dmalloc((void **)&B->edges, hedges.size());
-  if (!rc) rc = dmalloc((void **)&B->cuts, ninst * sizeof(QCutTab));
-  if (!rc) rc = dmalloc((void **)&B->lotab, hlo.size());
+  if (!rc) rc = dmalloc((void **)&B->wins, hwins.size() * sizeof(QWin));
+  if (!rc) rc = dmalloc((void **)&B->wlo, hlo.size());
   if (!rc) rc = dmalloc(&B->phase, (size_t)ninst * p * kQPh * item);
   if (!rc) rc = dmalloc((void **)&B->params, 2 * ninst * p * sizeof(double));
   if (!rc) rc = dmalloc((void **)&B->partial, std::max<uint64_t>(slot, 1) * sizeof(double));
   if (!rc) rc = dmalloc((void **)&B->out, ninst * (1 + 2 * p) * sizeof(double));
-  // launch sequence and tables
+  // launch sequence and tables: one launch per pass index, each instance's
+  // entry carrying its own pass
   std::vector<QEntry> tabs;
-  auto add = [&](int kind, int layer, int bit, int nb, auto pred) {
-    QaoaBatch::Launch L{kind, layer, bit, nb, tabs.size(), 0, 0};
+  auto add = [&](int kind, int layer, auto pass_of) {  // pass_of(j) < 0: not in this launch
+    QaoaBatch::Launch L{kind, layer, tabs.size(), 0, 0};
     for (int j = 0; j < (int)ninst; ++j) {
-      const QInstDev &q = B->host_inst[j];
-      if (!pred(q)) continue;
-      tabs.push_back(QEntry{j, 0, L.blocks});
-      L.blocks += (uint64_t)q.nchunks;
+      const int64_t ps = pass_of(j);
+      if (ps < 0) continue;
+      tabs.push_back(QEntry{j, (uint32_t)ps, L.blocks});
+      L.blocks += (uint64_t)B->host_inst[j].nchunks;
       L.count++;
     }
     if (L.count) B->launches.push_back(L);
   };
-  auto all = [](const QInstDev &) { return true; };
-  auto full = [](const QInstDev &q) { return q.lc == kQL; };
-  auto small = [](const QInstDev &q) { return q.lc < kQL; };
-  auto above = [](int h0) { return [h0](const QInstDev &q) { return q.n > h0; }; };
-  for (int l = 0; l < (int)p; ++l) {
-    add(0, l, 0, 1, full);  // nb = 1: full-chunk kernel
-    add(0, l, 0, 0, small);
-    for (int h0 = kQL; h0 < B->maxn; h0 += kQGM) add(5, l, h0, 0, above(h0));
-  }
-  add(2, 0, 0, 0, all);
-  for (int l = (int)p - 1; l >= 0; --l) {
-    for (int h0 = kQL + ((B->maxn - kQL - 1) / kQGM) * kQGM; h0 >= kQL; h0 -= kQGM)
-      add(6, l, h0, 0, above(h0));
-    add(4, l, 0, 1, full);
-    add(4, l, 0, 0, small);
-  }
-  if (!rc) rc = dmalloc((void **)&B->tables, tabs.size() * sizeof(QEntry));
+  const auto &hi = B->host_inst;
+  const int P = (int)p;
+  auto small = [&](int j) -> int64_t { return hi[j].n < kQL ? 0 : -1; };
+  for (int l = 0; l < P; ++l) add(0, l, small);
+  add(2, 0, small);
+  for (int l = P - 1; l >= 0; --l) add(4, l, small);
+  int maxf = 0;
+  for (auto &q : hi)
+    if (q.n >= kQL) maxf = std::max(maxf, qsched_nfwd(q.nwin, P));
+  for (int i = 0; i < maxf; ++i)
+    add(1, 0, [&](int j) -> int64_t {
+      const QInstDev &q = hi[j];
+      return q.n < kQL || i >= qsched_nfwd(q.nwin, P) ? -1 : (int64_t)qsched_fwd(q.nwin, i);
+    });
+  add(3, 0, [&](int j) -> int64_t {
+    return hi[j].n < kQL ? -1 : (int64_t)qsched_turn(hi[j].nwin, P);
+  });
+  for (int i = 0; i < maxf; ++i)
+    for (int bnd = 1; bnd >= 0; --bnd)  // the two reverse kernels (k_qaoa_win_rev<R, BND>)
+      add(bnd ? 6 : 7, 0, [&](int j) -> int64_t {
+        const QInstDev &q = hi[j];
+        const int F = qsched_nfwd(q.nwin, P);
+        if (q.n < kQL || i >= F) return -1;
+        const uint32_t ps = qsched_rev(q.nwin, F - 1 - i);
+        return (((ps >> 4) & kQBnd) != 0) == (bnd != 0) ? (int64_t)ps : -1;
+      });
+  if (!rc) rc = dmalloc((void **)&B->tables, std::max<size_t>(tabs.size(), 1) * sizeof(QEntry));
   if (rc) {
     free_batch(B.get());
     return rc;
@@ -928,21 +1050,30 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
   if (hipMemcpy(B->inst, B->host_inst.data(), ninst * sizeof(QInstDev), hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemcpy(B->edges, hedges.data(), hedges.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(B->cuts, hcuts.data(), ninst * sizeof(QCutTab), hipMemcpyHostToDevice) !=
+      hipMemcpy(B->wins, hwins.data(), hwins.size() * sizeof(QWin), hipMemcpyHostToDevice) !=
           hipSuccess ||
-      hipMemcpy(B->lotab, hlo.data(), hlo.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(B->wlo, hlo.data(), hlo.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(B->tables, tabs.data(), tabs.size() * sizeof(QEntry), hipMemcpyHostToDevice) !=
           hipSuccess) {
     free_batch(B.get());
     return fail(SQ_ERR_HIP, "qaoa: upload failed");
   }
-  // HBM traffic of one run (reads + writes of the states)
+  // HBM traffic of one run (reads + writes of the states, in state-sized units)
   for (auto &L : B->launches) {
-    const double per = (L.kind == 0 || L.kind == 5) ? 2.0 : (L.kind == 2 ? 3.0 : 4.0);
     for (int k = 0; k < L.count; ++k) {
-      const QInstDev &q = B->host_inst[tabs[L.tab_off + k].inst];
-      const double amps = std::ldexp(1.0, q.n);
-      B->traffic_bytes += per * amps * item * ((L.kind == 0 && L.layer == 0) ? 0.5 : 1.0);
+      const QEntry &e = tabs[L.tab_off + k];
+      const uint32_t fl = (e.pass >> 4) & 15;
+      double per = 0;
+      switch (L.kind) {
+        case 0: per = L.layer == 0 ? 1.0 : 2.0; break;
+        case 2: per = 3.0; break;
+        case 4: per = 4.0; break;
+        case 1: per = (fl & kQInit) ? 1.0 : 2.0; break;
+        case 3: per = 2.0; break;
+        case 6:
+        case 7: per = (fl & kQNoStore) ? 2.0 : 4.0; break;
+      }
+      B->traffic_bytes += per * std::ldexp(1.0, hi[e.inst].n) * item;
     }
   }
   *handle = B.release();
@@ -954,8 +1085,9 @@ static int enqueue_t(QaoaBatch *B, hipStream_t st) {
   using T = typename CV<R>::T;
   T *s = (T *)B->states;
   const T *ph = (const T *)B->phase;
-  const uint4 *lo = (const uint4 *)B->lotab;
+  const uint4 *wlo = (const uint4 *)B->wlo;
   const int p = B->p;
+  const int sc = tuning().qaoa_mid_scatter;
   const double *gam = B->params, *bet = B->params + (size_t)B->ninst * p;
   hipLaunchKernelGGL(k_qaoa_phases<R>, dim3(B->ninst * p), dim3(kBlock), 0, st, (T *)B->phase,
                      B->inst, gam, bet, p);
@@ -964,32 +1096,32 @@ static int enqueue_t(QaoaBatch *B, hipStream_t st) {
     const dim3 g((unsigned)L.blocks), bl(kBlock);
     switch (L.kind) {
       case 0:
-        if (L.nb)
-          hipLaunchKernelGGL(k_qaoa_low_fwd_full<R>, g, bl, 0, st, s, B->inst, B->cuts, lo, ph, p,
-                             tab, L.count, L.layer);
-        else
-          hipLaunchKernelGGL(k_qaoa_low_fwd<R>, g, bl, 0, st, s, B->inst, B->edges, ph, p, tab,
-                             L.count, L.layer);
+        hipLaunchKernelGGL(k_qaoa_low_fwd<R>, g, bl, 0, st, s, B->inst, B->edges, ph, p, tab,
+                           L.count, L.layer);
         break;
       case 2:
-        hipLaunchKernelGGL(k_qaoa_energy<R>, g, bl, 0, st, s, B->inst, B->edges, B->cuts, lo, tab,
-                           L.count, B->partial);
-        break;
-      case 5:
-        hipLaunchKernelGGL(k_qaoa_mid_fwd<R>, g, bl, 0, st, s, B->inst, ph, p, tab, L.count,
-                           L.layer, L.bit, tuning().qaoa_mid_scatter);
-        break;
-      case 6:
-        hipLaunchKernelGGL(k_qaoa_mid_rev<R>, g, bl, 0, st, s, B->inst, ph, p, tab, L.count,
-                           L.layer, L.bit, B->partial, tuning().qaoa_mid_scatter);
+        hipLaunchKernelGGL(k_qaoa_energy<R>, g, bl, 0, st, s, B->inst, B->edges, tab, L.count,
+                           B->partial);
         break;
       case 4:
-        if (L.nb)
-          hipLaunchKernelGGL(k_qaoa_low_rev_full<R>, g, bl, 0, st, s, B->inst, B->cuts, lo, ph, p,
-                             tab, L.count, L.layer, B->partial);
-        else
-          hipLaunchKernelGGL(k_qaoa_low_rev<R>, g, bl, 0, st, s, B->inst, B->edges, ph, p, tab,
-                             L.count, L.layer, B->partial);
+        hipLaunchKernelGGL(k_qaoa_low_rev<R>, g, bl, 0, st, s, B->inst, B->edges, ph, p, tab,
+                           L.count, L.layer, B->partial);
+        break;
+      case 1:
+        hipLaunchKernelGGL(k_qaoa_win_fwd<R>, g, bl, 0, st, s, B->inst, B->wins, wlo, ph, p, tab,
+                           L.count, sc);
+        break;
+      case 3:
+        hipLaunchKernelGGL(k_qaoa_win_turn<R>, g, bl, 0, st, s, B->inst, B->wins, wlo, ph, p,
+                           tab, L.count, B->partial, sc);
+        break;
+      case 6:
+        hipLaunchKernelGGL((k_qaoa_win_rev<R, true>), g, bl, 0, st, s, B->inst, B->wins, wlo, ph,
+                           p, tab, L.count, B->partial, sc);
+        break;
+      case 7:
+        hipLaunchKernelGGL((k_qaoa_win_rev<R, false>), g, bl, 0, st, s, B->inst, B->wins, wlo, ph,
+                           p, tab, L.count, B->partial, sc);
         break;
     }
   }

@@ -79,14 +79,17 @@ def test_qaoa_batch_energies_and_gradients(C, dtype, etol, gtol):
 def test_qaoa_gradients_match_general_adjoint(C):
     """The batch kernels against the general engine's adjoint on the same
     circuit (ZZ(2 gamma): dE/dgamma = 2 * sum of the ZZ-angle derivatives).
-    Sizes 13-20 give every width (1-8 bits) of the first pass over the bits
-    above 12, 24 and 26 a second pass (bits 20-23, 20-25) and cut-table
-    neighbours up to bit 25 (26 is the largest size of the benchmark batch)."""
+    Instances of n >= 12 qubits are swept in 7-bit windows above bits 0-4
+    (bits 5-11, 12-18, 19-25, 26-29): 12 has one window, 13-19 two, 20-26
+    three and 27 four, and the sizes give the last window every width 1-7;
+    with p = 3 every kind of pass (first, window-only, cost boundary, turn)
+    runs at every window count; 26 is the largest size of the benchmark
+    batch."""
     from snuqs_amd.circuits import qaoa_maxcut
     from snuqs_amd.qaoa import qaoa_batch
 
-    sizes = [14, 10, 17, 19, 13, 15, 20, 16, 24, 26]
-    p = 2
+    sizes = [14, 10, 17, 19, 13, 15, 20, 16, 24, 26, 12, 18, 27]
+    p = 3
     graphs, gam, bet = _batch(sizes, p, 5)
     out = qaoa_batch(sizes, graphs, p, dtype="complex128").run(gam, bet, use_graph=False)
     for j, n in enumerate(sizes):
