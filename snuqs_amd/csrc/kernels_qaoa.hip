@@ -165,39 +165,32 @@ __device__ __forceinline__ uint64_t slot_beta(const QInstDev &q, int l, int part
   return slot_gamma(q, l) + (uint64_t)q.nchunks * (1 + part);
 }
 
-// RX(2b) on the 4 register-index bits, with (c0, s0) = (cos b, sin b); the
-// inverse takes -s0.
+// RX(2b) / cos b on the register-index bits set in `mask` (uniform), with
+// t = tan b: per pair (x, y), x' = x - i t y and y' = y - i t x, one FMA per
+// component; the inverse takes -t.  Bits outside the mask take t = 0, an
+// exact identity: no branches, so the registers keep one assignment.
 template <typename T, typename R>
-__device__ __forceinline__ void qrx16(T (&v)[16], R c0, R s0) {
-#pragma unroll
-  for (int b = 0; b < 4; ++b)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      if (r & (1 << b)) continue;
-      const T x = v[r], y = v[r | (1 << b)];
-      v[r].x = c0 * x.x + s0 * y.y;
-      v[r].y = c0 * x.y - s0 * y.x;
-      v[r | (1 << b)].x = c0 * y.x + s0 * x.y;
-      v[r | (1 << b)].y = c0 * y.y - s0 * x.x;
-    }
-}
-
-// RX on the register-index bits set in `mask` (uniform), as qrx16
-template <typename T, typename R>
-__device__ __forceinline__ void qrx16m(T (&v)[16], R c0, R s0, int mask) {
+__device__ __forceinline__ void qrx16t(T (&v)[16], R t0, int mask) {
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    if (!((mask >> b) & 1)) continue;
+    const R t = ((mask >> b) & 1) ? t0 : R(0), nt = -t;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       if (r & (1 << b)) continue;
       const T x = v[r], y = v[r | (1 << b)];
-      v[r].x = c0 * x.x + s0 * y.y;
-      v[r].y = c0 * x.y - s0 * y.x;
-      v[r | (1 << b)].x = c0 * y.x + s0 * x.y;
-      v[r | (1 << b)].y = c0 * y.y - s0 * x.x;
+      v[r].x = fma(t, y.y, x.x);
+      v[r].y = fma(nt, y.x, x.y);
+      v[r | (1 << b)].x = fma(t, x.y, y.x);
+      v[r | (1 << b)].y = fma(nt, x.x, y.y);
     }
   }
+}
+
+// c^k for a uniform k (the scale factors of qrx16t)
+__device__ __forceinline__ double qpowc(double c, int k) {
+  double r = 1.0;
+  for (int i = 0; i < k; ++i) r *= c;
+  return r;
 }
 
 // Im(conj(l) a) of one amplitude pair in the state's precision: for complex64
@@ -230,9 +223,18 @@ __device__ __forceinline__ double qxpart16m(const T (&a)[16], const T (&l)[16], 
 }
 
 // Per (instance, layer): the cost-layer phases exp(-i g s_c), s_c = |E| - 2c for
-// c = 0..|E|, then (cos b, sin b) of the mixer, evaluated in FP64 and rounded to
-// the state's precision once per run by k_qaoa_phases.
-constexpr int kQPh = kQMaxEdges + 2;
+// c = 0..|E|, then the mixer as (cos b', sin b') and (tan b', cos b'),
+// evaluated in FP64 and rounded to the state's precision once per run by
+// k_qaoa_phases.  b' is b reduced to [-pi/4, pi/4] by a multiple of pi/2: the
+// energy is periodic in every b with period pi/2, since RX(pi) on every qubit
+// is -i X on every qubit, which flips every bit, leaves every cut, the cost
+// layers, the mixers and |+>^n unchanged, and only multiplies the state by
+// a global phase.  With |tan b'| <= 1 the window passes apply RX(2b') / cos b'
+// = [[1, -i t], [-i t, 1]] (one FMA per output component, growth at most
+// sqrt 2 per bit) and fold the cos b' factors into their other products.
+constexpr int kQPh = kQMaxEdges + 3;
+constexpr int kQMixCS = kQPh - 2;  // (cos b', sin b')
+constexpr int kQMixTC = kQPh - 1;  // (tan b', cos b')
 
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
@@ -249,9 +251,12 @@ __global__ void __launch_bounds__(kBlock)
     o[c] = T{(R)cs, (R)sn};
   }
   if (threadIdx.x == 0) {
+    const double b = bet[ii * p + layer];
+    const double br = b - M_PI_2 * nearbyint(b / M_PI_2);
     double sn, cs;
-    sincos(bet[ii * p + layer], &sn, &cs);
-    o[kQPh - 1] = T{(R)cs, (R)sn};
+    sincos(br, &sn, &cs);
+    o[kQMixCS] = T{(R)cs, (R)sn};
+    o[kQMixTC] = T{(R)(sn / cs), (R)cs};
   }
 }
 
@@ -405,21 +410,27 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = ld_nt(s + (m.t2 | qmid_r(r, m.W, h0)));
   }
+  // mixers as RX / cos (qrx16t): the cos factors of the pass, c_lp^W (pre) and
+  // c_lc^(W+5) (bnd), go into the cost phases, or into one product at the end
+  T mp{0, 1}, mc{0, 1};  // (tan, cos) of layers lp and lc
+  if (P.fl & kQPre) mp = ph[(size_t)(ii * p + P.lp) * kQPh + kQMixTC];
   QCut1 cc{};
   if (P.fl & kQBnd) {
     const T *phl = ph + (size_t)(ii * p + P.lc) * kQPh;
+    mc = phl[kQMixTC];
     if (t <= q.nedges) {
+      const double sg = qpowc((double)mp.y, m.W * ((P.fl & kQPre) != 0)) *
+                        qpowc((double)mc.y, m.W + kQC);
       const T z = phl[t];
-      ph_r[t] = z.x;
-      ph_i[t] = z.y;
+      ph_r[t] = (R)(sg * (double)z.x);
+      ph_i[t] = (R)(sg * (double)z.y);
     }
     cc = qcut_win(wins[wi], wlo + (size_t)wi * kBlock, q.n, m.base, t);
   }
   if (P.fl & kQPre) {
-    const T cb = ph[(size_t)(ii * p + P.lp) * kQPh + kQPh - 1];
-    qrx16m(v, cb.x, cb.y, m2);  // RX(2b) = [[c, -i s], [-i s, c]]
+    qrx16t(v, mp.x, m2);
     qtranspose(v, X, t, 2, 1);
-    qrx16m(v, cb.x, cb.y, m1);
+    qrx16t(v, mp.x, m1);
   } else if (!(P.fl & kQInit)) {
     qtranspose(v, X, t, 2, 1);
   }  // (|+> is the same in every layout)
@@ -433,14 +444,19 @@ __global__ void __launch_bounds__(kBlock)
       v[r].x = x;
       v[r].y = y;
     }
-    const T cb = ph[(size_t)(ii * p + P.lc) * kQPh + kQPh - 1];
-    qrx16m(v, cb.x, cb.y, m1 | 1);  // local bit 4 and the window's 5-7
-    qtranspose(v, X, t, 1, 0);      // writes the L1 slots this thread just read
-    qrx16(v, cb.x, cb.y);           // bits 0-3
+    qrx16t(v, mc.x, m1 | 1);    // local bit 4 and the window's 5-7
+    qtranspose(v, X, t, 1, 0);  // writes the L1 slots this thread just read
+    qrx16t(v, mc.x, 15);        // bits 0-3
     qtranspose(v, X, t, 0, 2);
-    qrx16m(v, cb.x, cb.y, m2);
+    qrx16t(v, mc.x, m2);
   } else {
     qtranspose(v, X, t, 1, 2);  // writes the L1 slots this thread just read
+    const R f = (R)qpowc((double)mp.y, m.W);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      v[r].x *= f;
+      v[r].y *= f;
+    }
   }
 #pragma unroll
   for (int r = 0; r < 16; ++r) st_nt(s + (m.t2 | qmid_r(r, m.W, h0)), v[r]);
@@ -476,17 +492,19 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r] = ld_nt(st + q.psi + m.base + (m.t2 | qmid_r(r, m.W, h0)));
   const QCut1 cc = qcut_win(wins[wi], wlo + (size_t)wi * kBlock, q.n, m.base, t);
-  R c0 = 1, s0 = 0;
+  // mixer as RX / cos (qrx16t): psi and lambda carry 1 / c^W from here on, and
+  // lambda's undo adds another 1 / c^W, so the energy, the partials and the
+  // stored lambda take c^(2W)
+  T mp{0, 1};
   if (pre) {
-    const T cb = ph[(size_t)(ii * p + P.lp) * kQPh + kQPh - 1];
-    c0 = cb.x;
-    s0 = cb.y;
-    qrx16m(v, c0, s0, m2);
+    mp = ph[(size_t)(ii * p + P.lp) * kQPh + kQMixTC];
+    qrx16t(v, mp.x, m2);
   }
+  const double f = qpowc((double)mp.y, 2 * m.W * pre);
 #pragma unroll
   for (int r = 0; r < 16; ++r) b[r] = v[r];
   qtranspose(v, X, t, 2, 1);
-  if (pre) qrx16m(v, c0, s0, m1);
+  if (pre) qrx16t(v, mp.x, m1);
   double acce = 0.0, accb = 0.0;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
@@ -497,19 +515,25 @@ __global__ void __launch_bounds__(kBlock)
   }
   if (pre) {
     accb += qxpart16m(v, l, m1);
-    qrx16m(l, c0, -s0, m1);
+    qrx16t(l, -mp.x, m1);
   }
   qtranspose(l, X, t, 1, 2);  // writes the L1 slots this thread read for psi
   if (pre) {
     accb += qxpart16m(b, l, m2);
-    qrx16m(l, c0, -s0, m2);
+    qrx16t(l, -mp.x, m2);
+    const R fr = (R)f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      l[r].x *= fr;
+      l[r].y *= fr;
+    }
   }
 #pragma unroll
   for (int r = 0; r < 16; ++r) st_nt(st + q.lam + m.base + (m.t2 | qmid_r(r, m.W, h0)), l[r]);
-  acce = qblock_sum(acce);
+  acce = qblock_sum(acce * f);
   if (t == 0) partial[slot_energy(q) + chunk] = acce;
   if (pre) {
-    accb = qblock_sum(accb);
+    accb = qblock_sum(accb * f);
     if (t == 0) partial[slot_beta(q, P.lp, P.g) + chunk] = accb;
   }
 }
@@ -551,37 +575,44 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
     a[r] = ld_nt(sa + o);
     l[r] = ld_nt(sl + o);
   }
-  R pc = 1, ps = 0;  // RX(-2 b_lp)
-  if (pre) {
-    const T cb = ph[(size_t)(ii * p + P.lp) * kQPh + kQPh - 1];
-    pc = cb.x;
-    ps = -cb.y;
-  }
+  // Mixers undone as RX / cos (qrx16t): psi and lambda carry the same factor
+  // u (uniform) at every point, and a partial taken there is divided by u^2.
+  // bnd: u = 1 / c_lc^k after k bits; the cost undo multiplies by
+  // c_lc^(W+5) c_lp^W (folded into its phases), so the pre part starts at
+  // u = c_lp^W and ends at 1.  Window only: u = 1 / c_lp^k, and one product
+  // by c_lp^W before the stores.
+  const int w2 = __popc(m2), w1 = __popc(m1);
+  T mp{0, 1};  // (tan, cos) of layer lp
+  if (pre) mp = ph[(size_t)(ii * p + P.lp) * kQPh + kQMixTC];
+  const double cp = mp.y;
   double accp = 0.0, accq = 0.0, accg = 0.0;  // beta of lc, beta of lp, gamma of lc
+  double cq = 1.0;
   if (bnd) {
     const T *phl = ph + (size_t)(ii * p + P.lc) * kQPh;
-    if (t <= q.nedges) {  // undo: exp(+i g s), the conjugate phases
+    const T mc = phl[kQMixTC];
+    cq = mc.y;
+    if (t <= q.nedges) {  // undo: exp(+i g s), the conjugate phases, times the factor
+      const double sg = qpowc(cq, m.W + kQC) * qpowc(cp, m.W * pre);
       const T z = phl[t];
-      ph_r[t] = z.x;
-      ph_i[t] = -z.y;
+      ph_r[t] = (R)(sg * (double)z.x);
+      ph_i[t] = (R)(-sg * (double)z.y);
     }
-    const T cb = phl[kQPh - 1];
-    const R c0 = cb.x, s0 = -cb.y;  // RX(-2 b_lc)
+    const R nt = -mc.x;  // RX(-2 b_lc) / cos
     accp += qxpart16m(a, l, m2);
-    qrx16m(a, c0, s0, m2);
-    qrx16m(l, c0, s0, m2);
+    qrx16t(a, nt, m2);
+    qrx16t(l, nt, m2);
     qtranspose(a, X, t, 2, 0);
     __syncthreads();
     qtranspose(l, X, t, 2, 0);
-    accp += qxpart16m(a, l, 15);  // bits 0-3
-    qrx16(a, c0, s0);
-    qrx16(l, c0, s0);
+    accp += qxpart16m(a, l, 15) * qpowc(cq, 2 * w2);  // bits 0-3
+    qrx16t(a, nt, 15);
+    qrx16t(l, nt, 15);
     qtranspose(a, X, t, 0, 1);  // writes the L0 slots this thread just read
     __syncthreads();
     qtranspose(l, X, t, 0, 1);
-    accp += qxpart16m(a, l, m1 | 1);
-    qrx16m(a, c0, s0, m1 | 1);
-    qrx16m(l, c0, s0, m1 | 1);
+    accp += qxpart16m(a, l, m1 | 1) * qpowc(cq, 2 * (w2 + 4));
+    qrx16t(a, nt, m1 | 1);
+    qrx16t(l, nt, m1 | 1);
     // gamma partial Im(conj(lambda) s psi) and undo the cost layer
     const QCut1 cc = qcut_win(wins[wi], wlo + (size_t)wi * kBlock, q.n, m.base, t);
 #pragma unroll
@@ -601,25 +632,35 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
     }
   } else {  // window only: its L2 part first
     accq += qxpart16m(a, l, m2);
-    qrx16m(a, pc, ps, m2);
-    qrx16m(l, pc, ps, m2);
+    qrx16t(a, -mp.x, m2);
+    qrx16t(l, -mp.x, m2);
     qtranspose(a, X, t, 2, 1);
     __syncthreads();
     qtranspose(l, X, t, 2, 1);
   }
   if (pre) {
-    accq += qxpart16m(a, l, m1);
-    qrx16m(a, pc, ps, m1);
-    qrx16m(l, pc, ps, m1);
+    accq += qxpart16m(a, l, m1) * (bnd ? 1.0 / qpowc(cp, 2 * m.W) : qpowc(cp, 2 * w2));
+    qrx16t(a, -mp.x, m1);
+    qrx16t(l, -mp.x, m1);
   }
   if (store) {
     qtranspose(a, X, t, 1, 2);  // writes the L1 slots this thread just read
     __syncthreads();
     qtranspose(l, X, t, 1, 2);
     if (bnd && pre) {
-      accq += qxpart16m(a, l, m2);
-      qrx16m(a, pc, ps, m2);
-      qrx16m(l, pc, ps, m2);
+      accq += qxpart16m(a, l, m2) / qpowc(cp, 2 * w2);
+      qrx16t(a, -mp.x, m2);
+      qrx16t(l, -mp.x, m2);
+    }
+    if (!bnd) {
+      const R f = (R)qpowc(cp, m.W);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        a[r].x *= f;
+        a[r].y *= f;
+        l[r].x *= f;
+        l[r].y *= f;
+      }
     }
     const uint32_t t2 = qfresh(m.t2);
 #pragma unroll
@@ -632,7 +673,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
   if (bnd) {
     accp = qblock_sum(accp);
     if (t == 0) partial[slot_beta(q, P.lc, P.g) + chunk] = accp;
-    accg = qblock_sum(accg);
+    accg = qblock_sum(accg * qpowc(cq, 2 * (m.W + kQC)));
     if (t == 0) partial[slot_gamma(q, P.lc) + chunk] = accg;
   }
   if (pre) {
@@ -682,7 +723,7 @@ __global__ void __launch_bounds__(kBlock)
     xi[j] = y;
   }
   __syncthreads();
-  const T cb = phl[kQPh - 1];
+  const T cb = phl[kQMixCS];
   const R c0 = cb.x, s0 = cb.y;  // RX(2b) = [[c, -i s], [-i s, c]]
   for (int bit = 0; bit < q.lc; ++bit) {
     for (int f = threadIdx.x; f < m / 2; f += kBlock) {
@@ -766,7 +807,7 @@ __global__ void __launch_bounds__(kBlock)
     yi[j] = w.y;
   }
   __syncthreads();
-  const T cb = phl[kQPh - 1];
+  const T cb = phl[kQMixCS];
   const R c0 = cb.x, s0 = cb.y;
   double accb = 0.0;
   for (int bit = 0; bit < q.lc; ++bit) {
