@@ -59,13 +59,17 @@ struct QInstDev {
 // Window-pass descriptor (launch-table entry): window g, flags, the layer lp
 // whose mixer the pass applies on the window bits first ("pre"), and the layer
 // lc whose cost it applies next, followed by lc's mixer on the window bits
-// and bits 0-4 ("bnd").  The reverse pass undoes them in the opposite order.
+// ("bnd").  Bits 0-4, held by every pass, take each layer's mixer in one pass
+// of that layer (kQLow): with its cost when the instance has at most two
+// windows, else in its first window-only pass, which has the registers and
+// time to spare.  The reverse pass undoes them in the opposite order.
 constexpr uint32_t kQInit = 1;     // forward: start from |+> (nothing read)
 constexpr uint32_t kQPre = 2;      // RX(2 b_lp) on the window bits
-constexpr uint32_t kQBnd = 4;      // cost(lc), then RX(2 b_lc) on window and bits 0-4
+constexpr uint32_t kQBnd = 4;      // cost(lc), then RX(2 b_lc) on the window bits
 constexpr uint32_t kQNoStore = 8;  // reverse: the sweep's last pass, partials only
+constexpr uint32_t kQLow = 16;     // bits 0-4 too (with lc if kQBnd, else with lp)
 __host__ __device__ constexpr uint32_t qpass(int g, uint32_t flags, int lp, int lc) {
-  return (uint32_t)g | flags << 4 | (uint32_t)lp << 8 | (uint32_t)lc << 16;
+  return (uint32_t)g | flags << 4 | (uint32_t)lp << 12 | (uint32_t)lc << 20;
 }
 
 // Launch table entry: instance, its pass, and its first block index in this launch.
@@ -203,9 +207,10 @@ __device__ __forceinline__ auto qimcl(const T &l, const T &a) {
 }
 
 // sum over the register bits b set in `mask` (uniform) and pairs (r, r^b) of
-// Im(conj(l_r) a_{r^b} + conj(l_{r^b}) a_r)  (= Im <l| X_b |a> over the pairs),
-// the two terms of a pair added in the state's precision (one more rounding
-// at the same scale), the pairs accumulated in FP64
+// Im(conj(l_r) a_{r^b} + conj(l_{r^b}) a_r)  (= Im <l| X_b |a> over the pairs):
+// the four products of a pair in one chain of FMAs in the state's precision
+// (one more rounding than qimcl, at the same scale), the pairs accumulated in
+// FP64
 template <typename T>
 __device__ __forceinline__ double qxpart16m(const T (&a)[16], const T (&l)[16], int mask) {
   double s = 0.0;
@@ -216,7 +221,8 @@ __device__ __forceinline__ double qxpart16m(const T (&a)[16], const T (&l)[16], 
     for (int r = 0; r < 16; ++r) {
       if (r & (1 << b)) continue;
       const int u = r | (1 << b);
-      s += (double)(qimcl(l[r], a[u]) + qimcl(l[u], a[r]));
+      const auto d = fma(l[u].x, a[r].y, fma(-l[u].y, a[r].x, qimcl(l[r], a[u])));
+      s += (double)d;
     }
   }
   return s;
@@ -372,7 +378,7 @@ struct QPassF {
   int g, fl, lp, lc;
 };
 __device__ __forceinline__ QPassF qpass_fields(uint32_t v) {
-  return {(int)(v & 15), (int)((v >> 4) & 15), (int)((v >> 8) & 255), (int)((v >> 16) & 255)};
+  return {(int)(v & 15), (int)((v >> 4) & 255), (int)((v >> 12) & 255), (int)((v >> 20) & 255)};
 }
 
 // Forward window pass: [RX(2 b_lp) on the window] [cost(lc), RX(2 b_lc) on the
@@ -410,31 +416,35 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = ld_nt(s + (m.t2 | qmid_r(r, m.W, h0)));
   }
-  // mixers as RX / cos (qrx16t): the cos factors of the pass, c_lp^W (pre) and
-  // c_lc^(W+5) (bnd), go into the cost phases, or into one product at the end
+  // mixers as RX / cos (qrx16t): the cos factors of the pass, c_lp^kp (pre)
+  // and c_lc^kc (bnd) for kp, kc rotated bits, go into the cost phases, or
+  // into one product at the end
+  const bool pre = (P.fl & kQPre) != 0, bnd = (P.fl & kQBnd) != 0;
+  const int low = (P.fl & kQLow) != 0;  // bits 0-4: L1 register bit 0, then L0
+  const int lowp = low && !bnd, lowc = low && bnd;
+  const int kp = pre ? m.W + kQC * lowp : 0, kc = m.W + kQC * lowc;
   T mp{0, 1}, mc{0, 1};  // (tan, cos) of layers lp and lc
-  if (P.fl & kQPre) mp = ph[(size_t)(ii * p + P.lp) * kQPh + kQMixTC];
+  if (pre) mp = ph[(size_t)(ii * p + P.lp) * kQPh + kQMixTC];
   QCut1 cc{};
-  if (P.fl & kQBnd) {
+  if (bnd) {
     const T *phl = ph + (size_t)(ii * p + P.lc) * kQPh;
     mc = phl[kQMixTC];
     if (t <= q.nedges) {
-      const double sg = qpowc((double)mp.y, m.W * ((P.fl & kQPre) != 0)) *
-                        qpowc((double)mc.y, m.W + kQC);
+      const double sg = qpowc((double)mp.y, kp) * qpowc((double)mc.y, kc);
       const T z = phl[t];
       ph_r[t] = (R)(sg * (double)z.x);
       ph_i[t] = (R)(sg * (double)z.y);
     }
     cc = qcut_win(wins[wi], wlo + (size_t)wi * kBlock, q.n, m.base, t);
   }
-  if (P.fl & kQPre) {
+  if (pre) {
     qrx16t(v, mp.x, m2);
     qtranspose(v, X, t, 2, 1);
-    qrx16t(v, mp.x, m1);
+    qrx16t(v, mp.x, m1 | lowp);
   } else if (!(P.fl & kQInit)) {
     qtranspose(v, X, t, 2, 1);
   }  // (|+> is the same in every layout)
-  if (P.fl & kQBnd) {
+  if (bnd) {
     if (P.fl & kQInit) __syncthreads();  // the phase table (no transpose since)
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -444,14 +454,21 @@ __global__ void __launch_bounds__(kBlock)
       v[r].x = x;
       v[r].y = y;
     }
-    qrx16t(v, mc.x, m1 | 1);    // local bit 4 and the window's 5-7
-    qtranspose(v, X, t, 1, 0);  // writes the L1 slots this thread just read
-    qrx16t(v, mc.x, 15);        // bits 0-3
+    qrx16t(v, mc.x, m1 | lowc);
+  }
+  // (bits 0-3 if kQLow) and back to the store layout; each transpose writes
+  // the slots this thread just read
+  if (low) {
+    qtranspose(v, X, t, 1, 0);
+    qrx16t(v, bnd ? mc.x : mp.x, 15);
     qtranspose(v, X, t, 0, 2);
+  } else {
+    qtranspose(v, X, t, 1, 2);
+  }
+  if (bnd) {
     qrx16t(v, mc.x, m2);
   } else {
-    qtranspose(v, X, t, 1, 2);  // writes the L1 slots this thread just read
-    const R f = (R)qpowc((double)mp.y, m.W);
+    const R f = (R)qpowc((double)mp.y, kp);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       v[r].x *= f;
@@ -487,7 +504,7 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t chunk = qmid_chunk(blockIdx.x - fe.first, q.n, scatter && P.g > 0);
   const QMid m = qmid_setup(chunk, q.n, h0, t);
   const int m2 = qmid_mask2(m.W), m1 = qmid_mask1(m.W);
-  const bool pre = (P.fl & kQPre) != 0;
+  const bool pre = (P.fl & kQPre) != 0;  // (never kQLow: see qsched_fwd)
   T v[16], b[16], l[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r] = ld_nt(st + q.psi + m.base + (m.t2 | qmid_r(r, m.W, h0)));
@@ -539,13 +556,16 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Reverse window pass on psi and lambda: the forward pass undone in reverse
-// order, each part's partial taken before it is undone (beta of lc over the
-// window and bits 0-4, gamma of lc at the cost, beta of lp over the window).
-// kQNoStore: the sweep's last pass; it stops at the gamma partial.  BND
-// selects the passes with kQBnd (their own instantiation: the cost and the
-// twelve-bit undo need far more registers than a window-only pass).
+// order, each part's partial taken before it is undone (beta of lc over its
+// bits, gamma of lc at the cost, beta of lp over its bits).  kQNoStore: the
+// sweep's last pass; it stops at the gamma partial.  BND selects the passes
+// with kQBnd (their own instantiation: the cost and the second mixer need
+// far more registers than a window-only pass).  Complex64: 3 waves per SIMD
+// (a few registers spill in the kQBnd instantiation; measured faster than 2
+// waves without spills, the LDS transposes' barriers wanting more resident
+// workgroups).
 template <typename R, bool BND>
-__global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
+__global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 3 : 1)
     k_qaoa_win_rev(typename CV<R>::T *__restrict__ st, const QInstDev *__restrict__ inst,
                    const QWin *__restrict__ wins, const uint4 *__restrict__ wlo,
                    const typename CV<R>::T *__restrict__ ph, int p,
@@ -577,42 +597,53 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
   }
   // Mixers undone as RX / cos (qrx16t): psi and lambda carry the same factor
   // u (uniform) at every point, and a partial taken there is divided by u^2.
-  // bnd: u = 1 / c_lc^k after k bits; the cost undo multiplies by
-  // c_lc^(W+5) c_lp^W (folded into its phases), so the pre part starts at
-  // u = c_lp^W and ends at 1.  Window only: u = 1 / c_lp^k, and one product
-  // by c_lp^W before the stores.
-  const int w2 = __popc(m2), w1 = __popc(m1);
+  // The first mixer undone (lc's if bnd, else lp's) on k1 bits leaves
+  // u = 1 / c^k1.  bnd: the cost undo multiplies by c_lc^k1 c_lp^W (folded
+  // into its phases), so the pre part starts at u = c_lp^W and ends at 1.
+  // Window only: one product by c_lp^k1 before the stores.
+  const int low = (P.fl & kQLow) != 0;  // bits 0-4: L0, then L1 register bit 0
+  const int w2 = __popc(m2), k1 = m.W + kQC * low;
   T mp{0, 1};  // (tan, cos) of layer lp
   if (pre) mp = ph[(size_t)(ii * p + P.lp) * kQPh + kQMixTC];
   const double cp = mp.y;
   double accp = 0.0, accq = 0.0, accg = 0.0;  // beta of lc, beta of lp, gamma of lc
-  double cq = 1.0;
+  double c1 = cp;
+  R nt = -mp.x;  // RX(-2b) / cos of the first mixer undone
   if (bnd) {
     const T *phl = ph + (size_t)(ii * p + P.lc) * kQPh;
     const T mc = phl[kQMixTC];
-    cq = mc.y;
+    c1 = mc.y;
+    nt = -mc.x;
     if (t <= q.nedges) {  // undo: exp(+i g s), the conjugate phases, times the factor
-      const double sg = qpowc(cq, m.W + kQC) * qpowc(cp, m.W * pre);
+      const double sg = qpowc(c1, k1) * qpowc(cp, m.W * pre);
       const T z = phl[t];
       ph_r[t] = (R)(sg * (double)z.x);
       ph_i[t] = (R)(-sg * (double)z.y);
     }
-    const R nt = -mc.x;  // RX(-2 b_lc) / cos
-    accp += qxpart16m(a, l, m2);
-    qrx16t(a, nt, m2);
-    qrx16t(l, nt, m2);
+  }
+  double acc1 = qxpart16m(a, l, m2);
+  qrx16t(a, nt, m2);
+  qrx16t(l, nt, m2);
+  if (low) {
     qtranspose(a, X, t, 2, 0);
     __syncthreads();
     qtranspose(l, X, t, 2, 0);
-    accp += qxpart16m(a, l, 15) * qpowc(cq, 2 * w2);  // bits 0-3
+    acc1 += qxpart16m(a, l, 15) * qpowc(c1, 2 * w2);  // bits 0-3
     qrx16t(a, nt, 15);
     qrx16t(l, nt, 15);
     qtranspose(a, X, t, 0, 1);  // writes the L0 slots this thread just read
     __syncthreads();
     qtranspose(l, X, t, 0, 1);
-    accp += qxpart16m(a, l, m1 | 1) * qpowc(cq, 2 * (w2 + 4));
-    qrx16t(a, nt, m1 | 1);
-    qrx16t(l, nt, m1 | 1);
+  } else {
+    qtranspose(a, X, t, 2, 1);
+    __syncthreads();
+    qtranspose(l, X, t, 2, 1);
+  }
+  acc1 += qxpart16m(a, l, m1 | low) * qpowc(c1, 2 * (w2 + 4 * low));
+  qrx16t(a, nt, m1 | low);
+  qrx16t(l, nt, m1 | low);
+  if (bnd) {
+    accp = acc1;
     // gamma partial Im(conj(lambda) s psi) and undo the cost layer
     const QCut1 cc = qcut_win(wins[wi], wlo + (size_t)wi * kBlock, q.n, m.base, t);
 #pragma unroll
@@ -630,18 +661,14 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
         l[r].y = w;
       }
     }
-  } else {  // window only: its L2 part first
-    accq += qxpart16m(a, l, m2);
-    qrx16t(a, -mp.x, m2);
-    qrx16t(l, -mp.x, m2);
-    qtranspose(a, X, t, 2, 1);
-    __syncthreads();
-    qtranspose(l, X, t, 2, 1);
-  }
-  if (pre) {
-    accq += qxpart16m(a, l, m1) * (bnd ? 1.0 / qpowc(cp, 2 * m.W) : qpowc(cp, 2 * w2));
-    qrx16t(a, -mp.x, m1);
-    qrx16t(l, -mp.x, m1);
+    accg *= qpowc(c1, 2 * k1);
+    if (pre) {
+      accq += qxpart16m(a, l, m1) / qpowc(cp, 2 * m.W);
+      qrx16t(a, -mp.x, m1);
+      qrx16t(l, -mp.x, m1);
+    }
+  } else {
+    accq = acc1;
   }
   if (store) {
     qtranspose(a, X, t, 1, 2);  // writes the L1 slots this thread just read
@@ -653,7 +680,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
       qrx16t(l, -mp.x, m2);
     }
     if (!bnd) {
-      const R f = (R)qpowc(cp, m.W);
+      const R f = (R)qpowc(cp, k1);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         a[r].x *= f;
@@ -673,7 +700,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 2 : 1)
   if (bnd) {
     accp = qblock_sum(accp);
     if (t == 0) partial[slot_beta(q, P.lc, P.g) + chunk] = accp;
-    accg = qblock_sum(accg * qpowc(cq, 2 * (m.W + kQC)));
+    accg = qblock_sum(accg);
     if (t == 0) partial[slot_gamma(q, P.lc) + chunk] = accg;
   }
   if (pre) {
@@ -929,11 +956,12 @@ static void free_batch(QaoaBatch *b) {
 // next cost) see every window once; for k = 1 every pass is a whole layer.
 static int qsched_nfwd(int k, int p) { return k == 1 ? p : p * (k - 1); }
 static uint32_t qsched_fwd(int k, int s) {
-  if (k == 1) return qpass(0, (s == 0 ? kQInit : 0) | kQBnd, 0, s);
+  if (k == 1) return qpass(0, (s == 0 ? kQInit : 0) | kQBnd | kQLow, 0, s);
   const int g = s % k, per = k - 1;
-  if (s == 0) return qpass(g, kQInit | kQBnd, 0, 0);
-  if (s % per == 0) return qpass(g, kQPre | kQBnd, s / per - 1, s / per);
-  return qpass(g, kQPre, s / per, 0);
+  const uint32_t lb = k <= 2 ? kQLow : 0;  // bits 0-4 with the cost, or (k >= 3)
+  if (s == 0) return qpass(g, kQInit | kQBnd | lb, 0, 0);  // in the first window-only pass
+  if (s % per == 0) return qpass(g, kQPre | kQBnd | lb, s / per - 1, s / per);
+  return qpass(g, kQPre | (s % per == 1 ? kQLow : 0), s / per, 0);
 }
 static uint32_t qsched_turn(int k, int p) {
   return k == 1 ? qpass(0, 0, 0, 0) : qpass((p * (k - 1)) % k, kQPre, p - 1, 0);
@@ -1103,7 +1131,7 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
   for (auto &L : B->launches) {
     for (int k = 0; k < L.count; ++k) {
       const QEntry &e = tabs[L.tab_off + k];
-      const uint32_t fl = (e.pass >> 4) & 15;
+      const uint32_t fl = (e.pass >> 4) & 255;
       double per = 0;
       switch (L.kind) {
         case 0: per = L.layer == 0 ? 1.0 : 2.0; break;
