@@ -155,3 +155,46 @@ def test_qaoa_rejects_repeated_edges(C):
     with pytest.raises(Exception, match="repeated edge"):
         qaoa_batch([14, 6], [[(0, 1), (2, 3)], [(0, 1), (1, 0)]], 1)
     qaoa_batch([14, 6], [[(0, 1), (2, 3)], [(0, 1), (1, 2)]], 1)
+
+
+def test_qaoa_largest_instances_vs_general_adjoint(C):
+    """The largest qubit counts a batch accepts: 30 qubits (four windows, the
+    last 4 bits wide, amplitude offsets up to 2^30 per state) and 28 (last
+    window 2 bits) beside a small instance, complex128 against the general
+    adjoint on the same circuit, and complex64 against complex128."""
+    import gc
+
+    from snuqs_amd.circuits import qaoa_maxcut, random_3_regular
+    from snuqs_amd.qaoa import qaoa_batch
+
+    if C.core.cuda.mem_info()[0] / 2**30 < 80:
+        pytest.skip("needs 80 GiB of free device memory")
+    rng = np.random.default_rng(30)
+    sizes = [30, 14, 28]
+    p = 2
+    graphs = [random_3_regular(n, rng) for n in sizes]
+    gam = rng.uniform(0, np.pi, (len(sizes), p))
+    bet = rng.uniform(0, np.pi / 2, (len(sizes), p))
+    out = {}
+    for dtype in ("complex128", "complex64"):
+        batch = qaoa_batch(sizes, graphs, p, dtype=dtype)
+        out[dtype] = batch.run(gam, bet, use_graph=False)
+        del batch
+        gc.collect()
+    o128, o64 = out["complex128"], out["complex64"]
+    for j, n in enumerate(sizes):
+        ops, pmap = qaoa_maxcut(n, graphs[j], gam[j], bet[j])
+        prog = C.functionals.Program(n, ops, dtype="complex128")
+        st = [C.result_types.StateVector(C.DeviceType.CUDA, n) for _ in range(3)]
+        paulis = ["ZZ"] * len(graphs[j])
+        qubits = [[a, b] for a, b in graphs[j]]
+        coeffs = [-0.5] * len(graphs[j])
+        e, g = prog.adjoint(st[0], st[1], st[2], [], pmap, 2 * p, paulis, qubits, coeffs)
+        e += 0.5 * len(graphs[j])
+        assert abs(o128[j, 0] - e) < 1e-10, (j, n)
+        assert np.abs(o128[j, 1:] - 2.0 * g).max() < 1e-8, (j, n)
+        del prog, st
+        gc.collect()
+    scale = np.array([len(g) ** 2 for g in graphs], dtype=float)[:, None]
+    assert np.all(np.abs(o64[:, 0] - o128[:, 0]) <= 1e-5 * np.abs(o128[:, 0]))
+    assert (np.abs(o64[:, 1:] - o128[:, 1:]) / scale).max() < 1e-5
