@@ -8,8 +8,8 @@
 // with C = 4 or 6 contiguous low bits: local bits 0..C-1 -> global bits
 // 0..C-1, local bits C..C+W-1 -> a window h0..h0+W-1 of higher bits, local
 // bits C+W..11 -> global bits C..11-W (W = 0: the identity, the "low" pass
-// over bits 0-11).  C = 4 (windows of up to 8 bits, the geometry of the
-// batched QAOA mid passes) loads 128 B runs (16 complex64 amplitudes); C = 6
+// over bits 0-11).  C = 4 (windows of up to 8 bits) loads 128 B runs (16
+// complex64 amplitudes; the batched QAOA window passes use C = 5); C = 6
 // (windows of up to 6 bits) loads and stores 512 B runs.  The 16 registers hold
 // local bits 8-11 (layout L2, the load layout), 4-7 (L1) or 0-3 (L0); a gate
 // whose local bit is in the register index is applied in registers, and the
