@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(kBlock)
   using T = typename CV<R>::T;
   const int ii = blockIdx.x / p, layer = blockIdx.x % p;
   const int ne = inst[ii].nedges;
-  const double g = gam[ii * p + layer];
+  const double g = gam[(size_t)ii * p + layer];
   T *o = ph + (size_t)blockIdx.x * kQPh;
   for (int c = threadIdx.x; c <= ne; c += kBlock) {
     double sn, cs;
@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(kBlock)
     o[c] = T{(R)cs, (R)sn};
   }
   if (threadIdx.x == 0) {
-    const double b = bet[ii * p + layer];
+    const double b = bet[(size_t)ii * p + layer];
     const double br = b - M_PI_2 * nearbyint(b / M_PI_2);
     double sn, cs;
     sincos(br, &sn, &cs);
@@ -424,10 +424,10 @@ __global__ void __launch_bounds__(kBlock)
   const int lowp = low && !bnd, lowc = low && bnd;
   const int kp = pre ? m.W + kQC * lowp : 0, kc = m.W + kQC * lowc;
   T mp{0, 1}, mc{0, 1};  // (tan, cos) of layers lp and lc
-  if (pre) mp = ph[(size_t)(ii * p + P.lp) * kQPh + kQMixTC];
+  if (pre) mp = ph[((size_t)ii * p + P.lp) * kQPh + kQMixTC];
   QCut1 cc{};
   if (bnd) {
-    const T *phl = ph + (size_t)(ii * p + P.lc) * kQPh;
+    const T *phl = ph + ((size_t)ii * p + P.lc) * kQPh;
     mc = phl[kQMixTC];
     if (t <= q.nedges) {
       const double sg = qpowc((double)mp.y, kp) * qpowc((double)mc.y, kc);
@@ -514,7 +514,7 @@ __global__ void __launch_bounds__(kBlock)
   // stored lambda take c^(2W)
   T mp{0, 1};
   if (pre) {
-    mp = ph[(size_t)(ii * p + P.lp) * kQPh + kQMixTC];
+    mp = ph[((size_t)ii * p + P.lp) * kQPh + kQMixTC];
     qrx16t(v, mp.x, m2);
   }
   const double f = qpowc((double)mp.y, 2 * m.W * pre);
@@ -604,13 +604,13 @@ __global__ void __launch_bounds__(kBlock, sizeof(R) == 4 ? 3 : 1)
   const int low = (P.fl & kQLow) != 0;  // bits 0-4: L0, then L1 register bit 0
   const int w2 = __popc(m2), k1 = m.W + kQC * low;
   T mp{0, 1};  // (tan, cos) of layer lp
-  if (pre) mp = ph[(size_t)(ii * p + P.lp) * kQPh + kQMixTC];
+  if (pre) mp = ph[((size_t)ii * p + P.lp) * kQPh + kQMixTC];
   const double cp = mp.y;
   double accp = 0.0, accq = 0.0, accg = 0.0;  // beta of lc, beta of lp, gamma of lc
   double c1 = cp;
   R nt = -mp.x;  // RX(-2b) / cos of the first mixer undone
   if (bnd) {
-    const T *phl = ph + (size_t)(ii * p + P.lc) * kQPh;
+    const T *phl = ph + ((size_t)ii * p + P.lc) * kQPh;
     const T mc = phl[kQMixTC];
     c1 = mc.y;
     nt = -mc.x;
@@ -726,7 +726,7 @@ __global__ void __launch_bounds__(kBlock)
   const int m = 1 << q.lc;
   const uint64_t base = chunk << q.lc;
   const uint8_t *e = edges + 2 * q.edge_off;
-  const T *phl = ph + (size_t)(ii * p + layer) * kQPh;
+  const T *phl = ph + ((size_t)ii * p + layer) * kQPh;
   if ((int)threadIdx.x <= q.nedges) {
     const T z = phl[threadIdx.x];
     ph_r[threadIdx.x] = z.x;
@@ -820,7 +820,7 @@ __global__ void __launch_bounds__(kBlock)
   const int m = 1 << q.lc;
   const uint64_t base = chunk << q.lc;
   const uint8_t *e = edges + 2 * q.edge_off;
-  const T *phl = ph + (size_t)(ii * p + layer) * kQPh;
+  const T *phl = ph + ((size_t)ii * p + layer) * kQPh;
   if ((int)threadIdx.x <= q.nedges) {  // undo: exp(+i g s), the conjugate phases
     const T z = phl[threadIdx.x];
     ph_r[threadIdx.x] = z.x;
@@ -974,6 +974,8 @@ int qaoa_create(void **handle, sq_dtype_t dt, uint32_t ninst, const uint32_t *nq
                 const uint32_t *nedges, const uint32_t *edges, uint32_t p) {
   if (ninst == 0 || p == 0) return fail(SQ_ERR_INVALID, "qaoa: empty batch");
   if (p > (uint32_t)kQMaxLayers) return fail(SQ_ERR_INVALID, "qaoa: at most 256 layers");
+  if ((uint64_t)ninst * p > (uint64_t)INT32_MAX)  // one phase-table workgroup each
+    return fail(SQ_ERR_INVALID, "qaoa: too many (instance, layer) pairs");
   auto B = std::make_unique<QaoaBatch>();
   B->dt = dt;
   B->ninst = (int)ninst;
