@@ -322,6 +322,60 @@ def run_batch_bench(C, n=20, layers=8, nsets=64, nterms=64, weight=4, reps=3):
     }
 
 
+def run_cbatch_bench(C, ncirc=64, sizes=tuple(range(12, 23)), depth=10, reps=3):
+    """Circuits of uneven qubit counts: 64 random layered circuits (a seeded
+    pairing with a Haar 4x4 Unitary per pair and RotZ on every qubit per
+    layer), depth 10, qubits uniform over 12..22, complex64, each fused into
+    <= 5-qubit blocks.  One batch = every state set to |0..0> and every
+    circuit applied, launch s applying block s of each circuit (hipGraph);
+    against the same fused circuits run one after another as Programs
+    (programs built before the clock, each run through its own hipGraph)."""
+    from snuqs_amd.cbatch import circuit_batch
+    from snuqs_amd.circuits import random_layered
+    from snuqs_amd.transpile import fuse_operations
+
+    rng = np.random.default_rng(77)
+    nq = [int(rng.choice(sizes)) for _ in range(ncirc)]
+    circs = [(n, random_layered(n, depth, seed=1000 + j)) for j, n in enumerate(nq)]
+    cb = circuit_batch(circs, dtype="complex64")
+    cb.run(use_graph=True)  # capture
+    C.core.cuda.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        cb.run(use_graph=True)
+    C.core.cuda.device_synchronize()
+    dt_b = (time.perf_counter() - t0) / reps
+    progs = [C.functionals.Program(n, fuse_operations(ops, 5), dtype="complex64") for n, ops in circs]
+    svs = [C.result_types.StateVector(C.DeviceType.CUDA, n, dtype="complex64") for n in nq]
+    for p, sv in zip(progs, svs):  # capture each program's graph
+        C.functionals.initialize_basis_z(sv)
+        p.run(sv, use_graph=True)
+    C.core.cuda.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for p, sv in zip(progs, svs):
+            C.functionals.initialize_basis_z(sv)
+            p.run(sv, use_graph=True)
+    C.core.cuda.device_synchronize()
+    dt_s = (time.perf_counter() - t0) / reps
+    err = max(float(np.abs(cb.state(j) - np.array(svs[j].cpu())).max()) for j in range(ncirc))
+    st = cb.stats()
+    log(f"circuit batch: {dt_b * 1e3:.1f} ms per {ncirc} circuits ({dt_s * 1e3:.1f} ms one by one)")
+    return {
+        "config": f"{ncirc} random layered circuits of depth {depth}, qubits uniform over "
+                  f"{sizes[0]}..{sizes[-1]}, complex64, fused into <= 5-qubit blocks",
+        "ms_per_batch": dt_b * 1e3,
+        "circuits_per_s": ncirc / dt_b,
+        "ms_sequential": dt_s * 1e3,
+        "speedup": dt_s / dt_b,
+        "launches_per_batch": st["launches"],
+        "launches_sequential": int(sum(p.stats()["launches"] for p in progs)),
+        "hbm_GBps": st["traffic_bytes"] / dt_b / 1e9,
+        "state_GiB": st["state_bytes"] / 2**30,
+        "max_abs_state_diff_batch_vs_sequential": err,
+    }
+
+
 def host_memory_gib():
     info = {}
     with open("/proc/meminfo") as f:
@@ -421,6 +475,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-qaoa", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-cbatch", action="store_true")
     ap.add_argument("--skip-host-tier", action="store_true")
     ap.add_argument("--skip-configs", action="store_true",
                     help="skip the config 1, config 2 complex128 and config 5 legs")
@@ -510,6 +565,7 @@ def main():
     others = None if args.skip_configs else run_other_configs(C, args.depth)
     qaoa = None if args.skip_qaoa else run_qaoa_bench(C)
     batch = None if args.skip_batch else run_batch_bench(C)
+    cbatch = None if args.skip_cbatch else run_cbatch_bench(C)
     host_tier = None if args.skip_host_tier else run_host_tier_bench(C)
     cpu = None if args.skip_cpu else cpu_baseline(n, args.depth, args.cpu_seconds)
 
@@ -560,6 +616,7 @@ def main():
         "adjoint": adj,
         "qaoa_batch": qaoa,
         "parameter_batch": batch,
+        "circuit_batch": cbatch,
         "host_tier": host_tier,
         "other_configs": others,
         "cpu_baseline": cpu,

@@ -377,6 +377,40 @@ int sq_qaoa_stats(void *batch, uint64_t *launches, double *state_bytes,
                   double *traffic_bytes);
 int sq_qaoa_destroy(void *batch);
 
+/* ---- Circuits of uneven qubit counts as one launch sequence (no reference
+ * counterpart: the reference runs one circuit per simulation and has no
+ * batching; transpile.py:233-234 is where it would go).  For callers with
+ * many circuits, each too small to fill the GPU alone (e.g. the circuits of
+ * an ensemble, a sweep or a training batch).
+ * num_circuits circuits; circuit j has num_qubits[j] (1..30) qubits and
+ * num_gates[j] gates; num_targets (1..5 each), targets and matrices hold
+ * every gate of circuit 0, then of circuit 1, ..., in the layout of
+ * sq_program_create (reference qubit order, targets[0] the most significant
+ * bit of a matrix's local index; 2^k x 2^k complex128 row-major).  Every
+ * gate runs as a dense matrix on its targets.  The batch owns the states
+ * (one device buffer).  sq_cbatch_run sets every state to |0...0> and
+ * applies every circuit: step s applies gate s of every circuit that has
+ * one, in one launch for the gates that run on the 5-target matrix-core
+ * kernels and one for the others (at most 2 max(num_gates) + 2 launches,
+ * captured into a hipGraph on the first use_graph call and replayed after).
+ * Asynchronous on `stream`.
+ * Errors (SQ_ERR_INVALID): no circuits, a qubit count outside 1..30, a gate
+ * with 0 or more than 5 targets or more targets than qubits, repeated or
+ * out-of-range targets. */
+int sq_cbatch_create(void **batch, sq_dtype_t dtype, uint32_t num_circuits,
+                     const uint32_t *num_qubits, const uint32_t *num_gates,
+                     const uint32_t *num_targets, const uint32_t *targets,
+                     const double *matrices);
+int sq_cbatch_run(void *batch, int use_graph, void *stream);
+/* Device-to-device copy of circuit j's state into dst (a device state of
+ * 2^num_qubits[j] elements of the batch's dtype), ordered on `stream`. */
+int sq_cbatch_copy_state(void *batch, uint32_t circuit, void *dst,
+                         uint64_t num_elems, void *stream);
+/* Kernel launches per run, bytes of all states, HBM bytes one run moves. */
+int sq_cbatch_stats(void *batch, uint64_t *launches, double *state_bytes,
+                    double *traffic_bytes);
+int sq_cbatch_destroy(void *batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,7 @@ HIP_SOURCES = [
     "kernels_sample.hip",
     "kernels_qaoa.hip",
     "kernels_layer.hip",
+    "kernels_cbatch.hip",
     "capi.hip",
     "host_tier.cpp",
 ]

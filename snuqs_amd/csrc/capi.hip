@@ -18,6 +18,7 @@
 
 #include "kernels.hpp"
 #include "kernels_qaoa.hpp"
+#include "kernels_cbatch.hpp"
 #include "kernels_reduce.hpp"
 
 namespace sq {
@@ -1931,6 +1932,33 @@ int sq_qaoa_stats(void *batch, uint64_t *launches, double *state_bytes, double *
   return qaoa_stats(batch, launches, state_bytes, traffic_bytes);
 }
 int sq_qaoa_destroy(void *batch) { return qaoa_destroy(batch); }
+
+// ---- circuits of uneven qubit counts ------------------------------------------
+int sq_cbatch_create(void **batch, sq_dtype_t dt, uint32_t ncirc, const uint32_t *nq,
+                     const uint32_t *ng, const uint32_t *nt, const uint32_t *targets,
+                     const double *matrices) {
+  if (!batch || !nq || !ng) return fail(SQ_ERR_INVALID, "null argument");
+  *batch = nullptr;
+  if (!valid_dtype(dt)) return fail(SQ_ERR_INVALID, "bad dtype");
+  uint64_t total = 0;
+  for (uint32_t j = 0; j < ncirc; ++j) total += ng[j];
+  if (total && (!nt || !targets || !matrices)) return fail(SQ_ERR_INVALID, "null argument");
+  return cbatch_create(batch, dt, ncirc, nq, ng, nt, targets, matrices);
+}
+int sq_cbatch_run(void *batch, int use_graph, void *stream) {
+  if (!batch) return fail(SQ_ERR_INVALID, "null batch");
+  return cbatch_run(batch, use_graph, (hipStream_t)stream);
+}
+int sq_cbatch_copy_state(void *batch, uint32_t circuit, void *dst, uint64_t num_elems,
+                         void *stream) {
+  if (!batch || !dst) return fail(SQ_ERR_INVALID, "null argument");
+  return cbatch_copy_state(batch, circuit, dst, num_elems, (hipStream_t)stream);
+}
+int sq_cbatch_stats(void *batch, uint64_t *launches, double *state_bytes, double *traffic_bytes) {
+  if (!batch) return fail(SQ_ERR_INVALID, "null batch");
+  return cbatch_stats(batch, launches, state_bytes, traffic_bytes);
+}
+int sq_cbatch_destroy(void *batch) { return cbatch_destroy(batch); }
 
 }  // extern "C"
 

@@ -195,7 +195,7 @@ int make_plan(const double *matrix, const uint32_t *targets, int k,
     // free of controls (the complex64 tiles' 16 B accesses) and enough free
     // positions left for full tiles.
     const int nfixed = __builtin_popcountll(p.fixed_mask);
-    if (kk == 4 && !(p.fixed_mask & 1) && nbits - nfixed - 5 >= 10) {
+    if (kk == 4 && !(flags & kPlanExact) && !(p.fixed_mask & 1) && nbits - nfixed - 5 >= 10) {
       uint64_t used = p.fixed_mask;
       for (int j = 0; j < kk; ++j) used |= 1ull << tb[j];
       int nb = 0;

@@ -498,6 +498,66 @@ class Program {
   void *prog_ = nullptr;
 };
 
+// Circuits of uneven qubit counts as one launch sequence (sq_cbatch_*).
+class CircuitBatch {
+ public:
+  CircuitBatch(const std::vector<std::pair<size_t, std::vector<std::shared_ptr<GateOperation>>>> &circuits,
+               sq_dtype_t dt)
+      : dt_(dt) {
+    std::vector<uint32_t> nq, ng, nt, tg;
+    std::vector<double> mats;
+    for (auto &c : circuits) {
+      nq.push_back((uint32_t)c.first);
+      ng.push_back((uint32_t)c.second.size());
+      for (auto &op : c.second) {
+        auto t = op->get_targets();
+        if (t.size() != op->num_matrix_qubits())
+          throw py::value_error("gate " + op->formatted_string() + ": targets do not match matrix");
+        nt.push_back((uint32_t)t.size());
+        for (size_t q : t) tg.push_back((uint32_t)q);
+        const double *m = mat_ptr(*op);
+        mats.insert(mats.end(), m, m + 2 * op->matrix().size());
+      }
+    }
+    nq_ = nq;
+    check(sq_cbatch_create(&h_, dt, (uint32_t)nq.size(), nq.data(), ng.data(), nt.data(),
+                           tg.data(), mats.data()));
+  }
+  ~CircuitBatch() {
+    if (h_) sq_cbatch_destroy(h_);
+  }
+  void run(bool use_graph, std::shared_ptr<Stream> stream) {
+    py::gil_scoped_release nogil;
+    check(sq_cbatch_run(h_, use_graph ? 1 : 0, sraw(stream)));
+  }
+  void copy_state(uint32_t j, StateVector &sv, std::shared_ptr<Stream> stream) {
+    sv.require_device("copy_state");
+    if (j >= nq_.size()) throw py::index_error("no circuit " + std::to_string(j));
+    if (sv.dtype() != dt_) throw py::value_error("state dtype differs from the batch's");
+    if (sv.num_qubits() != nq_[j])
+      throw py::value_error("state has " + std::to_string(sv.num_qubits()) + " qubits, circuit " +
+                            std::to_string(j) + " has " + std::to_string(nq_[j]));
+    py::gil_scoped_release nogil;
+    check(sq_cbatch_copy_state(h_, j, sv.ptr(), sv.num_elems(), sraw(stream)));
+  }
+  py::dict stats() const {
+    uint64_t l;
+    double sb, tb;
+    check(sq_cbatch_stats(h_, &l, &sb, &tb));
+    py::dict d;
+    d["launches"] = l;
+    d["state_bytes"] = sb;
+    d["traffic_bytes"] = tb;
+    return d;
+  }
+  std::vector<uint32_t> num_qubits() const { return nq_; }
+
+ private:
+  sq_dtype_t dt_;
+  std::vector<uint32_t> nq_;
+  void *h_ = nullptr;
+};
+
 // Batched QAOA MaxCut (sq_qaoa_*).
 class QaoaBatch {
  public:
@@ -1061,6 +1121,21 @@ PYBIND11_MODULE(_C, m) {
   }, py::arg("states"), py::arg("paulis"), py::arg("qubits"), py::arg("coeffs"), py::kw_only(),
      py::arg("stream") = nullptr,
      "Per-state energies sum_t c_t <psi_y|P_t|psi_y> (FP64) and per-term values.");
+
+  py::class_<CircuitBatch, std::shared_ptr<CircuitBatch>>(m_fn, "CircuitBatch")
+      .def(py::init([](const std::vector<std::pair<size_t, std::vector<std::shared_ptr<GateOperation>>>> &circuits,
+                       std::string dtype) {
+             return std::make_shared<CircuitBatch>(circuits, parse_dtype(dtype));
+           }), py::arg("circuits"), py::kw_only(), py::arg("dtype") = "complex64",
+           "Circuits of uneven qubit counts, [(num_qubits, gates), ...], as one launch sequence.")
+      .def("run", &CircuitBatch::run, py::kw_only(), py::arg("use_graph") = true,
+           py::arg("stream") = nullptr,
+           "Every state <- |0...0>, then every circuit's gates (launch s: gate s of each circuit).")
+      .def("copy_state", &CircuitBatch::copy_state, py::arg("circuit"), py::arg("sv"),
+           py::kw_only(), py::arg("stream") = nullptr,
+           "Copy circuit j's state into a device StateVector of its qubit count and dtype.")
+      .def("stats", &CircuitBatch::stats)
+      .def_property_readonly("num_qubits", &CircuitBatch::num_qubits);
 
   py::class_<QaoaBatch, std::shared_ptr<QaoaBatch>>(m_fn, "QaoaBatch")
       .def(py::init([](const std::vector<uint32_t> &nq,

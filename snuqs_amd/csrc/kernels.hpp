@@ -142,6 +142,18 @@ inline IdxMap make_idxmap(uint64_t ins_mask, uint64_t fixed_val) {
 
 // Launch one plan on a state.  `dev_matrix` is the device copy of a DENSE
 // plan with kk >= 4 in the layout of fused_block_3m; unused otherwise.
+// Circuit batches on the 5-target MFMA kernels: the bytes of one table
+// entry; the entry of a DENSE plan on 5 targets of a state of nelems
+// amplitudes at amplitude offset psi, its matrix appended to g3m (4 reals per
+// element, the layout of the 4-5 target kernels), false when the plan does
+// not run on those kernels; one launch over a table of `count` entries.
+size_t cbatch_mfma_entry_bytes();
+bool cbatch_mfma_entry(const GatePlan &p, bool c64, uint64_t nelems, uint64_t psi,
+                       uint64_t first_block, std::vector<double> *g3m, void *entry,
+                       uint64_t *nblocks);
+int launch_cbatch_mfma(sq_dtype_t dt, void *states, const void *g3m, const void *tab, int count,
+                       uint64_t blocks, hipStream_t st);
+
 int launch_plan(const GatePlan &p, void *sv, sq_dtype_t dt, uint64_t nelems,
                 hipStream_t stream, const void *dev_matrix);
 

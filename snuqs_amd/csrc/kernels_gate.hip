@@ -15,6 +15,7 @@
 //   k_mfma4_*, k_mfma5_* : DENSE plans on 4 / 5 targets (fused blocks) on the
 //              matrix cores
 //   k_dense_lds : DENSE plans on 4-5 targets for small states (VALU)
+#include <cstring>
 #include <vector>
 
 #include "kernels.hpp"
@@ -560,23 +561,29 @@ struct TileRange {
   uint64_t first, end, step;
   int rbits;  // ORD 2: log2(ntiles), else 0
 };
+// Workgroup wg of nwg (the launch's grid, or a circuit batch entry's share).
 template <int ORD>
-__device__ __forceinline__ TileRange tile_range(uint64_t ntiles, int wave) {
+__device__ __forceinline__ TileRange tile_range(uint64_t ntiles, int wave, uint64_t wg,
+                                                uint64_t nwg) {
   TileRange r;
   r.rbits = 0;
   if (ORD == 1) {
-    const uint64_t per = ((ntiles + gridDim.x - 1) / gridDim.x + 3) / 4 * 4;
-    r.first = (uint64_t)blockIdx.x * per + wave;
-    const uint64_t e = (uint64_t)(blockIdx.x + 1) * per;
+    const uint64_t per = ((ntiles + nwg - 1) / nwg + 3) / 4 * 4;
+    r.first = wg * per + wave;
+    const uint64_t e = (wg + 1) * per;
     r.end = e < ntiles ? e : ntiles;
     r.step = 4;
   } else {
-    r.first = (uint64_t)blockIdx.x * 4 + wave;
+    r.first = wg * 4 + wave;
     r.end = ntiles;
-    r.step = (uint64_t)gridDim.x * 4;
+    r.step = nwg * 4;
     if (ORD == 2) r.rbits = __builtin_ctzll(ntiles);
   }
   return r;
+}
+template <int ORD>
+__device__ __forceinline__ TileRange tile_range(uint64_t ntiles, int wave) {
+  return tile_range<ORD>(ntiles, wave, blockIdx.x, gridDim.x);
 }
 // The tile a walk position stands for.
 template <int ORD>
@@ -605,10 +612,13 @@ __host__ __device__ constexpr uint32_t swz_c64(uint32_t x) {
 // complex128: element (c, r) at slot 16r + c, swizzled by XOR (r & 15).
 __host__ __device__ constexpr uint32_t swz_c128(uint32_t x) { return x ^ ((x >> 4) & 15u); }
 
-template <bool PIPE, int ORD>
+// IM / SA: IdxMap and StageArgs by value (kernel arguments), or const
+// references into a circuit batch's entry table (read in place: the index
+// map is indexed at run time).
+template <bool PIPE, int ORD, typename IM = IdxMap, typename SA = StageArgs>
 __device__ __forceinline__ void
-    k_mfma5_c64_body(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
-                const float4 *__restrict__ G) {
+    k_mfma5_c64_body(float2 *__restrict__ sv, uint64_t ntiles, IM im, SA sa,
+                const float4 *__restrict__ G, uint64_t wg, uint64_t nwg) {
   constexpr int D = 32, NS = 16;
   __shared__ float4 As[D * D];      // [o][r], 16 KB
   __shared__ float2 Xs[4][D * 32];  // one tile per wave, 32 KB
@@ -617,7 +627,7 @@ __device__ __forceinline__ void
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 31, half = lane >> 5;
   float2 *X = Xs[wave];
-  TileRange tr = tile_range<ORD>(ntiles, wave);
+  TileRange tr = tile_range<ORD>(ntiles, wave, wg, nwg);
   uint64_t tile = tr.first;
   if (tile >= tr.end) return;
   // j = 2 lane + 128 k + e: lane bits are tile bits 1-6, k bits tile bits 7-9
@@ -704,13 +714,14 @@ template <bool PIPE, int ORD>
 __global__ void __launch_bounds__(kBlock)
     k_mfma5_c64(float2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
                 const float4 *__restrict__ G) {
-  k_mfma5_c64_body<PIPE, ORD>(sv, ntiles, im, sa, G);
+  k_mfma5_c64_body<PIPE, ORD>(sv, ntiles, im, sa, G, blockIdx.x, gridDim.x);
 }
 template <bool PIPE, int ORD>
 __global__ void __launch_bounds__(kBlock)
     kb_mfma5_c64(float2 *__restrict__ sv, uint64_t sstride, uint64_t ntiles, IdxMap im,
                  StageArgs sa, const float4 *__restrict__ G, uint64_t gstride) {
-  k_mfma5_c64_body<PIPE, ORD>(sv + blockIdx.y * sstride, ntiles, im, sa, G + blockIdx.y * gstride);
+  k_mfma5_c64_body<PIPE, ORD>(sv + blockIdx.y * sstride, ntiles, im, sa, G + blockIdx.y * gstride,
+                              blockIdx.x, gridDim.x);
 }
 
 
@@ -826,10 +837,10 @@ __global__ void __launch_bounds__(kBlock)
 #undef SQ_KW
 }
 
-template <bool PIPE, int ORD>
+template <bool PIPE, int ORD, typename IM = IdxMap, typename SA = StageArgs>
 __device__ __forceinline__ void
-    k_mfma5_c128_body(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
-                 const typename Q4<double>::T *__restrict__ G) {
+    k_mfma5_c128_body(double2 *__restrict__ sv, uint64_t ntiles, IM im, SA sa,
+                 const typename Q4<double>::T *__restrict__ G, uint64_t wg, uint64_t nwg) {
   using D4 = typename Q4<double>::T;
   constexpr int D = 32, NS = 8, NT = 2;
   __shared__ D4 As[D * D];           // [o][r], 32 KB
@@ -839,7 +850,7 @@ __device__ __forceinline__ void
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, q = lane >> 4;
   double2 *X = Xs[wave];
-  TileRange tr = tile_range<ORD>(ntiles, wave);
+  TileRange tr = tile_range<ORD>(ntiles, wave, wg, nwg);
   uint64_t tile = tr.first;
   if (tile >= tr.end) return;
   // j = lane + 64 k: lane bits are tile bits 0-5, k bits tile bits 6-8
@@ -932,14 +943,14 @@ template <bool PIPE, int ORD>
 __global__ void __launch_bounds__(kBlock)
     k_mfma5_c128(double2 *__restrict__ sv, uint64_t ntiles, IdxMap im, StageArgs sa,
                  const typename Q4<double>::T *__restrict__ G) {
-  k_mfma5_c128_body<PIPE, ORD>(sv, ntiles, im, sa, G);
+  k_mfma5_c128_body<PIPE, ORD>(sv, ntiles, im, sa, G, blockIdx.x, gridDim.x);
 }
 template <bool PIPE, int ORD>
 __global__ void __launch_bounds__(kBlock)
     kb_mfma5_c128(double2 *__restrict__ sv, uint64_t sstride, uint64_t ntiles, IdxMap im,
                   StageArgs sa, const typename Q4<double>::T *__restrict__ G, uint64_t gstride) {
   k_mfma5_c128_body<PIPE, ORD>(sv + blockIdx.y * sstride, ntiles, im, sa,
-                               G + blockIdx.y * gstride);
+                               G + blockIdx.y * gstride, blockIdx.x, gridDim.x);
 }
 
 
@@ -1543,6 +1554,79 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
       break;
     }
   }
+  SQ_HIP(hipGetLastError());
+  return SQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Circuit batches (kernels_cbatch.hip): gates of several circuits in one
+// launch on the 5-target MFMA kernels, each entry its own state, tiles,
+// workgroup range and matrix.
+// ---------------------------------------------------------------------------
+struct CBMfma {
+  uint64_t psi;          // amplitude offset of the circuit's state
+  uint64_t ntiles;       // tiles of the gate
+  uint64_t first_block;  // first workgroup of the entry in its launch
+  uint64_t nblocks;      // its workgroups (the tile loop's grid)
+  uint64_t g;            // offset of its matrix (3m layout, 4-real elements)
+  IdxMap im;
+  StageArgs sa;
+};
+
+template <bool C64, bool PIPE>
+__global__ void __launch_bounds__(kBlock)
+    k_cbatch_mfma5(void *__restrict__ states, const void *__restrict__ g3m,
+                   const CBMfma *__restrict__ tab, int count) {
+  int lo = 0, hi = count - 1;  // the entry holding this workgroup (uniform)
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].first_block <= blockIdx.x) lo = mid; else hi = mid - 1;
+  }
+  const CBMfma &m = tab[__builtin_amdgcn_readfirstlane(lo)];
+  const uint64_t wg = blockIdx.x - m.first_block;
+  if constexpr (C64)
+    k_mfma5_c64_body<PIPE, 0, const IdxMap &, const StageArgs &>(
+        (float2 *)states + m.psi, m.ntiles, m.im, m.sa, (const float4 *)g3m + m.g, wg, m.nblocks);
+  else
+    k_mfma5_c128_body<PIPE, 0, const IdxMap &, const StageArgs &>(
+        (double2 *)states + m.psi, m.ntiles, m.im, m.sa,
+        (const typename Q4<double>::T *)g3m + m.g, wg, m.nblocks);
+}
+
+size_t cbatch_mfma_entry_bytes() { return sizeof(CBMfma); }
+
+bool cbatch_mfma_entry(const GatePlan &p, bool c64, uint64_t nelems, uint64_t psi,
+                       uint64_t first_block, std::vector<double> *g3m, void *entry,
+                       uint64_t *nblocks) {
+  if (p.kind != Kind::DENSE || p.kk != 5) return false;
+  const uint64_t tile_groups = c64 ? 32 : 16;
+  const uint64_t nfree = nelems >> (__builtin_popcountll(p.fixed_mask) + p.kk);
+  CBMfma e{};
+  if (nfree < tile_groups || nfree % tile_groups || !make_stage(p, c64, &e.sa)) return false;
+  e.psi = psi;
+  e.ntiles = nfree / tile_groups;
+  e.first_block = first_block;
+  const uint64_t cap = 256ull * (uint64_t)tuning().mfma_blocks_per_cu[c64 ? 0 : 1];
+  e.nblocks = std::min<uint64_t>((e.ntiles + 3) / 4, cap);
+  e.im = make_idxmap(p.fixed_mask | target_mask(p), p.fixed_val);
+  e.g = g3m->size() / 4;
+  const std::vector<double> b = fused_block_3m(p);
+  g3m->insert(g3m->end(), b.begin(), b.end());
+  std::memcpy(entry, &e, sizeof(e));
+  *nblocks = e.nblocks;
+  return true;
+}
+
+int launch_cbatch_mfma(sq_dtype_t dt, void *states, const void *g3m, const void *tab, int count,
+                       uint64_t blocks, hipStream_t st) {
+  const bool c64 = dt == SQ_COMPLEX64;
+  const bool pipe = tuning().mfma5_pipeline[c64 ? 0 : 1] != 0;
+  const CBMfma *t = (const CBMfma *)tab;
+  const dim3 g((unsigned)blocks), b(kBlock);
+  if (c64 && pipe) hipLaunchKernelGGL((k_cbatch_mfma5<true, true>), g, b, 0, st, states, g3m, t, count);
+  else if (c64) hipLaunchKernelGGL((k_cbatch_mfma5<true, false>), g, b, 0, st, states, g3m, t, count);
+  else if (pipe) hipLaunchKernelGGL((k_cbatch_mfma5<false, true>), g, b, 0, st, states, g3m, t, count);
+  else hipLaunchKernelGGL((k_cbatch_mfma5<false, false>), g, b, 0, st, states, g3m, t, count);
   SQ_HIP(hipGetLastError());
   return SQ_OK;
 }

@@ -83,6 +83,9 @@ struct GatePlan {
 // classification: a DENSE plan on all k targets (PHASE for k = 0), the form
 // every matrix of a gate shares whatever its parameters.
 constexpr int kPlanDense = 1;
+// kPlanExact: keep a dense plan on exactly its own targets (no widening of a
+// 4-target block to 5 targets for the MFMA kernels).
+constexpr int kPlanExact = 2;
 int make_plan(const double *matrix, const uint32_t *targets, int k,
               int num_qubits, int nbits, GatePlan *plan, int flags = 0);
 
