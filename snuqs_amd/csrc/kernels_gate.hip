@@ -146,10 +146,12 @@ struct DiagArgs {
   int kk;
 };
 
-template <typename R, int U>
+// A: DiagArgs by value (kernel argument) or a const reference into a batch's
+// per-state table (read in place: its entries are indexed at run time, and a
+// copy would sit in scratch).
+template <typename R, int U, typename A = DiagArgs<R>>
 __device__ __forceinline__ void
-    k_diag_body(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im,
-           DiagArgs<R> a) {
+    k_diag_body(typename CV<R>::T *__restrict__ sv, uint64_t nfree, IdxMap im, A a) {
   using T = typename CV<R>::T;
   __shared__ R sdr[1 << kMaxK], sdi[1 << kMaxK];
   if (threadIdx.x < (1u << a.kk)) {
@@ -191,7 +193,7 @@ template <typename R, int U>
 __global__ void __launch_bounds__(kBlock)
     kb_diag(typename CV<R>::T *__restrict__ sv, uint64_t sstride, uint64_t nfree, IdxMap im,
             const DiagArgs<R> *__restrict__ tab) {
-  k_diag_body<R, U>(sv + blockIdx.y * sstride, nfree, im, tab[blockIdx.y]);
+  k_diag_body<R, U, const DiagArgs<R> &>(sv + blockIdx.y * sstride, nfree, im, tab[blockIdx.y]);
 }
 
 
