@@ -279,6 +279,8 @@ int cbatch_create(void **handle, sq_dtype_t dt, uint32_t ncirc, const uint32_t *
       L.count++;
       tabs.push_back(e);
     }
+    if (L.blocks > (uint64_t)INT32_MAX || M.blocks > (uint64_t)INT32_MAX)
+      return fail(SQ_ERR_INVALID, "circuit batch: too many circuits for one launch per step");
     if (L.count) B->launches.push_back(L);
     if (M.count) B->launches.push_back(M);
   }
