@@ -341,8 +341,11 @@ int sq_sample(const double *probs, uint64_t num_elems, const double *uniforms,
  * The reverse sweep applies G_g^dagger to the state and to lambda = H psi
  * (one FP64 inner product per entry).  num_checkpoints device buffers of
  * num_elems amplitudes store forward states at evenly spaced gates; the
- * reverse sweep reloads them instead of undoing those segments, bounding the
- * drift of complex64 reverse sweeps.  work0/work1: scratch states.
+ * reverse sweep continues from them instead of undoing those segments,
+ * bounding the drift of complex64 reverse sweeps.  work0/work1: scratch
+ * states.  After the call sv, work0, work1 and the checkpoints hold
+ * unspecified states (the reverse sweep moves onto each checkpoint's buffer
+ * instead of copying it back).
  * Host arrays: deriv_*, dmatrices, xmask/zmask/ny/coeffs, energy, grads. */
 int sq_adjoint_gradient(void *program, uint32_t num_derivs,
                         const uint32_t *deriv_gate, const uint32_t *deriv_param,

@@ -1764,13 +1764,18 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     SQ_HIP(hipMemcpyAsync(gslots, hg, grad_b, hipMemcpyHostToDevice, st));
   }
   SQ_TRY(launch_inner(sv, work0, dt, n, partial, eslot, nullptr, 0.0, st));
-  // reverse sweep
+  // reverse sweep, on `cur`: psi's buffer until the first checkpoint bound
+  // is reached, then that checkpoint's buffer, which holds psi at that point
+  // (a switch of buffers instead of a copy back; each checkpoint is reached
+  // once, and sv, the checkpoints and the work states are scratch after the
+  // call)
+  void *cur = sv;
   int cidx = (int)num_checkpoints - 1;
   // after a run or layer whose lowest gate is a checkpoint bound, psi is
-  // reloaded from the checkpoint (the state before that gate)
+  // the checkpoint (the state before that gate)
   auto restore_below = [&](uint32_t lo) -> int {
     if (cidx >= 0 && bound[cidx] == lo) {
-      SQ_HIP(hipMemcpyAsync(sv, checkpoints[cidx], n * itemsize(dt), hipMemcpyDeviceToDevice, st));
+      cur = checkpoints[cidx];
       while (cidx >= 0 && bound[cidx] == lo) --cidx;
     }
     return SQ_OK;
@@ -1797,14 +1802,14 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       const Run &r = runs[ri];
       if (r.nslots)  // derivative terms only; the low pass undoes the phases
         SQ_TRY(launch_adjoint_diag_run(
-            sv, work0, dt, n, rgates_dev + r.first, (int)r.m, (int)r.nslots,
+            cur, work0, dt, n, rgates_dev + r.first, (int)r.m, (int)r.nslots,
             rslots_dev + r.first_slot, distinct_params(r.first_slot, r.nslots),
             rows_at(r.nslots, diag_run_blocks(n), r.first_slot), fin, st, false));
       const Layer &L = layers[rfuse[ri]];
       auto pass = [&](uint32_t i, const FuseDesc *fz) {
         const LPass &ps = lpasses[i];
         return launch_adjoint_layer(
-            sv, work0, dt, n, ps.h0, ps.W, ps.C, ps.count, ps.pos, ps.gm, ps.dm, ps.slot,
+            cur, work0, dt, n, ps.h0, ps.W, ps.C, ps.count, ps.pos, ps.gm, ps.dm, ps.slot,
             (int)ps.nslots, rslots_dev + ps.first_slot, distinct_params(ps.first_slot, ps.nslots),
             rows_at(ps.nslots, layer_pass_blocks(n), ps.first_slot), fin, st, fz);
       };
@@ -1817,7 +1822,7 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     }
     if (run_top[g] >= 0) {
       const Run &r = runs[run_top[g]];
-      SQ_TRY(launch_adjoint_diag_run(sv, work0, dt, n, rgates_dev + r.first, (int)r.m,
+      SQ_TRY(launch_adjoint_diag_run(cur, work0, dt, n, rgates_dev + r.first, (int)r.m,
                                      (int)r.nslots, rslots_dev + r.first_slot,
                                      distinct_params(r.first_slot, r.nslots),
                                      rows_at(r.nslots, diag_run_blocks(n), r.first_slot), fin,
@@ -1831,7 +1836,7 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
       for (uint32_t i = L.first_pass; i < L.first_pass + L.npass; ++i) {
         const LPass &ps = lpasses[i];
         SQ_TRY(launch_adjoint_layer(
-            sv, work0, dt, n, ps.h0, ps.W, ps.C, ps.count, ps.pos, ps.gm, ps.dm, ps.slot,
+            cur, work0, dt, n, ps.h0, ps.W, ps.C, ps.count, ps.pos, ps.gm, ps.dm, ps.slot,
             (int)ps.nslots, rslots_dev + ps.first_slot, distinct_params(ps.first_slot, ps.nslots),
             rows_at(ps.nslots, layer_pass_blocks(n), ps.first_slot), fin, st));
       }
@@ -1842,7 +1847,7 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     if (grp_top[g] >= 0) {
       const Group &gr = groups[grp_top[g]];
       SQ_TRY(launch_adjoint_step_group(
-          sv, work0, dt, n, (int)gr.k, gr.bits, gr.ginv, gr.dgg, rslots_dev + gr.first_slot,
+          cur, work0, dt, n, (int)gr.k, gr.bits, gr.ginv, gr.dgg, rslots_dev + gr.first_slot,
           distinct_params(gr.first_slot, gr.k),
           rows_at(gr.k, step_group_blocks(n, (int)gr.k, gr.bits), gr.first_slot), fin, st));
       g = (int)gr.lo;
@@ -1850,7 +1855,7 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
     }
     bool restored = false;
     if (cidx >= 0 && bound[cidx] == (uint32_t)g) {
-      SQ_HIP(hipMemcpyAsync(sv, checkpoints[cidx], n * itemsize(dt), hipMemcpyDeviceToDevice, st));
+      cur = checkpoints[cidx];
       while (cidx >= 0 && bound[cidx] == (uint32_t)g) --cidx;
       restored = true;
     }
@@ -1866,13 +1871,13 @@ int sq_adjoint_gradient(void *program, uint32_t num_derivs, const uint32_t *deri
         }
       const size_t doff = doffs[d];
       const int bit = p->nq - 1 - (int)p->targets[p->toff[g]];
-      SQ_TRY(launch_adjoint_step1(sv, work0, dt, n, bit, gi, dmatrices + doff, restored ? 0 : 1,
+      SQ_TRY(launch_adjoint_step1(cur, work0, dt, n, bit, gi, dmatrices + doff, restored ? 0 : 1,
                                   partial, gslots + 2 * deriv_param[d], two, st));
       continue;
     }
-    if (!restored) SQ_TRY(launch_plan(p->inv[g], sv, dt, n, st, p->inv_dev[g]));
+    if (!restored) SQ_TRY(launch_plan(p->inv[g], cur, dt, n, st, p->inv_dev[g]));
     for (uint32_t d : by_gate[g]) {
-      SQ_HIP(hipMemcpyAsync(work1, sv, n * itemsize(dt), hipMemcpyDeviceToDevice, st));
+      SQ_HIP(hipMemcpyAsync(work1, cur, n * itemsize(dt), hipMemcpyDeviceToDevice, st));
       SQ_TRY(launch_plan(dplans[d], work1, dt, n, st, dmat_ptr[d]));
       // gslots[2p] += 2 Re <lambda|work1>
       SQ_TRY(launch_inner(work0, work1, dt, n, partial, gslots + 2 * deriv_param[d], two,

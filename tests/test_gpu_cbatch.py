@@ -107,3 +107,22 @@ def test_cbatch_copy_state_checks(C):
     batch.copy_state(1, sv)
     a = np.array(sv.cpu())
     assert a[0] == 1 and np.count_nonzero(a) == 1
+
+
+def test_cbatch_many_circuits(C):
+    """300 circuits (more entries per launch than a workgroup stages in LDS,
+    so entries are found by the global search), 2-11 qubits, fused (5-target
+    blocks on states with full tiles take the MFMA entry table), complex64
+    against the oracle."""
+    from snuqs_amd.cbatch import circuit_batch
+    from snuqs_amd.circuits import random_layered
+
+    rng = np.random.default_rng(300)
+    sizes = [int(rng.integers(2, 12)) for _ in range(300)]
+    circs = [(n, random_layered(n, 3, seed=5000 + j)) for j, n in enumerate(sizes)]
+    batch = circuit_batch(circs, dtype="complex64")
+    batch.run()
+    for j in rng.choice(len(circs), 40, replace=False):
+        n, ops = circs[j]
+        got = batch.state(int(j)).astype(np.complex128)
+        assert np.abs(got - run_oracle(n, ops)).max() < 2e-5, (j, n)
