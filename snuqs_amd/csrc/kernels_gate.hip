@@ -697,7 +697,7 @@ __device__ __forceinline__ void
     wave_lds_sync();
     if (!more) break;
     tile = ntile;
-    if (PIPE) {
+    if constexpr (PIPE) {
       base = nbase;
 #pragma unroll
       for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
@@ -824,7 +824,7 @@ __global__ void __launch_bounds__(kBlock)
     wave_lds_sync();
     if (!more) break;
     tile = ntile;
-    if (PIPE) {
+    if constexpr (PIPE) {
       base = nbase;
 #pragma unroll
       for (int k = 0; k < NK; ++k) cur[k] = nxt[k];
@@ -927,7 +927,7 @@ __device__ __forceinline__ void
     wave_lds_sync();
     if (!more) break;
     tile = ntile;
-    if (PIPE) {
+    if constexpr (PIPE) {
       base = nbase;
 #pragma unroll
       for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
