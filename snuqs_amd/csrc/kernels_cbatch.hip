@@ -242,12 +242,16 @@ int cbatch_create(void **handle, sq_dtype_t dt, uint32_t ncirc, const uint32_t *
   for (uint32_t s = 0; s < maxg; ++s) {
     CBatch::Launch L{0, tabs.size(), 0, 0};
     CBatch::Launch M{1, mtabs.size() / mbytes, 0, 0};
+    uint64_t mtiles = 0;  // tiles of the step's MFMA entries (sets their grid)
+    for (uint32_t j = 0; j < ncirc; ++j)
+      if (s < ng[j]) mtiles += cbatch_mfma_tiles(wplans[j][s], c64, 1ull << nq[j]);
+    const uint64_t tpw = batch_tiles_per_wave(mtiles);
     for (uint32_t j = 0; j < ncirc; ++j) {
       if (s >= ng[j]) continue;
       B->traffic_bytes += 2.0 * std::ldexp(1.0, (int)nq[j]) * (double)item;
       std::vector<uint8_t> ent(mbytes);
       uint64_t nb = 0;
-      if (cbatch_mfma_entry(wplans[j][s], c64, 1ull << nq[j], B->host_psi[j], M.blocks, &g3,
+      if (cbatch_mfma_entry(wplans[j][s], c64, 1ull << nq[j], B->host_psi[j], M.blocks, tpw, &g3,
                             ent.data(), &nb)) {
         mtabs.insert(mtabs.end(), ent.begin(), ent.end());
         M.blocks += nb;

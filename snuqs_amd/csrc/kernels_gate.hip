@@ -1597,9 +1597,30 @@ __global__ void __launch_bounds__(kBlock)
 
 size_t cbatch_mfma_entry_bytes() { return sizeof(CBMfma); }
 
+// Tiles per wave of a batch launch of the MFMA kernels over `total_tiles`
+// tiles: up to the mfma_batch_tiles knob (a wave that walks several tiles
+// overlaps the next tile's loads with this one's products and stages the
+// matrix once for all of them), but never so many that the launch drops below
+// kBatchMinBlocks workgroups (8 per CU).
+constexpr uint64_t kBatchMinBlocks = 2048;
+uint64_t batch_tiles_per_wave(uint64_t total_tiles) {
+  uint64_t t = (uint64_t)tuning().mfma_batch_tiles;
+  while (t > 1 && total_tiles < 4 * t * kBatchMinBlocks) t >>= 1;
+  return t;
+}
+
+uint64_t cbatch_mfma_tiles(const GatePlan &p, bool c64, uint64_t nelems) {
+  if (p.kind != Kind::DENSE || p.kk != 5) return 0;
+  const uint64_t tile_groups = c64 ? 32 : 16;
+  const uint64_t nfree = nelems >> (__builtin_popcountll(p.fixed_mask) + p.kk);
+  StageArgs sa{};
+  if (nfree < tile_groups || nfree % tile_groups || !make_stage(p, c64, &sa)) return 0;
+  return nfree / tile_groups;
+}
+
 bool cbatch_mfma_entry(const GatePlan &p, bool c64, uint64_t nelems, uint64_t psi,
-                       uint64_t first_block, std::vector<double> *g3m, void *entry,
-                       uint64_t *nblocks) {
+                       uint64_t first_block, uint64_t tiles_per_wave,
+                       std::vector<double> *g3m, void *entry, uint64_t *nblocks) {
   if (p.kind != Kind::DENSE || p.kk != 5) return false;
   const uint64_t tile_groups = c64 ? 32 : 16;
   const uint64_t nfree = nelems >> (__builtin_popcountll(p.fixed_mask) + p.kk);
@@ -1609,7 +1630,8 @@ bool cbatch_mfma_entry(const GatePlan &p, bool c64, uint64_t nelems, uint64_t ps
   e.ntiles = nfree / tile_groups;
   e.first_block = first_block;
   const uint64_t cap = 256ull * (uint64_t)tuning().mfma_blocks_per_cu[c64 ? 0 : 1];
-  e.nblocks = std::min<uint64_t>((e.ntiles + 3) / 4, cap);
+  const uint64_t per = 4 * std::max<uint64_t>(tiles_per_wave, 1);  // tiles per workgroup
+  e.nblocks = std::min<uint64_t>((e.ntiles + per - 1) / per, cap);
   e.im = make_idxmap(p.fixed_mask | target_mask(p), p.fixed_val);
   e.g = g3m->size() / 4;
   const std::vector<double> b = fused_block_3m(p);
@@ -1821,7 +1843,8 @@ static int launch_batch_t(const GatePlan &p, uint64_t nelems, uint32_t nsets,
     case BatchKernel::MFMA5: {
       const int di = sizeof(R) == 4 ? 0 : 1;
       const Tuning &tu = tuning();
-      uint64_t nb = (b.work + 3) / 4;
+      const uint64_t per = 4 * batch_tiles_per_wave(b.work * nsets);  // tiles per workgroup
+      uint64_t nb = (b.work + per - 1) / per;
       const uint64_t cap = 256ull * (uint64_t)tu.mfma_blocks_per_cu[di];
       if (nb > cap) nb = cap;
       const dim3 g((unsigned)nb, nsets);

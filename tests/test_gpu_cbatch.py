@@ -126,3 +126,30 @@ def test_cbatch_many_circuits(C):
         n, ops = circs[j]
         got = batch.state(int(j)).astype(np.complex128)
         assert np.abs(got - run_oracle(n, ops)).max() < 2e-5, (j, n)
+
+
+def test_cbatch_tiles_per_wave_bitwise(C):
+    """Steps whose 5-target entries hold enough tiles for 4 tiles per wave
+    (40 circuits of 20 qubits) give the same bits as 1 tile per wave, and
+    agree with Programs run alone."""
+    from snuqs_amd.cbatch import circuit_batch
+    from snuqs_amd.circuits import random_layered
+    from snuqs_amd.transpile import fuse_operations
+
+    circs = [(20, random_layered(20, 3, seed=700 + j)) for j in range(40)]
+    saved = C.core.tune_get("mfma_batch_tiles")
+    states = {}
+    try:
+        for tiles in (1, 4):
+            C.core.tune_set("mfma_batch_tiles", tiles)
+            batch = circuit_batch(circs, dtype="complex64")
+            batch.run()
+            states[tiles] = [batch.state(j) for j in (0, 13, 39)]
+            del batch
+    finally:
+        C.core.tune_set("mfma_batch_tiles", saved)
+    for j, a, b in zip((0, 13, 39), states[1], states[4]):
+        assert np.array_equal(a, b), j
+        sv = device_state(C, 20, "complex64")
+        C.functionals.Program(20, fuse_operations(circs[j][1], 5), dtype="complex64").run(sv)
+        assert np.abs(b.astype(np.complex128) - to_np(C, sv)).max() < 2e-5, j
