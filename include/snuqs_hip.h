@@ -89,9 +89,10 @@ int sq_copy(void *dst, const void *src, size_t bytes, void *stream);
  * with 512 B runs where that needs no more passes than 8-bit windows with
  * 128 B runs; default 1) and "adjoint_fuse_runs" (0-3, diagonal runs folded
  * into the low pass of the layer next to them: bit 0 forward, bit 1 reverse;
- * default 3) and "mfma_batch_tiles" (1-64, tiles per wave of the 4-5 target
- * MFMA kernels in parameter and circuit batches, which size their grids by it
- * when built, lowered where a launch would drop below 2048 workgroups;
+ * default 3) and "mfma_tiles_per_wave" (1-64, tiles per wave of the 4-5 target
+ * MFMA kernels in programs without a tuned shape and in parameter and circuit
+ * batches (which size their grids by it when built), lowered where a launch
+ * would drop below 2048 workgroups; the grid cap above still applies;
  * default 4).  A gate tuned by
  * sq_program_tune uses its own shape instead.  No reference counterpart. */
 int sq_tune_set(const char *key, int value);

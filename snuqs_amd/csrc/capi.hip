@@ -167,7 +167,7 @@ static int *tune_slot(const std::string &k) {
       {"adjoint_wide_windows", [] { return &tuning().adjoint_wide_windows; }},
       {"adjoint_fuse_runs", [] { return &tuning().adjoint_fuse_runs; }},
       {"mfma5_wide_c64", [] { return &tuning().mfma5_wide; }},
-      {"mfma_batch_tiles", [] { return &tuning().mfma_batch_tiles; }},
+      {"mfma_tiles_per_wave", [] { return &tuning().mfma_tiles_per_wave; }},
   };
   for (auto &e : table)
     if (k == e.name) return e.slot();
@@ -185,8 +185,8 @@ int sq_tune_set(const char *key, int value) {
   const bool flag = k.rfind("mfma5_pipeline", 0) == 0 || k.rfind("mfma5_wide", 0) == 0 ||
                     k == "qaoa_mid_scatter" || k == "tune_scatter" || k == "adjoint_layers" ||
                     k == "adjoint_wide_windows";
-  if (k == "mfma_batch_tiles" && (value < 1 || value > 64))
-    return fail(SQ_ERR_INVALID, "mfma_batch_tiles must be in [1, 64]");
+  if (k == "mfma_tiles_per_wave" && (value < 1 || value > 64))
+    return fail(SQ_ERR_INVALID, "mfma_tiles_per_wave must be in [1, 64]");
   if (k == "adjoint_fuse_runs" && (value < 0 || value > 3))
     return fail(SQ_ERR_INVALID, "adjoint_fuse_runs must be in [0, 3]");
   *slot = flag ? (value ? 1 : 0) : value;

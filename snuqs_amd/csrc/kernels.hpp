@@ -122,7 +122,7 @@ struct Tuning {
   int mfma5_pipeline[2] = {1, 1};        // 5-target MFMA: prefetch the next tile
   int mfma5_order[2] = {0, 0};           // 5-target MFMA: 0 cyclic tiles, 1 blocked, 2 scattered
   int mfma5_wide = 0;                    // complex64 5-target MFMA: 64-group tiles
-  int mfma_batch_tiles = 4;              // parameter / circuit batches: tiles per wave of the MFMA kernels
+  int mfma_tiles_per_wave = 4;           // 4-5 target MFMA, untuned programs and batches: tiles per wave
   int qaoa_mid_scatter = 0;              // QAOA window passes above bit 11: bit-reversed chunk order
   int tune_scatter = 1;                  // sq_program_tune also tries tile order 2
   int adjoint_layers = 1;                // adjoint sweeps: layer passes (12 / 8 bits a pass)
@@ -155,9 +155,10 @@ uint64_t cbatch_mfma_tiles(const GatePlan &p, bool c64, uint64_t nelems);
 bool cbatch_mfma_entry(const GatePlan &p, bool c64, uint64_t nelems, uint64_t psi,
                        uint64_t first_block, uint64_t tiles_per_wave,
                        std::vector<double> *g3m, void *entry, uint64_t *nblocks);
-// Tiles per wave of a batch launch of the MFMA kernels over total_tiles tiles
-// (the mfma_batch_tiles knob, lowered to keep 2048 workgroups or more).
-uint64_t batch_tiles_per_wave(uint64_t total_tiles);
+// Tiles per wave of a launch of the 4-5 target MFMA kernels over total_tiles
+// tiles (untuned programs and batches; the mfma_tiles_per_wave knob, lowered
+// to keep 2048 workgroups or more).
+uint64_t mfma_wave_tiles(uint64_t total_tiles);
 int launch_cbatch_mfma(sq_dtype_t dt, void *states, const void *g3m, const void *tab, int count,
                        uint64_t blocks, hipStream_t st);
 

@@ -245,7 +245,7 @@ int cbatch_create(void **handle, sq_dtype_t dt, uint32_t ncirc, const uint32_t *
     uint64_t mtiles = 0;  // tiles of the step's MFMA entries (sets their grid)
     for (uint32_t j = 0; j < ncirc; ++j)
       if (s < ng[j]) mtiles += cbatch_mfma_tiles(wplans[j][s], c64, 1ull << nq[j]);
-    const uint64_t tpw = batch_tiles_per_wave(mtiles);
+    const uint64_t tpw = mfma_wave_tiles(mtiles);
     for (uint32_t j = 0; j < ncirc; ++j) {
       if (s >= ng[j]) continue;
       B->traffic_bytes += 2.0 * std::ldexp(1.0, (int)nq[j]) * (double)item;

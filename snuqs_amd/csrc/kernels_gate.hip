@@ -1505,6 +1505,10 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
           int ord = p.shape.order >= 0 ? p.shape.order : tu.mfma5_order[di];
           if (ord < 0 || ord > 2) ord = 0;
           const uint64_t cap = 256ull * (uint64_t)bpc;
+          // untuned: each wave walks several tiles where the grid keeps 2048
+          // workgroups or more (a tuned shape keeps its own grid)
+          const uint64_t tpw = p.shape.blocks_per_cu > 0 ? 1 : mfma_wave_tiles(ntiles);
+          blocks = (ntiles + 4 * tpw - 1) / (4 * tpw);
           if (blocks > cap) blocks = cap;
           using Q = typename Q4<R>::T;
           if constexpr (c64) {
@@ -1515,7 +1519,9 @@ static int launch_plan_t(const GatePlan &p, typename CV<R>::T *sv,
                                  (float2 *)sv, ntiles, im, off, (const Q *)gdev);
             } else if (wide && ntiles % 2 == 0 && make_stage_wide(p, &sw)) {
               const uint64_t wt = ntiles / 2;  // 64-group tiles
-              const uint64_t wb = (wt + 3) / 4 < cap ? (wt + 3) / 4 : cap;
+              const uint64_t wtpw = p.shape.blocks_per_cu > 0 ? 1 : mfma_wave_tiles(wt);
+              const uint64_t wb0 = (wt + 4 * wtpw - 1) / (4 * wtpw);
+              const uint64_t wb = wb0 < cap ? wb0 : cap;
               auto k = mfma5_kernel(k_mfma5w_c64<true, 0>, k_mfma5w_c64<true, 1>,
                                     k_mfma5w_c64<true, 2>, k_mfma5w_c64<false, 0>,
                                     k_mfma5w_c64<false, 1>, k_mfma5w_c64<false, 2>, pipe,
@@ -1597,15 +1603,16 @@ __global__ void __launch_bounds__(kBlock)
 
 size_t cbatch_mfma_entry_bytes() { return sizeof(CBMfma); }
 
-// Tiles per wave of a batch launch of the MFMA kernels over `total_tiles`
-// tiles: up to the mfma_batch_tiles knob (a wave that walks several tiles
-// overlaps the next tile's loads with this one's products and stages the
-// matrix once for all of them), but never so many that the launch drops below
-// kBatchMinBlocks workgroups (8 per CU).
-constexpr uint64_t kBatchMinBlocks = 2048;
-uint64_t batch_tiles_per_wave(uint64_t total_tiles) {
-  uint64_t t = (uint64_t)tuning().mfma_batch_tiles;
-  while (t > 1 && total_tiles < 4 * t * kBatchMinBlocks) t >>= 1;
+// Tiles per wave of a launch of the 4-5 target MFMA kernels over
+// `total_tiles` tiles (untuned programs, parameter and circuit batches): up
+// to the mfma_tiles_per_wave knob (a wave that walks several tiles overlaps
+// the next tile's loads with this one's products and stages the block matrix
+// once for all of them), but never so many that the launch drops below
+// kMinWaveTileBlocks workgroups (8 per CU).
+constexpr uint64_t kMinWaveTileBlocks = 2048;
+uint64_t mfma_wave_tiles(uint64_t total_tiles) {
+  uint64_t t = (uint64_t)tuning().mfma_tiles_per_wave;
+  while (t > 1 && total_tiles < 4 * t * kMinWaveTileBlocks) t >>= 1;
   return t;
 }
 
@@ -1843,7 +1850,7 @@ static int launch_batch_t(const GatePlan &p, uint64_t nelems, uint32_t nsets,
     case BatchKernel::MFMA5: {
       const int di = sizeof(R) == 4 ? 0 : 1;
       const Tuning &tu = tuning();
-      const uint64_t per = 4 * batch_tiles_per_wave(b.work * nsets);  // tiles per workgroup
+      const uint64_t per = 4 * mfma_wave_tiles(b.work * nsets);  // tiles per workgroup
       uint64_t nb = (b.work + per - 1) / per;
       const uint64_t cap = 256ull * (uint64_t)tu.mfma_blocks_per_cu[di];
       if (nb > cap) nb = cap;

@@ -56,8 +56,8 @@ def test_tuning_knobs_round_trip_without_gpu(tmp_path):
             _C.core.tune_set("no_such_knob", 1)
         for bad in (0, 65):
             with pytest.raises(ValueError):
-                _C.core.tune_set("mfma_batch_tiles", bad)
-        assert _C.core.tune_get("mfma_batch_tiles") == 4
+                _C.core.tune_set("mfma_tiles_per_wave", bad)
+        assert _C.core.tune_get("mfma_tiles_per_wave") == 4
     finally:
         for k, v in saved.items():
             _C.core.tune_set(k, v)

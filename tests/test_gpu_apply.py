@@ -436,3 +436,33 @@ def test_fused_permutation_blocks(C, dtype):
     got = run_engine(C, n, fused, dtype, init)
     ref = run_oracle(n, ops, init.astype(np.complex128))
     assert np.abs(got - ref).max() <= TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype,n", [("complex64", 26), ("complex128", 24)])
+def test_program_tiles_per_wave_bitwise(C, dtype, n):
+    """Untuned programs large enough for the 4-5 target kernels to walk 4
+    tiles per wave (mfma_tiles_per_wave 4) give the same bits as 1 tile per
+    wave (narrow and, for complex64, wide tiles)."""
+    from snuqs_amd.circuits import random_layered
+    from snuqs_amd.transpile import fuse_operations
+
+    ops = fuse_operations(random_layered(n, 4, seed=n), 5)
+    assert any(len(op.targets) == 5 for op in ops)
+    keys = ("mfma_tiles_per_wave", "mfma5_wide_c64")
+    saved = [C.core.tune_get(k) for k in keys]
+    out = {}
+    try:
+        for wide in ((0, 1) if dtype == "complex64" else (0,)):
+            C.core.tune_set(keys[1], wide)
+            for tiles in (1, 4):
+                C.core.tune_set(keys[0], tiles)
+                sv = C.result_types.StateVector(C.DeviceType.CUDA, n, dtype=dtype)
+                C.functionals.initialize_basis_z(sv)
+                C.functionals.Program(n, ops, dtype=dtype).run(sv)
+                out[(wide, tiles)] = np.array(sv.cpu(), copy=True)
+                del sv
+    finally:
+        for k, v in zip(keys, saved):
+            C.core.tune_set(k, v)
+    for wide in ((0, 1) if dtype == "complex64" else (0,)):
+        assert np.array_equal(out[(wide, 1)], out[(wide, 4)]), wide

@@ -207,7 +207,7 @@ def test_batch_argument_errors(C):
 @pytest.mark.parametrize("dtype", ["complex64", "complex128"])
 def test_batch_tiles_per_wave_bitwise(C, dtype):
     """32 sets of 20 qubits: enough tiles for the 5-target blocks to walk 4
-    tiles per wave (mfma_batch_tiles 4) as well as 1; both grids give the same
+    tiles per wave (mfma_tiles_per_wave 4) as well as 1; both grids give the same
     bits, equal to separate programs."""
     from snuqs_amd.batch import fuse_operations_batch
     from snuqs_amd.circuits import hea_ansatz
@@ -217,11 +217,11 @@ def test_batch_tiles_per_wave_bitwise(C, dtype):
     thetas = rng.uniform(0, 2 * np.pi, (nsets, 2 * n * layers))
     ss = fuse_operations_batch([hea_ansatz(n, layers, t)[0] for t in thetas], 5)
     assert any(len(op.targets) == 5 for op in ss[0])
-    saved = C.core.tune_get("mfma_batch_tiles")
+    saved = C.core.tune_get("mfma_tiles_per_wave")
     got = {}
     try:
         for tiles in (1, 4):
-            C.core.tune_set("mfma_batch_tiles", tiles)
+            C.core.tune_set("mfma_tiles_per_wave", tiles)
             prog = C.functionals.BatchProgram(n, ss, dtype=dtype)
             b = C.result_types.StateBatch(nsets, n, dtype=dtype)
             b.initialize_basis_z()
@@ -229,7 +229,7 @@ def test_batch_tiles_per_wave_bitwise(C, dtype):
             got[tiles] = b.cpu()
             del prog, b
     finally:
-        C.core.tune_set("mfma_batch_tiles", saved)
+        C.core.tune_set("mfma_tiles_per_wave", saved)
     assert np.array_equal(got[1], got[4])
     want = _separate(C, n, [ss[0], ss[17], ss[31]], dtype)
     assert np.array_equal(got[4][[0, 17, 31]], want)

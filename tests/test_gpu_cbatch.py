@@ -137,17 +137,17 @@ def test_cbatch_tiles_per_wave_bitwise(C):
     from snuqs_amd.transpile import fuse_operations
 
     circs = [(20, random_layered(20, 3, seed=700 + j)) for j in range(40)]
-    saved = C.core.tune_get("mfma_batch_tiles")
+    saved = C.core.tune_get("mfma_tiles_per_wave")
     states = {}
     try:
         for tiles in (1, 4):
-            C.core.tune_set("mfma_batch_tiles", tiles)
+            C.core.tune_set("mfma_tiles_per_wave", tiles)
             batch = circuit_batch(circs, dtype="complex64")
             batch.run()
             states[tiles] = [batch.state(j) for j in (0, 13, 39)]
             del batch
     finally:
-        C.core.tune_set("mfma_batch_tiles", saved)
+        C.core.tune_set("mfma_tiles_per_wave", saved)
     for j, a, b in zip((0, 13, 39), states[1], states[4]):
         assert np.array_equal(a, b), j
         sv = device_state(C, 20, "complex64")
