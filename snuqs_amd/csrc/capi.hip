@@ -630,11 +630,14 @@ int sq_program_tune(void *program, void *sv, uint64_t n, int reps, void *stream,
   // after launch, mispredicts it); every gate then keeps its fastest shape.
   std::vector<GatePlan::Shape> cands;
   for (int order = 0; order < (tuning().tune_scatter ? 3 : 2); ++order) {
-    for (int bpc : {64, 256, 1024}) cands.push_back({bpc, order, 1, 0});
+    // grid caps of 64-1024 blocks per CU: at 2^20 tiles, 16 down to 1 tile
+    // per wave
+    for (int bpc : {64, 128, 256, 512, 1024}) cands.push_back({bpc, order, 1, 0});
     for (int bpc : {256, 1024}) cands.push_back({bpc, order, 0, 0});
-    if (p->dt == SQ_COMPLEX64)  // 64-group tiles: longer runs, half the waves per CU
-      for (int pipe = 0; pipe < 2; ++pipe)
-        for (int bpc : {256, 1024}) cands.push_back({bpc, order, pipe, 1});
+    if (p->dt == SQ_COMPLEX64) {  // 64-group tiles: longer runs, half the waves per CU
+      for (int bpc : {128, 256, 512, 1024}) cands.push_back({bpc, order, 1, 1});
+      for (int bpc : {256, 1024}) cands.push_back({bpc, order, 0, 1});
+    }
   }
   std::vector<hipEvent_t> ev(ng + 1);
   for (auto &e : ev) SQ_HIP(hipEventCreate(&e));
