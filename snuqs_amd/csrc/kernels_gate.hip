@@ -545,7 +545,9 @@ __global__ void __launch_bounds__(kBlock)
 // of element j is linear over GF(2) in j's bits (per-bit weights XORed; an
 // XOR swizzle keeps both access orders free of bank conflicts), so lanes
 // need one address constant plus uniform per-k terms.  Loads of the next
-// tile are issued before the current tile's MFMAs (PIPE).
+// tile are issued before the current tile's MFMAs (PIPE), and a wave's first
+// tile is requested before the workgroup stages the block matrix in LDS, so
+// the two latencies overlap.
 // ---------------------------------------------------------------------------
 // Tiles of wave `wave` of this workgroup, by tile order ORD:
 //   0 cyclic    tile 4b + w, then + 4 x grid (concurrent waves on neighbouring
@@ -624,14 +626,12 @@ __device__ __forceinline__ void
   constexpr int D = 32, NS = 16;
   __shared__ float4 As[D * D];      // [o][r], 16 KB
   __shared__ float2 Xs[4][D * 32];  // one tile per wave, 32 KB
-  for (int e = threadIdx.x; e < D * D; e += kBlock) As[e] = G[e];
-  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 31, half = lane >> 5;
   float2 *X = Xs[wave];
   TileRange tr = tile_range<ORD>(ntiles, wave, wg, nwg);
   uint64_t tile = tr.first;
-  if (tile >= tr.end) return;
+  const bool has = tile < tr.end;
   // j = 2 lane + 128 k + e: lane bits are tile bits 1-6, k bits tile bits 7-9
   uint64_t la = 0;
   uint32_t lw = 0;
@@ -647,9 +647,16 @@ __device__ __forceinline__ void
   ((((k) & 1) ? sa.w[7] : 0u) ^ (((k) & 2) ? sa.w[8] : 0u) ^ (((k) & 4) ? sa.w[9] : 0u))
   const uint32_t w0 = sa.w[0];
   float4 cur[8], nxt[8];
-  uint64_t base = deposit(tile_at<ORD>(tile, tr) * 32, im) + la;
+  uint64_t base = 0;
+  if (has) {
+    base = deposit(tile_at<ORD>(tile, tr) * 32, im) + la;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) cur[k] = ld_nt(reinterpret_cast<const float4 *>(sv + base + SQ_KA(k)));
+    for (int k = 0; k < 8; ++k) cur[k] = ld_nt(reinterpret_cast<const float4 *>(sv + base + SQ_KA(k)));
+  }
+  // the block matrix is staged while the first tile's loads are in flight
+  for (int e = threadIdx.x; e < D * D; e += kBlock) As[e] = G[e];
+  __syncthreads();
+  if (!has) return;
   while (true) {
     const uint64_t ntile = tile + tr.step;
     const bool more = ntile < tr.end;
@@ -746,14 +753,12 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int D = 32, NS = 16, NK = 16;
   __shared__ float4 As[D * D];          // [o][r], 16 KB
   __shared__ float2 Xs[4][2 * D * 32];  // one tile per wave, 64 KB
-  for (int e = threadIdx.x; e < D * D; e += kBlock) As[e] = G[e];
-  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 31, half = lane >> 5;
   float2 *X = Xs[wave];
   TileRange tr = tile_range<ORD>(ntiles, wave);
   uint64_t tile = tr.first;
-  if (tile >= tr.end) return;
+  const bool has = tile < tr.end;
   // j = 2 lane + 128 k + e: lane bits are tile bits 1-6, k bits tile bits 7-10
   uint64_t la = 0;
   uint32_t lw = 0;
@@ -771,10 +776,17 @@ __global__ void __launch_bounds__(kBlock)
    (((k) & 8) ? sa.w[10] : 0u))
   const uint32_t w0 = sa.w[0];
   float4 cur[NK], nxt[NK];
-  uint64_t base = deposit(tile_at<ORD>(tile, tr) * 64, im) + la;
+  uint64_t base = 0;
+  if (has) {
+    base = deposit(tile_at<ORD>(tile, tr) * 64, im) + la;
 #pragma unroll
-  for (int k = 0; k < NK; ++k)
-    cur[k] = ld_nt(reinterpret_cast<const float4 *>(sv + base + SQ_KA(k)));
+    for (int k = 0; k < NK; ++k)
+      cur[k] = ld_nt(reinterpret_cast<const float4 *>(sv + base + SQ_KA(k)));
+  }
+  // the block matrix is staged while the first tile's loads are in flight
+  for (int e = threadIdx.x; e < D * D; e += kBlock) As[e] = G[e];
+  __syncthreads();
+  if (!has) return;
   while (true) {
     const uint64_t ntile = tile + tr.step;
     const bool more = ntile < tr.end;
@@ -847,14 +859,12 @@ __device__ __forceinline__ void
   constexpr int D = 32, NS = 8, NT = 2;
   __shared__ D4 As[D * D];           // [o][r], 32 KB
   __shared__ double2 Xs[4][D * 16];  // one tile per wave, 32 KB
-  for (int e = threadIdx.x; e < D * D; e += kBlock) As[e] = G[e];
-  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, q = lane >> 4;
   double2 *X = Xs[wave];
   TileRange tr = tile_range<ORD>(ntiles, wave, wg, nwg);
   uint64_t tile = tr.first;
-  if (tile >= tr.end) return;
+  const bool has = tile < tr.end;
   // j = lane + 64 k: lane bits are tile bits 0-5, k bits tile bits 6-8
   uint64_t la = 0;
   uint32_t lw = 0;
@@ -869,9 +879,16 @@ __device__ __forceinline__ void
 #define SQ_KW(k) \
   ((((k) & 1) ? sa.w[6] : 0u) ^ (((k) & 2) ? sa.w[7] : 0u) ^ (((k) & 4) ? sa.w[8] : 0u))
   double2 cur[8], nxt[8];
-  uint64_t base = deposit(tile_at<ORD>(tile, tr) * 16, im) + la;
+  uint64_t base = 0;
+  if (has) {
+    base = deposit(tile_at<ORD>(tile, tr) * 16, im) + la;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) cur[k] = ld_nt(sv + base + SQ_KA(k));
+    for (int k = 0; k < 8; ++k) cur[k] = ld_nt(sv + base + SQ_KA(k));
+  }
+  // the block matrix is staged while the first tile's loads are in flight
+  for (int e = threadIdx.x; e < D * D; e += kBlock) As[e] = G[e];
+  __syncthreads();
+  if (!has) return;
   while (true) {
     const uint64_t ntile = tile + tr.step;
     const bool more = ntile < tr.end;
