@@ -37,9 +37,10 @@ struct CBGate {
   uint64_t first_block;  // first workgroup of the gate in its launch
   uint64_t nblocks;      // its workgroups (grid stride over its groups)
   uint64_t mat;          // element offset of its matrix (plan order, row-major)
-  uint64_t off[1 << kMaxK];  // amplitude offset of each local index in a group
   IdxMap im;             // group index -> amplitude index (target bits 0)
   int k, pad0, pad1, pad2;
+  int tb[kMaxK];         // amplitude bit of target b (local index bit b)
+  int pad3[8 - kMaxK];
 };
 
 // The entry whose workgroup range holds workgroup b (first_block ascending).
@@ -64,21 +65,40 @@ __device__ __forceinline__ int cb_find(const CBGate *tab, int count, uint64_t b)
   return __builtin_amdgcn_readfirstlane(lo);
 }
 
-// y = M x on every group of the gate assigned to this workgroup.  M is in
-// LDS as (re, im) pairs, row-major: a complex64 row is read two entries per
-// 16-byte access.
+// y = M x on every group of the gate assigned to this workgroup.  M is
+// staged in LDS as (re, im) pairs, row-major (a complex64 row is read two
+// entries per 16-byte access), after the workgroup's first groups are
+// requested: the two latencies overlap.  A group's amplitude offsets come
+// from the target bits (uniform), so no table sits between the entry and
+// the loads.  Every thread of the workgroup must call it (it synchronises).
 template <typename R, int K>
 __device__ __forceinline__ void cb_body(typename CV<R>::T *__restrict__ sv, const CBGate &g,
-                                        uint64_t wg, const typename CV<R>::T *M,
-                                        const uint64_t *offs) {
+                                        uint64_t wg, const typename CV<R>::T *__restrict__ mat,
+                                        typename CV<R>::T *M) {
   using T = typename CV<R>::T;
   constexpr int D = 1 << K;
-  const uint64_t stride = g.nblocks * kBlock;
-  for (uint64_t f = wg * kBlock + threadIdx.x; f < g.nfree; f += stride) {
-    const uint64_t base = deposit(f, g.im);
-    T x[D];
+  int tb[K];
 #pragma unroll
-    for (int c = 0; c < D; ++c) x[c] = ld_nt(sv + base + offs[c]);
+  for (int b = 0; b < K; ++b) tb[b] = g.tb[b];
+  auto offc = [&](int c) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int b = 0; b < K; ++b)
+      if ((c >> b) & 1) o += 1ull << tb[b];
+    return o;
+  };
+  const uint64_t stride = g.nblocks * kBlock;
+  uint64_t f = wg * kBlock + threadIdx.x;
+  T x[D];
+  uint64_t base = 0;
+  if (f < g.nfree) {
+    base = deposit(f, g.im);
+#pragma unroll
+    for (int c = 0; c < D; ++c) x[c] = ld_nt(sv + base + offc(c));
+  }
+  for (int e = threadIdx.x; e < D * D; e += kBlock) M[e] = mat[e];
+  __syncthreads();
+  while (f < g.nfree) {
     // 1-2 targets: rows unrolled; 3-5: one row at a time (the D inputs stay
     // in registers, the row's entries are LDS broadcasts; unrolled rows would
     // hold the whole matrix in registers)
@@ -106,7 +126,13 @@ __device__ __forceinline__ void cb_body(typename CV<R>::T *__restrict__ sv, cons
       T o;
       o.x = yr;
       o.y = yi;
-      st_nt(sv + base + offs[r], o);
+      st_nt(sv + base + offc(r), o);
+    }
+    f += stride;
+    if (f < g.nfree) {
+      base = deposit(f, g.im);
+#pragma unroll
+      for (int c = 0; c < D; ++c) x[c] = ld_nt(sv + base + offc(c));
     }
   }
 }
@@ -118,21 +144,17 @@ __global__ void __launch_bounds__(kBlock)
   using T = typename CV<R>::T;
   constexpr int DM = 1 << kMaxK;
   __shared__ __align__(16) T M[DM * DM];
-  __shared__ uint64_t offs[DM];
   const int e = cb_find(tab, count, blockIdx.x);
   const CBGate &g = tab[e];
-  const int k = g.k, d = 1 << k;
-  for (int x = threadIdx.x; x < d * d; x += kBlock) M[x] = mats[g.mat + x];
-  if ((int)threadIdx.x < d) offs[threadIdx.x] = g.off[threadIdx.x];
-  __syncthreads();
   T *sv = st + g.psi;
+  const T *mat = mats + g.mat;
   const uint64_t wg = blockIdx.x - g.first_block;
-  switch (k) {
-    case 1: cb_body<R, 1>(sv, g, wg, M, offs); break;
-    case 2: cb_body<R, 2>(sv, g, wg, M, offs); break;
-    case 3: cb_body<R, 3>(sv, g, wg, M, offs); break;
-    case 4: cb_body<R, 4>(sv, g, wg, M, offs); break;
-    default: cb_body<R, 5>(sv, g, wg, M, offs); break;
+  switch (g.k) {
+    case 1: cb_body<R, 1>(sv, g, wg, mat, M); break;
+    case 2: cb_body<R, 2>(sv, g, wg, mat, M); break;
+    case 3: cb_body<R, 3>(sv, g, wg, mat, M); break;
+    case 4: cb_body<R, 4>(sv, g, wg, mat, M); break;
+    default: cb_body<R, 5>(sv, g, wg, mat, M); break;
   }
 }
 
@@ -264,13 +286,10 @@ int cbatch_create(void **handle, sq_dtype_t dt, uint32_t ncirc, const uint32_t *
       e.k = p.kk;
       e.nfree = 1ull << (nq[j] - p.kk);
       uint64_t tmask = 0;
-      for (int r = 0; r < (1 << p.kk); ++r) {
-        uint64_t o = 0;
-        for (int b = 0; b < p.kk; ++b)
-          if ((r >> b) & 1) o += 1ull << p.tbit[b];
-        e.off[r] = o;
+      for (int b = 0; b < p.kk; ++b) {
+        tmask |= 1ull << p.tbit[b];
+        e.tb[b] = p.tbit[b];
       }
-      for (int b = 0; b < p.kk; ++b) tmask |= 1ull << p.tbit[b];
       e.im = make_idxmap(tmask, 0);
       e.mat = hm.size() / 2;
       for (const cd &z : p.m) {
