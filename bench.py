@@ -480,6 +480,9 @@ def main():
     ap.add_argument("--skip-configs", action="store_true",
                     help="skip the config 1, config 2 complex128 and config 5 legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--gate-times", default=None, metavar="PATH",
+                    help="write the fused circuit's per-gate times (HIP events, mean over the "
+                         "timed steps) with each gate's target bits and class to PATH as JSON")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -526,6 +529,15 @@ def main():
     flops_exec = 0.75 * flops
     norm = C.functionals.norm2(sv)
     log(f"fused: {wall / args.steps * 1e3:.1f} ms/step, {value:.0f} GB/s, norm^2 {norm:.12f}")
+    if args.gate_times:
+        rows = []
+        for g, (op, (cls, kk, b)) in enumerate(zip(fused, info)):
+            ms = float(acc_ms[g]) / args.steps
+            rows.append({"gate": g, "bits": sorted(n - 1 - q for q in op.targets), "class": int(cls),
+                         "kk": int(kk), "ms": ms, "GBps": b / (ms * 1e-3) / 1e9 if ms > 0 else None})
+        with open(args.gate_times, "w") as f:
+            json.dump({"qubits": n, "dtype": dtype, "steps": args.steps, "gates": rows}, f)
+        log(f"per-gate times written to {args.gate_times}")
 
     # copy-kernel ceiling on the same state size
     tmp = C.result_types.StateVector(C.DeviceType.CUDA, n, dtype=dtype)
