@@ -109,23 +109,29 @@ def test_cbatch_copy_state_checks(C):
     assert a[0] == 1 and np.count_nonzero(a) == 1
 
 
-def test_cbatch_many_circuits(C):
+@pytest.mark.parametrize("dtype,tol", [("complex64", 2e-5), ("complex128", 1e-12)])
+def test_cbatch_many_circuits(C, dtype, tol):
     """300 circuits (more entries per launch than a workgroup stages in LDS,
     so entries are found by the global search), 2-11 qubits, fused (5-target
-    blocks on states with full tiles take the MFMA entry table), complex64
-    against the oracle."""
+    blocks on states with full tiles take the MFMA entry table) against the
+    oracle; a second run of the batch gives the same bits."""
     from snuqs_amd.cbatch import circuit_batch
     from snuqs_amd.circuits import random_layered
 
     rng = np.random.default_rng(300)
     sizes = [int(rng.integers(2, 12)) for _ in range(300)]
     circs = [(n, random_layered(n, 3, seed=5000 + j)) for j, n in enumerate(sizes)]
-    batch = circuit_batch(circs, dtype="complex64")
+    batch = circuit_batch(circs, dtype=dtype)
     batch.run()
-    for j in rng.choice(len(circs), 40, replace=False):
+    picks = [int(j) for j in rng.choice(len(circs), 40, replace=False)]
+    first = {j: batch.state(j) for j in picks}
+    for j in picks:
         n, ops = circs[j]
-        got = batch.state(int(j)).astype(np.complex128)
-        assert np.abs(got - run_oracle(n, ops)).max() < 2e-5, (j, n)
+        got = first[j].astype(np.complex128)
+        assert np.abs(got - run_oracle(n, ops)).max() < tol, (j, n)
+    batch.run()
+    for j in picks:
+        assert np.array_equal(batch.state(j), first[j]), j
 
 
 def test_cbatch_tiles_per_wave_bitwise(C):
