@@ -465,6 +465,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--tune-reps", type=int, default=2,
+                    help="timed program runs per candidate shape when tuning")
     ap.add_argument("--no-tune", action="store_true",
                     help="run with the default launch shapes (no per-gate tuning)")
     ap.add_argument("--qubits", type=int, default=30)
@@ -507,7 +509,7 @@ def main():
         # per-gate launch shapes of the 5-target MFMA kernels, chosen by timing
         # on this state before the timed region (the state is reset after)
         t0 = time.perf_counter()
-        ntuned = prog.tune(sv, reps=2)
+        ntuned = prog.tune(sv, reps=args.tune_reps)
         tuning = {"tuned_gates": int(ntuned), "seconds": time.perf_counter() - t0}
         log(f"tuned {ntuned} gates in {tuning['seconds']:.1f} s")
     C.functionals.initialize_basis_z(sv)
